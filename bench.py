@@ -1,0 +1,141 @@
+"""Benchmark of the MPC-MD hot path on one MI355X.
+
+Workload (the single-GPU configuration the headline metric is quoted on):
+plain MPC-MD, cubic box L=64 (262,144 unit cells), rho=10 solvent particles
+per cell (2,621,440), FP32 solvent state with 64-bit fixed-point cell sums;
+one star of f=20 arms x (15 A + 15 B) beads (601 monomers, mass 10); shear
+rate 0.05, SRD angle 130 deg, collision time h=0.1, MD dt=0.01, 10
+velocity-Verlet substeps per collision.  A "step" is one collision interval
+(10 MD substeps + one collision), replayed from a HIP graph.
+
+Prints one JSON line: particle-updates/s (solvent particles x collisions per
+second), the roofline of the collision pipeline measured with HIP events on
+the library's stream, and the FP64 CPU oracle timed on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 1000] [--warmup 100]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MPC particle-updates/s (L=64,rho=10); % HBM peak; star-steps/s learned vs MPC"
+HBM_PEAK_GBS = 8000.0
+B_ALG = 48  # bytes per particle-update: FP32 position+velocity read once and written once
+
+WORKLOAD = dict(L=(64, 64, 64), n_c=10, shear=0.05, step_update=10, mpc_dt=0.1, md_dt=0.01,
+                arms=20, A=15, B=15, mass=10.0, seed=1)
+
+
+def make_system(w):
+    from starpolymersmd_amd import MPCSystem, make_params
+
+    g = MPCSystem(make_params(L=w["L"], n_c=w["n_c"], shear=w["shear"], step_update=w["step_update"],
+                              md_dt=w["md_dt"], seed=w["seed"], lam=1.0, kT=1.0))
+    g.add_star(w["A"], w["B"], w["arms"], w["mass"])
+    g.md_init()
+    g.initialize()
+    return g
+
+
+def cpu_baseline(w, intervals):
+    """FP64 oracle (faithful single-threaded restatement of the reference
+    CPU path) on the same workload shape for a bounded number of intervals."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_binding as ob
+
+    o = ob.Oracle(L=w["L"], n_c=w["n_c"], shear=w["shear"], step_update=w["step_update"], md_dt=w["md_dt"],
+                  rng_mode=ob.RNG_PHILOX, seed=w["seed"])
+    o.add_star(w["A"], w["B"], w["arms"], w["mass"])
+    o.md_init()
+    o.initialize()
+    S = w["step_update"]
+    o.propagate()
+    o.step(0, w["mpc_dt"])  # first collision outside the timed region
+    t0 = time.perf_counter()
+    t = 1
+    for _ in range(intervals):
+        for _ in range(S):
+            o.propagate()
+            o.step(t, w["mpc_dt"])
+            t += 1
+    el = time.perf_counter() - t0
+    n = o.n_fluid
+    return {"value": n * intervals / el, "unit": "particle-updates/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle, L=64 rho=10 f=20 15A+15B, {intervals} collision intervals "
+                      f"(10 MD substeps each), 1 thread, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--cpu-intervals", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the MPC box would be "
+              "slab-decomposed along z with per-face halo sums; not built). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    w = WORKLOAD
+    S, h = w["step_update"], w["mpc_dt"]
+    g = make_system(w)
+    n_f = g.num_fluid()
+    t = 0
+    g.run(t, 1, h)  # t=0: first collision
+    t += 1
+    g.run(t, a.warmup * S, h)
+    t += a.warmup * S
+    g.sync()
+    t0 = time.perf_counter()
+    g.run(t, a.steps * S, h)
+    g.sync()
+    el = time.perf_counter() - t0
+    t += a.steps * S
+    ms_per_step = el * 1e3 / a.steps
+    value = n_f * a.steps / el
+
+    # roofline pass: HIP events around every launch on the library's stream
+    g.profile(True)
+    g.profile_reset()
+    g.run(t, a.profile_steps * S, h)
+    g.sync()
+    t += a.profile_steps * S
+    prof = {k: g.profile_read(k)[0] for k in ("md", "bin", "scan", "scatter", "collide")}
+    g.profile(False)
+    pipe_ms = prof["bin"] + prof["scan"] + prof["scatter"] + prof["collide"]
+    achieved = B_ALG * n_f / (pipe_ms * 1e-3) / 1e9
+    T = g.temperature()
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "plain MPC-MD L=64 rho=10 (2,621,440 solvent), star f=20 15A+15B "
+                               "(601 monomers), shear 0.05, h=0.1, dt=0.01, 10 MD substeps/collision",
+                   "solvent_particles": n_f, "step": "one collision interval", "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "collision pipeline k_bin+k_scan+k_scatter+k_collide",
+                     "alg_bytes_per_launch": B_ALG * n_f, "avg_ms": pipe_ms,
+                     "per_kernel_ms": prof},
+        "md_us_per_interval": prof["md"] * 1e3,
+        "temperature": T,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, a.cpu_intervals)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,159 @@
+/*
+ * spmd.h -- C ABI of the MI355X MPC-MD library (libspmd.so).
+ *
+ * The reference has no plugin ABI: its hot path is the C++ classes MPC
+ * (star-polymers/lib/Hydrodynamics/MPC.h:25-122), Thermostat_None
+ * (lib/Thermostats/Thermostat_None.h:14) and Box (lib/Box.h:30-121).  Each
+ * entry point below replaces one of their members, cited beside it; the C++
+ * facade in starpolymersmd_amd/facade/ puts the reference's class names and
+ * signatures back on top of this ABI.
+ *
+ * Conventions
+ *  - Every function returns an int status (SPMD_OK = 0).  No exceptions cross
+ *    the ABI; spmd_last_error() gives a message for the last failure.
+ *  - Host pointers are caller-owned and copied synchronously.
+ *  - One HIP stream per handle; a handle is not thread-safe (the reference is
+ *    single-threaded with a global RNG, lib/Rand.cpp:6).
+ *  - A handle holds one or more replicas (independent boxes).  The
+ *    single-system reference API is replica 0 of a handle made by
+ *    spmd_create().  Per-replica calls take the replica index.
+ *  - Device state: solvent positions/velocities FP32 (structure of arrays),
+ *    monomers FP64, per-cell sums 64-bit fixed point, delrx FP64.
+ */
+#ifndef SPMD_H
+#define SPMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SPMD_OK = 0,
+    SPMD_ERR_ARG = 1,            /* invalid argument */
+    SPMD_ERR_HIP = 2,            /* HIP runtime failure */
+    SPMD_ERR_OOM = 3,            /* device allocation failed */
+    SPMD_ERR_CELL_OOB = 4,       /* MPC::sort cell index out of range (MPC.cpp:212-219,228-233 raise SIGSEGV) */
+    SPMD_ERR_VERLET_OOB = 5,     /* Box::update_VerletLists grid bounds (Box.cpp:133-142 raise SIGSEGV) */
+    SPMD_ERR_VERLET_OVERFLOW = 6,/* a monomer has more Verlet neighbours than the list capacity */
+    SPMD_ERR_UNSUPPORTED = 7,    /* e.g. the angular-momentum-conserving collision (MPC.cpp:258-269) */
+    SPMD_ERR_STATE = 8,          /* call out of order */
+    SPMD_ERR_RANK_OVERFLOW = 9,  /* more than 65535 particles in one cell */
+    SPMD_ERR_NO_DEVICE = 10,     /* no usable gfx950 device */
+    SPMD_ERR_SCAN = 11           /* cell-offset scan did not complete (bounded wait expired) */
+};
+
+/* Box ctor (Box.cpp:4-27) + MPC ctor (MPC.cpp:10-32) + Thermostat_None dt
+ * (Thermostat_None.cpp:5-12) parameters. */
+typedef struct spmd_params {
+    int32_t  L[3];            /* box; odd sizes rounded up to even (Box.cpp:14-16) */
+    double   kT;              /* Temperature (Box, MPC) */
+    int32_t  n_c;             /* solvent per cell = MPC's N_c (star-polymers.cpp:354 passes Mass_mono) */
+    double   shear;           /* Lees-Edwards shear rate (MPC aShear) */
+    uint32_t step_update;     /* collide on every step_update-th MPC::step call (MPC.cpp:119) */
+    double   alpha_deg;       /* SRD rotation angle (MPC.cpp:23-24: 130) */
+    double   lambda;          /* B-B attraction (Box Lambda) */
+    double   md_dt;           /* velocity-Verlet step (Thermostat DeltaT) */
+    uint64_t seed;            /* Philox4x32-10 key (replaces mt19937_64(1), Rand.cpp:6) */
+    int32_t  angular_momentum;/* MPC angular_mom flag; must be 0 */
+} spmd_params;
+
+typedef struct spmd_sys spmd_sys;
+
+const char* spmd_error_string(int code);
+const char* spmd_last_error(const spmd_sys* s);
+const char* spmd_version(void);
+
+/* MPC::MPC(Box&, T, N_c, shear, step, angular_mom) MPC.h:44 with its Box. */
+int spmd_create(const spmd_params* p, spmd_sys** out);
+/* Many independent replicas advanced together (no reference counterpart:
+ * the reference runs one process per parameter point, runExecuter.py:57-98).
+ * step_update and md_dt must agree across replicas. */
+int spmd_create_batch(const spmd_params* p, uint32_t n_replicas, spmd_sys** out);
+int spmd_destroy(spmd_sys* s);
+int spmd_num_replicas(const spmd_sys* s, uint32_t* n);
+/* Thermostat::dtime(double&) Thermostat.cpp:11-13 */
+int spmd_set_md_dt(spmd_sys* s, double dt);
+
+/* ---- solvent (MPC::Fluid, MPC.h:43) ---- */
+/* MPC::initialize() MPC.cpp:34-55 (Philox draws), all replicas. */
+int spmd_fluid_initialize(spmd_sys* s);
+/* MPC::NumberOfParticles() MPC.h:91 */
+int spmd_num_fluid(const spmd_sys* s, uint32_t replica, uint64_t* n);
+/* Fluid[i].Position / .Velocity, xyz interleaved per particle.  set: n must
+ * equal N_c*L^3.  get: device order (cell order of the last collision);
+ * ids (optional) gives each particle's index at spmd_set_fluid time when
+ * tracking is on. */
+int spmd_set_fluid(spmd_sys* s, uint32_t replica, const float* pos, const float* vel, uint64_t n);
+int spmd_get_fluid(spmd_sys* s, uint32_t replica, float* pos, float* vel, uint32_t* ids);
+/* carry particle ids through the cell sort (parity hooks; adds 8 B per
+ * particle per collision) */
+int spmd_set_tracking(spmd_sys* s, int on);
+
+/* ---- star (Box::Molecules, Box.h:50) ---- */
+/* Box::add_star(A, B, Arms, Mass, Bond, AnchorBond) Box.cpp:39-46 ->
+ * Molecule::initialize_open_star Molecule.cpp:126-176 (Philox draws). */
+int spmd_add_star(spmd_sys* s, uint32_t replica, uint32_t A, uint32_t B, uint32_t arms,
+                  double mass, double bond, double anchor_bond);
+int spmd_num_monomers(const spmd_sys* s, uint32_t replica, uint32_t* n);
+/* Molecules[0].Monomers[i].Position/Velocity/Force, xyz interleaved */
+int spmd_set_monomers(spmd_sys* s, uint32_t replica, const double* pos, const double* vel);
+int spmd_get_monomers(spmd_sys* s, uint32_t replica, double* pos, double* vel, double* force);
+
+/* ---- stepping ---- */
+/* Thermostat_None ctor work (Thermostat_None.cpp:5-12): Box::update_VerletLists
+ * + Box::calculate_forces_verlet for every replica. */
+int spmd_md_init(spmd_sys* s);
+/* Thermostat_None::propagate(calc_epot) Thermostat_None.cpp:21-38 (all
+ * replicas).  Enqueued lazily: consecutive calls run as one launch. */
+int spmd_md_propagate(spmd_sys* s, int calc_epot);
+/* MPC::step(t, dt) MPC.cpp:116-142 (all replicas), including the delrx
+ * advance on every call (MPC.cpp:117-118). */
+int spmd_mpc_step(spmd_sys* s, int64_t t, double dt);
+/* The caller loop of star-polymers.cpp:412-536 without I/O: for t in
+ * [t0, t0+n): Thermostat_None::propagate(false); MPC::step(t, mpc_dt).
+ * Whole collision intervals are replayed from a captured HIP graph. */
+int spmd_run(spmd_sys* s, int64_t t0, int64_t n, double mpc_dt);
+/* wait for all enqueued work and report device-side errors */
+int spmd_sync(spmd_sys* s);
+
+/* ---- parity hooks ---- */
+int spmd_set_delrx(spmd_sys* s, uint32_t replica, double delrx);
+int spmd_get_delrx(spmd_sys* s, uint32_t replica, double* delrx);
+/* randoms for the next collision of one replica: shift[3] in [-0.5,0.5),
+ * axis_u[2*L^3] = (u_phi, u_theta) per cell (MPC.cpp:244-245),
+ * gamma[L^3] = Gamma(1.5(n-1), kT) variate per cell (MPC.cpp:289). */
+int spmd_inject_collision_randoms(spmd_sys* s, uint32_t replica, const double* shift,
+                                  const double* axis_u, const double* gamma);
+/* shift, recentring flag (MPC::check_bounds) and COM of the last collision */
+int spmd_last_collision(spmd_sys* s, uint32_t replica, double* shift, int* recentered,
+                        double* com_pos, double* com_vel);
+/* MPCParticle::CellIndex of the last MPC::sort() (MPC.cpp:209,225), by
+ * particle id (tracking on) and by monomer index. */
+int spmd_get_cell_index(spmd_sys* s, uint32_t replica, uint32_t* fluid_cells, uint32_t* mono_cells);
+
+/* ---- observables ---- */
+/* MPC::calculateCurrentTemperature MPC.cpp:405-409 over the cell lists of
+ * the last collision. */
+int spmd_temperature(spmd_sys* s, uint32_t replica, double* T);
+/* Molecule::Epot after the last propagate (Box.cpp:243-286). */
+int spmd_epot(spmd_sys* s, uint32_t replica, double* epot);
+
+/* ---- tuning (no reference counterpart) ---- */
+/* particles per thread in the bin/scatter kernels (1,2,4,8) and cells per
+ * collide tile (64,128,256); chosen by starpolymersmd_amd.autotune. */
+int spmd_set_tuning(spmd_sys* s, int ppt, int tile_cells);
+int spmd_get_tuning(const spmd_sys* s, int* ppt, int* tile_cells);
+
+/* ---- timing (device events on the handle's stream) ---- */
+/* average duration in ms of each kernel class over the launches since the
+ * last reset: names "md", "bin", "scan", "scatter", "collide". */
+int spmd_profile_enable(spmd_sys* s, int on);
+int spmd_profile_read(spmd_sys* s, const char* name, double* avg_ms, int64_t* count);
+int spmd_profile_reset(spmd_sys* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

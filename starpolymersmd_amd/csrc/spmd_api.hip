@@ -1,0 +1,870 @@
+/*
+ * spmd_api.hip -- C ABI implementation (include/spmd.h): handle and device
+ * memory management, the lazy batching of Thermostat_None::propagate calls,
+ * MPC::step semantics (delrx cadence, collision every step_update calls),
+ * HIP-graph replay of whole collision intervals, parity hooks, observables.
+ */
+#pragma clang fp contract(off)
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/spmd.h"
+#include "spmd_internal.h"
+
+using namespace spmd;
+
+namespace {
+
+struct StarHost {
+    uint32_t n = 0;
+    std::vector<double> pos, vel, force, mass;
+    std::vector<uint8_t> btype, anchor;
+    std::vector<std::pair<uint32_t, uint32_t>> bonds;  /* directed (owner, other) */
+};
+
+struct ProfEvents {
+    std::vector<hipEvent_t> ev;  /* start/stop pairs */
+};
+
+const char* kProfNames[5] = {"md", "bin", "scan", "scatter", "collide"};
+
+}  // namespace
+
+struct spmd_sys {
+    std::vector<spmd_params> params;
+    std::vector<RepDev> reps;
+    std::vector<StarHost> stars;
+    DevState d{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::string err;
+    bool tracking = false;
+    bool mono_dirty = true;        /* device monomer buffers need (re)building */
+    uint32_t step_update = 1;
+    double md_dt = 0.01;
+    /* lazily batched work */
+    int pending_md = 0;
+    int pending_epot = 0;
+    int pending_delrx = 0;
+    double pending_dt = 0.0;
+    bool collided = false;
+    /* graphs of one collision interval, per starting buffer */
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    double graph_h = -1.0;
+    /* profiling */
+    bool prof = false;
+    ProfEvents pev[5];
+    std::vector<size_t> alloc_bytes;
+};
+
+namespace {
+
+int fail(spmd_sys* s, int code, const std::string& msg) {
+    if (s) s->err = msg;
+    return code;
+}
+
+#define HIPCHK(s, call)                                                                        \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) return fail((s), SPMD_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dalloc(spmd_sys* s, T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) return fail(s, SPMD_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return SPMD_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+void clear_graphs(spmd_sys* s) {
+    for (auto& g : s->graph)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
+
+int upload_tables(spmd_sys* s) {
+    DevState& d = s->d;
+    const uint32_t ppb = 256u * (uint32_t)d.ppt;
+    std::vector<uint32_t> pb(d.n_rep + 1, 0), tb(d.n_rep + 1, 0);
+    for (int r = 0; r < d.n_rep; ++r) {
+        pb[r + 1] = pb[r] + (s->reps[r].n_fluid + ppb - 1) / ppb;
+        tb[r + 1] = tb[r] + (s->reps[r].ncells + d.tile_cells - 1) / d.tile_cells;
+    }
+    d.n_pblk = pb[d.n_rep];
+    d.n_tiles = tb[d.n_rep];
+    HIPCHK(s, hipMemcpy(d.pblk_base, pb.data(), pb.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(d.tile_base, tb.data(), tb.size() * 4, hipMemcpyHostToDevice));
+    size_t need = std::max<size_t>(4ull * (d.n_pblk + d.n_rep), 2ull * d.n_tiles);
+    dfree(d.red);
+    if (int rc = dalloc(s, &d.red, need)) return rc;
+    clear_graphs(s);
+    return SPMD_OK;
+}
+
+/* (Re)build the device monomer buffers of all replicas from the host
+ * stars: monomer arrays, bond adjacency, Verlet lists. */
+int build_monomers(spmd_sys* s) {
+    DevState& d = s->d;
+    uint32_t mt = 0;
+    std::vector<uint32_t> adj_off;
+    std::vector<uint32_t> adj;
+    std::vector<double> pos, vel, force, mass;
+    std::vector<uint8_t> bt;
+    for (int r = 0; r < d.n_rep; ++r) {
+        StarHost& st = s->stars[r];
+        RepDev& rp = s->reps[r];
+        rp.mono_base = mt;
+        rp.n_mono = st.n;
+        rp.adj_base = (uint32_t)adj_off.size();
+        std::vector<std::vector<uint32_t>> lists(st.n);
+        for (auto& b : st.bonds) {
+            const uint32_t fl = st.anchor[b.first] ? kBondAnchor : 0u;
+            lists[b.first].push_back(b.second | fl);
+            lists[b.second].push_back(b.first | fl);
+        }
+        for (uint32_t i = 0; i < st.n; ++i) {
+            adj_off.push_back((uint32_t)adj.size());
+            for (uint32_t e : lists[i]) adj.push_back(e);
+        }
+        adj_off.push_back((uint32_t)adj.size());
+        pos.insert(pos.end(), st.pos.begin(), st.pos.end());
+        vel.insert(vel.end(), st.vel.begin(), st.vel.end());
+        force.insert(force.end(), st.force.begin(), st.force.end());
+        mass.insert(mass.end(), st.mass.begin(), st.mass.end());
+        bt.insert(bt.end(), st.btype.begin(), st.btype.end());
+        mt += st.n;
+    }
+    d.m_tot = mt;
+    dfree(d.mpos); dfree(d.mvel); dfree(d.mforce); dfree(d.mvpos); dfree(d.mmass); dfree(d.mbtype);
+    dfree(d.vl); dfree(d.vl_n); dfree(d.adj_off); dfree(d.adj); dfree(d.msorted);
+    if (int rc = dalloc(s, &d.mpos, 3ull * mt)) return rc;
+    if (int rc = dalloc(s, &d.mvel, 3ull * mt)) return rc;
+    if (int rc = dalloc(s, &d.mforce, 3ull * mt)) return rc;
+    if (int rc = dalloc(s, &d.mvpos, 3ull * mt)) return rc;
+    if (int rc = dalloc(s, &d.mmass, mt)) return rc;
+    if (int rc = dalloc(s, &d.mbtype, mt)) return rc;
+    if (int rc = dalloc(s, &d.vl, (size_t)mt * kVcap)) return rc;
+    if (int rc = dalloc(s, &d.vl_n, mt)) return rc;
+    if (int rc = dalloc(s, &d.adj_off, adj_off.size())) return rc;
+    if (int rc = dalloc(s, &d.adj, adj.size())) return rc;
+    if (int rc = dalloc(s, &d.msorted, mt)) return rc;
+    if (mt) {
+        HIPCHK(s, hipMemcpy(d.mpos, pos.data(), pos.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(s, hipMemcpy(d.mvel, vel.data(), vel.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(s, hipMemcpy(d.mforce, force.data(), force.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(s, hipMemcpy(d.mvpos, pos.data(), pos.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(s, hipMemcpy(d.mmass, mass.data(), mass.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(s, hipMemcpy(d.mbtype, bt.data(), bt.size(), hipMemcpyHostToDevice));
+        HIPCHK(s, hipMemset(d.vl_n, 0, (size_t)mt * 2));
+    }
+    if (!adj.empty()) HIPCHK(s, hipMemcpy(d.adj, adj.data(), adj.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(d.adj_off, adj_off.data(), adj_off.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(d.reps, s->reps.data(), s->reps.size() * sizeof(RepDev), hipMemcpyHostToDevice));
+    s->mono_dirty = false;
+    clear_graphs(s);
+    return SPMD_OK;
+}
+
+/* Pull device monomer state back into the host stars (before a rebuild). */
+int download_monomers(spmd_sys* s) {
+    if (s->mono_dirty || s->d.m_tot == 0) return SPMD_OK;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    for (int r = 0; r < s->d.n_rep; ++r) {
+        StarHost& st = s->stars[r];
+        const uint32_t mb = s->reps[r].mono_base;
+        if (!st.n) continue;
+        HIPCHK(s, hipMemcpy(st.pos.data(), s->d.mpos + 3ull * mb, 24ull * st.n, hipMemcpyDeviceToHost));
+        HIPCHK(s, hipMemcpy(st.vel.data(), s->d.mvel + 3ull * mb, 24ull * st.n, hipMemcpyDeviceToHost));
+        HIPCHK(s, hipMemcpy(st.force.data(), s->d.mforce + 3ull * mb, 24ull * st.n, hipMemcpyDeviceToHost));
+    }
+    return SPMD_OK;
+}
+
+int ensure_monomers(spmd_sys* s) { return s->mono_dirty ? build_monomers(s) : SPMD_OK; }
+
+void prof_begin(spmd_sys* s, int k, hipEvent_t* e0) {
+    *e0 = nullptr;
+    if (!s->prof) return;
+    if (hipEventCreate(e0) == hipSuccess) (void)hipEventRecord(*e0, s->stream);
+    s->pev[k].ev.push_back(*e0);
+}
+void prof_end(spmd_sys* s, int k) {
+    if (!s->prof) return;
+    hipEvent_t e1 = nullptr;
+    if (hipEventCreate(&e1) == hipSuccess) (void)hipEventRecord(e1, s->stream);
+    s->pev[k].ev.push_back(e1);
+}
+
+#define LAUNCH(s, k, call)                                                                   \
+    do {                                                                                      \
+        hipEvent_t e0_;                                                                       \
+        prof_begin((s), (k), &e0_);                                                           \
+        hipError_t e_ = (call);                                                               \
+        prof_end((s), (k));                                                                   \
+        if (e_ != hipSuccess) return fail((s), SPMD_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+/* One collision for all replicas (MPC.cpp:121-141), preceded by the pending
+ * MD substeps and delrx advances. */
+int enqueue_collision(spmd_sys* s, int n_md, int epot_last, int n_delrx, double delrx_dt, double h) {
+    DevState& d = s->d;
+    LAUNCH(s, 0, launch_md_prologue(d, s->stream, n_md, epot_last, 0, 1, n_delrx, delrx_dt, s->md_dt));
+    LAUNCH(s, 1, launch_bin(d, s->stream, h));
+    LAUNCH(s, 2, launch_scan(d, s->stream));
+    LAUNCH(s, 3, launch_scatter(d, s->stream, h));
+    LAUNCH(s, 4, launch_collide(d, s->stream));
+    d.cur ^= 1;
+    s->collided = true;
+    return SPMD_OK;
+}
+
+int flush_md(spmd_sys* s) {
+    if (s->pending_md == 0 && s->pending_delrx == 0) return SPMD_OK;
+    if (int rc = ensure_monomers(s)) return rc;
+    LAUNCH(s, 0, launch_md_prologue(s->d, s->stream, s->pending_md, s->pending_epot, 0, 0, s->pending_delrx,
+                                    s->pending_dt, s->md_dt));
+    s->pending_md = 0;
+    s->pending_delrx = 0;
+    return SPMD_OK;
+}
+
+int check_status(spmd_sys* s) {
+    std::vector<RepCtl> c(s->d.n_rep);
+    HIPCHK(s, hipMemcpy(c.data(), s->d.ctl, c.size() * sizeof(RepCtl), hipMemcpyDeviceToHost));
+    uint32_t st = 0;
+    int rbad = -1;
+    for (int r = 0; r < s->d.n_rep; ++r)
+        if (c[r].status) { st |= c[r].status; if (rbad < 0) rbad = r; }
+    if (!st) return SPMD_OK;
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "device status 0x%x (first replica %d)", st, rbad);
+    if (st & ST_SCAN_TIMEOUT) return fail(s, SPMD_ERR_SCAN, std::string("cell-offset scan wait expired; ") + buf);
+    if (st & ST_CELL_OOB) return fail(s, SPMD_ERR_CELL_OOB, std::string("cell index out of range; ") + buf);
+    if (st & ST_VERLET_OOB) return fail(s, SPMD_ERR_VERLET_OOB, std::string("monomer outside Verlet grid; ") + buf);
+    if (st & ST_VERLET_OVERFLOW) return fail(s, SPMD_ERR_VERLET_OVERFLOW, std::string("Verlet list overflow; ") + buf);
+    return fail(s, SPMD_ERR_RANK_OVERFLOW, std::string("cell occupancy overflow; ") + buf);
+}
+
+bool valid_rep(const spmd_sys* s, uint32_t r) { return s && r < (uint32_t)s->d.n_rep; }
+
+/* Molecule::initialize_open_star (Molecule.cpp:126-176) with Philox
+ * velocities, then Box::set_center_of_mass_to_zero (Box.cpp:74-79). */
+void build_star(StarHost& st, const RepDev& rep, const spmd_params& p, uint32_t A, uint32_t B, uint32_t arms,
+                double mass, double bond, double anchor_bond) {
+    const double kPi = 3.14159265358979323846;
+    const uint32_t per_arm = A + B, N = per_arm * arms + 1;
+    st.n = N;
+    st.pos.assign(3ull * N, 0.0);
+    st.vel.assign(3ull * N, 0.0);
+    st.force.assign(3ull * N, 0.0);
+    st.mass.assign(N, mass);
+    st.btype.assign(N, 0);
+    st.anchor.assign(N, 0);
+    st.bonds.clear();
+    st.anchor[0] = 1;
+    double avg[3] = {0, 0, 0};
+    const unsigned n = (unsigned)std::ceil(std::sqrt(arms));
+    unsigned arm_count = 0;
+    std::vector<std::vector<uint32_t>> nb(N);  /* Neighbors via push_front: newest first */
+    for (unsigned i = 0; i < n; i++) {
+        const double phi = 2. * kPi * i / n;
+        for (unsigned j = 1; j < n + 1; j++) {
+            if (arm_count >= arms) break;
+            const double theta = kPi * j / (n + 1);
+            const double dir[3] = {std::cos(phi) * std::sin(theta), std::sin(phi) * std::sin(theta), std::cos(theta)};
+            for (unsigned k = 1; k <= per_arm; k++) {
+                const unsigned idx = arm_count * per_arm + k;
+                if (k > A) st.btype[idx] = 1;
+                if (k == 1) {
+                    for (int q = 0; q < 3; ++q) st.pos[3 * idx + q] = anchor_bond * dir[q];
+                    nb[0].insert(nb[0].begin(), idx);
+                    nb[idx].insert(nb[idx].begin(), idx + 1);
+                } else {
+                    for (int q = 0; q < 3; ++q) st.pos[3 * idx + q] = (anchor_bond + (k - 1) * bond) * dir[q];
+                    if (k != per_arm) nb[idx].insert(nb[idx].begin(), idx + 1);
+                }
+                uint32_t w[4];
+                draw4(p.seed, rep.replica, TAG_STAR_INIT, idx, 0, 0, w);
+                for (int l = 0; l < 3; l++) st.vel[3 * idx + l] = u01(w[l]) - 0.5;
+                for (int q = 0; q < 3; ++q) avg[q] += st.vel[3 * idx + q];
+            }
+            arm_count++;
+        }
+    }
+    for (uint32_t i = 0; i < N; ++i)
+        for (uint32_t o : nb[i]) st.bonds.emplace_back(i, o);
+    for (int q = 0; q < 3; ++q) avg[q] /= (double)N;
+    for (uint32_t i = 0; i < N; ++i)
+        for (int q = 0; q < 3; ++q) st.vel[3 * i + q] -= avg[q];
+    /* Molecule::calculate_Ekin (Molecule.cpp:60-74) */
+    double cv[3] = {0, 0, 0};
+    for (uint32_t i = 0; i < N; ++i)
+        for (int q = 0; q < 3; ++q) cv[q] += st.vel[3 * i + q];
+    for (int q = 0; q < 3; ++q) cv[q] /= (double)N;
+    double ek = 0.0;
+    for (uint32_t i = 0; i < N; ++i) {
+        const double r0 = st.vel[3 * i] - cv[0], r1 = st.vel[3 * i + 1] - cv[1], r2 = st.vel[3 * i + 2] - cv[2];
+        ek += mass * ((r0 * r0 + r1 * r1) + r2 * r2);
+    }
+    ek /= 2.0;
+    const double scale = std::sqrt(3. * (double)N * p.kT / (ek * 2.0));
+    for (auto& v : st.vel) v *= scale;
+    double com[3] = {0, 0, 0};
+    for (uint32_t i = 0; i < N; ++i)
+        for (int q = 0; q < 3; ++q) com[q] += st.pos[3 * i + q];
+    for (int q = 0; q < 3; ++q) com[q] /= (double)N;
+    for (uint32_t i = 0; i < N; ++i)
+        for (int q = 0; q < 3; ++q) st.pos[3 * i + q] -= com[q];
+}
+
+int check_device(spmd_sys* s) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(s, SPMD_ERR_NO_DEVICE, "no HIP device");
+    HIPCHK(s, hipGetDevice(&s->device));
+    hipDeviceProp_t prop;
+    HIPCHK(s, hipGetDeviceProperties(&prop, s->device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(s, SPMD_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", library built for gfx950");
+    return SPMD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* spmd_version(void) { return "starpolymersmd-amd 0.1 (gfx950)"; }
+
+const char* spmd_error_string(int code) {
+    switch (code) {
+    case SPMD_OK: return "ok";
+    case SPMD_ERR_ARG: return "invalid argument";
+    case SPMD_ERR_HIP: return "HIP runtime error";
+    case SPMD_ERR_OOM: return "device out of memory";
+    case SPMD_ERR_CELL_OOB: return "cell index out of range";
+    case SPMD_ERR_VERLET_OOB: return "monomer outside the Verlet grid";
+    case SPMD_ERR_VERLET_OVERFLOW: return "Verlet list overflow";
+    case SPMD_ERR_UNSUPPORTED: return "unsupported option";
+    case SPMD_ERR_STATE: return "call out of order";
+    case SPMD_ERR_RANK_OVERFLOW: return "cell occupancy overflow";
+    case SPMD_ERR_NO_DEVICE: return "no usable gfx950 device";
+    case SPMD_ERR_SCAN: return "cell-offset scan did not complete";
+    default: return "unknown error";
+    }
+}
+
+const char* spmd_last_error(const spmd_sys* s) { return s ? s->err.c_str() : "null handle"; }
+
+int spmd_create_batch(const spmd_params* p, uint32_t n_rep, spmd_sys** out) {
+    if (!p || !out || n_rep == 0) return SPMD_ERR_ARG;
+    *out = nullptr;
+    auto* s = new spmd_sys;
+    if (int rc = check_device(s)) { delete s; return rc; }
+    s->params.assign(p, p + n_rep);
+    s->step_update = p[0].step_update ? p[0].step_update : 1;
+    s->md_dt = p[0].md_dt;
+    DevState& d = s->d;
+    d.n_rep = (int)n_rep;
+    d.ppt = 4;
+    d.tile_cells = 128;
+    uint64_t nt = 0, ct = 0;
+    for (uint32_t r = 0; r < n_rep; ++r) {
+        const spmd_params& q = p[r];
+        if (q.angular_momentum) { delete s; return SPMD_ERR_UNSUPPORTED; }
+        if (q.step_update != p[0].step_update || q.md_dt != p[0].md_dt || q.n_c < 0 || q.kT <= 0.0) {
+            delete s;
+            return SPMD_ERR_ARG;
+        }
+        RepDev rp{};
+        for (int i = 0; i < 3; ++i) {
+            int l = q.L[i];
+            if (l % 2) l++;                      /* Box.cpp:14-16 */
+            if (l < 4 || l > 1024) { delete s; return SPMD_ERR_ARG; }
+            rp.box.L[i] = l;
+            rp.box.invL[i] = 1.0 / l;
+            rp.box.half[i] = l * 0.5;
+            rp.box.iL[i] = l;
+            rp.vgrid[i] = int(l / 2.0);          /* Box.cpp:20-25 */
+            rp.vside[i] = l / (double)rp.vgrid[i];
+        }
+        rp.box.shear = q.shear;
+        rp.box.shear_on = q.shear != 0.;
+        rp.kT = q.kT;
+        rp.c = std::cos(q.alpha_deg * 3.14159265358979323846 / 180.0);
+        rp.s = std::sin(q.alpha_deg * 3.14159265358979323846 / 180.0);
+        rp.lambda = q.lambda;
+        rp.seed = q.seed;
+        rp.replica = r;
+        rp.ncells = (uint32_t)(rp.box.iL[0] * rp.box.iL[1] * rp.box.iL[2]);
+        if (rp.ncells > 4096u * 4096u) { delete s; return SPMD_ERR_ARG; }
+        rp.cell_base = ct;
+        rp.part_base = nt;
+        const uint64_t nf = (uint64_t)q.n_c * rp.ncells;
+        rp.n_fluid = (uint32_t)nf;
+        ct += rp.ncells;
+        nt += nf;
+        s->reps.push_back(rp);
+    }
+    if (nt >= (1ull << 32) || ct >= (1ull << 31)) { delete s; return SPMD_ERR_ARG; }
+    s->stars.resize(n_rep);
+    d.n_tot = nt;
+    d.c_tot = ct;
+    d.n_scan_tiles = (uint32_t)((ct + kScanTile - 1) / kScanTile);
+    HIPCHK(s, hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    int rc = SPMD_OK;
+    for (int b = 0; b < 2 && !rc; ++b) {
+        rc = rc ? rc : dalloc(s, &d.x[b], nt);
+        rc = rc ? rc : dalloc(s, &d.y[b], nt);
+        rc = rc ? rc : dalloc(s, &d.z[b], nt);
+        rc = rc ? rc : dalloc(s, &d.vx[b], nt);
+        rc = rc ? rc : dalloc(s, &d.vy[b], nt);
+        rc = rc ? rc : dalloc(s, &d.vz[b], nt);
+    }
+    rc = rc ? rc : dalloc(s, &d.rank, nt);
+    rc = rc ? rc : dalloc(s, &d.counts, ct + 16);
+    rc = rc ? rc : dalloc(s, &d.offsets, ct + 16);
+    rc = rc ? rc : dalloc(s, &d.scan_flags, d.n_scan_tiles);
+    rc = rc ? rc : dalloc(s, &d.scan_ticket, 1);
+    rc = rc ? rc : dalloc(s, &d.reps, n_rep);
+    rc = rc ? rc : dalloc(s, &d.ctl, n_rep);
+    rc = rc ? rc : dalloc(s, &d.pblk_base, n_rep + 1);
+    rc = rc ? rc : dalloc(s, &d.tile_base, n_rep + 1);
+    if (rc) { spmd_destroy(s); return rc; }
+    HIPCHK(s, hipMemset(d.counts, 0, (ct + 16) * 4));
+    HIPCHK(s, hipMemset(d.offsets, 0, (ct + 16) * 4));
+    HIPCHK(s, hipMemset(d.scan_flags, 0, d.n_scan_tiles * 8ull));
+    HIPCHK(s, hipMemset(d.scan_ticket, 0, 4));
+    for (int b = 0; b < 2; ++b) {
+        HIPCHK(s, hipMemset(d.x[b], 0, nt * 4)); HIPCHK(s, hipMemset(d.y[b], 0, nt * 4));
+        HIPCHK(s, hipMemset(d.z[b], 0, nt * 4)); HIPCHK(s, hipMemset(d.vx[b], 0, nt * 4));
+        HIPCHK(s, hipMemset(d.vy[b], 0, nt * 4)); HIPCHK(s, hipMemset(d.vz[b], 0, nt * 4));
+    }
+    std::vector<RepCtl> c(n_rep);
+    std::memset(c.data(), 0, c.size() * sizeof(RepCtl));
+    HIPCHK(s, hipMemcpy(d.ctl, c.data(), c.size() * sizeof(RepCtl), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(d.reps, s->reps.data(), s->reps.size() * sizeof(RepDev), hipMemcpyHostToDevice));
+    if ((rc = upload_tables(s))) { spmd_destroy(s); return rc; }
+    if ((rc = build_monomers(s))) { spmd_destroy(s); return rc; }
+    *out = s;
+    return SPMD_OK;
+}
+
+int spmd_create(const spmd_params* p, spmd_sys** out) { return spmd_create_batch(p, 1, out); }
+
+int spmd_destroy(spmd_sys* s) {
+    if (!s) return SPMD_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    clear_graphs(s);
+    for (auto& pe : s->pev)
+        for (auto e : pe.ev) if (e) (void)hipEventDestroy(e);
+    DevState& d = s->d;
+    for (int b = 0; b < 2; ++b) {
+        dfree(d.x[b]); dfree(d.y[b]); dfree(d.z[b]); dfree(d.vx[b]); dfree(d.vy[b]); dfree(d.vz[b]);
+        dfree(d.ids[b]);
+    }
+    dfree(d.rank); dfree(d.counts); dfree(d.offsets); dfree(d.scan_flags); dfree(d.scan_ticket);
+    dfree(d.mpos); dfree(d.mvel); dfree(d.mforce); dfree(d.mvpos); dfree(d.mmass); dfree(d.mbtype);
+    dfree(d.vl); dfree(d.vl_n); dfree(d.adj_off); dfree(d.adj); dfree(d.msorted);
+    dfree(d.inj_axis); dfree(d.inj_gamma); dfree(d.reps); dfree(d.ctl); dfree(d.pblk_base);
+    dfree(d.tile_base); dfree(d.red);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return SPMD_OK;
+}
+
+int spmd_num_replicas(const spmd_sys* s, uint32_t* n) {
+    if (!s || !n) return SPMD_ERR_ARG;
+    *n = (uint32_t)s->d.n_rep;
+    return SPMD_OK;
+}
+
+int spmd_set_md_dt(spmd_sys* s, double dt) {
+    if (!s || !(dt > 0)) return SPMD_ERR_ARG;
+    if (s->pending_md) if (int rc = flush_md(s)) return rc;
+    s->md_dt = dt;
+    clear_graphs(s);
+    return SPMD_OK;
+}
+
+int spmd_set_tuning(spmd_sys* s, int ppt, int tile_cells) {
+    if (!s) return SPMD_ERR_ARG;
+    if (ppt != 1 && ppt != 2 && ppt != 4 && ppt != 8) return fail(s, SPMD_ERR_ARG, "ppt must be 1,2,4,8");
+    if (tile_cells != 64 && tile_cells != 128 && tile_cells != 256)
+        return fail(s, SPMD_ERR_ARG, "tile_cells must be 64,128,256");
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    s->d.ppt = ppt;
+    s->d.tile_cells = tile_cells;
+    return upload_tables(s);
+}
+
+int spmd_get_tuning(const spmd_sys* s, int* ppt, int* tile_cells) {
+    if (!s) return SPMD_ERR_ARG;
+    if (ppt) *ppt = s->d.ppt;
+    if (tile_cells) *tile_cells = s->d.tile_cells;
+    return SPMD_OK;
+}
+
+int spmd_fluid_initialize(spmd_sys* s) {
+    if (!s) return SPMD_ERR_ARG;
+    if (int rc = flush_md(s)) return rc;
+    LAUNCH(s, 1, launch_fluid_init(s->d, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    s->collided = false;
+    return SPMD_OK;
+}
+
+int spmd_num_fluid(const spmd_sys* s, uint32_t r, uint64_t* n) {
+    if (!valid_rep(s, r) || !n) return SPMD_ERR_ARG;
+    *n = s->reps[r].n_fluid;
+    return SPMD_OK;
+}
+
+int spmd_set_fluid(spmd_sys* s, uint32_t r, const float* pos, const float* vel, uint64_t n) {
+    if (!valid_rep(s, r) || !pos || !vel) return SPMD_ERR_ARG;
+    if (n != s->reps[r].n_fluid) return fail(s, SPMD_ERR_ARG, "fluid size must equal N_c*L^3");
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    const DevState& d = s->d;
+    const uint64_t b = s->reps[r].part_base;
+    std::vector<float> t(n);
+    float* dst[6] = {d.x[d.cur], d.y[d.cur], d.z[d.cur], d.vx[d.cur], d.vy[d.cur], d.vz[d.cur]};
+    for (int c = 0; c < 6; ++c) {
+        const float* src = c < 3 ? pos : vel;
+        for (uint64_t i = 0; i < n; ++i) t[i] = src[3 * i + (c % 3)];
+        HIPCHK(s, hipMemcpy(dst[c] + b, t.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    if (d.ids[d.cur]) {
+        std::vector<uint32_t> id(n);
+        for (uint64_t i = 0; i < n; ++i) id[i] = (uint32_t)i;
+        HIPCHK(s, hipMemcpy(d.ids[d.cur] + b, id.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    s->collided = false;
+    return SPMD_OK;
+}
+
+int spmd_get_fluid(spmd_sys* s, uint32_t r, float* pos, float* vel, uint32_t* ids) {
+    if (!valid_rep(s, r)) return SPMD_ERR_ARG;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    const DevState& d = s->d;
+    const uint64_t n = s->reps[r].n_fluid, b = s->reps[r].part_base;
+    std::vector<float> t(n);
+    const float* src[6] = {d.x[d.cur], d.y[d.cur], d.z[d.cur], d.vx[d.cur], d.vy[d.cur], d.vz[d.cur]};
+    for (int c = 0; c < 6; ++c) {
+        float* dst = c < 3 ? pos : vel;
+        if (!dst) continue;
+        HIPCHK(s, hipMemcpy(t.data(), src[c] + b, n * 4, hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < n; ++i) dst[3 * i + (c % 3)] = t[i];
+    }
+    if (ids) {
+        if (!d.ids[d.cur]) return fail(s, SPMD_ERR_STATE, "particle tracking is off");
+        HIPCHK(s, hipMemcpy(ids, d.ids[d.cur] + b, n * 4, hipMemcpyDeviceToHost));
+    }
+    return SPMD_OK;
+}
+
+int spmd_set_tracking(spmd_sys* s, int on) {
+    if (!s) return SPMD_ERR_ARG;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    DevState& d = s->d;
+    if (on && !d.ids[0]) {
+        for (int b = 0; b < 2; ++b)
+            if (int rc = dalloc(s, &d.ids[b], d.n_tot)) return rc;
+        std::vector<uint32_t> id(d.n_tot);
+        for (int r = 0; r < d.n_rep; ++r)
+            for (uint32_t i = 0; i < s->reps[r].n_fluid; ++i) id[s->reps[r].part_base + i] = i;
+        HIPCHK(s, hipMemcpy(d.ids[d.cur], id.data(), d.n_tot * 4, hipMemcpyHostToDevice));
+    } else if (!on) {
+        dfree(d.ids[0]);
+        dfree(d.ids[1]);
+    }
+    s->tracking = on != 0;
+    clear_graphs(s);
+    return SPMD_OK;
+}
+
+int spmd_add_star(spmd_sys* s, uint32_t r, uint32_t A, uint32_t B, uint32_t arms, double mass, double bond,
+                  double anchor_bond) {
+    if (!valid_rep(s, r) || arms == 0 || A + B < 2 || !(mass > 0)) return SPMD_ERR_ARG;
+    if ((A + B) * arms + 1 > (uint32_t)kMaxMono) return fail(s, SPMD_ERR_ARG, "star larger than 2048 monomers");
+    if (int rc = flush_md(s)) return rc;
+    if (int rc = download_monomers(s)) return rc;
+    build_star(s->stars[r], s->reps[r], s->params[r], A, B, arms, mass, bond, anchor_bond);
+    s->mono_dirty = true;
+    return ensure_monomers(s);
+}
+
+int spmd_num_monomers(const spmd_sys* s, uint32_t r, uint32_t* n) {
+    if (!valid_rep(s, r) || !n) return SPMD_ERR_ARG;
+    *n = s->stars[r].n;
+    return SPMD_OK;
+}
+
+int spmd_set_monomers(spmd_sys* s, uint32_t r, const double* pos, const double* vel) {
+    if (!valid_rep(s, r)) return SPMD_ERR_ARG;
+    if (int rc = flush_md(s)) return rc;
+    if (int rc = ensure_monomers(s)) return rc;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    const uint32_t n = s->stars[r].n, mb = s->reps[r].mono_base;
+    if (!n) return SPMD_OK;
+    if (pos) {
+        HIPCHK(s, hipMemcpy(s->d.mpos + 3ull * mb, pos, 24ull * n, hipMemcpyHostToDevice));
+        std::memcpy(s->stars[r].pos.data(), pos, 24ull * n);
+    }
+    if (vel) {
+        HIPCHK(s, hipMemcpy(s->d.mvel + 3ull * mb, vel, 24ull * n, hipMemcpyHostToDevice));
+        std::memcpy(s->stars[r].vel.data(), vel, 24ull * n);
+    }
+    return SPMD_OK;
+}
+
+int spmd_get_monomers(spmd_sys* s, uint32_t r, double* pos, double* vel, double* force) {
+    if (!valid_rep(s, r)) return SPMD_ERR_ARG;
+    if (int rc = flush_md(s)) return rc;
+    if (int rc = ensure_monomers(s)) return rc;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    const uint32_t n = s->stars[r].n, mb = s->reps[r].mono_base;
+    if (!n) return SPMD_OK;
+    if (pos) HIPCHK(s, hipMemcpy(pos, s->d.mpos + 3ull * mb, 24ull * n, hipMemcpyDeviceToHost));
+    if (vel) HIPCHK(s, hipMemcpy(vel, s->d.mvel + 3ull * mb, 24ull * n, hipMemcpyDeviceToHost));
+    if (force) HIPCHK(s, hipMemcpy(force, s->d.mforce + 3ull * mb, 24ull * n, hipMemcpyDeviceToHost));
+    return SPMD_OK;
+}
+
+int spmd_md_init(spmd_sys* s) {
+    if (!s) return SPMD_ERR_ARG;
+    if (int rc = flush_md(s)) return rc;
+    if (int rc = ensure_monomers(s)) return rc;
+    LAUNCH(s, 0, launch_md_prologue(s->d, s->stream, 0, 0, 1, 0, 0, 0.0, s->md_dt));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return check_status(s);
+}
+
+int spmd_md_propagate(spmd_sys* s, int calc_epot) {
+    if (!s) return SPMD_ERR_ARG;
+    s->pending_md++;
+    s->pending_epot = calc_epot != 0;
+    return SPMD_OK;
+}
+
+int spmd_mpc_step(spmd_sys* s, int64_t t, double dt) {
+    if (!s) return SPMD_ERR_ARG;
+    if (s->pending_delrx && dt != s->pending_dt) {
+        const int keep = s->pending_md;
+        s->pending_md = 0;
+        if (int rc = flush_md(s)) return rc;
+        s->pending_md = keep;
+    }
+    s->pending_dt = dt;
+    s->pending_delrx++;
+    if (t % (int64_t)s->step_update) return SPMD_OK;
+    if (int rc = ensure_monomers(s)) return rc;
+    const int rc = enqueue_collision(s, s->pending_md, s->pending_epot, s->pending_delrx, dt, dt);
+    s->pending_md = 0;
+    s->pending_delrx = 0;
+    return rc;
+}
+
+int spmd_run(spmd_sys* s, int64_t t0, int64_t n, double mpc_dt) {
+    if (!s || n < 0) return SPMD_ERR_ARG;
+    const int64_t S = s->step_update;
+    int64_t t = t0;
+    const int64_t end = t0 + n;
+    /* single calls until the start of a collision interval: the call
+     * after a collision call, with nothing pending */
+    auto at_start = [&]() {
+        return s->pending_md == 0 && s->pending_delrx == 0 && t >= 1 && (t - 1) % S == 0;
+    };
+    while (t < end && !at_start()) {
+        if (int rc = spmd_md_propagate(s, 0)) return rc;
+        if (int rc = spmd_mpc_step(s, t, mpc_dt)) return rc;
+        ++t;
+    }
+    if (int rc = ensure_monomers(s)) return rc;
+    /* whole intervals t = kS+1 .. (k+1)S: replay the captured graph */
+    while (end - t >= S && at_start()) {
+        if (s->prof) {
+            if (int rc = enqueue_collision(s, (int)S, 0, (int)S, mpc_dt, mpc_dt)) return rc;
+        } else {
+            const int c = s->d.cur;
+            if (!s->graph[c] || s->graph_h != mpc_dt) {
+                if (s->graph_h != mpc_dt) clear_graphs(s);
+                hipGraph_t g;
+                HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+                DevState& d = s->d;
+                hipError_t e = launch_md_prologue(d, s->stream, (int)S, 0, 0, 1, (int)S, mpc_dt, s->md_dt);
+                if (e == hipSuccess) e = launch_bin(d, s->stream, mpc_dt);
+                if (e == hipSuccess) e = launch_scan(d, s->stream);
+                if (e == hipSuccess) e = launch_scatter(d, s->stream, mpc_dt);
+                if (e == hipSuccess) e = launch_collide(d, s->stream);
+                hipError_t e2 = hipStreamEndCapture(s->stream, &g);
+                if (e != hipSuccess || e2 != hipSuccess)
+                    return fail(s, SPMD_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e ? e : e2));
+                HIPCHK(s, hipGraphInstantiate(&s->graph[c], g, nullptr, nullptr, 0));
+                (void)hipGraphDestroy(g);
+                s->graph_h = mpc_dt;
+            }
+            HIPCHK(s, hipGraphLaunch(s->graph[c], s->stream));
+            s->d.cur ^= 1;
+            s->collided = true;
+        }
+        t += S;
+    }
+    for (; t < end; ++t) {
+        if (int rc = spmd_md_propagate(s, 0)) return rc;
+        if (int rc = spmd_mpc_step(s, t, mpc_dt)) return rc;
+    }
+    return SPMD_OK;
+}
+
+int spmd_sync(spmd_sys* s) {
+    if (!s) return SPMD_ERR_ARG;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return check_status(s);
+}
+
+int spmd_set_delrx(spmd_sys* s, uint32_t r, double delrx) {
+    if (!valid_rep(s, r)) return SPMD_ERR_ARG;
+    if (int rc = flush_md(s)) return rc;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(&s->d.ctl[r].delrx, &delrx, 8, hipMemcpyHostToDevice));
+    return SPMD_OK;
+}
+
+int spmd_get_delrx(spmd_sys* s, uint32_t r, double* delrx) {
+    if (!valid_rep(s, r) || !delrx) return SPMD_ERR_ARG;
+    if (int rc = flush_md(s)) return rc;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(delrx, &s->d.ctl[r].delrx, 8, hipMemcpyDeviceToHost));
+    return SPMD_OK;
+}
+
+int spmd_inject_collision_randoms(spmd_sys* s, uint32_t r, const double* shift, const double* axis_u,
+                                  const double* gamma) {
+    if (!valid_rep(s, r) || !shift || !axis_u || !gamma) return SPMD_ERR_ARG;
+    DevState& d = s->d;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    if (!d.inj_axis) {
+        if (int rc = dalloc(s, &d.inj_axis, 2 * d.c_tot)) return rc;
+        if (int rc = dalloc(s, &d.inj_gamma, d.c_tot)) return rc;
+        clear_graphs(s);
+    }
+    const RepDev& rp = s->reps[r];
+    HIPCHK(s, hipMemcpy(d.inj_axis + 2 * rp.cell_base, axis_u, 16ull * rp.ncells, hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(d.inj_gamma + rp.cell_base, gamma, 8ull * rp.ncells, hipMemcpyHostToDevice));
+    RepCtl c;
+    HIPCHK(s, hipMemcpy(&c, d.ctl + r, sizeof c, hipMemcpyDeviceToHost));
+    for (int i = 0; i < 3; ++i) c.inj_shift[i] = shift[i];
+    c.inj_pending = 1;
+    HIPCHK(s, hipMemcpy(d.ctl + r, &c, sizeof c, hipMemcpyHostToDevice));
+    return SPMD_OK;
+}
+
+int spmd_last_collision(spmd_sys* s, uint32_t r, double* shift, int* recentered, double* com_pos,
+                        double* com_vel) {
+    if (!valid_rep(s, r)) return SPMD_ERR_ARG;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    RepCtl c;
+    HIPCHK(s, hipMemcpy(&c, s->d.ctl + r, sizeof c, hipMemcpyDeviceToHost));
+    for (int i = 0; i < 3; ++i) {
+        if (shift) shift[i] = c.shift[i];
+        if (com_pos) com_pos[i] = c.com[i];
+        if (com_vel) com_vel[i] = c.com_vel[i];
+    }
+    if (recentered) *recentered = c.recenter;
+    return SPMD_OK;
+}
+
+int spmd_get_cell_index(spmd_sys* s, uint32_t r, uint32_t* fluid_cells, uint32_t* mono_cells) {
+    if (!valid_rep(s, r)) return SPMD_ERR_ARG;
+    if (!s->collided) return fail(s, SPMD_ERR_STATE, "no collision since the fluid was set");
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    const DevState& d = s->d;
+    const RepDev& rp = s->reps[r];
+    if (fluid_cells) {
+        if (!d.ids[d.cur]) return fail(s, SPMD_ERR_STATE, "particle tracking is off");
+        std::vector<uint32_t> off(rp.ncells + 1), ids(rp.n_fluid);
+        HIPCHK(s, hipMemcpy(off.data(), d.offsets + rp.cell_base, 4ull * (rp.ncells + 1), hipMemcpyDeviceToHost));
+        HIPCHK(s, hipMemcpy(ids.data(), d.ids[d.cur] + rp.part_base, 4ull * rp.n_fluid, hipMemcpyDeviceToHost));
+        for (uint32_t c = 0; c < rp.ncells; ++c)
+            for (uint32_t p = off[c]; p < off[c + 1]; ++p) fluid_cells[ids[p - rp.part_base]] = c;
+    }
+    if (mono_cells && rp.n_mono) {
+        std::vector<uint64_t> ms(rp.n_mono);
+        HIPCHK(s, hipMemcpy(ms.data(), d.msorted + rp.mono_base, 8ull * rp.n_mono, hipMemcpyDeviceToHost));
+        for (uint64_t k : ms) mono_cells[(uint32_t)k] = (uint32_t)((k >> 32) - rp.cell_base);
+    }
+    return SPMD_OK;
+}
+
+int spmd_temperature(spmd_sys* s, uint32_t r, double* T) {
+    if (!valid_rep(s, r) || !T) return SPMD_ERR_ARG;
+    if (!s->collided) return fail(s, SPMD_ERR_STATE, "no collision since the fluid was set");
+    if (int rc = flush_md(s)) return rc;
+    const RepDev& rp = s->reps[r];
+    const uint32_t nt = (rp.ncells + s->d.tile_cells - 1) / s->d.tile_cells;
+    std::vector<uint32_t> tb(2);
+    HIPCHK(s, hipMemcpy(tb.data(), s->d.tile_base + r, 8, hipMemcpyDeviceToHost));
+    HIPCHK(s, launch_temperature(s->d, s->stream, (int)r, nt));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    std::vector<double> red(2ull * nt);
+    HIPCHK(s, hipMemcpy(red.data(), s->d.red + 2ull * tb[0], red.size() * 8, hipMemcpyDeviceToHost));
+    double ek = 0.0, filled = 0.0;
+    for (uint32_t i = 0; i < nt; ++i) { ek += red[2 * i]; filled += red[2 * i + 1]; }
+    *T = 2. * ek / (3. * ((double)rp.n_fluid - filled));
+    return SPMD_OK;
+}
+
+int spmd_epot(spmd_sys* s, uint32_t r, double* epot) {
+    if (!valid_rep(s, r) || !epot) return SPMD_ERR_ARG;
+    if (int rc = flush_md(s)) return rc;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(epot, &s->d.ctl[r].epot, 8, hipMemcpyDeviceToHost));
+    return SPMD_OK;
+}
+
+int spmd_profile_enable(spmd_sys* s, int on) {
+    if (!s) return SPMD_ERR_ARG;
+    s->prof = on != 0;
+    return SPMD_OK;
+}
+
+int spmd_profile_reset(spmd_sys* s) {
+    if (!s) return SPMD_ERR_ARG;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    for (auto& pe : s->pev) {
+        for (auto e : pe.ev) if (e) (void)hipEventDestroy(e);
+        pe.ev.clear();
+    }
+    return SPMD_OK;
+}
+
+int spmd_profile_read(spmd_sys* s, const char* name, double* avg_ms, int64_t* count) {
+    if (!s || !name) return SPMD_ERR_ARG;
+    int k = -1;
+    for (int i = 0; i < 5; ++i)
+        if (!std::strcmp(name, kProfNames[i])) k = i;
+    if (k < 0) return fail(s, SPMD_ERR_ARG, "unknown kernel class");
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    double tot = 0.0;
+    int64_t n = 0;
+    auto& ev = s->pev[k].ev;
+    for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+        float ms = 0.f;
+        if (ev[i] && ev[i + 1] && hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) { tot += ms; n++; }
+    }
+    if (avg_ms) *avg_ms = n ? tot / n : 0.0;
+    if (count) *count = n;
+    return SPMD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,122 @@
+/*
+ * spmd_internal.h -- device-resident state of a handle and the kernel
+ * launchers shared by spmd_kernels.hip and spmd_api.hip.
+ *
+ * Layout in HBM (all replicas of a handle concatenated):
+ *   solvent   x,y,z,vx,vy,vz   float[N_tot] each, two copies (cur/alt): the
+ *             cell sort writes alt, which becomes cur after the collision
+ *   rank      uint16[N_tot]    slot of a particle inside its new cell
+ *   counts    uint32[C_tot]    particles per cell (zeroed by the scan)
+ *   offsets   uint32[C_tot+1]  exclusive scan of counts = first particle
+ *                              of each cell in the sorted arrays
+ *   monomers  pos/vel/force/vpos double[3*M_tot], mass double[M_tot],
+ *             btype uint8[M_tot], Verlet lists uint16[M_tot*VCAP],
+ *             bond CSR (adj_off uint32[M_tot+R], adj uint32[2*bonds])
+ *   msorted   uint64[M_tot]    (global cell << 32 | monomer) of the last
+ *                              collision, sorted, for the collide tiles
+ *   RepDev    per-replica constants, RepCtl per-replica step state
+ */
+#ifndef SPMD_INTERNAL_H
+#define SPMD_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "spmd_math.h"
+
+namespace spmd {
+
+constexpr int kVcap = 128;         /* Verlet-list capacity per monomer */
+constexpr int kMaxMono = 2048;     /* monomers per replica (one MD workgroup) */
+constexpr int kMdThreads = 512;
+constexpr int kScanItems = 16;     /* counts per thread in the scan */
+constexpr int kScanThreads = 256;
+constexpr int kScanTile = kScanItems * kScanThreads;
+
+enum : uint32_t {
+    ST_CELL_OOB = 1u << 0,
+    ST_VERLET_OOB = 1u << 1,
+    ST_VERLET_OVERFLOW = 1u << 2,
+    ST_RANK_OVERFLOW = 1u << 3,
+    ST_SCAN_TIMEOUT = 1u << 4,
+};
+
+/* bond entry: partner index | anchor-FENE flag in bit 31 */
+constexpr uint32_t kBondAnchor = 0x80000000u;
+
+struct RepDev {
+    Box box;               /* delrx field unused here: lives in RepCtl */
+    double kT, c, s, lambda;
+    uint64_t seed;
+    uint32_t replica;
+    uint32_t ncells;
+    uint64_t cell_base;
+    uint64_t part_base;
+    uint32_t n_fluid;
+    uint32_t mono_base, n_mono;
+    uint32_t adj_base;     /* index of this replica's adj_off[0] */
+    int vgrid[3];          /* Verlet grid cells per side (Box.cpp:20-22) */
+    double vside[3];       /* Verlet grid cell side (Box.cpp:23-25) */
+};
+
+struct RepCtl {
+    double delrx;
+    double shift[3];
+    double com[3], com_vel[3];   /* check_bounds recentring of this collision */
+    double vcom[3];              /* COM at the last Verlet rebuild */
+    double epot;
+    double inj_shift[3];
+    uint32_t k_cur, k_done;      /* Philox collision counter */
+    int32_t recenter;
+    int32_t inj_pending, inj_active;
+    uint32_t status;
+    uint32_t n_rebuild;
+    uint32_t pad;
+};
+
+struct DevState {
+    int n_rep;
+    uint64_t n_tot;        /* solvent particles, all replicas */
+    uint64_t c_tot;        /* cells, all replicas */
+    uint32_t m_tot;        /* monomers, all replicas */
+    float *x[2], *y[2], *z[2], *vx[2], *vy[2], *vz[2];
+    uint32_t* ids[2];      /* null unless tracking */
+    int cur;
+    uint16_t* rank;
+    uint32_t* counts;
+    uint32_t* offsets;
+    uint64_t* scan_flags;  /* per scan tile: status<<62 | value */
+    uint32_t* scan_ticket;
+    uint32_t n_scan_tiles;
+    double *mpos, *mvel, *mforce, *mvpos, *mmass;
+    uint8_t* mbtype;
+    uint16_t* vl;
+    uint16_t* vl_n;
+    uint32_t *adj_off, *adj;
+    uint64_t* msorted;
+    double* inj_axis;      /* 2 per cell, all replicas */
+    double* inj_gamma;     /* 1 per cell */
+    RepDev* reps;          /* device copy */
+    RepCtl* ctl;           /* device */
+    uint32_t* pblk_base;   /* particle-block prefix per replica (n_rep+1) */
+    uint32_t* tile_base;   /* collide-tile prefix per replica (n_rep+1) */
+    double* red;           /* scratch for reductions: 4 doubles per block */
+    uint32_t n_pblk;       /* total particle blocks */
+    uint32_t n_tiles;      /* total collide tiles */
+    int ppt;               /* particles per thread in bin/scatter */
+    int tile_cells;        /* cells per collide tile */
+};
+
+/* launchers (spmd_kernels.hip) */
+hipError_t launch_md_prologue(const DevState& d, hipStream_t st, int n_md, int epot_last, int md_init,
+                              int do_collision, int n_delrx, double delrx_dt, double md_dt);
+hipError_t launch_bin(const DevState& d, hipStream_t st, double h);
+hipError_t launch_scan(const DevState& d, hipStream_t st);
+hipError_t launch_scatter(const DevState& d, hipStream_t st, double h);
+hipError_t launch_collide(const DevState& d, hipStream_t st);
+hipError_t launch_fluid_init(const DevState& d, hipStream_t st);
+hipError_t launch_temperature(const DevState& d, hipStream_t st, int replica, uint32_t ntiles);
+
+}  // namespace spmd
+
+#endif

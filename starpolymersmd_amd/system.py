@@ -1,0 +1,191 @@
+"""Python mirror of the reference's MPC / Thermostat_None / Box hot-path API.
+
+`MPCSystem` owns one handle of libspmd.so (one or more replicas) and exposes
+the reference's call names (MPC::initialize, MPC::step, MPC::Fluid,
+Thermostat_None::propagate, Box::add_star, Box::Molecules ...) with numpy
+arrays at the boundary.  Every method calls the HIP library; nothing is
+computed on the host.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import Params, SpmdError, load, make_params, ptr
+
+
+class MPCSystem:
+    def __init__(self, params: Params | list[Params]):
+        lib = load()
+        self._lib = lib
+        h = C.c_void_p()
+        if isinstance(params, Params):
+            rc = lib.spmd_create(C.byref(params), C.byref(h))
+            self.n_replicas = 1
+        else:
+            arr = (Params * len(params))(*params)
+            rc = lib.spmd_create_batch(arr, len(params), C.byref(h))
+            self.n_replicas = len(params)
+        if rc:
+            raise SpmdError(rc, lib.spmd_error_string(rc).decode())
+        self.h = h
+
+    # ------------------------------------------------------------ helpers
+    def _chk(self, rc: int):
+        if rc:
+            raise SpmdError(rc, self._lib.spmd_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.spmd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------ solvent
+    def num_fluid(self, r: int = 0) -> int:
+        n = C.c_uint64()
+        self._chk(self._lib.spmd_num_fluid(self.h, r, C.byref(n)))
+        return n.value
+
+    def initialize(self):
+        """MPC::initialize() (MPC.cpp:34-55) for every replica."""
+        self._chk(self._lib.spmd_fluid_initialize(self.h))
+
+    def set_fluid(self, pos: np.ndarray, vel: np.ndarray, r: int = 0):
+        pos = np.ascontiguousarray(pos, dtype=np.float32)
+        vel = np.ascontiguousarray(vel, dtype=np.float32)
+        self._chk(self._lib.spmd_set_fluid(self.h, r, ptr(pos, C.c_float), ptr(vel, C.c_float), pos.shape[0]))
+
+    def get_fluid(self, r: int = 0, ids: bool = False):
+        n = self.num_fluid(r)
+        pos = np.empty((n, 3), np.float32)
+        vel = np.empty((n, 3), np.float32)
+        idv = np.empty(n, np.uint32) if ids else None
+        self._chk(self._lib.spmd_get_fluid(self.h, r, ptr(pos, C.c_float), ptr(vel, C.c_float),
+                                           ptr(idv, C.c_uint32)))
+        return (pos, vel, idv) if ids else (pos, vel)
+
+    def set_tracking(self, on: bool = True):
+        self._chk(self._lib.spmd_set_tracking(self.h, int(on)))
+
+    # ------------------------------------------------------------ star
+    def add_star(self, A, B, arms, mass, bond=1.01, anchor_bond=2.0, r: int = 0):
+        """Box::add_star (Box.cpp:39-46)."""
+        self._chk(self._lib.spmd_add_star(self.h, r, A, B, arms, mass, bond, anchor_bond))
+
+    def num_monomers(self, r: int = 0) -> int:
+        n = C.c_uint32()
+        self._chk(self._lib.spmd_num_monomers(self.h, r, C.byref(n)))
+        return n.value
+
+    def get_monomers(self, r: int = 0):
+        n = self.num_monomers(r)
+        pos, vel, frc = (np.empty((n, 3)) for _ in range(3))
+        self._chk(self._lib.spmd_get_monomers(self.h, r, ptr(pos, C.c_double), ptr(vel, C.c_double),
+                                              ptr(frc, C.c_double)))
+        return pos, vel, frc
+
+    def set_monomers(self, pos=None, vel=None, r: int = 0):
+        p = None if pos is None else np.ascontiguousarray(pos, dtype=np.float64)
+        v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64)
+        self._chk(self._lib.spmd_set_monomers(self.h, r, ptr(p, C.c_double), ptr(v, C.c_double)))
+
+    # ------------------------------------------------------------ stepping
+    def md_init(self):
+        """Thermostat_None ctor (Thermostat_None.cpp:5-12)."""
+        self._chk(self._lib.spmd_md_init(self.h))
+
+    def set_md_dt(self, dt: float):
+        self._chk(self._lib.spmd_set_md_dt(self.h, dt))
+
+    def propagate(self, calc_epot: bool = False):
+        """Thermostat_None::propagate (Thermostat_None.cpp:21-38)."""
+        self._chk(self._lib.spmd_md_propagate(self.h, int(calc_epot)))
+
+    def step(self, t: int, dt: float):
+        """MPC::step (MPC.cpp:116-142)."""
+        self._chk(self._lib.spmd_mpc_step(self.h, t, dt))
+
+    def run(self, t0: int, n: int, mpc_dt: float):
+        """star-polymers.cpp:412-536 loop body, n calls from t0."""
+        self._chk(self._lib.spmd_run(self.h, t0, n, mpc_dt))
+
+    def sync(self):
+        self._chk(self._lib.spmd_sync(self.h))
+
+    # ------------------------------------------------------------ hooks
+    def set_delrx(self, v: float, r: int = 0):
+        self._chk(self._lib.spmd_set_delrx(self.h, r, v))
+
+    def get_delrx(self, r: int = 0) -> float:
+        v = C.c_double()
+        self._chk(self._lib.spmd_get_delrx(self.h, r, C.byref(v)))
+        return v.value
+
+    def inject(self, shift, axis_u, gamma, r: int = 0):
+        s = np.ascontiguousarray(shift, dtype=np.float64)
+        a = np.ascontiguousarray(axis_u, dtype=np.float64)
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        self._chk(self._lib.spmd_inject_collision_randoms(self.h, r, ptr(s, C.c_double), ptr(a, C.c_double),
+                                                          ptr(g, C.c_double)))
+
+    def last_collision(self, r: int = 0):
+        sh, cp, cv = np.empty(3), np.empty(3), np.empty(3)
+        rec = C.c_int()
+        self._chk(self._lib.spmd_last_collision(self.h, r, ptr(sh, C.c_double), C.byref(rec),
+                                                ptr(cp, C.c_double), ptr(cv, C.c_double)))
+        return sh, bool(rec.value), cp, cv
+
+    def cell_index(self, r: int = 0):
+        n = self.num_fluid(r)
+        fc = np.empty(n, np.uint32)
+        mc = np.empty(max(self.num_monomers(r), 1), np.uint32)
+        self._chk(self._lib.spmd_get_cell_index(self.h, r, ptr(fc, C.c_uint32), ptr(mc, C.c_uint32)))
+        return fc, mc[: self.num_monomers(r)]
+
+    # ------------------------------------------------------------ observables
+    def temperature(self, r: int = 0) -> float:
+        v = C.c_double()
+        self._chk(self._lib.spmd_temperature(self.h, r, C.byref(v)))
+        return v.value
+
+    def epot(self, r: int = 0) -> float:
+        v = C.c_double()
+        self._chk(self._lib.spmd_epot(self.h, r, C.byref(v)))
+        return v.value
+
+    # ------------------------------------------------------------ tuning/profiling
+    def set_tuning(self, ppt: int, tile_cells: int):
+        self._chk(self._lib.spmd_set_tuning(self.h, ppt, tile_cells))
+
+    def get_tuning(self):
+        a, b = C.c_int(), C.c_int()
+        self._chk(self._lib.spmd_get_tuning(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def profile(self, on: bool = True):
+        self._chk(self._lib.spmd_profile_enable(self.h, int(on)))
+
+    def profile_reset(self):
+        self._chk(self._lib.spmd_profile_reset(self.h))
+
+    def profile_read(self, name: str):
+        ms = C.c_double()
+        n = C.c_int64()
+        self._chk(self._lib.spmd_profile_read(self.h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+
+__all__ = ["MPCSystem", "make_params", "Params", "SpmdError"]
