@@ -80,7 +80,19 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--cpu-intervals", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run N MiB device-to-device copies (counter calibration)")
     a = ap.parse_args()
+    if a.copy_probe:
+        import torch
+
+        x = torch.empty(a.copy_probe << 20, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_mib": a.copy_probe, "bytes_read": x.numel(), "bytes_written": y.numel()}))
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the MPC box would be "
               "slab-decomposed along z with per-face halo sums; not built). Run with --gpus 1.",
