@@ -38,13 +38,13 @@ def make_system(w):
 
     g = MPCSystem(make_params(L=w["L"], n_c=w["n_c"], shear=w["shear"], step_update=w["step_update"],
                               md_dt=w["md_dt"], seed=w["seed"], lam=1.0, kT=1.0))
-    g.add_star(w["A"], w["B"], w["arms"], w["mass"])
+    g.add_star(w["A"], w["B"], w["arms"], w["mass"], coil=True)
     g.md_init()
     g.initialize()
     return g
 
 
-def cpu_baseline(w, intervals):
+def cpu_baseline(w, intervals, gpu_star):
     """FP64 oracle (faithful single-threaded restatement of the reference
     CPU path) on the same workload shape for a bounded number of intervals."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -52,7 +52,9 @@ def cpu_baseline(w, intervals):
 
     o = ob.Oracle(L=w["L"], n_c=w["n_c"], shear=w["shear"], step_update=w["step_update"], md_dt=w["md_dt"],
                   rng_mode=ob.RNG_PHILOX, seed=w["seed"])
-    o.add_star(w["A"], w["B"], w["arms"], w["mass"])
+    o.add_star(w["A"], w["B"], w["arms"], w["mass"])  # topology; positions/velocities from the library
+    pos, vel, _ = gpu_star
+    o.set_monomers(pos, vel)
     o.md_init()
     o.initialize()
     S = w["step_update"]
@@ -103,6 +105,7 @@ def main():
     S, h = w["step_update"], w["mpc_dt"]
     g = make_system(w)
     n_f = g.num_fluid()
+    star0 = g.get_monomers()
     t = 0
     g.run(t, 1, h)  # t=0: first collision
     t += 1
@@ -145,7 +148,7 @@ def main():
         "temperature": T,
     }
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, a.cpu_intervals)
+        out["cpu_baseline"] = cpu_baseline(w, a.cpu_intervals, star0)
     print(json.dumps(out), flush=True)
 
 

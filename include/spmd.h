@@ -96,6 +96,12 @@ int spmd_set_tracking(spmd_sys* s, int on);
  * Molecule::initialize_open_star Molecule.cpp:126-176 (Philox draws). */
 int spmd_add_star(spmd_sys* s, uint32_t replica, uint32_t A, uint32_t B, uint32_t arms,
                   double mass, double bond, double anchor_bond);
+/* Same star with seeded random-coil arms (no reference counterpart: the
+ * reference builds straight arms only, which for 30-bead arms leave the
+ * Verlet grid of Box.cpp:129-142).  Bead k>=2 sits at distance `bond` from
+ * bead k-1 with r^2 >= 1 to all other beads. */
+int spmd_add_star_coil(spmd_sys* s, uint32_t replica, uint32_t A, uint32_t B, uint32_t arms,
+                       double mass, double bond, double anchor_bond);
 int spmd_num_monomers(const spmd_sys* s, uint32_t replica, uint32_t* n);
 /* Molecules[0].Monomers[i].Position/Velocity/Force, xyz interleaved */
 int spmd_set_monomers(spmd_sys* s, uint32_t replica, const double* pos, const double* vel);
@@ -139,6 +145,10 @@ int spmd_get_cell_index(spmd_sys* s, uint32_t replica, uint32_t* fluid_cells, ui
 int spmd_temperature(spmd_sys* s, uint32_t replica, double* T);
 /* Molecule::Epot after the last propagate (Box.cpp:243-286). */
 int spmd_epot(spmd_sys* s, uint32_t replica, double* epot);
+
+/* Verlet rebuilds so far and entries of the current (half) lists of one
+ * replica's star -- diagnostics, no reference counterpart. */
+int spmd_md_stats(spmd_sys* s, uint32_t replica, uint32_t* n_rebuild, uint32_t* vl_entries);
 
 /* ---- tuning (no reference counterpart) ---- */
 /* particles per thread in the bin/scatter kernels (1,2,4,8) and cells per

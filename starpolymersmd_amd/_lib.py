@@ -69,6 +69,8 @@ _SIGS = {
     "spmd_set_tracking": (C.c_int, [C.c_void_p, C.c_int]),
     "spmd_add_star": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                 C.c_double, C.c_double, C.c_double]),
+    "spmd_add_star_coil": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                     C.c_double, C.c_double, C.c_double]),
     "spmd_num_monomers": (C.c_int, [C.c_void_p, C.c_uint32, _u32]),
     "spmd_set_monomers": (C.c_int, [C.c_void_p, C.c_uint32, _d, _d]),
     "spmd_get_monomers": (C.c_int, [C.c_void_p, C.c_uint32, _d, _d, _d]),
@@ -84,6 +86,7 @@ _SIGS = {
     "spmd_get_cell_index": (C.c_int, [C.c_void_p, C.c_uint32, _u32, _u32]),
     "spmd_temperature": (C.c_int, [C.c_void_p, C.c_uint32, _d]),
     "spmd_epot": (C.c_int, [C.c_void_p, C.c_uint32, _d]),
+    "spmd_md_stats": (C.c_int, [C.c_void_p, C.c_uint32, _u32, _u32]),
     "spmd_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "spmd_get_tuning": (C.c_int, [C.c_void_p, _P(C.c_int), _P(C.c_int)]),
     "spmd_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),

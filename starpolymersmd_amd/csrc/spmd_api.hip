@@ -30,7 +30,7 @@ struct ProfEvents {
     std::vector<hipEvent_t> ev;  /* start/stop pairs */
 };
 
-const char* kProfNames[5] = {"md", "bin", "scan", "scatter", "collide"};
+const char* kProfNames[6] = {"md", "bin", "scan", "scatter", "collide", "prologue"};
 
 }  // namespace
 
@@ -40,6 +40,8 @@ struct spmd_sys {
     std::vector<StarHost> stars;
     DevState d{};
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;    /* MD kernel beside the speculative solvent sort */
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int device = 0;
     std::string err;
     bool tracking = false;
@@ -57,7 +59,7 @@ struct spmd_sys {
     double graph_h = -1.0;
     /* profiling */
     bool prof = false;
-    ProfEvents pev[5];
+    ProfEvents pev[6];
     std::vector<size_t> alloc_bytes;
 };
 
@@ -147,6 +149,15 @@ int build_monomers(spmd_sys* s) {
         mt += st.n;
     }
     d.m_tot = mt;
+    /* MD kernel LDS: stride = largest star; Verlet capacity = what is left
+     * of the 160 KiB workgroup limit */
+    int ns = 1;
+    for (auto& st : s->stars) ns = std::max<int>(ns, (int)st.n);
+    ns = (ns + 1) & ~1;
+    d.md_ns = ns;
+    const size_t budget = 160u * 1024u - 4096u;  /* leave room for the kernel's static LDS */
+    const size_t fixed = md_lds_bytes(ns, 0);
+    d.md_vl_cap = fixed < budget ? (uint32_t)((budget - fixed) / 2) : 0u;
     dfree(d.mpos); dfree(d.mvel); dfree(d.mforce); dfree(d.mvpos); dfree(d.mmass); dfree(d.mbtype);
     dfree(d.vl); dfree(d.vl_n); dfree(d.adj_off); dfree(d.adj); dfree(d.msorted);
     if (int rc = dalloc(s, &d.mpos, 3ull * mt)) return rc;
@@ -194,38 +205,80 @@ int download_monomers(spmd_sys* s) {
 
 int ensure_monomers(spmd_sys* s) { return s->mono_dirty ? build_monomers(s) : SPMD_OK; }
 
-void prof_begin(spmd_sys* s, int k, hipEvent_t* e0) {
-    *e0 = nullptr;
+void prof_begin(spmd_sys* s, int k, hipStream_t st) {
     if (!s->prof) return;
-    if (hipEventCreate(e0) == hipSuccess) (void)hipEventRecord(*e0, s->stream);
-    s->pev[k].ev.push_back(*e0);
+    hipEvent_t e0 = nullptr;
+    if (hipEventCreate(&e0) == hipSuccess) (void)hipEventRecord(e0, st);
+    s->pev[k].ev.push_back(e0);
 }
-void prof_end(spmd_sys* s, int k) {
+void prof_end(spmd_sys* s, int k, hipStream_t st) {
     if (!s->prof) return;
     hipEvent_t e1 = nullptr;
-    if (hipEventCreate(&e1) == hipSuccess) (void)hipEventRecord(e1, s->stream);
+    if (hipEventCreate(&e1) == hipSuccess) (void)hipEventRecord(e1, st);
     s->pev[k].ev.push_back(e1);
 }
 
-#define LAUNCH(s, k, call)                                                                   \
+#define LAUNCH(s, k, st, call)                                                               \
     do {                                                                                      \
-        hipEvent_t e0_;                                                                       \
-        prof_begin((s), (k), &e0_);                                                           \
+        prof_begin((s), (k), (st));                                                           \
         hipError_t e_ = (call);                                                               \
-        prof_end((s), (k));                                                                   \
+        prof_end((s), (k), (st));                                                             \
         if (e_ != hipSuccess) return fail((s), SPMD_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
     } while (0)
 
-/* One collision for all replicas (MPC.cpp:121-141), preceded by the pending
- * MD substeps and delrx advances. */
+/* One collision for all replicas (MPC.cpp:121-141), preceded by the
+ * pending MD substeps and delrx advances.  The MD kernel runs on the side
+ * stream beside the solvent sort, which assumes no recentring; if the MD's
+ * check_bounds recentres a replica, the redo kernels sort that replica
+ * again (they return at once otherwise).  Used for direct calls and for
+ * graph capture alike. */
+hipError_t enqueue_collision_ops(spmd_sys* s, int n_md, int epot_last, int n_delrx, double delrx_dt, double h) {
+    const DevState& d = s->d;
+    hipError_t e;
+    prof_begin(s, 5, s->stream);
+    if ((e = launch_prologue(d, s->stream, 1, n_delrx, delrx_dt)) != hipSuccess) return e;
+    prof_end(s, 5, s->stream);
+    if (d.m_tot == 0) {
+        prof_begin(s, 1, s->stream);
+        if ((e = launch_bin(d, s->stream, h, kSortNormal)) != hipSuccess) return e;
+        prof_end(s, 1, s->stream);
+        prof_begin(s, 2, s->stream);
+        if ((e = launch_scan(d, s->stream)) != hipSuccess) return e;
+        prof_end(s, 2, s->stream);
+        prof_begin(s, 3, s->stream);
+        if ((e = launch_scatter(d, s->stream, h, kSortNormal)) != hipSuccess) return e;
+        prof_end(s, 3, s->stream);
+    } else {
+        if ((e = hipEventRecord(s->ev_fork, s->stream)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(s->side, s->ev_fork, 0)) != hipSuccess) return e;
+        prof_begin(s, 0, s->side);
+        if ((e = launch_md(d, s->side, n_md, epot_last, 0, 1, s->md_dt)) != hipSuccess) return e;
+        prof_end(s, 0, s->side);
+        if ((e = hipEventRecord(s->ev_join, s->side)) != hipSuccess) return e;
+        prof_begin(s, 1, s->stream);
+        if ((e = launch_bin(d, s->stream, h, kSortSpec)) != hipSuccess) return e;
+        prof_end(s, 1, s->stream);
+        prof_begin(s, 2, s->stream);
+        if ((e = launch_scan(d, s->stream)) != hipSuccess) return e;
+        prof_end(s, 2, s->stream);
+        prof_begin(s, 3, s->stream);
+        if ((e = launch_scatter(d, s->stream, h, kSortSpec)) != hipSuccess) return e;
+        prof_end(s, 3, s->stream);
+        if ((e = hipStreamWaitEvent(s->stream, s->ev_join, 0)) != hipSuccess) return e;
+        if ((e = launch_bin(d, s->stream, h, kSortRedo)) != hipSuccess) return e;
+        if ((e = launch_scan_redo(d, s->stream)) != hipSuccess) return e;
+        if ((e = launch_scatter(d, s->stream, h, kSortRedo)) != hipSuccess) return e;
+    }
+    prof_begin(s, 4, s->stream);
+    if ((e = launch_collide(d, s->stream)) != hipSuccess) return e;
+    prof_end(s, 4, s->stream);
+    return hipSuccess;
+}
+
 int enqueue_collision(spmd_sys* s, int n_md, int epot_last, int n_delrx, double delrx_dt, double h) {
-    DevState& d = s->d;
-    LAUNCH(s, 0, launch_md_prologue(d, s->stream, n_md, epot_last, 0, 1, n_delrx, delrx_dt, s->md_dt));
-    LAUNCH(s, 1, launch_bin(d, s->stream, h));
-    LAUNCH(s, 2, launch_scan(d, s->stream));
-    LAUNCH(s, 3, launch_scatter(d, s->stream, h));
-    LAUNCH(s, 4, launch_collide(d, s->stream));
-    d.cur ^= 1;
+    const hipError_t e = enqueue_collision_ops(s, n_md, epot_last, n_delrx, delrx_dt, h);
+    if (e != hipSuccess) return fail(s, SPMD_ERR_HIP, std::string("collision launch: ") + hipGetErrorString(e));
+    s->d.cur ^= 1;
     s->collided = true;
     return SPMD_OK;
 }
@@ -233,8 +286,8 @@ int enqueue_collision(spmd_sys* s, int n_md, int epot_last, int n_delrx, double 
 int flush_md(spmd_sys* s) {
     if (s->pending_md == 0 && s->pending_delrx == 0) return SPMD_OK;
     if (int rc = ensure_monomers(s)) return rc;
-    LAUNCH(s, 0, launch_md_prologue(s->d, s->stream, s->pending_md, s->pending_epot, 0, 0, s->pending_delrx,
-                                    s->pending_dt, s->md_dt));
+    LAUNCH(s, 5, s->stream, launch_prologue(s->d, s->stream, 0, s->pending_delrx, s->pending_dt));
+    LAUNCH(s, 0, s->stream, launch_md(s->d, s->stream, s->pending_md, s->pending_epot, 0, 0, s->md_dt));
     s->pending_md = 0;
     s->pending_delrx = 0;
     return SPMD_OK;
@@ -327,6 +380,75 @@ void build_star(StarHost& st, const RepDev& rep, const spmd_params& p, uint32_t 
     for (int q = 0; q < 3; ++q) com[q] /= (double)N;
     for (uint32_t i = 0; i < N; ++i)
         for (int q = 0; q < 3; ++q) st.pos[3 * i + q] -= com[q];
+}
+
+/* Random-coil arms (no reference counterpart; the reference only builds
+ * straight arms, which for long arms reach past the COM-centred Verlet grid
+ * of Box.cpp:129-142).  Topology, bead types, masses and velocities are
+ * those of build_star; bead k >= 2 of an arm is placed at distance `bond`
+ * from bead k-1 in a Philox-drawn direction, accepted when it keeps r^2 >=
+ * 1 to every other bead and does not fold back towards the core. */
+void coil_positions(StarHost& st, const RepDev& rep, const spmd_params& p, uint32_t A, uint32_t B, uint32_t arms,
+                    double bond, double anchor_bond) {
+    const double kPi = 3.14159265358979323846;
+    const uint32_t per_arm = A + B, N = st.n;
+    std::vector<double> P(3ull * N, 0.0);
+    std::vector<uint8_t> placed(N, 0);
+    placed[0] = 1;
+    const unsigned n = (unsigned)std::ceil(std::sqrt(arms));
+    /* first beads on the reference's (phi, theta) grid, all before any arm grows */
+    unsigned arm_count = 0;
+    for (unsigned i = 0; i < n; i++) {
+        const double phi = 2. * kPi * i / n;
+        for (unsigned j = 1; j < n + 1; j++) {
+            if (arm_count >= arms) break;
+            const double theta = kPi * j / (n + 1);
+            const double dir[3] = {std::cos(phi) * std::sin(theta), std::sin(phi) * std::sin(theta), std::cos(theta)};
+            const unsigned idx = arm_count * per_arm + 1;
+            for (int q = 0; q < 3; ++q) P[3 * idx + q] = anchor_bond * dir[q];
+            placed[idx] = 1;
+            arm_count++;
+        }
+    }
+    auto min_d2 = [&](const double* c, uint32_t skip) {
+        double m = 1e300;
+        for (uint32_t q = 0; q < N; ++q) {
+            if (!placed[q] || q == skip) continue;
+            const double dx = c[0] - P[3 * q], dy = c[1] - P[3 * q + 1], dz = c[2] - P[3 * q + 2];
+            const double r2 = (dx * dx + dy * dy) + dz * dz;
+            if (r2 < m) m = r2;
+        }
+        return m;
+    };
+    /* grow all arms one bead at a time */
+    for (unsigned k = 2; k <= per_arm; k++) {
+        for (unsigned a = 0; a < arms; a++) {
+            const unsigned idx = a * per_arm + k;
+            const double* prev = &P[3 * (idx - 1)];
+            const double rprev = std::sqrt((prev[0] * prev[0] + prev[1] * prev[1]) + prev[2] * prev[2]);
+            double best[3] = {0, 0, 0}, best_m = -1.0;
+            for (uint32_t att = 0; att < 512; ++att) {
+                uint32_t w[4];
+                draw4(p.seed, rep.replica, TAG_STAR_INIT, idx, 1, att, w);
+                const double z = 2.0 * u01(w[0]) - 1.0, ph = 2. * kPi * u01(w[1]);
+                const double sz = std::sqrt(1.0 - z * z);
+                const double c[3] = {prev[0] + bond * sz * std::cos(ph), prev[1] + bond * sz * std::sin(ph),
+                                     prev[2] + bond * z};
+                const double rc = std::sqrt((c[0] * c[0] + c[1] * c[1]) + c[2] * c[2]);
+                const double m = rc >= rprev - 0.5 ? min_d2(c, idx - 1) : -0.5;
+                if (m > best_m) { best_m = m; best[0] = c[0]; best[1] = c[1]; best[2] = c[2]; }
+                if (m >= 1.0) break;
+            }
+            for (int q = 0; q < 3; ++q) P[3 * idx + q] = best[q];
+            placed[idx] = 1;
+        }
+    }
+    double com[3] = {0, 0, 0};
+    for (uint32_t i = 0; i < N; ++i)
+        for (int q = 0; q < 3; ++q) com[q] += P[3 * i + q];
+    for (int q = 0; q < 3; ++q) com[q] /= (double)N;
+    for (uint32_t i = 0; i < N; ++i)
+        for (int q = 0; q < 3; ++q) st.pos[3 * i + q] = P[3 * i + q] - com[q];
 }
 
 int check_device(spmd_sys* s) {
@@ -422,6 +544,9 @@ int spmd_create_batch(const spmd_params* p, uint32_t n_rep, spmd_sys** out) {
     d.c_tot = ct;
     d.n_scan_tiles = (uint32_t)((ct + kScanTile - 1) / kScanTile);
     HIPCHK(s, hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIPCHK(s, hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
+    HIPCHK(s, hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
+    HIPCHK(s, hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
     int rc = SPMD_OK;
     for (int b = 0; b < 2 && !rc; ++b) {
         rc = rc ? rc : dalloc(s, &d.x[b], nt);
@@ -478,6 +603,10 @@ int spmd_destroy(spmd_sys* s) {
     dfree(d.vl); dfree(d.vl_n); dfree(d.adj_off); dfree(d.adj); dfree(d.msorted);
     dfree(d.inj_axis); dfree(d.inj_gamma); dfree(d.reps); dfree(d.ctl); dfree(d.pblk_base);
     dfree(d.tile_base); dfree(d.red);
+    if (s->side) (void)hipStreamSynchronize(s->side);
+    if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+    if (s->ev_join) (void)hipEventDestroy(s->ev_join);
+    if (s->side) (void)hipStreamDestroy(s->side);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return SPMD_OK;
@@ -518,7 +647,7 @@ int spmd_get_tuning(const spmd_sys* s, int* ppt, int* tile_cells) {
 int spmd_fluid_initialize(spmd_sys* s) {
     if (!s) return SPMD_ERR_ARG;
     if (int rc = flush_md(s)) return rc;
-    LAUNCH(s, 1, launch_fluid_init(s->d, s->stream));
+    LAUNCH(s, 1, s->stream, launch_fluid_init(s->d, s->stream));
     HIPCHK(s, hipStreamSynchronize(s->stream));
     s->collided = false;
     return SPMD_OK;
@@ -603,6 +732,18 @@ int spmd_add_star(spmd_sys* s, uint32_t r, uint32_t A, uint32_t B, uint32_t arms
     return ensure_monomers(s);
 }
 
+int spmd_add_star_coil(spmd_sys* s, uint32_t r, uint32_t A, uint32_t B, uint32_t arms, double mass, double bond,
+                       double anchor_bond) {
+    if (!valid_rep(s, r) || arms == 0 || A + B < 2 || !(mass > 0) || !(bond > 0)) return SPMD_ERR_ARG;
+    if ((A + B) * arms + 1 > (uint32_t)kMaxMono) return fail(s, SPMD_ERR_ARG, "star larger than 2048 monomers");
+    if (int rc = flush_md(s)) return rc;
+    if (int rc = download_monomers(s)) return rc;
+    build_star(s->stars[r], s->reps[r], s->params[r], A, B, arms, mass, bond, anchor_bond);
+    coil_positions(s->stars[r], s->reps[r], s->params[r], A, B, arms, bond, anchor_bond);
+    s->mono_dirty = true;
+    return ensure_monomers(s);
+}
+
 int spmd_num_monomers(const spmd_sys* s, uint32_t r, uint32_t* n) {
     if (!valid_rep(s, r) || !n) return SPMD_ERR_ARG;
     *n = s->stars[r].n;
@@ -644,7 +785,7 @@ int spmd_md_init(spmd_sys* s) {
     if (!s) return SPMD_ERR_ARG;
     if (int rc = flush_md(s)) return rc;
     if (int rc = ensure_monomers(s)) return rc;
-    LAUNCH(s, 0, launch_md_prologue(s->d, s->stream, 0, 0, 1, 0, 0, 0.0, s->md_dt));
+    LAUNCH(s, 0, s->stream, launch_md(s->d, s->stream, 0, 0, 1, 0, s->md_dt));
     HIPCHK(s, hipStreamSynchronize(s->stream));
     return check_status(s);
 }
@@ -700,12 +841,7 @@ int spmd_run(spmd_sys* s, int64_t t0, int64_t n, double mpc_dt) {
                 if (s->graph_h != mpc_dt) clear_graphs(s);
                 hipGraph_t g;
                 HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-                DevState& d = s->d;
-                hipError_t e = launch_md_prologue(d, s->stream, (int)S, 0, 0, 1, (int)S, mpc_dt, s->md_dt);
-                if (e == hipSuccess) e = launch_bin(d, s->stream, mpc_dt);
-                if (e == hipSuccess) e = launch_scan(d, s->stream);
-                if (e == hipSuccess) e = launch_scatter(d, s->stream, mpc_dt);
-                if (e == hipSuccess) e = launch_collide(d, s->stream);
+                const hipError_t e = enqueue_collision_ops(s, (int)S, 0, (int)S, mpc_dt, mpc_dt);
                 hipError_t e2 = hipStreamEndCapture(s->stream, &g);
                 if (e != hipSuccess || e2 != hipSuccess)
                     return fail(s, SPMD_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e ? e : e2));
@@ -832,6 +968,24 @@ int spmd_epot(spmd_sys* s, uint32_t r, double* epot) {
     return SPMD_OK;
 }
 
+int spmd_md_stats(spmd_sys* s, uint32_t r, uint32_t* n_rebuild, uint32_t* vl_entries) {
+    if (!valid_rep(s, r)) return SPMD_ERR_ARG;
+    if (int rc = flush_md(s)) return rc;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    RepCtl c;
+    HIPCHK(s, hipMemcpy(&c, s->d.ctl + r, sizeof c, hipMemcpyDeviceToHost));
+    if (n_rebuild) *n_rebuild = c.n_rebuild;
+    if (vl_entries) {
+        const uint32_t n = s->stars[r].n;
+        std::vector<uint16_t> cnt(n);
+        if (n) HIPCHK(s, hipMemcpy(cnt.data(), s->d.vl_n + s->reps[r].mono_base, 2ull * n, hipMemcpyDeviceToHost));
+        uint32_t t = 0;
+        for (uint16_t c2 : cnt) t += c2;
+        *vl_entries = t;
+    }
+    return SPMD_OK;
+}
+
 int spmd_profile_enable(spmd_sys* s, int on) {
     if (!s) return SPMD_ERR_ARG;
     s->prof = on != 0;
@@ -851,7 +1005,7 @@ int spmd_profile_reset(spmd_sys* s) {
 int spmd_profile_read(spmd_sys* s, const char* name, double* avg_ms, int64_t* count) {
     if (!s || !name) return SPMD_ERR_ARG;
     int k = -1;
-    for (int i = 0; i < 5; ++i)
+    for (int i = 0; i < 6; ++i)
         if (!std::strcmp(name, kProfNames[i])) k = i;
     if (k < 0) return fail(s, SPMD_ERR_ARG, "unknown kernel class");
     HIPCHK(s, hipStreamSynchronize(s->stream));

@@ -71,7 +71,7 @@ struct RepCtl {
     int32_t inj_pending, inj_active;
     uint32_t status;
     uint32_t n_rebuild;
-    uint32_t pad;
+    int32_t redo;                /* the speculative sort (no recentring) must be redone */
 };
 
 struct DevState {
@@ -103,16 +103,26 @@ struct DevState {
     double* red;           /* scratch for reductions: 4 doubles per block */
     uint32_t n_pblk;       /* total particle blocks */
     uint32_t n_tiles;      /* total collide tiles */
+    int md_ns;             /* LDS stride of the MD kernel: max monomers per replica (>= 1) */
+    uint32_t md_vl_cap;    /* Verlet entries the MD kernel keeps in LDS */
     int ppt;               /* particles per thread in bin/scatter */
     int tile_cells;        /* cells per collide tile */
 };
 
-/* launchers (spmd_kernels.hip) */
-hipError_t launch_md_prologue(const DevState& d, hipStream_t st, int n_md, int epot_last, int md_init,
-                              int do_collision, int n_delrx, double delrx_dt, double md_dt);
-hipError_t launch_bin(const DevState& d, hipStream_t st, double h);
+/* bytes of dynamic LDS the MD kernel needs for stride ns and cap entries */
+size_t md_lds_bytes(int ns, uint32_t cap);
+
+/* launchers (spmd_kernels.hip).  Sort modes: kSortNormal applies the
+ * recentring flag, kSortSpec assumes none (runs beside the MD kernel),
+ * kSortRedo redoes the replicas whose MD found a recentring. */
+enum { kSortNormal = 0, kSortSpec = 1, kSortRedo = 2 };
+hipError_t launch_prologue(const DevState& d, hipStream_t st, int do_collision, int n_delrx, double delrx_dt);
+hipError_t launch_md(const DevState& d, hipStream_t st, int n_md, int epot_last, int md_init, int do_collision,
+                     double md_dt);
+hipError_t launch_bin(const DevState& d, hipStream_t st, double h, int mode);
 hipError_t launch_scan(const DevState& d, hipStream_t st);
-hipError_t launch_scatter(const DevState& d, hipStream_t st, double h);
+hipError_t launch_scan_redo(const DevState& d, hipStream_t st);
+hipError_t launch_scatter(const DevState& d, hipStream_t st, double h, int mode);
 hipError_t launch_collide(const DevState& d, hipStream_t st);
 hipError_t launch_fluid_init(const DevState& d, hipStream_t st);
 hipError_t launch_temperature(const DevState& d, hipStream_t st, int replica, uint32_t ntiles);

@@ -4,15 +4,20 @@
  * uses MFMA.
  *
  * Per collision (MPC::step, star-polymers/lib/Hydrodynamics/MPC.cpp:116-142)
- * the library launches, on one stream, for all replicas at once:
- *   k_md_prologue  one workgroup per replica: the pending velocity-Verlet
- *                  substeps (Thermostat_None::propagate), the delrx advance,
- *                  check_bounds, the grid shift, monomer cell keys (sorted)
+ * the library launches, for all replicas at once:
+ *   k_prologue     per replica: delrx advance of the pending MPC::step calls,
+ *                  the collision's Philox counter and grid shift
+ *   k_md           (second stream) one workgroup per replica: the pending
+ *                  velocity-Verlet substeps (Thermostat_None::propagate),
+ *                  check_bounds, sorted monomer cell keys
  *   k_bin          per solvent particle: recentre, stream, LE wrap, shifted
- *                  cell index, slot in the cell via a wave-aggregated atomic
- *   k_scan         decoupled look-back exclusive scan of the cell counts
+ *                  cell index, slot in the cell via run-aggregated atomics
+ *   k_scan         single-pass exclusive scan of the cell counts
  *   k_scatter      per solvent particle: recompute the streamed state and
  *                  its cell, write it to its slot of the cell-sorted arrays
+ *   (redo)         k_bin/k_scan_redo/k_scatter again for replicas whose MD
+ *                  step recentred (bin and scatter ran beside k_md assuming
+ *                  it would not); they return at once otherwise
  *   k_collide      per tile of cells: momentum and energy sums in LDS
  *                  (64-bit fixed point, order independent), MBS thermostat,
  *                  SRD rotation, write-back
@@ -57,23 +62,29 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
 
-/* Slot of each lane's particle inside its cell: one atomicAdd per distinct
- * cell in the wave (leader peeling over a ballot). */
+/* Slot of each lane's particle inside its cell.  Lanes are grouped into
+ * runs of equal consecutive keys (the particles arrive in the previous
+ * collision's cell order, so runs are long); every run head reserves its
+ * whole run with one atomicAdd, and all heads of the wave issue theirs in
+ * the same instruction -- one round trip per wave. */
 __device__ __forceinline__ uint32_t wave_rank(uint32_t* counts, uint32_t key, bool ok) {
-    uint64_t active = __ballot(ok);
-    uint32_t rank = 0;
-    const uint64_t lt = lanemask_lt();
-    while (active) {
-        const int leader = __ffsll((unsigned long long)active) - 1;
-        const uint32_t lk = __shfl(key, leader);
-        const uint64_t same = __ballot(ok && key == lk) & active;
-        uint32_t base = 0;
-        if ((int)__lane_id() == leader) base = atomicAdd(&counts[lk], (uint32_t)__popcll(same));
-        base = __shfl(base, leader);
-        if (ok && key == lk) rank = base + (uint32_t)__popcll(same & lt);
-        active &= ~same;
+    const uint32_t lane = __lane_id();
+    const uint32_t prev_key = __shfl_up(key, 1);
+    const int prev_ok = __shfl_up((int)ok, 1);
+    const bool head = ok && (lane == 0 || !prev_ok || prev_key != key);
+    const uint64_t heads = __ballot(head);
+    const uint64_t stops = heads | __ballot(!ok);
+    const uint64_t le = (2ull << lane) - 1ull;   /* lanes <= this one (lane 63: all) */
+    uint32_t base = 0;
+    if (head) {
+        const uint64_t above = stops & ~le;
+        const uint32_t next = above ? (uint32_t)(__ffsll((unsigned long long)above) - 1) : 64u;
+        base = atomicAdd(&counts[key], next - lane);
     }
-    return rank;
+    const uint64_t hm = heads & le;
+    const int h = hm ? 63 - __clzll((long long)hm) : (int)lane;
+    const uint32_t b = __shfl(base, h);
+    return ok ? b + (lane - (uint32_t)h) : 0u;
 }
 
 /* MPC::check_bounds fluid part (MPC.cpp:183-188) when this collision
@@ -122,231 +133,518 @@ __device__ double block_sum(double v, double* scratch) {
 
 /* ======================================================= MD + prologue */
 
-__global__ __launch_bounds__(kMdThreads) void k_md_prologue(DevState d, int n_md, int epot_last, int md_init,
-                                                           int do_coll, int n_delrx, double delrx_dt, double dt) {
-    constexpr int PER = kMaxMono / kMdThreads;
-    __shared__ double sx[kMaxMono], sy[kMaxMono], sz[kMaxMono];
-    __shared__ uint8_t sbt[kMaxMono];
-    __shared__ double sred[kMdThreads / 64];
-    __shared__ double s_shift[3];
+/* Dynamic LDS of k_md_prologue for a stride of ns monomers (host computes
+ * the same size with md_lds_bytes) followed by cap Verlet entries. */
+constexpr int kGridMax = 4096;   /* cells of the Verlet rebuild grid kept in LDS */
+/* Verlet list radii: the reference keeps every pair within 2.0 (Box.cpp:9-10)
+ * and rebuilds once a monomer moved 0.25 = (2.0 - 1.5)/2.  Only B-B pairs
+ * interact beyond r^2 = 1.0594631, so the other pairs need a list radius of
+ * only 1.0293 + 0.5 for the same rebuild rule; the set of interacting pairs
+ * -- and so every force -- is unchanged. */
+constexpr double kListBB2 = 4.0;
+constexpr double kListAA2 = 2.34;   /* (sqrt(1.0594631) + 0.5)^2 = 2.3388 */
 
+struct MdLds {
+    double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *px, *py, *pz, *im, *red, *shift;
+    uint32_t* vl_off;
+    uint32_t* gcell;   /* grid cell starts (kGridMax + 1) */
+    uint32_t* gcid;    /* grid cell of each monomer (ns) */
+    uint32_t* aoff;    /* bond adjacency offsets (ns + 1) */
+    uint32_t* adj;     /* bond adjacency (2 ns) */
+    uint16_t* gmem;    /* monomers sorted by grid cell (ns) */
+    uint16_t* vl;
+    uint8_t* bt;
+    int* flags;
+};
+
+__device__ __forceinline__ MdLds md_carve(unsigned char* base, int ns) {
+    MdLds L;
+    double* p = reinterpret_cast<double*>(base);
+    L.x = p; L.y = p + ns; L.z = p + 2 * ns;
+    L.vx = p + 3 * ns; L.vy = p + 4 * ns; L.vz = p + 5 * ns;
+    L.fx = p + 6 * ns; L.fy = p + 7 * ns; L.fz = p + 8 * ns;
+    L.px = p + 9 * ns; L.py = p + 10 * ns; L.pz = p + 11 * ns;   /* Verlet reference positions */
+    L.im = p + 12 * ns;                                           /* masses */
+    L.red = p + 13 * ns;         /* 32 doubles */
+    L.shift = L.red + 32;        /* 4 doubles */
+    L.flags = reinterpret_cast<int*>(L.shift + 4);   /* 8 ints */
+    L.vl_off = reinterpret_cast<uint32_t*>(L.flags + 8);
+    L.gcell = L.vl_off + ns + 1;
+    L.gcid = L.gcell + kGridMax + 1;
+    L.aoff = L.gcid + ns;
+    L.adj = L.aoff + ns + 1;
+    L.gmem = reinterpret_cast<uint16_t*>(L.adj + 2 * ns);
+    L.bt = reinterpret_cast<uint8_t*>(L.gmem + ns);
+    L.vl = reinterpret_cast<uint16_t*>(L.bt + ((ns + 7) & ~7));
+    return L;
+}
+
+size_t md_lds_bytes(int ns, uint32_t cap) {
+    return (size_t)13 * ns * 8 + 36 * 8 + 8 * 4 + (size_t)(ns + 1) * 4 + (size_t)(kGridMax + 1) * 4 +
+           (size_t)ns * 4 + (size_t)(ns + 1) * 4 + (size_t)2 * ns * 4 + (size_t)ns * 2 + ((ns + 7) & ~7) +
+           (size_t)cap * 2 + 16;
+}
+
+template <int NT>
+__device__ double block_sum_all(double v, double* scratch) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) scratch[w] = v;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) t += scratch[i];
+    return t;
+}
+
+/* One workgroup per replica; all per-monomer state in LDS.  The pair loops
+ * give each monomer a group of kMdGroup lanes striding over its Verlet list,
+ * combined by a fixed shuffle tree (deterministic). */
+constexpr int kMdGroup = 1;
+
+/* Forces of monomer i's half list (every pair once, j > i), added to both
+ * monomers' 64-bit fixed-point LDS accumulators (order independent, so the
+ * list order does not change the result); bonds from both sides in bond
+ * order, in registers. */
+__device__ __forceinline__ void md_pair_force(const MdLds& S, const RepDev& rep, int i, const uint16_t* lst,
+                                              uint32_t nvl, int gl, double& fx, double& fy, double& fz) {
+#pragma clang fp contract(fast)
+    unsigned long long* qx = reinterpret_cast<unsigned long long*>(S.fx);
+    unsigned long long* qy = reinterpret_cast<unsigned long long*>(S.fy);
+    unsigned long long* qz = reinterpret_cast<unsigned long long*>(S.fz);
+    const double xi = S.x[i], yi = S.y[i], zi = S.z[i];
+    const bool bi = S.bt[i] != 0;
+    for (uint32_t q = gl; q < nvl; q += kMdGroup) {
+        const int j = lst[q];
+        const double dx = S.x[j] - xi, dy = S.y[j] - yi, dz = S.z[j] - zi;
+        const double r2 = (dx * dx + dy * dy) + dz * dz;
+        const double fa = pair_force_fast(r2, bi && S.bt[j] != 0, rep.lambda);
+        if (fa != 0.0) {
+            const unsigned long long gx = to_fix(dx * fa), gy = to_fix(dy * fa), gz = to_fix(dz * fa);
+            atomicAdd(&qx[j], gx); atomicAdd(&qy[j], gy); atomicAdd(&qz[j], gz);
+            atomicAdd(&qx[i], 0ull - gx); atomicAdd(&qy[i], 0ull - gy); atomicAdd(&qz[i], 0ull - gz);
+        }
+    }
+    const uint32_t a0 = S.aoff[i], a1 = S.aoff[i + 1];
+    for (uint32_t q = a0 + gl; q < a1; q += kMdGroup) {
+        const uint32_t e = S.adj[q];
+        const int j = (int)(e & ~kBondAnchor);
+        const double dx = S.x[j] - xi, dy = S.y[j] - yi, dz = S.z[j] - zi;
+        const double r2 = (dx * dx + dy * dy) + dz * dz;
+        const double fa = fene_force_fast(r2, (e & kBondAnchor) ? 20.25 : 2.25);
+        fx -= dx * fa; fy -= dy * fa; fz -= dz * fa;
+    }
+}
+
+/* Molecule::Epot contributions of monomer i (Box.cpp:257-277): its half-list
+ * pairs (each pair once) and half of every bond from each side. */
+__device__ __forceinline__ double md_pair_epot(const MdLds& S, const RepDev& rep, int i, const uint16_t* lst,
+                                            uint32_t nvl, int gl) {
+    const double xi = S.x[i], yi = S.y[i], zi = S.z[i];
+    const bool bi = S.bt[i] != 0;
+    unsigned long long epq = 0ull;  /* fixed point: independent of list order */
+    for (uint32_t q = gl; q < nvl; q += kMdGroup) {
+        const int j = lst[q];
+        const double dx = S.x[j] - xi, dy = S.y[j] - yi, dz = S.z[j] - zi;
+        epq += to_fix(pair_potential((dx * dx + dy * dy) + dz * dz, bi && S.bt[j] != 0, rep.lambda));
+    }
+    double ep = from_fix(epq);
+    const uint32_t a0 = S.aoff[i], a1 = S.aoff[i + 1];
+    for (uint32_t q = a0 + gl; q < a1; q += kMdGroup) {
+        const uint32_t e = S.adj[q];
+        const int j = (int)(e & ~kBondAnchor);
+        const double dx = S.x[j] - xi, dy = S.y[j] - yi, dz = S.z[j] - zi;
+        const double r2 = (dx * dx + dy * dy) + dz * dz;
+        ep += 0.5 * ((e & kBondAnchor) ? fene_anchor_potential(r2) : fene_potential(r2));
+    }
+    return ep;
+}
+
+/* Exclusive scan of a[0..n) in place (n <= 4*kMdThreads), total in a[n]
+ * and returned; each thread scans a contiguous chunk, deterministic. */
+__device__ uint32_t md_block_scan(uint32_t* a, int n, double* scratch_d) {
+    uint32_t* wsum = reinterpret_cast<uint32_t*>(scratch_d);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int per = (n + kMdThreads - 1) / kMdThreads;
+    const int i0 = tid * per;
+    uint32_t t = 0;
+    for (int q = 0; q < per; ++q) t += i0 + q < n ? a[i0 + q] : 0u;
+    uint32_t inc = t;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(inc, o);
+        if (lane >= o) inc += u;
+    }
+    __syncthreads();
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < kMdThreads / 64; ++q) {
+        const uint32_t u = wsum[q];
+        pre += q < w ? u : 0u;
+        tot += u;
+    }
+    uint32_t run = pre + inc - t;
+    for (int q = 0; q < per; ++q)
+        if (i0 + q < n) { const uint32_t v = a[i0 + q]; a[i0 + q] = run; run += v; }
+    if (tid == 0) a[n] = tot;
+    __syncthreads();
+    return tot;
+}
+
+template <bool MAX>
+__device__ double block_extreme(double v, double* scratch) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double u = __shfl_xor(v, o);
+        v = MAX ? fmax(v, u) : fmin(v, u);
+    }
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) scratch[w] = v;
+    __syncthreads();
+    double t = scratch[0];
+#pragma unroll
+    for (int i = 1; i < kMdThreads / 64; ++i) t = MAX ? fmax(t, scratch[i]) : fmin(t, scratch[i]);
+    return t;
+}
+
+/* MPC::check_bounds (MPC.cpp:160-190) on the monomers; a recentring also
+ * marks the speculative solvent sort of this collision for redo.  Separate
+ * function so its registers do not burden the MD loops. */
+__device__ __forceinline__ void md_check_bounds(const MdLds& S, const RepDev& rep, RepCtl* ctl, int N) {
+    constexpr int NT = kMdThreads;
+    const int tid = threadIdx.x;
+    bool out = false;
+    for (int i = tid; i < N; i += NT) {
+        out |= fabs(S.x[i]) >= (rep.box.L[0] * 0.5 - 0.5);
+        out |= fabs(S.y[i]) >= (rep.box.L[1] * 0.5 - 0.5);
+        out |= fabs(S.z[i]) >= (rep.box.L[2] * 0.5 - 0.5);
+    }
+    const bool rec = __syncthreads_or(out) != 0;
+    if (rec) {
+        double cp[3], cv[3];
+        for (int c = 0; c < 3; ++c) {
+            const double* a = c == 0 ? S.x : (c == 1 ? S.y : S.z);
+            const double* b = c == 0 ? S.vx : (c == 1 ? S.vy : S.vz);
+            double ap = 0.0, av = 0.0;
+            for (int i = tid; i < N; i += NT) { ap += a[i]; av += b[i]; }
+            cp[c] = block_sum_all<NT>(ap, S.red) / (double)N;
+            cv[c] = block_sum_all<NT>(av, S.red) / (double)N;
+        }
+        __syncthreads();
+        for (int i = tid; i < N; i += NT) {
+            S.x[i] -= cp[0]; S.y[i] -= cp[1]; S.z[i] -= cp[2];
+            S.vx[i] -= cv[0]; S.vy[i] -= cv[1]; S.vz[i] -= cv[2];
+        }
+        if (tid == 0)
+            for (int c = 0; c < 3; ++c) { ctl->com[c] = cp[c]; ctl->com_vel[c] = cv[c]; }
+    }
+    if (tid == 0) {
+        ctl->recenter = rec ? 1 : 0;
+        ctl->redo = rec ? 1 : 0;
+        for (int c = 0; c < 3; ++c) S.shift[c] = ctl->shift[c];
+    }
+    __syncthreads();
+}
+
+/* Box::update_VerletLists (Box.cpp:105-180): every pair with r^2 <= 4
+ * -- the COM-centred 27-cell grid of side >= 2 selects exactly these --
+ * plus the grid bounds check that raises SIGSEGV in the reference.
+ * Returns whether the lists were built in LDS (else in global memory). */
+__device__ __forceinline__ bool md_rebuild(const MdLds& S, const DevState& d, const RepDev& rep, RepCtl* ctl, int N,
+                                        uint32_t cap) {
+    constexpr int NT = kMdThreads;
+    constexpr int NG = NT / kMdGroup;
+    const int tid = threadIdx.x;
+    const int grp = tid / kMdGroup, gl = tid % kMdGroup;
+    const uint32_t mb = rep.mono_base;
+    bool lds_vl;
+        double cs[3];
+    for (int c = 0; c < 3; ++c) {
+        const double* a = c == 0 ? S.x : (c == 1 ? S.y : S.z);
+        double acc = 0.0;
+        for (int i = tid; i < N; i += NT) acc += a[i];
+        cs[c] = block_sum_all<NT>(acc, S.red) / (double)N;
+    }
+    if (tid == 0) { ctl->vcom[0] = cs[0]; ctl->vcom[1] = cs[1]; ctl->vcom[2] = cs[2]; ctl->n_rebuild++; }
+    for (int i = tid; i < N; i += NT) {
+        const double xi[3] = {S.x[i], S.y[i], S.z[i]};
+        bool oob = false;
+        for (int c = 0; c < 3; ++c) {
+            const int cn = (int)((xi[c] - cs[c] + rep.box.L[c] * 0.5) / rep.vside[c]);
+            oob |= cn < 0 || cn >= rep.vgrid[c];
+        }
+        if (oob) atomicOr(&ctl->status, ST_VERLET_OOB);
+        S.px[i] = xi[0]; S.py[i] = xi[1]; S.pz[i] = xi[2];
+    }
+    /* half list (j > i) of candidate pairs from a grid of side 2 over the star's bounding
+     * box (cells sorted by monomer index: deterministic lists); an
+     * all-pairs scan when the box needs more than kGridMax cells */
+    double lo[3], hi[3];
+    int gdim[3];
+    for (int c = 0; c < 3; ++c) {
+        const double* a = c == 0 ? S.x : (c == 1 ? S.y : S.z);
+        double mn = 1e300, mx = -1e300;
+        for (int i = tid; i < N; i += NT) { mn = fmin(mn, a[i]); mx = fmax(mx, a[i]); }
+        lo[c] = block_extreme<false>(mn, S.red);
+        hi[c] = block_extreme<true>(mx, S.red);
+        gdim[c] = (int)((hi[c] - lo[c]) * 0.5) + 1;
+    }
+    const long ncell = (long)gdim[0] * gdim[1] * gdim[2];
+    const bool use_grid = ncell <= kGridMax;
+    if (use_grid) {
+        for (int c = tid; c <= (int)ncell; c += NT) S.gcell[c] = 0u;
+        __syncthreads();
+        for (int i = tid; i < N; i += NT) {
+            int cc[3];
+            const double xi[3] = {S.x[i], S.y[i], S.z[i]};
+            for (int c = 0; c < 3; ++c) cc[c] = min((int)((xi[c] - lo[c]) * 0.5), gdim[c] - 1);
+            const uint32_t cid = (uint32_t)(cc[0] + gdim[0] * (cc[1] + gdim[1] * cc[2]));
+            S.gcid[i] = cid;
+            atomicAdd(&S.gcell[cid], 1u);
+        }
+        __syncthreads();
+        md_block_scan(S.gcell, (int)ncell, S.red);
+        for (int i = tid; i < N; i += NT) {
+            const uint32_t slot = atomicAdd(&S.gcell[S.gcid[i]], 1u);
+            S.gmem[slot] = (uint16_t)i;
+        }
+        __syncthreads();
+        /* gcell[c] now ends cell c.  Members stay in arrival order: the
+         * forces are summed in fixed point, so list order does not matter. */
+        __syncthreads();
+    }
+    /* visit the candidates of monomer i on lane gl; fn(j) per pair */
+    auto visit = [&](int i, auto&& fn) {
+        const double xi = S.x[i], yi = S.y[i], zi = S.z[i];
+        const bool bi = S.bt[i] != 0;
+        if (use_grid) {
+            const uint32_t cid = S.gcid[i];
+            const int cx = (int)(cid % (uint32_t)gdim[0]);
+            const int cy = (int)((cid / (uint32_t)gdim[0]) % (uint32_t)gdim[1]);
+            const int cz = (int)(cid / ((uint32_t)gdim[0] * (uint32_t)gdim[1]));
+            for (int q = gl; q < 27; q += kMdGroup) {
+                const int nx = cx + q % 3 - 1, ny = cy + (q / 3) % 3 - 1, nz = cz + q / 9 - 1;
+                if (nx < 0 || ny < 0 || nz < 0 || nx >= gdim[0] || ny >= gdim[1] || nz >= gdim[2]) continue;
+                const int nc = nx + gdim[0] * (ny + gdim[1] * nz);
+                const uint32_t a0 = nc ? S.gcell[nc - 1] : 0u, a1 = S.gcell[nc];
+                for (uint32_t p = a0; p < a1; ++p) {
+                    const int j = S.gmem[p];
+                    const double dx = S.x[j] - xi, dy = S.y[j] - yi, dz = S.z[j] - zi;
+                    if (j > i && ((dx * dx + dy * dy) + dz * dz) <= ((bi && S.bt[j]) ? kListBB2 : kListAA2)) fn(j);
+                }
+            }
+        } else {
+            for (int j = gl; j < N; j += kMdGroup) {
+                const double dx = S.x[j] - xi, dy = S.y[j] - yi, dz = S.z[j] - zi;
+                if (j > i && ((dx * dx + dy * dy) + dz * dz) <= ((bi && S.bt[j]) ? kListBB2 : kListAA2)) fn(j);
+            }
+        }
+    };
+    /* pass 1: counts */
+    for (int i = grp; i < N; i += NG) {
+        uint32_t c = 0;
+        visit(i, [&](int) { ++c; });
+#pragma unroll
+        for (int o = kMdGroup / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, kMdGroup);
+        if (gl == 0) S.vl_off[i] = c;
+    }
+    __syncthreads();
+    bool over = false;
+    for (int i = tid; i < N; i += NT) over |= S.vl_off[i] > (uint32_t)kVcap;
+    if (over) atomicOr(&ctl->status, ST_VERLET_OVERFLOW);
+    lds_vl = md_block_scan(S.vl_off, N, S.red) <= cap;
+    /* pass 2: fill, lane-major order inside a monomer's list */
+    for (int i = grp; i < N; i += NG) {
+        uint32_t c = 0;
+        visit(i, [&](int) { ++c; });
+        uint32_t pre = c;
+#pragma unroll
+        for (int o = 1; o < kMdGroup; o <<= 1) {
+            const uint32_t t = __shfl_up(pre, o, kMdGroup);
+            if (gl >= o) pre += t;
+        }
+        uint32_t w = pre - c;
+        const uint32_t total = S.vl_off[i + 1] - S.vl_off[i];
+        const uint32_t base = S.vl_off[i];
+        visit(i, [&](int j) {
+            if (lds_vl) S.vl[base + w] = (uint16_t)j;
+            else if (w < (uint32_t)kVcap) d.vl[(size_t)(mb + i) * kVcap + w] = (uint16_t)j;
+            ++w;
+        });
+        if (gl == 0 && !lds_vl) d.vl_n[mb + i] = (uint16_t)(total < (uint32_t)kVcap ? total : kVcap);
+    }
+    __syncthreads();
+    return lds_vl;
+}
+
+__global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epot_last, int md_init, int do_coll,
+                                                  double dt, int ns, uint32_t cap) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int NT = kMdThreads;
+    constexpr int NG = NT / kMdGroup;
+    const MdLds S = md_carve(smem, ns);
     const int r = blockIdx.x;
     const int tid = threadIdx.x;
+    const int grp = tid / kMdGroup, gl = tid % kMdGroup;
     const RepDev& rep = d.reps[r];
     RepCtl* ctl = d.ctl + r;
     const int N = (int)rep.n_mono;
     const uint32_t mb = rep.mono_base;
     const double dth = dt * 0.50;
 
-    double x[PER][3], v[PER][3], f[PER][3], m[PER];
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int i = tid + k * kMdThreads;
-        if (i < N) {
-            for (int c = 0; c < 3; ++c) {
-                x[k][c] = d.mpos[3 * (mb + i) + c];
-                v[k][c] = d.mvel[3 * (mb + i) + c];
-                f[k][c] = d.mforce[3 * (mb + i) + c];
+    for (int i = tid; i < N; i += NT) {
+        const size_t g = 3 * (size_t)(mb + i);
+        S.x[i] = d.mpos[g]; S.y[i] = d.mpos[g + 1]; S.z[i] = d.mpos[g + 2];
+        S.vx[i] = d.mvel[g]; S.vy[i] = d.mvel[g + 1]; S.vz[i] = d.mvel[g + 2];
+        S.fx[i] = d.mforce[g]; S.fy[i] = d.mforce[g + 1]; S.fz[i] = d.mforce[g + 2];
+        S.px[i] = d.mvpos[g]; S.py[i] = d.mvpos[g + 1]; S.pz[i] = d.mvpos[g + 2];
+        S.im[i] = dth / d.mmass[mb + i];  /* half-kick factor dt/(2m) */
+        S.bt[i] = d.mbtype[mb + i];
+    }
+    const uint32_t abase = N > 0 ? d.adj_off[rep.adj_base] : 0u;
+    for (int i = tid; i <= N; i += NT) S.aoff[i] = d.adj_off[rep.adj_base + i] - abase;
+    __syncthreads();
+    for (uint32_t q = tid; N > 0 && q < S.aoff[N]; q += NT) S.adj[q] = d.adj[abase + q];
+    /* Verlet lists: global fixed-capacity lists -> LDS CSR when they fit */
+    bool lds_vl = false, rebuilt = false;
+    if (N > 0 && !md_init) {
+        for (int i = tid; i < N; i += NT) S.vl_off[i] = d.vl_n[mb + i];
+        __syncthreads();
+        lds_vl = md_block_scan(S.vl_off, N, S.red) <= cap;
+        if (lds_vl) {
+            for (int i = grp; i < N; i += NG) {
+                const uint32_t o = S.vl_off[i], n = S.vl_off[i + 1] - o;
+                for (uint32_t q = gl; q < n; q += kMdGroup) S.vl[o + q] = d.vl[(size_t)(mb + i) * kVcap + q];
             }
-            m[k] = d.mmass[mb + i];
-            sx[i] = x[k][0]; sy[i] = x[k][1]; sz[i] = x[k][2];
-            sbt[i] = d.mbtype[mb + i];
         }
     }
     __syncthreads();
 
-    /* Box::update_VerletLists (Box.cpp:105-180): every pair with r^2 <= 4
-     * -- the COM-centred 27-cell grid of side >= 2 selects exactly these --
-     * plus the grid bounds check that raises SIGSEGV in the reference. */
     auto rebuild = [&]() {
-        double cs[3];
-        for (int c = 0; c < 3; ++c) {
-            double acc = 0.0;
-#pragma unroll
-            for (int k = 0; k < PER; ++k)
-                if (tid + k * kMdThreads < N) acc += x[k][c];
-            cs[c] = block_sum<kMdThreads>(acc, sred) / (double)N;
-        }
-        if (tid == 0) { ctl->vcom[0] = cs[0]; ctl->vcom[1] = cs[1]; ctl->vcom[2] = cs[2]; ctl->n_rebuild++; }
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int i = tid + k * kMdThreads;
-            if (i >= N) continue;
-            bool oob = false;
-            for (int c = 0; c < 3; ++c) {
-                const int cn = (int)((x[k][c] - cs[c] + rep.box.L[c] * 0.5) / rep.vside[c]);
-                oob |= cn < 0 || cn >= rep.vgrid[c];
-            }
-            if (oob) atomicOr(&ctl->status, ST_VERLET_OOB);
-            int cnt = 0;
-            uint16_t* lst = d.vl + (size_t)(mb + i) * kVcap;
-            for (int j = 0; j < N; ++j) {
-                if (j == i) continue;
-                const double dx = sx[j] - x[k][0], dy = sy[j] - x[k][1], dz = sz[j] - x[k][2];
-                const double r2 = (dx * dx + dy * dy) + dz * dz;
-                if (r2 <= 4.0) {
-                    if (cnt < kVcap) lst[cnt] = (uint16_t)j;
-                    cnt++;
-                }
-            }
-            if (cnt > kVcap) atomicOr(&ctl->status, ST_VERLET_OVERFLOW);
-            d.vl_n[mb + i] = (uint16_t)(cnt < kVcap ? cnt : kVcap);
-            for (int c = 0; c < 3; ++c) d.mvpos[3 * (mb + i) + c] = x[k][c];
-        }
+        lds_vl = md_rebuild(S, d, rep, ctl, N, cap);
+        rebuilt = true;
     };
 
-    /* Box::calculate_forces_verlet (Box.cpp:243-286). */
+
+    /* Box::calculate_forces_verlet (Box.cpp:243-286) */
+    double epot_val = 0.0;
+    bool epot_set = false;
+    /* fixed-point force accumulators live in the force arrays during forces() */
+    unsigned long long* const qx = reinterpret_cast<unsigned long long*>(S.fx);
+    unsigned long long* const qy = reinterpret_cast<unsigned long long*>(S.fy);
+    unsigned long long* const qz = reinterpret_cast<unsigned long long*>(S.fz);
     auto forces = [&](bool calc_epot) {
         double ep = 0.0;
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int i = tid + k * kMdThreads;
-            if (i >= N) continue;
+        for (int i = grp; i < N; i += NG) {
             double fx = 0.0, fy = 0.0, fz = 0.0;
-            const bool bi = sbt[i] != 0;
-            const int nvl = d.vl_n[mb + i];
-            const uint16_t* lst = d.vl + (size_t)(mb + i) * kVcap;
-            for (int q = 0; q < nvl; ++q) {
-                const int j = lst[q];
-                const double dx = sx[j] - x[k][0], dy = sy[j] - x[k][1], dz = sz[j] - x[k][2];
-                const double r2 = (dx * dx + dy * dy) + dz * dz;
-                const bool bb = bi && sbt[j] != 0;
-                const double fa = pair_force(r2, bb, rep.lambda);
-                fx -= dx * fa; fy -= dy * fa; fz -= dz * fa;
-                if (calc_epot) ep += 0.5 * pair_potential(r2, bb, rep.lambda);
+            if (lds_vl) {
+                const uint16_t* lst = S.vl + S.vl_off[i];
+                const uint32_t nvl = S.vl_off[i + 1] - S.vl_off[i];
+                md_pair_force(S, rep, i, lst, nvl, gl, fx, fy, fz);
+                if (calc_epot) ep += md_pair_epot(S, rep, i, lst, nvl, gl);
+            } else {
+                const uint16_t* lst = d.vl + (size_t)(mb + i) * kVcap;
+                const uint32_t nvl = d.vl_n[mb + i];
+                md_pair_force(S, rep, i, lst, nvl, gl, fx, fy, fz);
+                if (calc_epot) ep += md_pair_epot(S, rep, i, lst, nvl, gl);
             }
-            const uint32_t a0 = d.adj_off[rep.adj_base + i], a1 = d.adj_off[rep.adj_base + i + 1];
-            for (uint32_t q = a0; q < a1; ++q) {
-                const uint32_t e = d.adj[q];
-                const int j = (int)(e & ~kBondAnchor);
-                const bool anchor = (e & kBondAnchor) != 0;
-                const double dx = sx[j] - x[k][0], dy = sy[j] - x[k][1], dz = sz[j] - x[k][2];
-                const double r2 = (dx * dx + dy * dy) + dz * dz;
-                const double fa = anchor ? fene_anchor_force(r2) : fene_force(r2);
-                fx -= dx * fa; fy -= dy * fa; fz -= dz * fa;
-                if (calc_epot) ep += 0.5 * (anchor ? fene_anchor_potential(r2) : fene_potential(r2));
+#pragma unroll
+            for (int o = kMdGroup / 2; o > 0; o >>= 1) {
+                fx += __shfl_xor(fx, o, kMdGroup);
+                fy += __shfl_xor(fy, o, kMdGroup);
+                fz += __shfl_xor(fz, o, kMdGroup);
             }
-            f[k][0] = fx; f[k][1] = fy; f[k][2] = fz;
+            if (gl == 0) {
+                atomicAdd(&qx[i], to_fix(fx));
+                atomicAdd(&qy[i], to_fix(fy));
+                atomicAdd(&qz[i], to_fix(fz));
+            }
         }
-        const double tot = block_sum<kMdThreads>(ep, sred);
-        if (tid == 0) ctl->epot = calc_epot ? tot : 0.0;
+        __syncthreads();
+        for (int i = tid; i < N; i += NT) {
+            S.fx[i] = from_fix(qx[i]);
+            S.fy[i] = from_fix(qy[i]);
+            S.fz[i] = from_fix(qz[i]);
+        }
+        if (calc_epot) {
+            const double tot = block_sum_all<NT>(ep, S.red);
+            epot_val = tot;
+        } else {
+            epot_val = 0.0;
+            __syncthreads();
+        }
     };
 
     if (N > 0) {
         if (md_init) {
             rebuild();
+            for (int i = tid; i < N; i += NT) { qx[i] = 0ull; qy[i] = 0ull; qz[i] = 0ull; }
+            __syncthreads();
             forces(false);
+            epot_set = true;
         }
         for (int s = 0; s < n_md; ++s) {
-            /* Thermostat_None::propagate (Thermostat_None.cpp:21-38) */
-#pragma unroll
-            for (int k = 0; k < PER; ++k) {
-                const int i = tid + k * kMdThreads;
-                if (i >= N) continue;
-                for (int c = 0; c < 3; ++c) {
-                    v[k][c] += (f[k][c] / m[k]) * dth;
-                    x[k][c] += v[k][c] * dt;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < PER; ++k) {
-                const int i = tid + k * kMdThreads;
-                if (i < N) { sx[i] = x[k][0]; sy[i] = x[k][1]; sz[i] = x[k][2]; }
-            }
-            /* Box::check_VerletLists (Box.cpp:182-194) */
+            /* Thermostat_None::propagate (Thermostat_None.cpp:21-38) and
+             * Box::check_VerletLists (Box.cpp:182-194) */
             bool need = false;
-#pragma unroll
-            for (int k = 0; k < PER; ++k) {
-                const int i = tid + k * kMdThreads;
-                if (i >= N) continue;
-                const double dx = x[k][0] - d.mvpos[3 * (mb + i)];
-                const double dy = x[k][1] - d.mvpos[3 * (mb + i) + 1];
-                const double dz = x[k][2] - d.mvpos[3 * (mb + i) + 2];
+            for (int i = tid; i < N; i += NT) {
+                const double h = S.im[i];
+                double vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
+                vx += S.fx[i] * h; vy += S.fy[i] * h; vz += S.fz[i] * h;
+                qx[i] = 0ull; qy[i] = 0ull; qz[i] = 0ull;
+                const double x = S.x[i] + vx * dt, y = S.y[i] + vy * dt, z = S.z[i] + vz * dt;
+                S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz;
+                S.x[i] = x; S.y[i] = y; S.z[i] = z;
+                const double dx = x - S.px[i], dy = y - S.py[i], dz = z - S.pz[i];
                 need |= sqrt((dx * dx + dy * dy) + dz * dz) > (2.0 - 1.5) * 0.5;
             }
-            if (__syncthreads_or(need)) {
-                rebuild();
-                __syncthreads();
-            }
+            if (__syncthreads_or(need)) rebuild();
             forces(epot_last && s == n_md - 1);
-#pragma unroll
-            for (int k = 0; k < PER; ++k) {
-                const int i = tid + k * kMdThreads;
-                if (i >= N) continue;
-                for (int c = 0; c < 3; ++c) v[k][c] += (f[k][c] / m[k]) * dth;
+            epot_set = true;
+            for (int i = tid; i < N; i += NT) {
+                const double h = S.im[i];
+                S.vx[i] += S.fx[i] * h; S.vy[i] += S.fy[i] * h; S.vz[i] += S.fz[i] * h;
             }
             __syncthreads();
+        }
+        if (rebuilt && lds_vl) {
+            for (int i = grp; i < N; i += NG) {
+                const uint32_t o = S.vl_off[i], n = S.vl_off[i + 1] - o;
+                for (uint32_t q = gl; q < n && q < (uint32_t)kVcap; q += kMdGroup)
+                    d.vl[(size_t)(mb + i) * kVcap + q] = S.vl[o + q];
+                if (gl == 0) d.vl_n[mb + i] = (uint16_t)(n < (uint32_t)kVcap ? n : kVcap);
+            }
         }
     }
 
-    if (tid == 0) {
-        /* MPC::step delrx advance, MPC.cpp:117-118, once per call */
-        double dl = ctl->delrx;
-        for (int q = 0; q < n_delrx; ++q) {
-            dl += rep.box.shear * rep.box.L[1] * delrx_dt;
-            dl -= rep.box.L[0] * round(dl / rep.box.L[0]);
-        }
-        ctl->delrx = dl;
+    if (tid == 0 && epot_set) ctl->epot = epot_val;
+    if (do_coll) md_check_bounds(S, rep, ctl, N);
+
+    /* write the monomer state back */
+    for (int i = tid; i < N; i += NT) {
+        const size_t g = 3 * (size_t)(mb + i);
+        d.mpos[g] = S.x[i]; d.mpos[g + 1] = S.y[i]; d.mpos[g + 2] = S.z[i];
+        d.mvel[g] = S.vx[i]; d.mvel[g + 1] = S.vy[i]; d.mvel[g + 2] = S.vz[i];
+        d.mforce[g] = S.fx[i]; d.mforce[g + 1] = S.fy[i]; d.mforce[g + 2] = S.fz[i];
+        d.mvpos[g] = S.px[i]; d.mvpos[g + 1] = S.py[i]; d.mvpos[g + 2] = S.pz[i];
     }
 
     if (do_coll) {
-        /* MPC::check_bounds, MPC.cpp:160-190 */
-        bool out = false;
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int i = tid + k * kMdThreads;
-            if (i >= N) continue;
-            for (int c = 0; c < 3; ++c) out |= fabs(x[k][c]) >= (rep.box.L[c] * 0.5 - 0.5);
-        }
-        const bool rec = __syncthreads_or(out) != 0;
-        if (rec) {
-            double cp[3], cv[3];
-            for (int c = 0; c < 3; ++c) {
-                double ap = 0.0, av = 0.0;
-#pragma unroll
-                for (int k = 0; k < PER; ++k)
-                    if (tid + k * kMdThreads < N) { ap += x[k][c]; av += v[k][c]; }
-                cp[c] = block_sum<kMdThreads>(ap, sred) / (double)N;
-                cv[c] = block_sum<kMdThreads>(av, sred) / (double)N;
-            }
-#pragma unroll
-            for (int k = 0; k < PER; ++k)
-                if (tid + k * kMdThreads < N)
-                    for (int c = 0; c < 3; ++c) { x[k][c] -= cp[c]; v[k][c] -= cv[c]; }
-            if (tid == 0)
-                for (int c = 0; c < 3; ++c) { ctl->com[c] = cp[c]; ctl->com_vel[c] = cv[c]; }
-        }
-        if (tid == 0) {
-            ctl->recenter = rec ? 1 : 0;
-            const uint32_t k = ctl->k_done;
-            if (ctl->inj_pending) {
-                for (int c = 0; c < 3; ++c) s_shift[c] = ctl->inj_shift[c];
-                ctl->inj_active = 1;
-                ctl->inj_pending = 0;
-            } else {
-                uint32_t w[4];
-                draw4(rep.seed, rep.replica, TAG_SHIFT, 0, k, 0, w);
-                for (int c = 0; c < 3; ++c) s_shift[c] = u01(w[c]) - 0.5;
-                ctl->inj_active = 0;
-            }
-            for (int c = 0; c < 3; ++c) ctl->shift[c] = s_shift[c];
-            ctl->k_cur = k;
-            ctl->k_done = k + 1;
-        }
-        __syncthreads();
         /* monomer cell keys (MPC::shiftParticles MPC.cpp:306-310 + sort
-         * MPC.cpp:221-235: shifted, never wrapped), bitonic-sorted */
-        uint64_t* keys = reinterpret_cast<uint64_t*>(sx);
+         * MPC.cpp:221-235: shifted, never wrapped), bitonic-sorted in LDS
+         * over the velocity/force arrays, which are written back above */
+        __syncthreads();
+        uint64_t* keys = reinterpret_cast<uint64_t*>(S.vx);
         int np2 = 1;
         while (np2 < N) np2 <<= 1;
-        for (int i = tid; i < np2; i += kMdThreads) keys[i] = ~0ull;
+        for (int i = tid; i < np2; i += NT) keys[i] = ~0ull;
         __syncthreads();
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int i = tid + k * kMdThreads;
-            if (i >= N) continue;
-            const uint32_t ci = cell_index(rep.box, x[k][0] + s_shift[0], x[k][1] + s_shift[1], x[k][2] + s_shift[2]);
+        for (int i = tid; i < N; i += NT) {
+            const uint32_t ci = cell_index(rep.box, S.x[i] + S.shift[0], S.y[i] + S.shift[1], S.z[i] + S.shift[2]);
             if (ci >= rep.ncells) atomicOr(&ctl->status, ST_CELL_OOB);
             const uint64_t g = ci < rep.ncells ? rep.cell_base + ci : 0xFFFFFFFFull;
             keys[i] = (g << 32) | (uint32_t)i;
@@ -354,7 +652,7 @@ __global__ __launch_bounds__(kMdThreads) void k_md_prologue(DevState d, int n_md
         __syncthreads();
         for (int kk = 2; kk <= np2; kk <<= 1) {
             for (int j = kk >> 1; j > 0; j >>= 1) {
-                for (int i = tid; i < np2; i += kMdThreads) {
+                for (int i = tid; i < np2; i += NT) {
                     const int ixj = i ^ j;
                     if (ixj > i) {
                         const uint64_t a = keys[i], b2 = keys[ixj];
@@ -365,33 +663,75 @@ __global__ __launch_bounds__(kMdThreads) void k_md_prologue(DevState d, int n_md
                 __syncthreads();
             }
         }
-        for (int i = tid; i < N; i += kMdThreads) d.msorted[mb + i] = keys[i];
-        if (r == 0) {
-            for (uint32_t t = tid; t < d.n_scan_tiles; t += kMdThreads) d.scan_flags[t] = 0ull;
-            if (tid == 0) *d.scan_ticket = 0u;
+        for (int i = tid; i < N; i += NT) d.msorted[mb + i] = keys[i];
+    }
+}
+
+/* ========================================================== prologue */
+
+/* Per replica, before anything else of a collision: the delrx advance of
+ * every pending MPC::step call (MPC.cpp:117-118), then the collision's
+ * Philox counter and grid shift (MPC.cpp:123).  Block 0 resets the scan
+ * state. */
+__global__ __launch_bounds__(64) void k_prologue(DevState d, int do_coll, int n_delrx, double delrx_dt) {
+    const int r = blockIdx.x;
+    const RepDev& rep = d.reps[r];
+    RepCtl* ctl = d.ctl + r;
+    if (threadIdx.x == 0) {
+        double dl = ctl->delrx;
+        for (int q = 0; q < n_delrx; ++q) {
+            dl += rep.box.shear * rep.box.L[1] * delrx_dt;
+            dl -= rep.box.L[0] * round(dl / rep.box.L[0]);
+        }
+        ctl->delrx = dl;
+        if (do_coll) {
+            const uint32_t k = ctl->k_done;
+            double sh[3];
+            if (ctl->inj_pending) {
+                for (int c = 0; c < 3; ++c) sh[c] = ctl->inj_shift[c];
+                ctl->inj_active = 1;
+                ctl->inj_pending = 0;
+            } else {
+                uint32_t w[4];
+                draw4(rep.seed, rep.replica, TAG_SHIFT, 0, k, 0, w);
+                for (int c = 0; c < 3; ++c) sh[c] = u01(w[c]) - 0.5;
+                ctl->inj_active = 0;
+            }
+            for (int c = 0; c < 3; ++c) ctl->shift[c] = sh[c];
+            ctl->k_cur = k;
+            ctl->k_done = k + 1;
+            ctl->recenter = 0;
+            ctl->redo = 0;
         }
     }
-
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int i = tid + k * kMdThreads;
-        if (i >= N) continue;
-        for (int c = 0; c < 3; ++c) {
-            d.mpos[3 * (mb + i) + c] = x[k][c];
-            d.mvel[3 * (mb + i) + c] = v[k][c];
-            d.mforce[3 * (mb + i) + c] = f[k][c];
-        }
+    if (do_coll && r == 0) {
+        for (uint32_t t = threadIdx.x; t < d.n_scan_tiles; t += 64) d.scan_flags[t] = 0ull;
+        if (threadIdx.x == 0) *d.scan_ticket = 0u;
     }
 }
 
 /* ============================================================== bin */
 
+/* Control words the sort kernels use; in the speculative mode the
+ * recentring fields (written concurrently by k_md) are not read. */
+__device__ __forceinline__ bool sort_ctl(const DevState& d, int r, int mode, RepCtl& c) {
+    const RepCtl* g = d.ctl + r;
+    if (mode == kSortRedo && !g->redo) return false;
+    c.delrx = g->delrx;
+    c.shift[0] = g->shift[0]; c.shift[1] = g->shift[1]; c.shift[2] = g->shift[2];
+    c.recenter = mode == kSortSpec ? 0 : g->recenter;
+    if (c.recenter)
+        for (int q = 0; q < 3; ++q) { c.com[q] = g->com[q]; c.com_vel[q] = g->com_vel[q]; }
+    return true;
+}
+
 template <int PPT>
-__global__ __launch_bounds__(256) void k_bin(DevState d, double h) {
+__global__ __launch_bounds__(256) void k_bin(DevState d, double h, int mode) {
     const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
     const int r = find_rep(d.pblk_base, d.n_rep, lb);
     const RepDev& rep = d.reps[r];
-    const RepCtl c = d.ctl[r];
+    RepCtl c;
+    if (!sort_ctl(d, r, mode, c)) return;
     Box b = rep.box;
     b.delrx = c.delrx;
     const uint64_t p0 = rep.part_base + (uint64_t)(lb - d.pblk_base[r]) * (256 * PPT);
@@ -465,34 +805,33 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(DevState d) {
         agg += s_wsum[q];
     }
     const uint32_t texcl = wpre + inc - run;
-    if (tid == 0) {
-        uint32_t prefix = 0;
-        if (tile == 0) {
-            __hip_atomic_store(&d.scan_flags[0], (2ull << 62) | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(&d.scan_flags[tile], (1ull << 62) | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            int64_t p = (int64_t)tile - 1;
-            uint32_t spins = 0;
-            while (p >= 0) {
-                const uint64_t f = __hip_atomic_load(&d.scan_flags[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                const uint64_t st = f >> 62;
-                if (st == 0) {
-                    if (++spins > (1u << 24)) {
-                        atomicOr(&d.ctl[0].status, ST_SCAN_TIMEOUT);
-                        break;
-                    }
+    /* publish this tile's aggregate, then wave 0 sums the aggregates of
+     * all earlier tiles 64 at a time (earlier tiles hold earlier tickets,
+     * so they are running and will publish; the wait is bounded) */
+    if (tid == 0)
+        __hip_atomic_store(&d.scan_flags[tile], (1ull << 62) | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (w == 0) {
+        uint32_t acc = 0;
+        bool timeout = false;
+        for (uint32_t b0 = 0; b0 < tile; b0 += 64) {
+            const uint32_t j = b0 + lane;
+            if (j < tile) {
+                uint64_t f;
+                uint32_t spins = 0;
+                while (((f = __hip_atomic_load(&d.scan_flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) >> 62) == 0) {
+                    if (++spins > (1u << 22)) { timeout = true; break; }
                     __builtin_amdgcn_s_sleep(1);
-                    continue;
                 }
-                prefix += (uint32_t)(f & 0xFFFFFFFFull);
-                if (st == 2) break;
-                --p;
+                acc += (uint32_t)(f & 0xFFFFFFFFull);
             }
-            __hip_atomic_store(&d.scan_flags[tile], (2ull << 62) | (uint64_t)(prefix + agg), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
         }
-        s_prefix = prefix;
-        if (base <= d.c_tot && (uint64_t)(tile + 1) * kScanTile >= d.c_tot) d.offsets[d.c_tot] = prefix + agg;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+        if (timeout) atomicOr(&d.ctl[0].status, ST_SCAN_TIMEOUT);
+        if (lane == 0) {
+            s_prefix = acc;
+            if ((uint64_t)(tile + 1) * kScanTile >= d.c_tot) d.offsets[d.c_tot] = acc + agg;
+        }
     }
     __syncthreads();
     const uint32_t off = s_prefix + texcl;
@@ -510,14 +849,48 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(DevState d) {
     }
 }
 
+/* Exclusive scan of one replica's cell counts when its speculative sort is
+ * redone (rare: only after a recentring); one workgroup per replica. */
+__global__ __launch_bounds__(1024) void k_scan_redo(DevState d) {
+    __shared__ uint32_t s_w[16];
+    __shared__ uint32_t s_carry;
+    const int r = blockIdx.x;
+    if (!d.ctl[r].redo) return;
+    const RepDev& rep = d.reps[r];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) s_carry = (uint32_t)rep.part_base;
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < rep.ncells; c0 += 1024) {
+        const uint32_t c = c0 + tid;
+        const uint32_t g = (uint32_t)rep.cell_base + c;
+        const uint32_t v = c < rep.ncells ? d.counts[g] : 0u;
+        uint32_t inc = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(inc, o);
+            if (lane >= o) inc += u;
+        }
+        if (lane == 63) s_w[w] = inc;
+        __syncthreads();
+        uint32_t pre = 0, tot = 0;
+        for (int q = 0; q < 16; ++q) { pre += q < w ? s_w[q] : 0u; tot += s_w[q]; }
+        const uint32_t base = s_carry;
+        if (c < rep.ncells) { d.offsets[g] = base + pre + inc - v; d.counts[g] = 0u; }
+        __syncthreads();
+        if (tid == 0) s_carry = base + tot;
+        __syncthreads();
+    }
+}
+
 /* =========================================================== scatter */
 
 template <int PPT>
-__global__ __launch_bounds__(256) void k_scatter(DevState d, double h) {
+__global__ __launch_bounds__(256) void k_scatter(DevState d, double h, int mode) {
     const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
     const int r = find_rep(d.pblk_base, d.n_rep, lb);
     const RepDev& rep = d.reps[r];
-    const RepCtl c = d.ctl[r];
+    RepCtl c;
+    if (!sort_ctl(d, r, mode, c)) return;
     Box b = rep.box;
     b.delrx = c.delrx;
     const uint64_t p0 = rep.part_base + (uint64_t)(lb - d.pblk_base[r]) * (256 * PPT);
@@ -547,9 +920,13 @@ __global__ __launch_bounds__(256) void k_scatter(DevState d, double h) {
 
 /* =========================================================== collide */
 
-/* One workgroup per tile of TC consecutive cells of one replica.  Particles
- * of the tile are contiguous in the sorted arrays; up to CAP of them are
- * staged in LDS, larger tiles re-read global memory on each pass. */
+/* One workgroup per tile of TC consecutive cells of one replica.  The
+ * tile's particles are contiguous in the sorted arrays; up to CAP of them
+ * are staged in LDS (larger tiles re-read global memory on each pass).
+ * Per cell, in this order (MPC.cpp:130-137): CMV over fluid and monomers,
+ * relative kinetic energy, MBS scale sqrt(Gamma/E) (only n >= 2), then
+ * v <- CMV + s R (v - CMV).  Sums are 64-bit fixed point in LDS, so their
+ * value does not depend on the order the particles arrive in. */
 template <int TC, int CAP>
 __global__ __launch_bounds__(256) void k_collide(DevState d) {
     constexpr int NT = 256;
@@ -557,11 +934,12 @@ __global__ __launch_bounds__(256) void k_collide(DevState d) {
     __shared__ unsigned long long s_px[TC], s_py[TC], s_pz[TC], s_e[TC], s_mm[TC];
     __shared__ uint32_t s_nm[TC];
     __shared__ double s_cmv[TC][3];
-    __shared__ double s_R[TC][9];
-    __shared__ double s_scale[TC];
+    __shared__ double s_g[TC];
+    __shared__ float s_R[TC][9];
     __shared__ float s_vx[CAP], s_vy[CAP], s_vz[CAP];
-    __shared__ uint16_t s_lc[CAP];
-    __shared__ uint32_t s_mrange[2];
+    __shared__ uint8_t s_lc[CAP];
+    __shared__ int8_t s_cy[CAP];
+    __shared__ uint32_t s_mr[2];
 
     const int tid = threadIdx.x;
     const uint32_t lt = xcd_remap(blockIdx.x, gridDim.x);
@@ -571,13 +949,12 @@ __global__ __launch_bounds__(256) void k_collide(DevState d) {
     const uint32_t c_lo = (lt - d.tile_base[r]) * TC;
     const uint32_t nc = min((uint32_t)TC, rep.ncells - c_lo);
     const uint32_t gc0 = (uint32_t)rep.cell_base + c_lo;
-    const int cur = d.cur ^ 1; /* the scatter wrote here */
-    float *VX = d.vx[cur], *VY = d.vy[cur], *VZ = d.vz[cur];
-    const float* Y = d.y[cur];
-    const int Ly = rep.box.iL[1], Lx = rep.box.iL[0];
+    const int buf = d.cur ^ 1; /* the scatter wrote here */
+    float *VX = d.vx[buf], *VY = d.vy[buf], *VZ = d.vz[buf];
+    const float* Y = d.y[buf];
+    const uint32_t Ly = (uint32_t)rep.box.iL[1], Lx = (uint32_t)rep.box.iL[0];
     const bool shear_on = rep.box.shear_on != 0;
-    const double shear_Ly_unit = rep.box.shear;  /* cy*shear*Ly evaluated as (cy*shear)*Ly */
-    const double LyD = rep.box.L[1];
+    const double shear = rep.box.shear, LyD = rep.box.L[1];
 
     for (uint32_t i = tid; i <= nc; i += NT) s_off[i] = d.offsets[gc0 + i];
     for (uint32_t i = tid; i < nc; i += NT) {
@@ -587,132 +964,34 @@ __global__ __launch_bounds__(256) void k_collide(DevState d) {
     if (tid == 0) {
         /* monomers of this tile: binary search in the sorted keys */
         const uint64_t* ms = d.msorted + rep.mono_base;
-        const uint32_t nm = rep.n_mono;
-        uint32_t lo = 0, hi = nm;
+        uint32_t lo = 0, hi = rep.n_mono;
         const uint64_t klo = (uint64_t)gc0 << 32;
         while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (ms[mid] < klo) lo = mid + 1; else hi = mid; }
-        uint32_t lo2 = lo, hi2 = nm;
+        uint32_t lo2 = lo, hi2 = rep.n_mono;
         const uint64_t khi = (uint64_t)(gc0 + nc) << 32;
         while (lo2 < hi2) { const uint32_t mid = (lo2 + hi2) >> 1; if (ms[mid] < khi) lo2 = mid + 1; else hi2 = mid; }
-        s_mrange[0] = lo; s_mrange[1] = lo2;
+        s_mr[0] = lo; s_mr[1] = lo2;
     }
     __syncthreads();
     const uint32_t p0 = s_off[0], np = s_off[nc] - p0;
     const bool fast = np <= (uint32_t)CAP;
-    const uint32_t m0 = s_mrange[0], m1 = s_mrange[1];
-
-    auto cell_of = [&](uint32_t q) -> uint32_t {
-        if (fast) return s_lc[q] & 0x0FFFu;
-        uint32_t lo = 0, hi = nc;  /* last cell with s_off[c] <= p0+q */
-        const uint32_t p = p0 + q;
-        while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (s_off[mid] <= p) lo = mid; else hi = mid; }
-        return lo;
-    };
-    /* y image index of the shifted image: only cells on the y faces can
-     * hold particles that the +shift wrap moved across (valid for Ly >= 4) */
-    auto image_cy = [&](uint32_t q, uint32_t lc) -> int {
-        if (!shear_on) return 0;
-        if (fast) return (int)(s_lc[q] >> 12) - 1;
-        const uint32_t gcell = c_lo + lc;
-        const int iy = (int)((gcell / (uint32_t)Lx) % (uint32_t)Ly);
-        if (iy == 0 && Y[p0 + q] > 0.0f) return 1;
-        if (iy == Ly - 1 && Y[p0 + q] < 0.0f) return -1;
-        return 0;
-    };
-    auto load_v = [&](uint32_t q, double& vx, double& vy, double& vz) {
-        if (fast) { vx = s_vx[q]; vy = s_vy[q]; vz = s_vz[q]; }
-        else { vx = VX[p0 + q]; vy = VY[p0 + q]; vz = VZ[p0 + q]; }
-    };
-
-    if (fast) {
-        for (uint32_t ci = tid; ci < nc; ci += NT) {
-            const uint32_t a = s_off[ci] - p0, e = s_off[ci + 1] - p0;
-            int tag = 1;
-            if (shear_on) {
-                const uint32_t gcell = c_lo + ci;
-                const int iy = (int)((gcell / (uint32_t)Lx) % (uint32_t)Ly);
-                tag = iy == 0 ? 3 : (iy == Ly - 1 ? 4 : 1);
-            }
-            for (uint32_t q = a; q < e; ++q) s_lc[q] = (uint16_t)(ci | (tag << 12));
-        }
-        __syncthreads();
-        for (uint32_t q = tid; q < np; q += NT) {
-            s_vx[q] = VX[p0 + q]; s_vy[q] = VY[p0 + q]; s_vz[q] = VZ[p0 + q];
-            const uint32_t t = s_lc[q] >> 12;
-            int cy = 0;
-            if (t == 3 && Y[p0 + q] > 0.0f) cy = 1;
-            if (t == 4 && Y[p0 + q] < 0.0f) cy = -1;
-            s_lc[q] = (uint16_t)((s_lc[q] & 0x0FFFu) | ((cy + 1) << 12));
-        }
-        __syncthreads();
-    }
-
-    /* MPC::calculateCMV (MPC.cpp:314-322): sum m*v and m per cell */
-    for (uint32_t q = tid; q < np; q += NT) {
-        const uint32_t lc = cell_of(q);
-        const int cy = image_cy(q, lc);
-        double vx, vy, vz;
-        load_v(q, vx, vy, vz);
-        vx -= (double)cy * shear_Ly_unit * LyD;
-        atomicAdd(&s_px[lc], to_fix(vx));
-        atomicAdd(&s_py[lc], to_fix(vy));
-        atomicAdd(&s_pz[lc], to_fix(vz));
-    }
-    for (uint32_t j = m0 + tid; j < m1; j += NT) {
-        const uint64_t key = d.msorted[rep.mono_base + j];
-        const uint32_t lc = (uint32_t)(key >> 32) - gc0;
-        const uint32_t mi = rep.mono_base + (uint32_t)key;
-        const double mm = d.mmass[mi];
-        atomicAdd(&s_px[lc], to_fix(d.mvel[3 * mi] * mm));
-        atomicAdd(&s_py[lc], to_fix(d.mvel[3 * mi + 1] * mm));
-        atomicAdd(&s_pz[lc], to_fix(d.mvel[3 * mi + 2] * mm));
-        atomicAdd(&s_mm[lc], to_fix(mm));
-        atomicAdd(&s_nm[lc], 1u);
-    }
-    __syncthreads();
-    for (uint32_t ci = tid; ci < nc; ci += NT) {
-        const double M = (double)(s_off[ci + 1] - s_off[ci]) + from_fix(s_mm[ci]);
-        double c0 = from_fix(s_px[ci]), c1 = from_fix(s_py[ci]), c2 = from_fix(s_pz[ci]);
-        if (M > 0) { c0 /= M; c1 /= M; c2 /= M; }
-        s_cmv[ci][0] = c0; s_cmv[ci][1] = c1; s_cmv[ci][2] = c2;
-    }
-    __syncthreads();
-
-    /* MPC::thermostat relative kinetic energy (MPC.cpp:277-279) */
-    for (uint32_t q = tid; q < np; q += NT) {
-        const uint32_t lc = cell_of(q);
-        const int cy = image_cy(q, lc);
-        double vx, vy, vz;
-        load_v(q, vx, vy, vz);
-        vx -= (double)cy * shear_Ly_unit * LyD;
-        const double dx = vx - s_cmv[lc][0], dy = vy - s_cmv[lc][1], dz = vz - s_cmv[lc][2];
-        atomicAdd(&s_e[lc], to_fix((dx * dx + dy * dy) + dz * dz));
-    }
-    for (uint32_t j = m0 + tid; j < m1; j += NT) {
-        const uint64_t key = d.msorted[rep.mono_base + j];
-        const uint32_t lc = (uint32_t)(key >> 32) - gc0;
-        const uint32_t mi = rep.mono_base + (uint32_t)key;
-        const double dx = d.mvel[3 * mi] - s_cmv[lc][0];
-        const double dy = d.mvel[3 * mi + 1] - s_cmv[lc][1];
-        const double dz = d.mvel[3 * mi + 2] - s_cmv[lc][2];
-        atomicAdd(&s_e[lc], to_fix(d.mmass[mi] * ((dx * dx + dy * dy) + dz * dz)));
-    }
-    __syncthreads();
-
-    /* per cell: MBS scale (MPC.cpp:287-289) and rotation (MPC.cpp:244-257) */
+    const uint32_t m0 = s_mr[0], m1 = s_mr[1];
     const uint32_t k = ctl->k_cur;
     const bool inj = ctl->inj_active != 0;
+
+    /* per cell: the Gamma variate (depends only on n) and the rotation --
+     * issued before the particle loads so the waves overlap them */
     for (uint32_t ci = tid; ci < nc; ci += NT) {
         const uint32_t cell = c_lo + ci;
-        const uint32_t n = (s_off[ci + 1] - s_off[ci]) + s_nm[ci];
-        double scale = 1.0;
-        if (n >= 2) {
-            const double ek = from_fix(s_e[ci]) * 0.5;
-            const double g = inj ? d.inj_gamma[gc0 + ci]
-                                 : gamma_mt(rep.seed, rep.replica, cell, k, 1.5 * (n - 1), rep.kT);
-            scale = sqrt(g / ek);
-        }
-        s_scale[ci] = n >= 2 ? scale : -1.0;
+        uint32_t nm = 0;
+        for (uint32_t j = m0; j < m1; ++j)
+            nm += (uint32_t)(d.msorted[rep.mono_base + j] >> 32) == gc0 + ci;
+        s_nm[ci] = nm;
+        const uint32_t n = (s_off[ci + 1] - s_off[ci]) + nm;
+        double g = 0.0;
+        if (n >= 2)
+            g = inj ? d.inj_gamma[gc0 + ci] : gamma_mt(rep.seed, rep.replica, cell, k, 1.5 * (n - 1), rep.kT);
+        s_g[ci] = g;
         double u1, u2;
         if (inj) {
             u1 = d.inj_axis[2 * (size_t)(gc0 + ci)];
@@ -729,16 +1008,99 @@ __global__ __launch_bounds__(256) void k_collide(DevState d) {
         const double a1 = sqrt(1 - th * th) * sin(phi);
         const double a2 = th;
         const double cc = rep.c, ss = rep.s;
-        double* R = s_R[ci];
-        R[0] = a0 * a0 + (1 - a0 * a0) * cc;
-        R[1] = a0 * a1 * (1 - cc) - a2 * ss;
-        R[2] = a0 * a2 * (1 - cc) + a1 * ss;
-        R[3] = a0 * a1 * (1 - cc) + a2 * ss;
-        R[4] = a1 * a1 + (1 - a1 * a1) * cc;
-        R[5] = a1 * a2 * (1 - cc) - a0 * ss;
-        R[6] = a0 * a2 * (1 - cc) - a1 * ss;
-        R[7] = a1 * a2 * (1 - cc) + a0 * ss;
-        R[8] = a2 * a2 + (1 - a2 * a2) * cc;
+        float* R = s_R[ci];
+        R[0] = (float)(a0 * a0 + (1 - a0 * a0) * cc);
+        R[1] = (float)(a0 * a1 * (1 - cc) - a2 * ss);
+        R[2] = (float)(a0 * a2 * (1 - cc) + a1 * ss);
+        R[3] = (float)(a0 * a1 * (1 - cc) + a2 * ss);
+        R[4] = (float)(a1 * a1 + (1 - a1 * a1) * cc);
+        R[5] = (float)(a1 * a2 * (1 - cc) - a0 * ss);
+        R[6] = (float)(a0 * a2 * (1 - cc) - a1 * ss);
+        R[7] = (float)(a1 * a2 * (1 - cc) + a0 * ss);
+        R[8] = (float)(a2 * a2 + (1 - a2 * a2) * cc);
+    }
+
+    /* last cell whose first particle is <= p0+q */
+    auto cell_of = [&](uint32_t q) -> uint32_t {
+        const uint32_t p = p0 + q;
+        uint32_t lo = 0, hi = nc;
+        while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (s_off[mid] <= p) lo = mid; else hi = mid; }
+        return lo;
+    };
+    /* y image index of the shifted image: only cells of the bottom/top y
+     * layer can hold particles the +shift wrap moved across the face, and
+     * their unshifted y sits on the opposite side (valid for Ly >= 4) */
+    auto image_cy = [&](uint32_t q, uint32_t lc) -> int {
+        if (!shear_on) return 0;
+        const uint32_t iy = ((c_lo + lc) / Lx) % Ly;
+        if (iy == 0 && Y[p0 + q] > 0.0f) return 1;
+        if (iy == Ly - 1 && Y[p0 + q] < 0.0f) return -1;
+        return 0;
+    };
+
+    /* MPC::calculateCMV (MPC.cpp:314-322) */
+    for (uint32_t q = tid; q < np; q += NT) {
+        const uint32_t lc = cell_of(q);
+        const int cy = image_cy(q, lc);
+        const float fx = VX[p0 + q], fy = VY[p0 + q], fz = VZ[p0 + q];
+        if (fast) { s_vx[q] = fx; s_vy[q] = fy; s_vz[q] = fz; s_lc[q] = (uint8_t)lc; s_cy[q] = (int8_t)cy; }
+        const double vx = (double)fx - (double)cy * shear * LyD;
+        atomicAdd(&s_px[lc], to_fix(vx));
+        atomicAdd(&s_py[lc], to_fix((double)fy));
+        atomicAdd(&s_pz[lc], to_fix((double)fz));
+    }
+    for (uint32_t j = m0 + tid; j < m1; j += NT) {
+        const uint64_t key = d.msorted[rep.mono_base + j];
+        const uint32_t lc = (uint32_t)(key >> 32) - gc0;
+        const uint32_t mi = rep.mono_base + (uint32_t)key;
+        const double mm = d.mmass[mi];
+        atomicAdd(&s_px[lc], to_fix(d.mvel[3 * mi] * mm));
+        atomicAdd(&s_py[lc], to_fix(d.mvel[3 * mi + 1] * mm));
+        atomicAdd(&s_pz[lc], to_fix(d.mvel[3 * mi + 2] * mm));
+        atomicAdd(&s_mm[lc], to_fix(mm));
+    }
+    __syncthreads();
+    for (uint32_t ci = tid; ci < nc; ci += NT) {
+        const double M = (double)(s_off[ci + 1] - s_off[ci]) + from_fix(s_mm[ci]);
+        double c0 = from_fix(s_px[ci]), c1 = from_fix(s_py[ci]), c2 = from_fix(s_pz[ci]);
+        if (M > 0) { c0 /= M; c1 /= M; c2 /= M; }
+        s_cmv[ci][0] = c0; s_cmv[ci][1] = c1; s_cmv[ci][2] = c2;
+    }
+    __syncthreads();
+
+    auto load = [&](uint32_t q, uint32_t& lc, int& cy, double& vx, double& vy, double& vz) {
+        if (fast) {
+            lc = s_lc[q]; cy = s_cy[q];
+            vx = s_vx[q]; vy = s_vy[q]; vz = s_vz[q];
+        } else {
+            lc = cell_of(q); cy = image_cy(q, lc);
+            vx = VX[p0 + q]; vy = VY[p0 + q]; vz = VZ[p0 + q];
+        }
+    };
+
+    /* MPC::thermostat relative kinetic energy (MPC.cpp:277-279) */
+    for (uint32_t q = tid; q < np; q += NT) {
+        uint32_t lc; int cy; double vx, vy, vz;
+        load(q, lc, cy, vx, vy, vz);
+        vx -= (double)cy * shear * LyD;
+        const double dx = vx - s_cmv[lc][0], dy = vy - s_cmv[lc][1], dz = vz - s_cmv[lc][2];
+        atomicAdd(&s_e[lc], to_fix((dx * dx + dy * dy) + dz * dz));
+    }
+    for (uint32_t j = m0 + tid; j < m1; j += NT) {
+        const uint64_t key = d.msorted[rep.mono_base + j];
+        const uint32_t lc = (uint32_t)(key >> 32) - gc0;
+        const uint32_t mi = rep.mono_base + (uint32_t)key;
+        const double dx = d.mvel[3 * mi] - s_cmv[lc][0];
+        const double dy = d.mvel[3 * mi + 1] - s_cmv[lc][1];
+        const double dz = d.mvel[3 * mi + 2] - s_cmv[lc][2];
+        atomicAdd(&s_e[lc], to_fix(d.mmass[mi] * ((dx * dx + dy * dy) + dz * dz)));
+    }
+    __syncthreads();
+
+    /* MBS scale sqrt(Gamma / E_rel) for n >= 2 (MPC.cpp:287-289) */
+    for (uint32_t ci = tid; ci < nc; ci += NT) {
+        const uint32_t n = (s_off[ci + 1] - s_off[ci]) + s_nm[ci];
+        s_g[ci] = n >= 2 ? sqrt(s_g[ci] / (from_fix(s_e[ci]) * 0.5)) : -1.0;
     }
     __syncthreads();
 
@@ -746,24 +1108,22 @@ __global__ __launch_bounds__(256) void k_collide(DevState d) {
      * 266-267); the image offset is undone as the unshift wrap does */
     auto apply = [&](uint32_t lc, double& vx, double& vy, double& vz) {
         const double c0 = s_cmv[lc][0], c1 = s_cmv[lc][1], c2 = s_cmv[lc][2];
-        const double sc = s_scale[lc];
+        const double sc = s_g[lc];
         if (sc >= 0.0) {
             vx = sc * vx + (1 - sc) * c0;
             vy = sc * vy + (1 - sc) * c1;
             vz = sc * vz + (1 - sc) * c2;
         }
-        const double* R = s_R[lc];
+        const float* R = s_R[lc];
         const double d0 = vx - c0, d1 = vy - c1, d2 = vz - c2;
-        vx = c0 + ((R[0] * d0 + R[1] * d1) + R[2] * d2);
-        vy = c1 + ((R[3] * d0 + R[4] * d1) + R[5] * d2);
-        vz = c2 + ((R[6] * d0 + R[7] * d1) + R[8] * d2);
+        vx = c0 + (((double)R[0] * d0 + (double)R[1] * d1) + (double)R[2] * d2);
+        vy = c1 + (((double)R[3] * d0 + (double)R[4] * d1) + (double)R[5] * d2);
+        vz = c2 + (((double)R[6] * d0 + (double)R[7] * d1) + (double)R[8] * d2);
     };
     for (uint32_t q = tid; q < np; q += NT) {
-        const uint32_t lc = cell_of(q);
-        const int cy = image_cy(q, lc);
-        double vx, vy, vz;
-        load_v(q, vx, vy, vz);
-        const double off = (double)cy * shear_Ly_unit * LyD;
+        uint32_t lc; int cy; double vx, vy, vz;
+        load(q, lc, cy, vx, vy, vz);
+        const double off = (double)cy * shear * LyD;
         vx -= off;
         apply(lc, vx, vy, vz);
         vx += off;
@@ -888,20 +1248,37 @@ __global__ __launch_bounds__(256) void k_temperature(DevState d, int replica) {
 
 /* ========================================================= launchers */
 
-hipError_t launch_md_prologue(const DevState& d, hipStream_t st, int n_md, int epot_last, int md_init,
-                              int do_collision, int n_delrx, double delrx_dt, double md_dt) {
-    hipLaunchKernelGGL(k_md_prologue, dim3(d.n_rep), dim3(kMdThreads), 0, st, d, n_md, epot_last, md_init,
-                       do_collision, n_delrx, delrx_dt, md_dt);
+hipError_t launch_prologue(const DevState& d, hipStream_t st, int do_collision, int n_delrx, double delrx_dt) {
+    hipLaunchKernelGGL(k_prologue, dim3(d.n_rep), dim3(64), 0, st, d, do_collision, n_delrx, delrx_dt);
     return hipGetLastError();
 }
 
-hipError_t launch_bin(const DevState& d, hipStream_t st, double h) {
+hipError_t launch_md(const DevState& d, hipStream_t st, int n_md, int epot_last, int md_init, int do_collision,
+                     double md_dt) {
+    if (d.m_tot == 0) return hipSuccess;
+    const int ns = d.md_ns;
+    const uint32_t cap = d.md_vl_cap;
+    const size_t lds = md_lds_bytes(ns, cap);
+    static size_t attr_set = 0;
+    if (lds > attr_set) {
+        const hipError_t e = hipFuncSetAttribute((const void*)k_md, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)lds);
+        if (e != hipSuccess) return e;
+        attr_set = lds;
+    }
+    hipLaunchKernelGGL(k_md, dim3(d.n_rep), dim3(kMdThreads), lds, st, d, n_md, epot_last, md_init, do_collision,
+                       md_dt, ns, cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin(const DevState& d, hipStream_t st, double h, int mode) {
+    if (d.n_pblk == 0) return hipSuccess;
     const dim3 g(d.n_pblk), b(256);
     switch (d.ppt) {
-    case 1: hipLaunchKernelGGL(k_bin<1>, g, b, 0, st, d, h); break;
-    case 2: hipLaunchKernelGGL(k_bin<2>, g, b, 0, st, d, h); break;
-    case 4: hipLaunchKernelGGL(k_bin<4>, g, b, 0, st, d, h); break;
-    case 8: hipLaunchKernelGGL(k_bin<8>, g, b, 0, st, d, h); break;
+    case 1: hipLaunchKernelGGL(k_bin<1>, g, b, 0, st, d, h, mode); break;
+    case 2: hipLaunchKernelGGL(k_bin<2>, g, b, 0, st, d, h, mode); break;
+    case 4: hipLaunchKernelGGL(k_bin<4>, g, b, 0, st, d, h, mode); break;
+    case 8: hipLaunchKernelGGL(k_bin<8>, g, b, 0, st, d, h, mode); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -912,13 +1289,19 @@ hipError_t launch_scan(const DevState& d, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_scatter(const DevState& d, hipStream_t st, double h) {
+hipError_t launch_scan_redo(const DevState& d, hipStream_t st) {
+    hipLaunchKernelGGL(k_scan_redo, dim3(d.n_rep), dim3(1024), 0, st, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(const DevState& d, hipStream_t st, double h, int mode) {
+    if (d.n_pblk == 0) return hipSuccess;
     const dim3 g(d.n_pblk), b(256);
     switch (d.ppt) {
-    case 1: hipLaunchKernelGGL(k_scatter<1>, g, b, 0, st, d, h); break;
-    case 2: hipLaunchKernelGGL(k_scatter<2>, g, b, 0, st, d, h); break;
-    case 4: hipLaunchKernelGGL(k_scatter<4>, g, b, 0, st, d, h); break;
-    case 8: hipLaunchKernelGGL(k_scatter<8>, g, b, 0, st, d, h); break;
+    case 1: hipLaunchKernelGGL(k_scatter<1>, g, b, 0, st, d, h, mode); break;
+    case 2: hipLaunchKernelGGL(k_scatter<2>, g, b, 0, st, d, h, mode); break;
+    case 4: hipLaunchKernelGGL(k_scatter<4>, g, b, 0, st, d, h, mode); break;
+    case 8: hipLaunchKernelGGL(k_scatter<8>, g, b, 0, st, d, h, mode); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -927,15 +1310,16 @@ hipError_t launch_scatter(const DevState& d, hipStream_t st, double h) {
 hipError_t launch_collide(const DevState& d, hipStream_t st) {
     const dim3 g(d.n_tiles), b(256);
     switch (d.tile_cells) {
-    case 64: hipLaunchKernelGGL((k_collide<64, 1024>), g, b, 0, st, d); break;
-    case 128: hipLaunchKernelGGL((k_collide<128, 2048>), g, b, 0, st, d); break;
-    case 256: hipLaunchKernelGGL((k_collide<256, 3584>), g, b, 0, st, d); break;
+    case 64: hipLaunchKernelGGL((k_collide<64, 896>), g, b, 0, st, d); break;
+    case 128: hipLaunchKernelGGL((k_collide<128, 1536>), g, b, 0, st, d); break;
+    case 256: hipLaunchKernelGGL((k_collide<256, 3072>), g, b, 0, st, d); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_fluid_init(const DevState& d, hipStream_t st) {
+    if (d.n_pblk == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fluid_init, dim3(d.n_pblk), dim3(256), 0, st, d, 0);
     hipLaunchKernelGGL(k_fluid_mean, dim3(d.n_rep), dim3(64), 0, st, d);
     hipLaunchKernelGGL(k_fluid_init, dim3(d.n_pblk), dim3(256), 0, st, d, 1);

@@ -139,6 +139,38 @@ SPMD_HD double rm24_of(double r2) {
     rm24 *= rm24;
     return rm24;
 }
+/* 1/x for x in a normal range: hardware reciprocal plus two Newton steps
+ * (relative error at the level of one ulp). */
+SPMD_HD double rcp_nr(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+#else
+    double r = 1.0 / x;
+#endif
+    r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    return r;
+}
+
+/* The MD kernels' forms of the force magnitudes: one reciprocal per pair,
+ * no divergent division.  Same functions as pair_force / fene_force up to
+ * rounding (the 48-24 core and B-B tail share 192 r^-2 r^-24 (r^-24 - 1/2),
+ * the tail scaled by lambda). */
+SPMD_HD double pair_force_fast(double r2, bool both_b, double lambda) {
+    if (!(r2 < 2.25)) return 0.0;
+    const double rm2 = rcp_nr(r2);
+    double rm24 = rm2 * rm2 * rm2;
+    rm24 *= rm24;
+    rm24 *= rm24;
+    const double f = 192.0 * rm2 * rm24 * (rm24 - 0.5);
+    const double w = r2 < 1.0594631 ? 1.0 : (both_b ? lambda : 0.0);
+    return f * w;
+}
+SPMD_HD double fene_force_fast(double r2, double R02) {
+    /* -30 / (1 - r2/R0^2) = -30 R0^2 / (R0^2 - r2); r2 == 0 gives 0 like the reference */
+    return r2 == 0.0 ? 0.0 : -30.0 * R02 * rcp_nr(R02 - r2);
+}
+
 /* non-bonded pair force magnitude; both_b = both beads B-type */
 SPMD_HD double pair_force(double r2, bool both_b, double lambda) {
     if (r2 < 1.0594631) {

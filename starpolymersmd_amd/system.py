@@ -81,9 +81,11 @@ class MPCSystem:
         self._chk(self._lib.spmd_set_tracking(self.h, int(on)))
 
     # ------------------------------------------------------------ star
-    def add_star(self, A, B, arms, mass, bond=1.01, anchor_bond=2.0, r: int = 0):
-        """Box::add_star (Box.cpp:39-46)."""
-        self._chk(self._lib.spmd_add_star(self.h, r, A, B, arms, mass, bond, anchor_bond))
+    def add_star(self, A, B, arms, mass, bond=1.01, anchor_bond=2.0, r: int = 0, coil: bool = False):
+        """Box::add_star (Box.cpp:39-46); coil=True places the arms as seeded
+        random coils instead of straight rays."""
+        fn = self._lib.spmd_add_star_coil if coil else self._lib.spmd_add_star
+        self._chk(fn(self.h, r, A, B, arms, mass, bond, anchor_bond))
 
     def num_monomers(self, r: int = 0) -> int:
         n = C.c_uint32()
@@ -165,6 +167,12 @@ class MPCSystem:
         v = C.c_double()
         self._chk(self._lib.spmd_epot(self.h, r, C.byref(v)))
         return v.value
+
+    def md_stats(self, r: int = 0):
+        """(Verlet rebuilds so far, entries of the current half lists)."""
+        a, b = C.c_uint32(), C.c_uint32()
+        self._chk(self._lib.spmd_md_stats(self.h, r, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     # ------------------------------------------------------------ tuning/profiling
     def set_tuning(self, ppt: int, tile_cells: int):
