@@ -157,17 +157,16 @@ int build_monomers(spmd_sys* s) {
     d.md_ns = ns;
     const size_t budget = 160u * 1024u - 4096u;  /* leave room for the kernel's static LDS */
     const size_t fixed = md_lds_bytes(ns, 0);
-    d.md_vl_cap = fixed < budget ? (uint32_t)((budget - fixed) / 2) : 0u;
+    d.md_vl_cap = fixed < budget ? (uint32_t)((budget - fixed) / 4) : 0u;
     dfree(d.mpos); dfree(d.mvel); dfree(d.mforce); dfree(d.mvpos); dfree(d.mmass); dfree(d.mbtype);
-    dfree(d.vl); dfree(d.vl_n); dfree(d.adj_off); dfree(d.adj); dfree(d.msorted);
+    dfree(d.vl); dfree(d.adj_off); dfree(d.adj); dfree(d.msorted);
     if (int rc = dalloc(s, &d.mpos, 3ull * mt)) return rc;
     if (int rc = dalloc(s, &d.mvel, 3ull * mt)) return rc;
     if (int rc = dalloc(s, &d.mforce, 3ull * mt)) return rc;
     if (int rc = dalloc(s, &d.mvpos, 3ull * mt)) return rc;
     if (int rc = dalloc(s, &d.mmass, mt)) return rc;
     if (int rc = dalloc(s, &d.mbtype, mt)) return rc;
-    if (int rc = dalloc(s, &d.vl, (size_t)mt * kVcap)) return rc;
-    if (int rc = dalloc(s, &d.vl_n, mt)) return rc;
+    if (int rc = dalloc(s, &d.vl, (size_t)mt * kVlPerMono)) return rc;
     if (int rc = dalloc(s, &d.adj_off, adj_off.size())) return rc;
     if (int rc = dalloc(s, &d.adj, adj.size())) return rc;
     if (int rc = dalloc(s, &d.msorted, mt)) return rc;
@@ -178,7 +177,10 @@ int build_monomers(spmd_sys* s) {
         HIPCHK(s, hipMemcpy(d.mvpos, pos.data(), pos.size() * 8, hipMemcpyHostToDevice));
         HIPCHK(s, hipMemcpy(d.mmass, mass.data(), mass.size() * 8, hipMemcpyHostToDevice));
         HIPCHK(s, hipMemcpy(d.mbtype, bt.data(), bt.size(), hipMemcpyHostToDevice));
-        HIPCHK(s, hipMemset(d.vl_n, 0, (size_t)mt * 2));
+    }
+    for (int r = 0; r < d.n_rep; ++r) {  /* no Verlet list until the next rebuild */
+        const uint32_t zero = 0u;
+        HIPCHK(s, hipMemcpy(&d.ctl[r].vl_count, &zero, 4, hipMemcpyHostToDevice));
     }
     if (!adj.empty()) HIPCHK(s, hipMemcpy(d.adj, adj.data(), adj.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(s, hipMemcpy(d.adj_off, adj_off.data(), adj_off.size() * 4, hipMemcpyHostToDevice));
@@ -600,7 +602,7 @@ int spmd_destroy(spmd_sys* s) {
     }
     dfree(d.rank); dfree(d.counts); dfree(d.offsets); dfree(d.scan_flags); dfree(d.scan_ticket);
     dfree(d.mpos); dfree(d.mvel); dfree(d.mforce); dfree(d.mvpos); dfree(d.mmass); dfree(d.mbtype);
-    dfree(d.vl); dfree(d.vl_n); dfree(d.adj_off); dfree(d.adj); dfree(d.msorted);
+    dfree(d.vl); dfree(d.adj_off); dfree(d.adj); dfree(d.msorted);
     dfree(d.inj_axis); dfree(d.inj_gamma); dfree(d.reps); dfree(d.ctl); dfree(d.pblk_base);
     dfree(d.tile_base); dfree(d.red);
     if (s->side) (void)hipStreamSynchronize(s->side);
@@ -975,14 +977,7 @@ int spmd_md_stats(spmd_sys* s, uint32_t r, uint32_t* n_rebuild, uint32_t* vl_ent
     RepCtl c;
     HIPCHK(s, hipMemcpy(&c, s->d.ctl + r, sizeof c, hipMemcpyDeviceToHost));
     if (n_rebuild) *n_rebuild = c.n_rebuild;
-    if (vl_entries) {
-        const uint32_t n = s->stars[r].n;
-        std::vector<uint16_t> cnt(n);
-        if (n) HIPCHK(s, hipMemcpy(cnt.data(), s->d.vl_n + s->reps[r].mono_base, 2ull * n, hipMemcpyDeviceToHost));
-        uint32_t t = 0;
-        for (uint16_t c2 : cnt) t += c2;
-        *vl_entries = t;
-    }
+    if (vl_entries) *vl_entries = c.vl_count;
     return SPMD_OK;
 }
 

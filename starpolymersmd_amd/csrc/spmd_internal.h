@@ -26,7 +26,7 @@
 
 namespace spmd {
 
-constexpr int kVcap = 128;         /* Verlet-list capacity per monomer */
+constexpr int kVlPerMono = 64;     /* global Verlet half-list entries reserved per monomer */
 constexpr int kMaxMono = 2048;     /* monomers per replica (one MD workgroup) */
 constexpr int kMdThreads = 512;
 constexpr int kScanItems = 16;     /* counts per thread in the scan */
@@ -72,6 +72,8 @@ struct RepCtl {
     uint32_t status;
     uint32_t n_rebuild;
     int32_t redo;                /* the speculative sort (no recentring) must be redone */
+    uint32_t vl_count;           /* entries of the star's Verlet half list */
+    uint32_t pad2;
 };
 
 struct DevState {
@@ -90,8 +92,7 @@ struct DevState {
     uint32_t n_scan_tiles;
     double *mpos, *mvel, *mforce, *mvpos, *mmass;
     uint8_t* mbtype;
-    uint16_t* vl;
-    uint16_t* vl_n;
+    uint32_t* vl;          /* Verlet half lists: i | j << 16, kVlPerMono per monomer */
     uint32_t *adj_off, *adj;
     uint64_t* msorted;
     double* inj_axis;      /* 2 per cell, all replicas */

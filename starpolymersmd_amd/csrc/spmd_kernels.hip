@@ -133,8 +133,6 @@ __device__ double block_sum(double v, double* scratch) {
 
 /* ======================================================= MD + prologue */
 
-/* Dynamic LDS of k_md_prologue for a stride of ns monomers (host computes
- * the same size with md_lds_bytes) followed by cap Verlet entries. */
 constexpr int kGridMax = 4096;   /* cells of the Verlet rebuild grid kept in LDS */
 /* Verlet list radii: the reference keeps every pair within 2.0 (Box.cpp:9-10)
  * and rebuilds once a monomer moved 0.25 = (2.0 - 1.5)/2.  Only B-B pairs
@@ -143,46 +141,43 @@ constexpr int kGridMax = 4096;   /* cells of the Verlet rebuild grid kept in LDS
  * -- and so every force -- is unchanged. */
 constexpr double kListBB2 = 4.0;
 constexpr double kListAA2 = 2.34;   /* (sqrt(1.0594631) + 0.5)^2 = 2.3388 */
+constexpr int kMdBatch = 4;         /* independent pairs per lane in flight */
 
+/* Dynamic LDS of k_md for a stride of ns monomers (host computes the same
+ * size with md_lds_bytes) followed by cap pair-list entries. */
 struct MdLds {
-    double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *px, *py, *pz, *im, *red, *shift;
-    uint32_t* vl_off;
+    double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *px, *py, *pz, *im, *red;
     uint32_t* gcell;   /* grid cell starts (kGridMax + 1) */
     uint32_t* gcid;    /* grid cell of each monomer (ns) */
-    uint32_t* aoff;    /* bond adjacency offsets (ns + 1) */
-    uint32_t* adj;     /* bond adjacency (2 ns) */
-    uint16_t* gmem;    /* monomers sorted by grid cell (ns) */
-    uint16_t* vl;
-    uint8_t* bt;
+    uint32_t* bl;      /* bonds: i | j << 12 | anchor << 24 (ns) */
+    uint32_t* pl;      /* half pair list: i | j << 16 (cap) */
+    uint16_t* gmem;    /* monomers ordered by grid cell (ns) */
+    uint8_t* bt;       /* monomer is B-type (ns) */
     int* flags;
 };
 
-__device__ __forceinline__ MdLds md_carve(unsigned char* base, int ns) {
+__device__ __forceinline__ MdLds md_carve(unsigned char* base, int ns, uint32_t cap) {
     MdLds L;
     double* p = reinterpret_cast<double*>(base);
     L.x = p; L.y = p + ns; L.z = p + 2 * ns;
     L.vx = p + 3 * ns; L.vy = p + 4 * ns; L.vz = p + 5 * ns;
     L.fx = p + 6 * ns; L.fy = p + 7 * ns; L.fz = p + 8 * ns;
     L.px = p + 9 * ns; L.py = p + 10 * ns; L.pz = p + 11 * ns;   /* Verlet reference positions */
-    L.im = p + 12 * ns;                                           /* masses */
-    L.red = p + 13 * ns;         /* 32 doubles */
-    L.shift = L.red + 32;        /* 4 doubles */
-    L.flags = reinterpret_cast<int*>(L.shift + 4);   /* 8 ints */
-    L.vl_off = reinterpret_cast<uint32_t*>(L.flags + 8);
-    L.gcell = L.vl_off + ns + 1;
+    L.im = p + 12 * ns;                                           /* dt/(2m) */
+    L.red = p + 13 * ns;                                          /* 80 doubles */
+    L.flags = reinterpret_cast<int*>(L.red + 80);                 /* 16 ints */
+    L.gcell = reinterpret_cast<uint32_t*>(L.flags + 16);
     L.gcid = L.gcell + kGridMax + 1;
-    L.aoff = L.gcid + ns;
-    L.adj = L.aoff + ns + 1;
-    L.gmem = reinterpret_cast<uint16_t*>(L.adj + 2 * ns);
+    L.bl = L.gcid + ns;
+    L.pl = L.bl + ns;
+    L.gmem = reinterpret_cast<uint16_t*>(L.pl + cap);
     L.bt = reinterpret_cast<uint8_t*>(L.gmem + ns);
-    L.vl = reinterpret_cast<uint16_t*>(L.bt + ((ns + 7) & ~7));
     return L;
 }
 
 size_t md_lds_bytes(int ns, uint32_t cap) {
-    return (size_t)13 * ns * 8 + 36 * 8 + 8 * 4 + (size_t)(ns + 1) * 4 + (size_t)(kGridMax + 1) * 4 +
-           (size_t)ns * 4 + (size_t)(ns + 1) * 4 + (size_t)2 * ns * 4 + (size_t)ns * 2 + ((ns + 7) & ~7) +
-           (size_t)cap * 2 + 16;
+    return (size_t)13 * ns * 8 + 80 * 8 + 16 * 4 + (size_t)(kGridMax + 1) * 4 + (size_t)ns * 4 + (size_t)ns * 4 +
+           (size_t)cap * 4 + (size_t)ns * 2 + (size_t)ns + 64;
 }
 
 template <int NT>
@@ -198,71 +193,8 @@ __device__ double block_sum_all(double v, double* scratch) {
     return t;
 }
 
-/* One workgroup per replica; all per-monomer state in LDS.  The pair loops
- * give each monomer a group of kMdGroup lanes striding over its Verlet list,
- * combined by a fixed shuffle tree (deterministic). */
-constexpr int kMdGroup = 1;
-
-/* Forces of monomer i's half list (every pair once, j > i), added to both
- * monomers' 64-bit fixed-point LDS accumulators (order independent, so the
- * list order does not change the result); bonds from both sides in bond
- * order, in registers. */
-__device__ __forceinline__ void md_pair_force(const MdLds& S, const RepDev& rep, int i, const uint16_t* lst,
-                                              uint32_t nvl, int gl, double& fx, double& fy, double& fz) {
-#pragma clang fp contract(fast)
-    unsigned long long* qx = reinterpret_cast<unsigned long long*>(S.fx);
-    unsigned long long* qy = reinterpret_cast<unsigned long long*>(S.fy);
-    unsigned long long* qz = reinterpret_cast<unsigned long long*>(S.fz);
-    const double xi = S.x[i], yi = S.y[i], zi = S.z[i];
-    const bool bi = S.bt[i] != 0;
-    for (uint32_t q = gl; q < nvl; q += kMdGroup) {
-        const int j = lst[q];
-        const double dx = S.x[j] - xi, dy = S.y[j] - yi, dz = S.z[j] - zi;
-        const double r2 = (dx * dx + dy * dy) + dz * dz;
-        const double fa = pair_force_fast(r2, bi && S.bt[j] != 0, rep.lambda);
-        if (fa != 0.0) {
-            const unsigned long long gx = to_fix(dx * fa), gy = to_fix(dy * fa), gz = to_fix(dz * fa);
-            atomicAdd(&qx[j], gx); atomicAdd(&qy[j], gy); atomicAdd(&qz[j], gz);
-            atomicAdd(&qx[i], 0ull - gx); atomicAdd(&qy[i], 0ull - gy); atomicAdd(&qz[i], 0ull - gz);
-        }
-    }
-    const uint32_t a0 = S.aoff[i], a1 = S.aoff[i + 1];
-    for (uint32_t q = a0 + gl; q < a1; q += kMdGroup) {
-        const uint32_t e = S.adj[q];
-        const int j = (int)(e & ~kBondAnchor);
-        const double dx = S.x[j] - xi, dy = S.y[j] - yi, dz = S.z[j] - zi;
-        const double r2 = (dx * dx + dy * dy) + dz * dz;
-        const double fa = fene_force_fast(r2, (e & kBondAnchor) ? 20.25 : 2.25);
-        fx -= dx * fa; fy -= dy * fa; fz -= dz * fa;
-    }
-}
-
-/* Molecule::Epot contributions of monomer i (Box.cpp:257-277): its half-list
- * pairs (each pair once) and half of every bond from each side. */
-__device__ __forceinline__ double md_pair_epot(const MdLds& S, const RepDev& rep, int i, const uint16_t* lst,
-                                            uint32_t nvl, int gl) {
-    const double xi = S.x[i], yi = S.y[i], zi = S.z[i];
-    const bool bi = S.bt[i] != 0;
-    unsigned long long epq = 0ull;  /* fixed point: independent of list order */
-    for (uint32_t q = gl; q < nvl; q += kMdGroup) {
-        const int j = lst[q];
-        const double dx = S.x[j] - xi, dy = S.y[j] - yi, dz = S.z[j] - zi;
-        epq += to_fix(pair_potential((dx * dx + dy * dy) + dz * dz, bi && S.bt[j] != 0, rep.lambda));
-    }
-    double ep = from_fix(epq);
-    const uint32_t a0 = S.aoff[i], a1 = S.aoff[i + 1];
-    for (uint32_t q = a0 + gl; q < a1; q += kMdGroup) {
-        const uint32_t e = S.adj[q];
-        const int j = (int)(e & ~kBondAnchor);
-        const double dx = S.x[j] - xi, dy = S.y[j] - yi, dz = S.z[j] - zi;
-        const double r2 = (dx * dx + dy * dy) + dz * dz;
-        ep += 0.5 * ((e & kBondAnchor) ? fene_anchor_potential(r2) : fene_potential(r2));
-    }
-    return ep;
-}
-
-/* Exclusive scan of a[0..n) in place (n <= 4*kMdThreads), total in a[n]
- * and returned; each thread scans a contiguous chunk, deterministic. */
+/* Exclusive scan of a[0..n) in place, total in a[n] and returned; each
+ * thread scans a contiguous chunk, deterministic. */
 __device__ uint32_t md_block_scan(uint32_t* a, int n, double* scratch_d) {
     uint32_t* wsum = reinterpret_cast<uint32_t*>(scratch_d);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -294,26 +226,195 @@ __device__ uint32_t md_block_scan(uint32_t* a, int n, double* scratch_d) {
     return tot;
 }
 
-template <bool MAX>
-__device__ double block_extreme(double v, double* scratch) {
+/* Forces of kMdBatch independent list entries per lane: half-list pairs
+ * (Newton's third law) or bonds, added to both monomers' 64-bit fixed-point
+ * LDS accumulators (the sum does not depend on the order of the entries). */
+template <bool BONDS>
+__device__ __forceinline__ void md_forces_flat(const MdLds& S, const uint32_t* list, uint32_t n, double lambda) {
+#pragma clang fp contract(fast)
+    constexpr int NT = kMdThreads;
+    unsigned long long* qx = reinterpret_cast<unsigned long long*>(S.fx);
+    unsigned long long* qy = reinterpret_cast<unsigned long long*>(S.fy);
+    unsigned long long* qz = reinterpret_cast<unsigned long long*>(S.fz);
+    for (uint32_t q0 = threadIdx.x; q0 < n; q0 += NT * kMdBatch) {
+        int ii[kMdBatch], jj[kMdBatch];
+        unsigned long long gx[kMdBatch], gy[kMdBatch], gz[kMdBatch];
+        bool ok[kMdBatch];
+#pragma unroll
+        for (int k = 0; k < kMdBatch; ++k) {
+            const uint32_t q = q0 + k * NT;
+            ok[k] = q < n;
+            const uint32_t e = ok[k] ? list[q] : 0u;
+            int i, j;
+            bool anchor = false;
+            if (BONDS) { i = e & 0xFFF; j = (e >> 12) & 0xFFF; anchor = (e >> 24) & 1; }
+            else { i = e & 0xFFFF; j = e >> 16; }
+            ii[k] = i; jj[k] = j;
+            const double dx = S.x[j] - S.x[i], dy = S.y[j] - S.y[i], dz = S.z[j] - S.z[i];
+            const double r2 = (dx * dx + dy * dy) + dz * dz;
+            double fa;
+            if (BONDS) fa = fene_force_fast(r2, anchor ? 20.25 : 2.25);
+            else fa = pair_force_fast(r2, S.bt[i] && S.bt[j], lambda);
+            if (!ok[k]) fa = 0.0;
+            ok[k] = ok[k] && fa != 0.0;
+            gx[k] = to_fix(dx * fa); gy[k] = to_fix(dy * fa); gz[k] = to_fix(dz * fa);
+        }
+#pragma unroll
+        for (int k = 0; k < kMdBatch; ++k) {
+            if (!ok[k]) continue;
+            atomicAdd(&qx[jj[k]], gx[k]); atomicAdd(&qy[jj[k]], gy[k]); atomicAdd(&qz[jj[k]], gz[k]);
+            atomicAdd(&qx[ii[k]], 0ull - gx[k]); atomicAdd(&qy[ii[k]], 0ull - gy[k]); atomicAdd(&qz[ii[k]], 0ull - gz[k]);
+        }
+    }
+}
+
+/* Molecule::Epot (Box.cpp:257-277): every Verlet pair once (the reference
+ * adds 0.5 U from each side) plus every bond; per-thread fixed point, then
+ * a block sum. */
+__device__ double md_epot_flat(const MdLds& S, const uint32_t* pl, uint32_t np, uint32_t nb, double lambda) {
+    unsigned long long acc = 0ull;
+    for (uint32_t q = threadIdx.x; q < np; q += kMdThreads) {
+        const uint32_t e = pl[q];
+        const int i = e & 0xFFFF, j = e >> 16;
+        const double dx = S.x[j] - S.x[i], dy = S.y[j] - S.y[i], dz = S.z[j] - S.z[i];
+        acc += to_fix(pair_potential((dx * dx + dy * dy) + dz * dz, S.bt[i] && S.bt[j], lambda));
+    }
+    for (uint32_t q = threadIdx.x; q < nb; q += kMdThreads) {
+        const uint32_t e = S.bl[q];
+        const int i = e & 0xFFF, j = (e >> 12) & 0xFFF;
+        const double dx = S.x[j] - S.x[i], dy = S.y[j] - S.y[i], dz = S.z[j] - S.z[i];
+        const double r2 = (dx * dx + dy * dy) + dz * dz;
+        acc += to_fix(((e >> 24) & 1) ? fene_anchor_potential(r2) : fene_potential(r2));
+    }
+    return block_sum_all<kMdThreads>(from_fix(acc), S.red);
+}
+
+/* Box::update_VerletLists (Box.cpp:105-180): the half list of every pair
+ * inside the list radius, through a grid of side 2 over the star's bounding
+ * box (an all-pairs scan when the box needs more than kGridMax cells), plus
+ * the COM-centred grid bounds check that raises SIGSEGV in the reference.
+ * Entries are appended with an LDS counter; when they exceed the LDS
+ * capacity the list is rebuilt into the replica's global region.  Returns
+ * the entry count; *in_lds tells where the list is. */
+__device__ uint32_t md_rebuild(const MdLds& S, uint32_t* gpl, uint32_t gcap, const RepDev& rep, RepCtl* ctl, int N,
+                               uint32_t cap, bool* in_lds) {
+    constexpr int NT = kMdThreads;
+    const int tid = threadIdx.x;
+    /* COM of the molecule (Box::center_of_mass_reference_frame) and box */
+    double lc[9];
+    for (int c = 0; c < 3; ++c) { lc[c] = 0.0; lc[3 + c] = 1e300; lc[6 + c] = -1e300; }
+    for (int i = tid; i < N; i += NT) {
+        const double v[3] = {S.x[i], S.y[i], S.z[i]};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { lc[c] += v[c]; lc[3 + c] = fmin(lc[3 + c], v[c]); lc[6 + c] = fmax(lc[6 + c], v[c]); }
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-        const double u = __shfl_xor(v, o);
-        v = MAX ? fmax(v, u) : fmin(v, u);
-    }
-    const int w = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) scratch[w] = v;
-    __syncthreads();
-    double t = scratch[0];
 #pragma unroll
-    for (int i = 1; i < kMdThreads / 64; ++i) t = MAX ? fmax(t, scratch[i]) : fmin(t, scratch[i]);
-    return t;
+        for (int c = 0; c < 3; ++c) {
+            lc[c] += __shfl_xor(lc[c], o);
+            lc[3 + c] = fmin(lc[3 + c], __shfl_xor(lc[3 + c], o));
+            lc[6 + c] = fmax(lc[6 + c], __shfl_xor(lc[6 + c], o));
+        }
+    }
+    const int w = tid >> 6;
+    __syncthreads();
+    if ((tid & 63) == 0)
+#pragma unroll
+        for (int c = 0; c < 9; ++c) S.red[w * 9 + c] = lc[c];
+    __syncthreads();
+    double cs[3], lo[3], hi[3];
+    int gdim[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double a = 0.0, mn = 1e300, mx = -1e300;
+        for (int q = 0; q < NT / 64; ++q) {
+            a += S.red[q * 9 + c];
+            mn = fmin(mn, S.red[q * 9 + 3 + c]);
+            mx = fmax(mx, S.red[q * 9 + 6 + c]);
+        }
+        cs[c] = a / (double)N; lo[c] = mn; hi[c] = mx;
+        gdim[c] = (int)((hi[c] - lo[c]) * 0.5) + 1;
+    }
+    if (tid == 0) { ctl->vcom[0] = cs[0]; ctl->vcom[1] = cs[1]; ctl->vcom[2] = cs[2]; ctl->n_rebuild++; }
+    bool oob = false;
+    for (int i = tid; i < N; i += NT) {
+        const double v[3] = {S.x[i], S.y[i], S.z[i]};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int cn = (int)((v[c] - cs[c] + rep.box.L[c] * 0.5) / rep.vside[c]);
+            oob |= cn < 0 || cn >= rep.vgrid[c];
+        }
+        S.px[i] = v[0]; S.py[i] = v[1]; S.pz[i] = v[2];
+    }
+    if (oob) atomicOr(&ctl->status, ST_VERLET_OOB);
+    const long ncell = (long)gdim[0] * gdim[1] * gdim[2];
+    const bool use_grid = ncell <= kGridMax;
+    if (use_grid) {
+        for (int c = tid; c <= (int)ncell; c += NT) S.gcell[c] = 0u;
+        __syncthreads();
+        for (int i = tid; i < N; i += NT) {
+            const int cx = min((int)((S.x[i] - lo[0]) * 0.5), gdim[0] - 1);
+            const int cy = min((int)((S.y[i] - lo[1]) * 0.5), gdim[1] - 1);
+            const int cz = min((int)((S.z[i] - lo[2]) * 0.5), gdim[2] - 1);
+            const uint32_t cid = (uint32_t)(cx + gdim[0] * (cy + gdim[1] * cz));
+            S.gcid[i] = cid;
+            atomicAdd(&S.gcell[cid], 1u);
+        }
+        __syncthreads();
+        md_block_scan(S.gcell, (int)ncell, S.red);
+        for (int i = tid; i < N; i += NT) S.gmem[atomicAdd(&S.gcell[S.gcid[i]], 1u)] = (uint16_t)i;
+        __syncthreads();  /* gcell[c] now ends cell c */
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        /* pass 0 into LDS; pass 1 into global memory only if LDS overflowed */
+        uint32_t* out = pass == 0 ? S.pl : gpl;
+        const uint32_t lcap = cap < gcap ? cap : gcap;  /* an LDS list must also fit its global copy */
+        const uint32_t ocap = pass == 0 ? lcap : gcap;
+        if (tid == 0) S.flags[1] = 0;
+        __syncthreads();
+        for (int i = tid; i < N; i += NT) {
+            const double xi = S.x[i], yi = S.y[i], zi = S.z[i];
+            const bool bi = S.bt[i] != 0;
+            auto cand = [&](int j) {
+                const double dx = S.x[j] - xi, dy = S.y[j] - yi, dz = S.z[j] - zi;
+                if (j > i && ((dx * dx + dy * dy) + dz * dz) <= ((bi && S.bt[j]) ? kListBB2 : kListAA2)) {
+                    const uint32_t slot = (uint32_t)atomicAdd(&S.flags[1], 1);
+                    if (slot < ocap) out[slot] = (uint32_t)i | ((uint32_t)j << 16);
+                }
+            };
+            if (use_grid) {
+                const uint32_t cid = S.gcid[i];
+                const int cx = (int)(cid % (uint32_t)gdim[0]);
+                const int cy = (int)((cid / (uint32_t)gdim[0]) % (uint32_t)gdim[1]);
+                const int cz = (int)(cid / ((uint32_t)gdim[0] * (uint32_t)gdim[1]));
+                for (int q = 0; q < 27; ++q) {
+                    const int nx = cx + q % 3 - 1, ny = cy + (q / 3) % 3 - 1, nz = cz + q / 9 - 1;
+                    if (nx < 0 || ny < 0 || nz < 0 || nx >= gdim[0] || ny >= gdim[1] || nz >= gdim[2]) continue;
+                    const int nc = nx + gdim[0] * (ny + gdim[1] * nz);
+                    const uint32_t a0 = nc ? S.gcell[nc - 1] : 0u, a1 = S.gcell[nc];
+                    for (uint32_t p = a0; p < a1; ++p) cand(S.gmem[p]);
+                }
+            } else {
+                for (int j = i + 1; j < N; ++j) cand(j);
+            }
+        }
+        __syncthreads();
+        const uint32_t total = (uint32_t)S.flags[1];
+        if (pass == 0 && total <= lcap) { *in_lds = true; __syncthreads(); return total; }
+        if (pass == 1) {
+            if (total > gcap && tid == 0) atomicOr(&ctl->status, ST_VERLET_OVERFLOW);
+            *in_lds = false;
+            __syncthreads();
+            return total < gcap ? total : gcap;
+        }
+        __syncthreads();
+    }
+    return 0;
 }
 
 /* MPC::check_bounds (MPC.cpp:160-190) on the monomers; a recentring also
- * marks the speculative solvent sort of this collision for redo.  Separate
- * function so its registers do not burden the MD loops. */
+ * marks the speculative solvent sort of this collision for redo. */
 __device__ __forceinline__ void md_check_bounds(const MdLds& S, const RepDev& rep, RepCtl* ctl, int N) {
     constexpr int NT = kMdThreads;
     const int tid = threadIdx.x;
@@ -326,6 +427,7 @@ __device__ __forceinline__ void md_check_bounds(const MdLds& S, const RepDev& re
     const bool rec = __syncthreads_or(out) != 0;
     if (rec) {
         double cp[3], cv[3];
+#pragma unroll
         for (int c = 0; c < 3; ++c) {
             const double* a = c == 0 ? S.x : (c == 1 ? S.y : S.z);
             const double* b = c == 0 ? S.vx : (c == 1 ? S.vy : S.vz);
@@ -345,156 +447,32 @@ __device__ __forceinline__ void md_check_bounds(const MdLds& S, const RepDev& re
     if (tid == 0) {
         ctl->recenter = rec ? 1 : 0;
         ctl->redo = rec ? 1 : 0;
-        for (int c = 0; c < 3; ++c) S.shift[c] = ctl->shift[c];
     }
     __syncthreads();
 }
 
-/* Box::update_VerletLists (Box.cpp:105-180): every pair with r^2 <= 4
- * -- the COM-centred 27-cell grid of side >= 2 selects exactly these --
- * plus the grid bounds check that raises SIGSEGV in the reference.
- * Returns whether the lists were built in LDS (else in global memory). */
-__device__ __forceinline__ bool md_rebuild(const MdLds& S, const DevState& d, const RepDev& rep, RepCtl* ctl, int N,
-                                        uint32_t cap) {
-    constexpr int NT = kMdThreads;
-    constexpr int NG = NT / kMdGroup;
-    const int tid = threadIdx.x;
-    const int grp = tid / kMdGroup, gl = tid % kMdGroup;
-    const uint32_t mb = rep.mono_base;
-    bool lds_vl;
-        double cs[3];
-    for (int c = 0; c < 3; ++c) {
-        const double* a = c == 0 ? S.x : (c == 1 ? S.y : S.z);
-        double acc = 0.0;
-        for (int i = tid; i < N; i += NT) acc += a[i];
-        cs[c] = block_sum_all<NT>(acc, S.red) / (double)N;
-    }
-    if (tid == 0) { ctl->vcom[0] = cs[0]; ctl->vcom[1] = cs[1]; ctl->vcom[2] = cs[2]; ctl->n_rebuild++; }
-    for (int i = tid; i < N; i += NT) {
-        const double xi[3] = {S.x[i], S.y[i], S.z[i]};
-        bool oob = false;
-        for (int c = 0; c < 3; ++c) {
-            const int cn = (int)((xi[c] - cs[c] + rep.box.L[c] * 0.5) / rep.vside[c]);
-            oob |= cn < 0 || cn >= rep.vgrid[c];
-        }
-        if (oob) atomicOr(&ctl->status, ST_VERLET_OOB);
-        S.px[i] = xi[0]; S.py[i] = xi[1]; S.pz[i] = xi[2];
-    }
-    /* half list (j > i) of candidate pairs from a grid of side 2 over the star's bounding
-     * box (cells sorted by monomer index: deterministic lists); an
-     * all-pairs scan when the box needs more than kGridMax cells */
-    double lo[3], hi[3];
-    int gdim[3];
-    for (int c = 0; c < 3; ++c) {
-        const double* a = c == 0 ? S.x : (c == 1 ? S.y : S.z);
-        double mn = 1e300, mx = -1e300;
-        for (int i = tid; i < N; i += NT) { mn = fmin(mn, a[i]); mx = fmax(mx, a[i]); }
-        lo[c] = block_extreme<false>(mn, S.red);
-        hi[c] = block_extreme<true>(mx, S.red);
-        gdim[c] = (int)((hi[c] - lo[c]) * 0.5) + 1;
-    }
-    const long ncell = (long)gdim[0] * gdim[1] * gdim[2];
-    const bool use_grid = ncell <= kGridMax;
-    if (use_grid) {
-        for (int c = tid; c <= (int)ncell; c += NT) S.gcell[c] = 0u;
-        __syncthreads();
-        for (int i = tid; i < N; i += NT) {
-            int cc[3];
-            const double xi[3] = {S.x[i], S.y[i], S.z[i]};
-            for (int c = 0; c < 3; ++c) cc[c] = min((int)((xi[c] - lo[c]) * 0.5), gdim[c] - 1);
-            const uint32_t cid = (uint32_t)(cc[0] + gdim[0] * (cc[1] + gdim[1] * cc[2]));
-            S.gcid[i] = cid;
-            atomicAdd(&S.gcell[cid], 1u);
-        }
-        __syncthreads();
-        md_block_scan(S.gcell, (int)ncell, S.red);
-        for (int i = tid; i < N; i += NT) {
-            const uint32_t slot = atomicAdd(&S.gcell[S.gcid[i]], 1u);
-            S.gmem[slot] = (uint16_t)i;
-        }
-        __syncthreads();
-        /* gcell[c] now ends cell c.  Members stay in arrival order: the
-         * forces are summed in fixed point, so list order does not matter. */
-        __syncthreads();
-    }
-    /* visit the candidates of monomer i on lane gl; fn(j) per pair */
-    auto visit = [&](int i, auto&& fn) {
-        const double xi = S.x[i], yi = S.y[i], zi = S.z[i];
-        const bool bi = S.bt[i] != 0;
-        if (use_grid) {
-            const uint32_t cid = S.gcid[i];
-            const int cx = (int)(cid % (uint32_t)gdim[0]);
-            const int cy = (int)((cid / (uint32_t)gdim[0]) % (uint32_t)gdim[1]);
-            const int cz = (int)(cid / ((uint32_t)gdim[0] * (uint32_t)gdim[1]));
-            for (int q = gl; q < 27; q += kMdGroup) {
-                const int nx = cx + q % 3 - 1, ny = cy + (q / 3) % 3 - 1, nz = cz + q / 9 - 1;
-                if (nx < 0 || ny < 0 || nz < 0 || nx >= gdim[0] || ny >= gdim[1] || nz >= gdim[2]) continue;
-                const int nc = nx + gdim[0] * (ny + gdim[1] * nz);
-                const uint32_t a0 = nc ? S.gcell[nc - 1] : 0u, a1 = S.gcell[nc];
-                for (uint32_t p = a0; p < a1; ++p) {
-                    const int j = S.gmem[p];
-                    const double dx = S.x[j] - xi, dy = S.y[j] - yi, dz = S.z[j] - zi;
-                    if (j > i && ((dx * dx + dy * dy) + dz * dz) <= ((bi && S.bt[j]) ? kListBB2 : kListAA2)) fn(j);
-                }
-            }
-        } else {
-            for (int j = gl; j < N; j += kMdGroup) {
-                const double dx = S.x[j] - xi, dy = S.y[j] - yi, dz = S.z[j] - zi;
-                if (j > i && ((dx * dx + dy * dy) + dz * dz) <= ((bi && S.bt[j]) ? kListBB2 : kListAA2)) fn(j);
-            }
-        }
-    };
-    /* pass 1: counts */
-    for (int i = grp; i < N; i += NG) {
-        uint32_t c = 0;
-        visit(i, [&](int) { ++c; });
-#pragma unroll
-        for (int o = kMdGroup / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, kMdGroup);
-        if (gl == 0) S.vl_off[i] = c;
-    }
-    __syncthreads();
-    bool over = false;
-    for (int i = tid; i < N; i += NT) over |= S.vl_off[i] > (uint32_t)kVcap;
-    if (over) atomicOr(&ctl->status, ST_VERLET_OVERFLOW);
-    lds_vl = md_block_scan(S.vl_off, N, S.red) <= cap;
-    /* pass 2: fill, lane-major order inside a monomer's list */
-    for (int i = grp; i < N; i += NG) {
-        uint32_t c = 0;
-        visit(i, [&](int) { ++c; });
-        uint32_t pre = c;
-#pragma unroll
-        for (int o = 1; o < kMdGroup; o <<= 1) {
-            const uint32_t t = __shfl_up(pre, o, kMdGroup);
-            if (gl >= o) pre += t;
-        }
-        uint32_t w = pre - c;
-        const uint32_t total = S.vl_off[i + 1] - S.vl_off[i];
-        const uint32_t base = S.vl_off[i];
-        visit(i, [&](int j) {
-            if (lds_vl) S.vl[base + w] = (uint16_t)j;
-            else if (w < (uint32_t)kVcap) d.vl[(size_t)(mb + i) * kVcap + w] = (uint16_t)j;
-            ++w;
-        });
-        if (gl == 0 && !lds_vl) d.vl_n[mb + i] = (uint16_t)(total < (uint32_t)kVcap ? total : kVcap);
-    }
-    __syncthreads();
-    return lds_vl;
-}
-
+/* One workgroup per replica (its star): the pending velocity-Verlet
+ * substeps (Thermostat_None::propagate, Thermostat_None.cpp:21-38) with all
+ * per-monomer state in LDS; then, for a collision, check_bounds and the
+ * monomers' sorted cell keys. */
 __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epot_last, int md_init, int do_coll,
                                                   double dt, int ns, uint32_t cap) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NT = kMdThreads;
-    constexpr int NG = NT / kMdGroup;
-    const MdLds S = md_carve(smem, ns);
+    MdLds S = md_carve(smem, ns, cap);
     const int r = blockIdx.x;
     const int tid = threadIdx.x;
-    const int grp = tid / kMdGroup, gl = tid % kMdGroup;
     const RepDev& rep = d.reps[r];
     RepCtl* ctl = d.ctl + r;
     const int N = (int)rep.n_mono;
     const uint32_t mb = rep.mono_base;
     const double dth = dt * 0.50;
+    unsigned long long* const qx = reinterpret_cast<unsigned long long*>(S.fx);
+    unsigned long long* const qy = reinterpret_cast<unsigned long long*>(S.fy);
+    unsigned long long* const qz = reinterpret_cast<unsigned long long*>(S.fz);
+    /* the replica's global pair-list region: 64 entries per monomer */
+    uint32_t* const gpl = d.vl + (size_t)mb * kVlPerMono;
+    const uint32_t gcap = (uint32_t)N * kVlPerMono;
 
     for (int i = tid; i < N; i += NT) {
         const size_t g = 3 * (size_t)(mb + i);
@@ -505,78 +483,57 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
         S.im[i] = dth / d.mmass[mb + i];  /* half-kick factor dt/(2m) */
         S.bt[i] = d.mbtype[mb + i];
     }
-    const uint32_t abase = N > 0 ? d.adj_off[rep.adj_base] : 0u;
-    for (int i = tid; i <= N; i += NT) S.aoff[i] = d.adj_off[rep.adj_base + i] - abase;
+    /* bonds, each once: the owner side of the adjacency (anchor flag kept) */
+    if (tid == 0) S.flags[2] = 0;
     __syncthreads();
-    for (uint32_t q = tid; N > 0 && q < S.aoff[N]; q += NT) S.adj[q] = d.adj[abase + q];
-    /* Verlet lists: global fixed-capacity lists -> LDS CSR when they fit */
-    bool lds_vl = false, rebuilt = false;
-    if (N > 0 && !md_init) {
-        for (int i = tid; i < N; i += NT) S.vl_off[i] = d.vl_n[mb + i];
-        __syncthreads();
-        lds_vl = md_block_scan(S.vl_off, N, S.red) <= cap;
-        if (lds_vl) {
-            for (int i = grp; i < N; i += NG) {
-                const uint32_t o = S.vl_off[i], n = S.vl_off[i + 1] - o;
-                for (uint32_t q = gl; q < n; q += kMdGroup) S.vl[o + q] = d.vl[(size_t)(mb + i) * kVcap + q];
+    {
+        const uint32_t abase = N > 0 ? d.adj_off[rep.adj_base] : 0u;
+        for (int i = tid; i < N; i += NT) {
+            const uint32_t a0 = d.adj_off[rep.adj_base + i] - abase, a1 = d.adj_off[rep.adj_base + i + 1] - abase;
+            for (uint32_t q = a0; q < a1; ++q) {
+                const uint32_t e = d.adj[abase + q];
+                const uint32_t j = e & ~kBondAnchor;
+                if ((int)j > i) {
+                    const int slot = atomicAdd(&S.flags[2], 1);
+                    if (slot < ns) S.bl[slot] = (uint32_t)i | (j << 12) | ((e & kBondAnchor) ? (1u << 24) : 0u);
+                }
             }
         }
     }
+    /* Verlet half list of the last rebuild -> LDS when it fits */
+    uint32_t npl = ctl->vl_count;
+    bool lds_vl = false;
+    if (N > 0 && !md_init && npl <= cap) {
+        for (uint32_t q = tid; q < npl; q += NT) S.pl[q] = gpl[q];
+        lds_vl = true;
+    }
     __syncthreads();
-
-    auto rebuild = [&]() {
-        lds_vl = md_rebuild(S, d, rep, ctl, N, cap);
-        rebuilt = true;
-    };
-
-
-    /* Box::calculate_forces_verlet (Box.cpp:243-286) */
+    const uint32_t nbond = (uint32_t)min(S.flags[2], ns);
+    bool rebuilt = false;
     double epot_val = 0.0;
     bool epot_set = false;
-    /* fixed-point force accumulators live in the force arrays during forces() */
-    unsigned long long* const qx = reinterpret_cast<unsigned long long*>(S.fx);
-    unsigned long long* const qy = reinterpret_cast<unsigned long long*>(S.fy);
-    unsigned long long* const qz = reinterpret_cast<unsigned long long*>(S.fz);
+
+    auto rebuild = [&]() {
+        bool in_lds = false;
+        npl = md_rebuild(S, gpl, gcap, rep, ctl, N, cap, &in_lds);
+        lds_vl = in_lds;
+        rebuilt = true;
+    };
+    /* Box::calculate_forces_verlet (Box.cpp:243-286); accumulators zeroed
+     * by the caller */
     auto forces = [&](bool calc_epot) {
-        double ep = 0.0;
-        for (int i = grp; i < N; i += NG) {
-            double fx = 0.0, fy = 0.0, fz = 0.0;
-            if (lds_vl) {
-                const uint16_t* lst = S.vl + S.vl_off[i];
-                const uint32_t nvl = S.vl_off[i + 1] - S.vl_off[i];
-                md_pair_force(S, rep, i, lst, nvl, gl, fx, fy, fz);
-                if (calc_epot) ep += md_pair_epot(S, rep, i, lst, nvl, gl);
-            } else {
-                const uint16_t* lst = d.vl + (size_t)(mb + i) * kVcap;
-                const uint32_t nvl = d.vl_n[mb + i];
-                md_pair_force(S, rep, i, lst, nvl, gl, fx, fy, fz);
-                if (calc_epot) ep += md_pair_epot(S, rep, i, lst, nvl, gl);
-            }
-#pragma unroll
-            for (int o = kMdGroup / 2; o > 0; o >>= 1) {
-                fx += __shfl_xor(fx, o, kMdGroup);
-                fy += __shfl_xor(fy, o, kMdGroup);
-                fz += __shfl_xor(fz, o, kMdGroup);
-            }
-            if (gl == 0) {
-                atomicAdd(&qx[i], to_fix(fx));
-                atomicAdd(&qy[i], to_fix(fy));
-                atomicAdd(&qz[i], to_fix(fz));
-            }
-        }
+        const uint32_t* pl = lds_vl ? S.pl : gpl;
+        md_forces_flat<false>(S, pl, npl, rep.lambda);
+        md_forces_flat<true>(S, S.bl, nbond, rep.lambda);
         __syncthreads();
         for (int i = tid; i < N; i += NT) {
             S.fx[i] = from_fix(qx[i]);
             S.fy[i] = from_fix(qy[i]);
             S.fz[i] = from_fix(qz[i]);
         }
-        if (calc_epot) {
-            const double tot = block_sum_all<NT>(ep, S.red);
-            epot_val = tot;
-        } else {
-            epot_val = 0.0;
-            __syncthreads();
-        }
+        if (calc_epot) epot_val = md_epot_flat(S, pl, npl, nbond, rep.lambda);
+        else { epot_val = 0.0; __syncthreads(); }
+        epot_set = true;
     };
 
     if (N > 0) {
@@ -585,11 +542,9 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
             for (int i = tid; i < N; i += NT) { qx[i] = 0ull; qy[i] = 0ull; qz[i] = 0ull; }
             __syncthreads();
             forces(false);
-            epot_set = true;
         }
         for (int s = 0; s < n_md; ++s) {
-            /* Thermostat_None::propagate (Thermostat_None.cpp:21-38) and
-             * Box::check_VerletLists (Box.cpp:182-194) */
+            /* first half kick, drift, Box::check_VerletLists (Box.cpp:182-194) */
             bool need = false;
             for (int i = tid; i < N; i += NT) {
                 const double h = S.im[i];
@@ -604,23 +559,17 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
             }
             if (__syncthreads_or(need)) rebuild();
             forces(epot_last && s == n_md - 1);
-            epot_set = true;
+            /* second half kick */
             for (int i = tid; i < N; i += NT) {
                 const double h = S.im[i];
                 S.vx[i] += S.fx[i] * h; S.vy[i] += S.fy[i] * h; S.vz[i] += S.fz[i] * h;
             }
             __syncthreads();
         }
-        if (rebuilt && lds_vl) {
-            for (int i = grp; i < N; i += NG) {
-                const uint32_t o = S.vl_off[i], n = S.vl_off[i + 1] - o;
-                for (uint32_t q = gl; q < n && q < (uint32_t)kVcap; q += kMdGroup)
-                    d.vl[(size_t)(mb + i) * kVcap + q] = S.vl[o + q];
-                if (gl == 0) d.vl_n[mb + i] = (uint16_t)(n < (uint32_t)kVcap ? n : kVcap);
-            }
-        }
+        if (rebuilt && lds_vl)
+            for (uint32_t q = tid; q < npl && q < gcap; q += NT) gpl[q] = S.pl[q];
+        if (rebuilt && tid == 0) ctl->vl_count = npl;
     }
-
     if (tid == 0 && epot_set) ctl->epot = epot_val;
     if (do_coll) md_check_bounds(S, rep, ctl, N);
 
@@ -639,12 +588,13 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
          * over the velocity/force arrays, which are written back above */
         __syncthreads();
         uint64_t* keys = reinterpret_cast<uint64_t*>(S.vx);
+        const double sh[3] = {ctl->shift[0], ctl->shift[1], ctl->shift[2]};
         int np2 = 1;
         while (np2 < N) np2 <<= 1;
         for (int i = tid; i < np2; i += NT) keys[i] = ~0ull;
         __syncthreads();
         for (int i = tid; i < N; i += NT) {
-            const uint32_t ci = cell_index(rep.box, S.x[i] + S.shift[0], S.y[i] + S.shift[1], S.z[i] + S.shift[2]);
+            const uint32_t ci = cell_index(rep.box, S.x[i] + sh[0], S.y[i] + sh[1], S.z[i] + sh[2]);
             if (ci >= rep.ncells) atomicOr(&ctl->status, ST_CELL_OOB);
             const uint64_t g = ci < rep.ncells ? rep.cell_base + ci : 0xFFFFFFFFull;
             keys[i] = (g << 32) | (uint32_t)i;
