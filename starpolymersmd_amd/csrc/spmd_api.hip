@@ -120,8 +120,7 @@ int upload_tables(spmd_sys* s) {
 int build_monomers(spmd_sys* s) {
     DevState& d = s->d;
     uint32_t mt = 0;
-    std::vector<uint32_t> adj_off;
-    std::vector<uint32_t> adj;
+    std::vector<uint32_t> bonds;
     std::vector<double> pos, vel, force, mass;
     std::vector<uint8_t> bt;
     for (int r = 0; r < d.n_rep; ++r) {
@@ -129,18 +128,14 @@ int build_monomers(spmd_sys* s) {
         RepDev& rp = s->reps[r];
         rp.mono_base = mt;
         rp.n_mono = st.n;
-        rp.adj_base = (uint32_t)adj_off.size();
-        std::vector<std::vector<uint32_t>> lists(st.n);
+        /* each bond once, lower index first; the anchor FENE applies when
+         * the bond's owner (its first monomer) is the core (Box.cpp:270-277) */
+        rp.bond_base = (uint32_t)bonds.size();
         for (auto& b : st.bonds) {
-            const uint32_t fl = st.anchor[b.first] ? kBondAnchor : 0u;
-            lists[b.first].push_back(b.second | fl);
-            lists[b.second].push_back(b.first | fl);
+            const uint32_t i = std::min(b.first, b.second), j = std::max(b.first, b.second);
+            bonds.push_back(i | (j << 12) | (st.anchor[b.first] ? (1u << 24) : 0u));
         }
-        for (uint32_t i = 0; i < st.n; ++i) {
-            adj_off.push_back((uint32_t)adj.size());
-            for (uint32_t e : lists[i]) adj.push_back(e);
-        }
-        adj_off.push_back((uint32_t)adj.size());
+        rp.n_bond = (uint32_t)bonds.size() - rp.bond_base;
         pos.insert(pos.end(), st.pos.begin(), st.pos.end());
         vel.insert(vel.end(), st.vel.begin(), st.vel.end());
         force.insert(force.end(), st.force.begin(), st.force.end());
@@ -159,7 +154,7 @@ int build_monomers(spmd_sys* s) {
     const size_t fixed = md_lds_bytes(ns, 0);
     d.md_vl_cap = fixed < budget ? (uint32_t)((budget - fixed) / 4) : 0u;
     dfree(d.mpos); dfree(d.mvel); dfree(d.mforce); dfree(d.mvpos); dfree(d.mmass); dfree(d.mbtype);
-    dfree(d.vl); dfree(d.adj_off); dfree(d.adj); dfree(d.msorted);
+    dfree(d.vl); dfree(d.bonds); dfree(d.mcell);
     if (int rc = dalloc(s, &d.mpos, 3ull * mt)) return rc;
     if (int rc = dalloc(s, &d.mvel, 3ull * mt)) return rc;
     if (int rc = dalloc(s, &d.mforce, 3ull * mt)) return rc;
@@ -167,9 +162,8 @@ int build_monomers(spmd_sys* s) {
     if (int rc = dalloc(s, &d.mmass, mt)) return rc;
     if (int rc = dalloc(s, &d.mbtype, mt)) return rc;
     if (int rc = dalloc(s, &d.vl, (size_t)mt * kVlPerMono)) return rc;
-    if (int rc = dalloc(s, &d.adj_off, adj_off.size())) return rc;
-    if (int rc = dalloc(s, &d.adj, adj.size())) return rc;
-    if (int rc = dalloc(s, &d.msorted, mt)) return rc;
+    if (int rc = dalloc(s, &d.bonds, std::max<size_t>(bonds.size(), 1))) return rc;
+    if (int rc = dalloc(s, &d.mcell, mt)) return rc;
     if (mt) {
         HIPCHK(s, hipMemcpy(d.mpos, pos.data(), pos.size() * 8, hipMemcpyHostToDevice));
         HIPCHK(s, hipMemcpy(d.mvel, vel.data(), vel.size() * 8, hipMemcpyHostToDevice));
@@ -182,8 +176,7 @@ int build_monomers(spmd_sys* s) {
         const uint32_t zero = 0u;
         HIPCHK(s, hipMemcpy(&d.ctl[r].vl_count, &zero, 4, hipMemcpyHostToDevice));
     }
-    if (!adj.empty()) HIPCHK(s, hipMemcpy(d.adj, adj.data(), adj.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(s, hipMemcpy(d.adj_off, adj_off.data(), adj_off.size() * 4, hipMemcpyHostToDevice));
+    if (!bonds.empty()) HIPCHK(s, hipMemcpy(d.bonds, bonds.data(), bonds.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(s, hipMemcpy(d.reps, s->reps.data(), s->reps.size() * sizeof(RepDev), hipMemcpyHostToDevice));
     s->mono_dirty = false;
     clear_graphs(s);
@@ -309,6 +302,8 @@ int check_status(spmd_sys* s) {
     if (st & ST_CELL_OOB) return fail(s, SPMD_ERR_CELL_OOB, std::string("cell index out of range; ") + buf);
     if (st & ST_VERLET_OOB) return fail(s, SPMD_ERR_VERLET_OOB, std::string("monomer outside Verlet grid; ") + buf);
     if (st & ST_VERLET_OVERFLOW) return fail(s, SPMD_ERR_VERLET_OVERFLOW, std::string("Verlet list overflow; ") + buf);
+    if (st & ST_TILE_MONO)
+        return fail(s, SPMD_ERR_RANK_OVERFLOW, std::string("more monomers in one collide tile than it holds; ") + buf);
     return fail(s, SPMD_ERR_RANK_OVERFLOW, std::string("cell occupancy overflow; ") + buf);
 }
 
@@ -602,7 +597,7 @@ int spmd_destroy(spmd_sys* s) {
     }
     dfree(d.rank); dfree(d.counts); dfree(d.offsets); dfree(d.scan_flags); dfree(d.scan_ticket);
     dfree(d.mpos); dfree(d.mvel); dfree(d.mforce); dfree(d.mvpos); dfree(d.mmass); dfree(d.mbtype);
-    dfree(d.vl); dfree(d.adj_off); dfree(d.adj); dfree(d.msorted);
+    dfree(d.vl); dfree(d.bonds); dfree(d.mcell);
     dfree(d.inj_axis); dfree(d.inj_gamma); dfree(d.reps); dfree(d.ctl); dfree(d.pblk_base);
     dfree(d.tile_base); dfree(d.red);
     if (s->side) (void)hipStreamSynchronize(s->side);
@@ -937,9 +932,9 @@ int spmd_get_cell_index(spmd_sys* s, uint32_t r, uint32_t* fluid_cells, uint32_t
             for (uint32_t p = off[c]; p < off[c + 1]; ++p) fluid_cells[ids[p - rp.part_base]] = c;
     }
     if (mono_cells && rp.n_mono) {
-        std::vector<uint64_t> ms(rp.n_mono);
-        HIPCHK(s, hipMemcpy(ms.data(), d.msorted + rp.mono_base, 8ull * rp.n_mono, hipMemcpyDeviceToHost));
-        for (uint64_t k : ms) mono_cells[(uint32_t)k] = (uint32_t)((k >> 32) - rp.cell_base);
+        std::vector<uint32_t> mc(rp.n_mono);
+        HIPCHK(s, hipMemcpy(mc.data(), d.mcell + rp.mono_base, 4ull * rp.n_mono, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < rp.n_mono; ++i) mono_cells[i] = (uint32_t)(mc[i] - rp.cell_base);
     }
     return SPMD_OK;
 }

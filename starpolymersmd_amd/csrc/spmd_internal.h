@@ -10,10 +10,11 @@
  *   offsets   uint32[C_tot+1]  exclusive scan of counts = first particle
  *                              of each cell in the sorted arrays
  *   monomers  pos/vel/force/vpos double[3*M_tot], mass double[M_tot],
- *             btype uint8[M_tot], Verlet lists uint16[M_tot*VCAP],
- *             bond CSR (adj_off uint32[M_tot+R], adj uint32[2*bonds])
- *   msorted   uint64[M_tot]    (global cell << 32 | monomer) of the last
- *                              collision, sorted, for the collide tiles
+ *             btype uint8[M_tot], Verlet half lists uint32[M_tot*64]
+ *             (i | j << 16, per replica from mono_base*64), bonds
+ *             uint32[bonds] (i | j << 12 | anchor << 24, each bond once)
+ *   mcell     uint32[M_tot]    global cell of each monomer at the last
+ *                              collision; each collide tile gathers its own
  *   RepDev    per-replica constants, RepCtl per-replica step state
  */
 #ifndef SPMD_INTERNAL_H
@@ -39,6 +40,7 @@ enum : uint32_t {
     ST_VERLET_OVERFLOW = 1u << 2,
     ST_RANK_OVERFLOW = 1u << 3,
     ST_SCAN_TIMEOUT = 1u << 4,
+    ST_TILE_MONO = 1u << 5,      /* more monomers in a collide tile than its list holds */
 };
 
 /* bond entry: partner index | anchor-FENE flag in bit 31 */
@@ -54,7 +56,7 @@ struct RepDev {
     uint64_t part_base;
     uint32_t n_fluid;
     uint32_t mono_base, n_mono;
-    uint32_t adj_base;     /* index of this replica's adj_off[0] */
+    uint32_t bond_base, n_bond;  /* this replica's entries of the bond list */
     int vgrid[3];          /* Verlet grid cells per side (Box.cpp:20-22) */
     double vside[3];       /* Verlet grid cell side (Box.cpp:23-25) */
 };
@@ -93,8 +95,8 @@ struct DevState {
     double *mpos, *mvel, *mforce, *mvpos, *mmass;
     uint8_t* mbtype;
     uint32_t* vl;          /* Verlet half lists: i | j << 16, kVlPerMono per monomer */
-    uint32_t *adj_off, *adj;
-    uint64_t* msorted;
+    uint32_t* bonds;
+    uint32_t* mcell;       /* global cell of each monomer at the last collision (~0u: none) */
     double* inj_axis;      /* 2 per cell, all replicas */
     double* inj_gamma;     /* 1 per cell */
     RepDev* reps;          /* device copy */

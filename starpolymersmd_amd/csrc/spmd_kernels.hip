@@ -133,25 +133,31 @@ __device__ double block_sum(double v, double* scratch) {
 
 /* ======================================================= MD + prologue */
 
-constexpr int kGridMax = 4096;   /* cells of the Verlet rebuild grid kept in LDS */
 /* Verlet list radii: the reference keeps every pair within 2.0 (Box.cpp:9-10)
  * and rebuilds once a monomer moved 0.25 = (2.0 - 1.5)/2.  Only B-B pairs
  * interact beyond r^2 = 1.0594631, so the other pairs need a list radius of
  * only 1.0293 + 0.5 for the same rebuild rule; the set of interacting pairs
- * -- and so every force -- is unchanged. */
-constexpr double kListBB2 = 4.0;
-constexpr double kListAA2 = 2.34;   /* (sqrt(1.0594631) + 0.5)^2 = 2.3388 */
-constexpr int kMdBatch = 4;         /* independent pairs per lane in flight */
+ * -- and so every force -- is unchanged.  The rebuild tests distances in
+ * FP32 against radii padded by 1e-4 (far above FP32 rounding of r^2 at
+ * these coordinates), so its list holds every pair the FP64 test would
+ * keep; extra pairs beyond the cutoffs contribute nothing. */
+constexpr float kListBB2f = 4.0001f;
+constexpr float kListAA2f = 2.3401f;   /* (sqrt(1.0594631) + 0.5)^2 = 2.3388 */
+constexpr int kMdBatch = 4;            /* independent pairs per lane in flight */
+constexpr int kGridMax = 4096;         /* cells of the rebuild grid */
 
-/* Dynamic LDS of k_md for a stride of ns monomers (host computes the same
- * size with md_lds_bytes) followed by cap pair-list entries. */
+/* Dynamic LDS of k_md for a stride of ns monomers (the host computes the
+ * same size with md_lds_bytes), with cap pair-list entries. */
 struct MdLds {
     double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *px, *py, *pz, *im, *red;
-    uint32_t* gcell;   /* grid cell starts (kGridMax + 1) */
-    uint32_t* gcid;    /* grid cell of each monomer (ns) */
+    /* rebuild scratch over the force accumulators (zeroed again after it):
+     * FP32 positions sorted by grid cell, w = bits of (i | B << 16), and
+     * each monomer's grid cell */
+    float4* sp4;       /* ns */
+    uint32_t* gcid;    /* ns */
+    uint32_t* gcell;   /* grid cell ends after the fill (kGridMax + 1) */
     uint32_t* bl;      /* bonds: i | j << 12 | anchor << 24 (ns) */
     uint32_t* pl;      /* half pair list: i | j << 16 (cap) */
-    uint16_t* gmem;    /* monomers ordered by grid cell (ns) */
     uint8_t* bt;       /* monomer is B-type (ns) */
     int* flags;
 };
@@ -166,18 +172,18 @@ __device__ __forceinline__ MdLds md_carve(unsigned char* base, int ns, uint32_t 
     L.im = p + 12 * ns;                                           /* dt/(2m) */
     L.red = p + 13 * ns;                                          /* 80 doubles */
     L.flags = reinterpret_cast<int*>(L.red + 80);                 /* 16 ints */
+    L.sp4 = reinterpret_cast<float4*>(L.fx);                      /* 16 ns <= 24 ns bytes */
+    L.gcid = reinterpret_cast<uint32_t*>(L.sp4 + ns);             /* + 4 ns */
     L.gcell = reinterpret_cast<uint32_t*>(L.flags + 16);
-    L.gcid = L.gcell + kGridMax + 1;
-    L.bl = L.gcid + ns;
+    L.bl = L.gcell + kGridMax + 1;
     L.pl = L.bl + ns;
-    L.gmem = reinterpret_cast<uint16_t*>(L.pl + cap);
-    L.bt = reinterpret_cast<uint8_t*>(L.gmem + ns);
+    L.bt = reinterpret_cast<uint8_t*>(L.pl + cap);
     return L;
 }
 
 size_t md_lds_bytes(int ns, uint32_t cap) {
-    return (size_t)13 * ns * 8 + 80 * 8 + 16 * 4 + (size_t)(kGridMax + 1) * 4 + (size_t)ns * 4 + (size_t)ns * 4 +
-           (size_t)cap * 4 + (size_t)ns * 2 + (size_t)ns + 64;
+    return (size_t)13 * ns * 8 + 80 * 8 + 16 * 4 + (size_t)(kGridMax + 1) * 4 + (size_t)ns * 4 + (size_t)cap * 4 +
+           (size_t)ns + 64;
 }
 
 template <int NT>
@@ -191,39 +197,6 @@ __device__ double block_sum_all(double v, double* scratch) {
 #pragma unroll
     for (int i = 0; i < NT / 64; ++i) t += scratch[i];
     return t;
-}
-
-/* Exclusive scan of a[0..n) in place, total in a[n] and returned; each
- * thread scans a contiguous chunk, deterministic. */
-__device__ uint32_t md_block_scan(uint32_t* a, int n, double* scratch_d) {
-    uint32_t* wsum = reinterpret_cast<uint32_t*>(scratch_d);
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int per = (n + kMdThreads - 1) / kMdThreads;
-    const int i0 = tid * per;
-    uint32_t t = 0;
-    for (int q = 0; q < per; ++q) t += i0 + q < n ? a[i0 + q] : 0u;
-    uint32_t inc = t;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = __shfl_up(inc, o);
-        if (lane >= o) inc += u;
-    }
-    __syncthreads();
-    if (lane == 63) wsum[w] = inc;
-    __syncthreads();
-    uint32_t pre = 0, tot = 0;
-#pragma unroll
-    for (int q = 0; q < kMdThreads / 64; ++q) {
-        const uint32_t u = wsum[q];
-        pre += q < w ? u : 0u;
-        tot += u;
-    }
-    uint32_t run = pre + inc - t;
-    for (int q = 0; q < per; ++q)
-        if (i0 + q < n) { const uint32_t v = a[i0 + q]; a[i0 + q] = run; run += v; }
-    if (tid == 0) a[n] = tot;
-    __syncthreads();
-    return tot;
 }
 
 /* Forces of kMdBatch independent list entries per lane: half-list pairs
@@ -289,19 +262,125 @@ __device__ double md_epot_flat(const MdLds& S, const uint32_t* pl, uint32_t np, 
     return block_sum_all<kMdThreads>(from_fix(acc), S.red);
 }
 
+/* Exclusive scan of a[0..n) in place, total in a[n] and returned; each
+ * thread scans a contiguous chunk, deterministic. */
+__device__ uint32_t md_block_scan(uint32_t* a, int n, double* scratch_d) {
+    uint32_t* wsum = reinterpret_cast<uint32_t*>(scratch_d);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int per = (n + kMdThreads - 1) / kMdThreads;
+    const int i0 = tid * per;
+    uint32_t t = 0;
+    for (int q = 0; q < per; ++q) t += i0 + q < n ? a[i0 + q] : 0u;
+    uint32_t inc = t;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(inc, o);
+        if (lane >= o) inc += u;
+    }
+    __syncthreads();
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < kMdThreads / 64; ++q) {
+        const uint32_t u = wsum[q];
+        pre += q < w ? u : 0u;
+        tot += u;
+    }
+    uint32_t run = pre + inc - t;
+    for (int q = 0; q < per; ++q)
+        if (i0 + q < n) { const uint32_t v = a[i0 + q]; a[i0 + q] = run; run += v; }
+    if (tid == 0) a[n] = tot;
+    __syncthreads();
+    return tot;
+}
+
+/* exclusive block scan of one value per thread; *total gets the sum */
+__device__ __forceinline__ uint32_t md_thread_scan(uint32_t v, double* scratch_d, uint32_t* total) {
+    uint32_t* wsum = reinterpret_cast<uint32_t*>(scratch_d);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(inc, o);
+        if (lane >= o) inc += u;
+    }
+    __syncthreads();
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < kMdThreads / 64; ++q) {
+        const uint32_t u = wsum[q];
+        pre += q < w ? u : 0u;
+        tot += u;
+    }
+    *total = tot;
+    return pre + inc - v;
+}
+
+/* The list candidates of the monomer at sorted slot p: the 3x3 rows of
+ * three neighbouring cells (x-fastest cells, so each row is one contiguous
+ * slot range).  WRITE=false counts the pairs j > i inside the list radius,
+ * WRITE=true stores them from out[o]. */
+template <bool WRITE>
+__device__ __forceinline__ uint32_t md_scan_cands(const MdLds& S, int p, const int gd[3], uint32_t* out,
+                                                  uint32_t o, uint32_t ocap) {
+    const float4 me = S.sp4[p];
+    const uint32_t im = __float_as_uint(me.w);
+    const int i = (int)(im & 0xFFFF);
+    const bool bi = (im >> 16) != 0;
+    const uint32_t cid = S.gcid[i];
+    const int cx = (int)(cid % (uint32_t)gd[0]);
+    const int cy = (int)((cid / (uint32_t)gd[0]) % (uint32_t)gd[1]);
+    const int cz = (int)(cid / ((uint32_t)gd[0] * (uint32_t)gd[1]));
+    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < gd[0] ? cx + 1 : gd[0] - 1;
+    uint32_t n = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z = cz + dz;
+        if (z < 0 || z >= gd[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = cy + dy;
+            if (y < 0 || y >= gd[1]) continue;
+            const int rowc = gd[0] * (y + gd[1] * z);
+            const uint32_t a0 = rowc + x0 ? S.gcell[rowc + x0 - 1] : 0u;
+            const uint32_t a1 = S.gcell[rowc + x1];
+            for (uint32_t q = a0; q < a1; q += 4) {
+                float4 c[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) c[k] = S.sp4[q + k < a1 ? q + k : a1 - 1];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t jm = __float_as_uint(c[k].w);
+                    const int j = (int)(jm & 0xFFFF);
+                    const float dx = c[k].x - me.x, dy2 = c[k].y - me.y, dz2 = c[k].z - me.z;
+                    const float r2 = __builtin_fmaf(dz2, dz2, __builtin_fmaf(dy2, dy2, dx * dx));
+                    const bool hit = q + k < a1 && j > i && r2 <= ((bi && (jm >> 16)) ? kListBB2f : kListAA2f);
+                    if (WRITE && hit && o + n < ocap) out[o + n] = (uint32_t)i | ((uint32_t)j << 16);
+                    n += hit ? 1u : 0u;
+                }
+            }
+        }
+    }
+    return n;
+}
+
 /* Box::update_VerletLists (Box.cpp:105-180): the half list of every pair
- * inside the list radius, through a grid of side 2 over the star's bounding
- * box (an all-pairs scan when the box needs more than kGridMax cells), plus
- * the COM-centred grid bounds check that raises SIGSEGV in the reference.
- * Entries are appended with an LDS counter; when they exceed the LDS
- * capacity the list is rebuilt into the replica's global region.  Returns
- * the entry count; *in_lds tells where the list is. */
+ * inside the list radius, plus the COM-centred grid bounds check that
+ * raises SIGSEGV in the reference.  The monomers are binned on a grid of
+ * side >= 2 over their bounding box and sorted by cell; each lane takes
+ * the monomers at its sorted slots, counts their pairs, and after a block
+ * scan writes them at its own offset -- into LDS when the list fits there,
+ * else into the replica's global region.  Leaves the force accumulators
+ * zeroed.  Returns the entry count; *in_lds tells where the list is. */
 __device__ uint32_t md_rebuild(const MdLds& S, uint32_t* gpl, uint32_t gcap, const RepDev& rep, RepCtl* ctl, int N,
                                uint32_t cap, bool* in_lds) {
     constexpr int NT = kMdThreads;
     const int tid = threadIdx.x;
-    /* COM of the molecule (Box::center_of_mass_reference_frame) and box */
+    /* COM of the molecule (Box::center_of_mass_reference_frame) and the
+     * bounding box: 9 values per wave through S.red */
     double lc[9];
+#pragma unroll
     for (int c = 0; c < 3; ++c) { lc[c] = 0.0; lc[3 + c] = 1e300; lc[6 + c] = -1e300; }
     for (int i = tid; i < N; i += NT) {
         const double v[3] = {S.x[i], S.y[i], S.z[i]};
@@ -309,32 +388,31 @@ __device__ uint32_t md_rebuild(const MdLds& S, uint32_t* gpl, uint32_t gcap, con
         for (int c = 0; c < 3; ++c) { lc[c] += v[c]; lc[3 + c] = fmin(lc[3 + c], v[c]); lc[6 + c] = fmax(lc[6 + c], v[c]); }
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
+    for (int o = 32; o > 0; o >>= 1)
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             lc[c] += __shfl_xor(lc[c], o);
             lc[3 + c] = fmin(lc[3 + c], __shfl_xor(lc[3 + c], o));
             lc[6 + c] = fmax(lc[6 + c], __shfl_xor(lc[6 + c], o));
         }
-    }
     const int w = tid >> 6;
     __syncthreads();
     if ((tid & 63) == 0)
 #pragma unroll
         for (int c = 0; c < 9; ++c) S.red[w * 9 + c] = lc[c];
     __syncthreads();
-    double cs[3], lo[3], hi[3];
-    int gdim[3];
+    double cs[3], lo[3], ext[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        double a = 0.0, mn = 1e300, mx = -1e300;
+        double sum = 0.0, a = 1e300, b = -1e300;
         for (int q = 0; q < NT / 64; ++q) {
-            a += S.red[q * 9 + c];
-            mn = fmin(mn, S.red[q * 9 + 3 + c]);
-            mx = fmax(mx, S.red[q * 9 + 6 + c]);
+            sum += S.red[q * 9 + c];
+            a = fmin(a, S.red[q * 9 + 3 + c]);
+            b = fmax(b, S.red[q * 9 + 6 + c]);
         }
-        cs[c] = a / (double)N; lo[c] = mn; hi[c] = mx;
-        gdim[c] = (int)((hi[c] - lo[c]) * 0.5) + 1;
+        cs[c] = sum / (double)N;
+        lo[c] = a;
+        ext[c] = b - a;
     }
     if (tid == 0) { ctl->vcom[0] = cs[0]; ctl->vcom[1] = cs[1]; ctl->vcom[2] = cs[2]; ctl->n_rebuild++; }
     bool oob = false;
@@ -348,69 +426,61 @@ __device__ uint32_t md_rebuild(const MdLds& S, uint32_t* gpl, uint32_t gcap, con
         S.px[i] = v[0]; S.py[i] = v[1]; S.pz[i] = v[2];
     }
     if (oob) atomicOr(&ctl->status, ST_VERLET_OOB);
-    const long ncell = (long)gdim[0] * gdim[1] * gdim[2];
-    const bool use_grid = ncell <= kGridMax;
-    if (use_grid) {
-        for (int c = tid; c <= (int)ncell; c += NT) S.gcell[c] = 0u;
-        __syncthreads();
-        for (int i = tid; i < N; i += NT) {
-            const int cx = min((int)((S.x[i] - lo[0]) * 0.5), gdim[0] - 1);
-            const int cy = min((int)((S.y[i] - lo[1]) * 0.5), gdim[1] - 1);
-            const int cz = min((int)((S.z[i] - lo[2]) * 0.5), gdim[2] - 1);
-            const uint32_t cid = (uint32_t)(cx + gdim[0] * (cy + gdim[1] * cz));
-            S.gcid[i] = cid;
-            atomicAdd(&S.gcell[cid], 1u);
+    /* grid: side >= 2 (the largest list radius), grown until the cells fit */
+    double side = 2.0;
+    int gd[3] = {1, 1, 1};
+    for (int it = 0; it < 64; ++it, side *= 1.25) {
+        long nc = 1;
+        int g[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double q = ext[c] / side;
+            g[c] = q < 4096.0 ? (int)q + 1 : 4097;  /* also catches inf/NaN */
+            nc *= g[c];
         }
-        __syncthreads();
-        md_block_scan(S.gcell, (int)ncell, S.red);
-        for (int i = tid; i < N; i += NT) S.gmem[atomicAdd(&S.gcell[S.gcid[i]], 1u)] = (uint16_t)i;
-        __syncthreads();  /* gcell[c] now ends cell c */
+        if (nc <= kGridMax) { gd[0] = g[0]; gd[1] = g[1]; gd[2] = g[2]; break; }
+    }  /* no fit (non-finite positions): one cell, every pair tested */
+    const int ncell = gd[0] * gd[1] * gd[2];
+    const double inv = 1.0 / side;
+    for (int c = tid; c <= ncell; c += NT) S.gcell[c] = 0u;
+    __syncthreads();
+    for (int i = tid; i < N; i += NT) {
+        const int cx = max(min((int)((S.x[i] - lo[0]) * inv), gd[0] - 1), 0);
+        const int cy = max(min((int)((S.y[i] - lo[1]) * inv), gd[1] - 1), 0);
+        const int cz = max(min((int)((S.z[i] - lo[2]) * inv), gd[2] - 1), 0);
+        const uint32_t cid = (uint32_t)(cx + gd[0] * (cy + gd[1] * cz));
+        S.gcid[i] = cid;
+        atomicAdd(&S.gcell[cid], 1u);
     }
-    for (int pass = 0; pass < 2; ++pass) {
-        /* pass 0 into LDS; pass 1 into global memory only if LDS overflowed */
-        uint32_t* out = pass == 0 ? S.pl : gpl;
-        const uint32_t lcap = cap < gcap ? cap : gcap;  /* an LDS list must also fit its global copy */
-        const uint32_t ocap = pass == 0 ? lcap : gcap;
-        if (tid == 0) S.flags[1] = 0;
-        __syncthreads();
-        for (int i = tid; i < N; i += NT) {
-            const double xi = S.x[i], yi = S.y[i], zi = S.z[i];
-            const bool bi = S.bt[i] != 0;
-            auto cand = [&](int j) {
-                const double dx = S.x[j] - xi, dy = S.y[j] - yi, dz = S.z[j] - zi;
-                if (j > i && ((dx * dx + dy * dy) + dz * dz) <= ((bi && S.bt[j]) ? kListBB2 : kListAA2)) {
-                    const uint32_t slot = (uint32_t)atomicAdd(&S.flags[1], 1);
-                    if (slot < ocap) out[slot] = (uint32_t)i | ((uint32_t)j << 16);
-                }
-            };
-            if (use_grid) {
-                const uint32_t cid = S.gcid[i];
-                const int cx = (int)(cid % (uint32_t)gdim[0]);
-                const int cy = (int)((cid / (uint32_t)gdim[0]) % (uint32_t)gdim[1]);
-                const int cz = (int)(cid / ((uint32_t)gdim[0] * (uint32_t)gdim[1]));
-                for (int q = 0; q < 27; ++q) {
-                    const int nx = cx + q % 3 - 1, ny = cy + (q / 3) % 3 - 1, nz = cz + q / 9 - 1;
-                    if (nx < 0 || ny < 0 || nz < 0 || nx >= gdim[0] || ny >= gdim[1] || nz >= gdim[2]) continue;
-                    const int nc = nx + gdim[0] * (ny + gdim[1] * nz);
-                    const uint32_t a0 = nc ? S.gcell[nc - 1] : 0u, a1 = S.gcell[nc];
-                    for (uint32_t p = a0; p < a1; ++p) cand(S.gmem[p]);
-                }
-            } else {
-                for (int j = i + 1; j < N; ++j) cand(j);
-            }
-        }
-        __syncthreads();
-        const uint32_t total = (uint32_t)S.flags[1];
-        if (pass == 0 && total <= lcap) { *in_lds = true; __syncthreads(); return total; }
-        if (pass == 1) {
-            if (total > gcap && tid == 0) atomicOr(&ctl->status, ST_VERLET_OVERFLOW);
-            *in_lds = false;
-            __syncthreads();
-            return total < gcap ? total : gcap;
-        }
-        __syncthreads();
+    __syncthreads();
+    md_block_scan(S.gcell, ncell, S.red);
+    for (int i = tid; i < N; i += NT) {
+        const uint32_t slot = atomicAdd(&S.gcell[S.gcid[i]], 1u);
+        S.sp4[slot] = make_float4((float)S.x[i], (float)S.y[i], (float)S.z[i],
+                                  __uint_as_float((uint32_t)i | (S.bt[i] ? 1u << 16 : 0u)));
     }
-    return 0;
+    __syncthreads();  /* gcell[c] now ends cell c */
+    /* count, scan, write */
+    uint32_t cnt = 0;
+    for (int p = tid; p < N; p += NT) cnt += md_scan_cands<false>(S, p, gd, nullptr, 0, 0);
+    uint32_t total;
+    const uint32_t off = md_thread_scan(cnt, S.red, &total);
+    const uint32_t lcap = cap < gcap ? cap : gcap;  /* an LDS list must also fit its global copy */
+    const bool lds = total <= lcap;
+    uint32_t* out = lds ? S.pl : gpl;
+    const uint32_t ocap = lds ? lcap : gcap;
+    uint32_t o = off;
+    for (int p = tid; p < N; p += NT) o += md_scan_cands<true>(S, p, gd, out, o, ocap);
+    if (!lds && total > gcap && tid == 0) atomicOr(&ctl->status, ST_VERLET_OVERFLOW);
+    __syncthreads();
+    /* the scratch lived in the force accumulators */
+    unsigned long long* qx = reinterpret_cast<unsigned long long*>(S.fx);
+    unsigned long long* qy = reinterpret_cast<unsigned long long*>(S.fy);
+    unsigned long long* qz = reinterpret_cast<unsigned long long*>(S.fz);
+    for (int i = tid; i < N; i += NT) { qx[i] = 0ull; qy[i] = 0ull; qz[i] = 0ull; }
+    __syncthreads();
+    *in_lds = lds;
+    return total < ocap ? total : ocap;
 }
 
 /* MPC::check_bounds (MPC.cpp:160-190) on the monomers; a recentring also
@@ -454,7 +524,7 @@ __device__ __forceinline__ void md_check_bounds(const MdLds& S, const RepDev& re
 /* One workgroup per replica (its star): the pending velocity-Verlet
  * substeps (Thermostat_None::propagate, Thermostat_None.cpp:21-38) with all
  * per-monomer state in LDS; then, for a collision, check_bounds and the
- * monomers' sorted cell keys. */
+ * monomers' cells. */
 __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epot_last, int md_init, int do_coll,
                                                   double dt, int ns, uint32_t cap) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -470,9 +540,10 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
     unsigned long long* const qx = reinterpret_cast<unsigned long long*>(S.fx);
     unsigned long long* const qy = reinterpret_cast<unsigned long long*>(S.fy);
     unsigned long long* const qz = reinterpret_cast<unsigned long long*>(S.fz);
-    /* the replica's global pair-list region: 64 entries per monomer */
+    /* the replica's global pair-list region */
     uint32_t* const gpl = d.vl + (size_t)mb * kVlPerMono;
     const uint32_t gcap = (uint32_t)N * kVlPerMono;
+    const uint32_t nbond = rep.n_bond < (uint32_t)ns ? rep.n_bond : (uint32_t)ns;
 
     for (int i = tid; i < N; i += NT) {
         const size_t g = 3 * (size_t)(mb + i);
@@ -483,23 +554,7 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
         S.im[i] = dth / d.mmass[mb + i];  /* half-kick factor dt/(2m) */
         S.bt[i] = d.mbtype[mb + i];
     }
-    /* bonds, each once: the owner side of the adjacency (anchor flag kept) */
-    if (tid == 0) S.flags[2] = 0;
-    __syncthreads();
-    {
-        const uint32_t abase = N > 0 ? d.adj_off[rep.adj_base] : 0u;
-        for (int i = tid; i < N; i += NT) {
-            const uint32_t a0 = d.adj_off[rep.adj_base + i] - abase, a1 = d.adj_off[rep.adj_base + i + 1] - abase;
-            for (uint32_t q = a0; q < a1; ++q) {
-                const uint32_t e = d.adj[abase + q];
-                const uint32_t j = e & ~kBondAnchor;
-                if ((int)j > i) {
-                    const int slot = atomicAdd(&S.flags[2], 1);
-                    if (slot < ns) S.bl[slot] = (uint32_t)i | (j << 12) | ((e & kBondAnchor) ? (1u << 24) : 0u);
-                }
-            }
-        }
-    }
+    for (uint32_t q = tid; q < nbond; q += NT) S.bl[q] = d.bonds[rep.bond_base + q];
     /* Verlet half list of the last rebuild -> LDS when it fits */
     uint32_t npl = ctl->vl_count;
     bool lds_vl = false;
@@ -508,7 +563,6 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
         lds_vl = true;
     }
     __syncthreads();
-    const uint32_t nbond = (uint32_t)min(S.flags[2], ns);
     bool rebuilt = false;
     double epot_val = 0.0;
     bool epot_set = false;
@@ -538,9 +592,7 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
 
     if (N > 0) {
         if (md_init) {
-            rebuild();
-            for (int i = tid; i < N; i += NT) { qx[i] = 0ull; qy[i] = 0ull; qz[i] = 0ull; }
-            __syncthreads();
+            rebuild();  /* leaves the accumulators zeroed */
             forces(false);
         }
         for (int s = 0; s < n_md; ++s) {
@@ -583,37 +635,14 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
     }
 
     if (do_coll) {
-        /* monomer cell keys (MPC::shiftParticles MPC.cpp:306-310 + sort
-         * MPC.cpp:221-235: shifted, never wrapped), bitonic-sorted in LDS
-         * over the velocity/force arrays, which are written back above */
-        __syncthreads();
-        uint64_t* keys = reinterpret_cast<uint64_t*>(S.vx);
+        /* monomer cells (MPC::shiftParticles MPC.cpp:306-310 + sort
+         * MPC.cpp:221-235: shifted, never wrapped) */
         const double sh[3] = {ctl->shift[0], ctl->shift[1], ctl->shift[2]};
-        int np2 = 1;
-        while (np2 < N) np2 <<= 1;
-        for (int i = tid; i < np2; i += NT) keys[i] = ~0ull;
-        __syncthreads();
         for (int i = tid; i < N; i += NT) {
             const uint32_t ci = cell_index(rep.box, S.x[i] + sh[0], S.y[i] + sh[1], S.z[i] + sh[2]);
             if (ci >= rep.ncells) atomicOr(&ctl->status, ST_CELL_OOB);
-            const uint64_t g = ci < rep.ncells ? rep.cell_base + ci : 0xFFFFFFFFull;
-            keys[i] = (g << 32) | (uint32_t)i;
+            d.mcell[mb + i] = ci < rep.ncells ? (uint32_t)rep.cell_base + ci : ~0u;
         }
-        __syncthreads();
-        for (int kk = 2; kk <= np2; kk <<= 1) {
-            for (int j = kk >> 1; j > 0; j >>= 1) {
-                for (int i = tid; i < np2; i += NT) {
-                    const int ixj = i ^ j;
-                    if (ixj > i) {
-                        const uint64_t a = keys[i], b2 = keys[ixj];
-                        const bool up = (i & kk) == 0;
-                        if ((a > b2) == up) { keys[i] = b2; keys[ixj] = a; }
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        for (int i = tid; i < N; i += NT) d.msorted[mb + i] = keys[i];
     }
 }
 
@@ -889,13 +918,15 @@ __global__ __launch_bounds__(256) void k_collide(DevState d) {
     __shared__ float s_vx[CAP], s_vy[CAP], s_vz[CAP];
     __shared__ uint8_t s_lc[CAP];
     __shared__ int8_t s_cy[CAP];
-    __shared__ uint32_t s_mr[2];
+    constexpr int MCAP = 2 * TC + 64;  /* monomers per tile (unit cells hold ~1 at most) */
+    __shared__ uint32_t s_ml[MCAP];    /* local cell << 16 | monomer */
+    __shared__ uint32_t s_mn;
 
     const int tid = threadIdx.x;
     const uint32_t lt = xcd_remap(blockIdx.x, gridDim.x);
     const int r = find_rep(d.tile_base, d.n_rep, lt);
     const RepDev& rep = d.reps[r];
-    const RepCtl* ctl = d.ctl + r;
+    RepCtl* ctl = d.ctl + r;
     const uint32_t c_lo = (lt - d.tile_base[r]) * TC;
     const uint32_t nc = min((uint32_t)TC, rep.ncells - c_lo);
     const uint32_t gc0 = (uint32_t)rep.cell_base + c_lo;
@@ -911,21 +942,22 @@ __global__ __launch_bounds__(256) void k_collide(DevState d) {
         s_px[i] = s_py[i] = s_pz[i] = s_e[i] = s_mm[i] = 0ull;
         s_nm[i] = 0u;
     }
-    if (tid == 0) {
-        /* monomers of this tile: binary search in the sorted keys */
-        const uint64_t* ms = d.msorted + rep.mono_base;
-        uint32_t lo = 0, hi = rep.n_mono;
-        const uint64_t klo = (uint64_t)gc0 << 32;
-        while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (ms[mid] < klo) lo = mid + 1; else hi = mid; }
-        uint32_t lo2 = lo, hi2 = rep.n_mono;
-        const uint64_t khi = (uint64_t)(gc0 + nc) << 32;
-        while (lo2 < hi2) { const uint32_t mid = (lo2 + hi2) >> 1; if (ms[mid] < khi) lo2 = mid + 1; else hi2 = mid; }
-        s_mr[0] = lo; s_mr[1] = lo2;
+    if (tid == 0) s_mn = 0u;
+    __syncthreads();
+    /* the replica's monomers in this tile's cells (any order: the sums
+     * are fixed point) */
+    for (uint32_t j = tid; j < rep.n_mono; j += NT) {
+        const uint32_t rel = d.mcell[rep.mono_base + j] - gc0;
+        if (rel < nc) {
+            const uint32_t slot = atomicAdd(&s_mn, 1u);
+            if (slot < (uint32_t)MCAP) { s_ml[slot] = (rel << 16) | j; atomicAdd(&s_nm[rel], 1u); }
+        }
     }
     __syncthreads();
     const uint32_t p0 = s_off[0], np = s_off[nc] - p0;
     const bool fast = np <= (uint32_t)CAP;
-    const uint32_t m0 = s_mr[0], m1 = s_mr[1];
+    const uint32_t nmt = s_mn < (uint32_t)MCAP ? s_mn : (uint32_t)MCAP;
+    if (tid == 0 && s_mn > (uint32_t)MCAP) atomicOr(&ctl->status, ST_TILE_MONO);
     const uint32_t k = ctl->k_cur;
     const bool inj = ctl->inj_active != 0;
 
@@ -933,11 +965,7 @@ __global__ __launch_bounds__(256) void k_collide(DevState d) {
      * issued before the particle loads so the waves overlap them */
     for (uint32_t ci = tid; ci < nc; ci += NT) {
         const uint32_t cell = c_lo + ci;
-        uint32_t nm = 0;
-        for (uint32_t j = m0; j < m1; ++j)
-            nm += (uint32_t)(d.msorted[rep.mono_base + j] >> 32) == gc0 + ci;
-        s_nm[ci] = nm;
-        const uint32_t n = (s_off[ci + 1] - s_off[ci]) + nm;
+        const uint32_t n = (s_off[ci + 1] - s_off[ci]) + s_nm[ci];
         double g = 0.0;
         if (n >= 2)
             g = inj ? d.inj_gamma[gc0 + ci] : gamma_mt(rep.seed, rep.replica, cell, k, 1.5 * (n - 1), rep.kT);
@@ -999,10 +1027,10 @@ __global__ __launch_bounds__(256) void k_collide(DevState d) {
         atomicAdd(&s_py[lc], to_fix((double)fy));
         atomicAdd(&s_pz[lc], to_fix((double)fz));
     }
-    for (uint32_t j = m0 + tid; j < m1; j += NT) {
-        const uint64_t key = d.msorted[rep.mono_base + j];
-        const uint32_t lc = (uint32_t)(key >> 32) - gc0;
-        const uint32_t mi = rep.mono_base + (uint32_t)key;
+    for (uint32_t j = tid; j < nmt; j += NT) {
+        const uint32_t e = s_ml[j];
+        const uint32_t lc = e >> 16;
+        const uint32_t mi = rep.mono_base + (e & 0xFFFFu);
         const double mm = d.mmass[mi];
         atomicAdd(&s_px[lc], to_fix(d.mvel[3 * mi] * mm));
         atomicAdd(&s_py[lc], to_fix(d.mvel[3 * mi + 1] * mm));
@@ -1036,10 +1064,10 @@ __global__ __launch_bounds__(256) void k_collide(DevState d) {
         const double dx = vx - s_cmv[lc][0], dy = vy - s_cmv[lc][1], dz = vz - s_cmv[lc][2];
         atomicAdd(&s_e[lc], to_fix((dx * dx + dy * dy) + dz * dz));
     }
-    for (uint32_t j = m0 + tid; j < m1; j += NT) {
-        const uint64_t key = d.msorted[rep.mono_base + j];
-        const uint32_t lc = (uint32_t)(key >> 32) - gc0;
-        const uint32_t mi = rep.mono_base + (uint32_t)key;
+    for (uint32_t j = tid; j < nmt; j += NT) {
+        const uint32_t e = s_ml[j];
+        const uint32_t lc = e >> 16;
+        const uint32_t mi = rep.mono_base + (e & 0xFFFFu);
         const double dx = d.mvel[3 * mi] - s_cmv[lc][0];
         const double dy = d.mvel[3 * mi + 1] - s_cmv[lc][1];
         const double dz = d.mvel[3 * mi + 2] - s_cmv[lc][2];
@@ -1079,10 +1107,10 @@ __global__ __launch_bounds__(256) void k_collide(DevState d) {
         vx += off;
         VX[p0 + q] = (float)vx; VY[p0 + q] = (float)vy; VZ[p0 + q] = (float)vz;
     }
-    for (uint32_t j = m0 + tid; j < m1; j += NT) {
-        const uint64_t key = d.msorted[rep.mono_base + j];
-        const uint32_t lc = (uint32_t)(key >> 32) - gc0;
-        const uint32_t mi = rep.mono_base + (uint32_t)key;
+    for (uint32_t j = tid; j < nmt; j += NT) {
+        const uint32_t e = s_ml[j];
+        const uint32_t lc = e >> 16;
+        const uint32_t mi = rep.mono_base + (e & 0xFFFFu);
         double vx = d.mvel[3 * mi], vy = d.mvel[3 * mi + 1], vz = d.mvel[3 * mi + 2];
         apply(lc, vx, vy, vz);
         d.mvel[3 * mi] = vx; d.mvel[3 * mi + 1] = vy; d.mvel[3 * mi + 2] = vz;
@@ -1159,6 +1187,7 @@ __global__ __launch_bounds__(64) void k_fluid_mean(DevState d) {
 template <int TC>
 __global__ __launch_bounds__(256) void k_temperature(DevState d, int replica) {
     __shared__ double sred[4];
+    __shared__ unsigned long long s_q[4][TC];  /* monomer sum m v (3) and m per cell, fixed point */
     const uint32_t lt = d.tile_base[replica] + blockIdx.x;
     const RepDev& rep = d.reps[replica];
     const uint32_t c_lo = blockIdx.x * TC;
@@ -1166,22 +1195,28 @@ __global__ __launch_bounds__(256) void k_temperature(DevState d, int replica) {
     const uint32_t nc = min((uint32_t)TC, rep.ncells - c_lo);
     const int cur = d.cur;
     const float *VX = d.vx[cur], *VY = d.vy[cur], *VZ = d.vz[cur];
-    const uint64_t* ms = d.msorted + rep.mono_base;
+    for (uint32_t i = threadIdx.x; i < 4 * TC; i += 256) (&s_q[0][0])[i] = 0ull;
+    __syncthreads();
+    /* monomers in the cells of the last collision */
+    for (uint32_t j = threadIdx.x; j < rep.n_mono; j += 256) {
+        const uint32_t rel = d.mcell[rep.mono_base + j] - gc0;
+        if (rel < nc) {
+            const uint32_t mi = rep.mono_base + j;
+            const double mm = d.mmass[mi];
+            atomicAdd(&s_q[0][rel], to_fix(d.mvel[3 * mi] * mm));
+            atomicAdd(&s_q[1][rel], to_fix(d.mvel[3 * mi + 1] * mm));
+            atomicAdd(&s_q[2][rel], to_fix(d.mvel[3 * mi + 2] * mm));
+            atomicAdd(&s_q[3][rel], to_fix(mm));
+        }
+    }
+    __syncthreads();
     double ek = 0.0, filled = 0.0;
     for (uint32_t ci = threadIdx.x; ci < nc; ci += 256) {
         const uint32_t a = d.offsets[gc0 + ci], e = d.offsets[gc0 + ci + 1];
         double s0 = 0.0, s1 = 0.0, s2 = 0.0, M = 0.0;
         for (uint32_t p = a; p < e; ++p) { s0 += VX[p]; s1 += VY[p]; s2 += VZ[p]; M += 1.0; }
-        /* monomers of this cell from the last collision's sorted keys */
-        const uint64_t key = (uint64_t)(gc0 + ci) << 32;
-        uint32_t lo = 0, hi = rep.n_mono;
-        while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (ms[mid] < key) lo = mid + 1; else hi = mid; }
-        for (uint32_t j = lo; j < rep.n_mono && (ms[j] >> 32) == (uint64_t)(gc0 + ci); ++j) {
-            const uint32_t mi = rep.mono_base + (uint32_t)ms[j];
-            const double mm = d.mmass[mi];
-            s0 += d.mvel[3 * mi] * mm; s1 += d.mvel[3 * mi + 1] * mm; s2 += d.mvel[3 * mi + 2] * mm;
-            M += mm;
-        }
+        s0 += from_fix(s_q[0][ci]); s1 += from_fix(s_q[1][ci]); s2 += from_fix(s_q[2][ci]);
+        M += from_fix(s_q[3][ci]);
         if (M > 0) { s0 /= M; s1 /= M; s2 /= M; }
         double e2 = 0.0;
         for (uint32_t p = a; p < e; ++p) {
