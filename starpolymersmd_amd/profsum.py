@@ -1,0 +1,120 @@
+"""Summarise rocprofv3 CSV output of the MPC-MD library as markdown.
+
+    python -m starpolymersmd_amd.profsum --stats <dir>/<name>_kernel_stats.csv \
+        [--fetch <fetch counter_collection.csv>] [--write <write counter_collection.csv>] \
+        [--copy-fetch <csv> --copy-write <csv> --copy-bytes N] [--particles N]
+
+Kernel statistics come from `rocprofv3 --kernel-trace --stats --output-format
+csv`; byte counters from `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE`
+(KiB per dispatch).  A plain device copy of known size, profiled the same
+way, gives the factor between the counters and real bytes.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+from collections import defaultdict
+
+PIPELINE = ("k_bin", "k_scan", "k_scatter", "k_collide")
+
+
+def short(name: str) -> str:
+    n = name.split("(")[0]
+    for k in ("k_md", "k_prologue", "k_bin", "k_scan_redo", "k_scan", "k_scatter", "k_collide",
+              "k_fluid_init", "k_fluid_mean", "k_temperature"):
+        if k in n:
+            return k
+    return n[:48]
+
+
+def load_stats(path):
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            rows.append((short(r["Name"]), int(r["Calls"]), float(r["TotalDurationNs"]), float(r["AverageNs"]),
+                         float(r["Percentage"])))
+    return rows
+
+
+def load_counter(path, counter):
+    """mean counter value per kernel (KiB per dispatch), dispatch count"""
+    per = defaultdict(list)
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter:
+                continue
+            per[short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    return {k: (sum(v) / len(v), len(v)) for k, v in per.items()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--stats", required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--copy-fetch")
+    ap.add_argument("--copy-write")
+    ap.add_argument("--copy-bytes", type=float, default=0.0)
+    ap.add_argument("--particles", type=int, default=0)
+    ap.add_argument("--bytes-per-particle", type=float, default=48.0)
+    a = ap.parse_args()
+
+    stats = load_stats(a.stats)
+    merged = defaultdict(lambda: [0, 0.0])
+    for k, n, tot, _, _ in stats:
+        merged[k][0] += n
+        merged[k][1] += tot
+    print("| kernel | calls | total ms | avg us |")
+    print("|---|---:|---:|---:|")
+    for k, (n, tot) in sorted(merged.items(), key=lambda kv: -kv[1][1]):
+        print(f"| {k} | {n} | {tot / 1e6:.3f} | {tot / n / 1e3:.2f} |")
+    # per collision: k_collide runs once per collision; k_bin and k_scatter
+    # run twice (speculative pass and the redo pass, which exits at once
+    # unless a recentring forced it), so totals are divided by collisions
+    ncoll = merged["k_collide"][0] if "k_collide" in merged else 1
+    print(f"\ncollisions profiled: {ncoll}")
+    print("\n| kernel | us per collision |")
+    print("|---|---:|")
+    for k in ("k_prologue", "k_md", "k_bin", "k_scan", "k_scan_redo", "k_scatter", "k_collide"):
+        if k in merged:
+            print(f"| {k} | {merged[k][1] / ncoll / 1e3:.2f} |")
+    pipe_us = sum(merged[k][1] / ncoll / 1e3 for k in PIPELINE if k in merged)
+    print(f"\ncollision pipeline ({' + '.join(PIPELINE)}): {pipe_us:.2f} us per collision")
+    if a.particles:
+        alg = a.particles * a.bytes_per_particle
+        print(f"algorithmic bytes per collision: {alg:.0f} ({a.bytes_per_particle:g} B x {a.particles} particles)")
+        print(f"achieved (algorithmic): {alg / (pipe_us * 1e-6) / 1e9:.1f} GB/s")
+
+    if a.fetch and a.write:
+        fs, ws = 1.0, 1.0
+        if a.copy_fetch and a.copy_write and a.copy_bytes:
+            cf = max(load_counter(a.copy_fetch, "FETCH_SIZE").values(), key=lambda t: t[0])[0] * 1024
+            cw = max(load_counter(a.copy_write, "WRITE_SIZE").values(), key=lambda t: t[0])[0] * 1024
+            fs, ws = a.copy_bytes / cf, a.copy_bytes / cw
+            print(f"\ncopy calibration ({a.copy_bytes:.0f} B copied): FETCH_SIZE reads {cf:.0f} B -> scale {fs:.3f}; "
+                  f"WRITE_SIZE reads {cw:.0f} B -> scale {ws:.3f}")
+        f = load_counter(a.fetch, "FETCH_SIZE")
+        w = load_counter(a.write, "WRITE_SIZE")
+        nf = f.get("k_collide", (0.0, 1))[1]
+        print("\nper collision (dispatch means x dispatches / collisions):")
+        print("\n| kernel | dispatches | FETCH KiB | WRITE KiB | scaled read MB | scaled write MB |")
+        print("|---|---:|---:|---:|---:|---:|")
+        tot = 0.0
+        for k in sorted(set(f) | set(w)):
+            fk, nk = f.get(k, (0.0, 0))
+            wk, _ = w.get(k, (0.0, 0))
+            per = nk / nf if k in PIPELINE else 1.0
+            rb, wb = fk * 1024 * fs * per, wk * 1024 * ws * per
+            if k in PIPELINE:
+                tot += rb + wb
+            print(f"| {k} | {nk} | {fk * per:.0f} | {wk * per:.0f} | {rb / 1e6:.1f} | {wb / 1e6:.1f} |")
+        print(f"\nHBM traffic of the collision pipeline: {tot:.0f} B per collision")
+        if pipe_us:
+            print(f"measured bandwidth of the pipeline: {tot / (pipe_us * 1e-6) / 1e9:.1f} GB/s")
+        if a.particles:
+            print(f"amplification over algorithmic bytes: {tot / (a.particles * a.bytes_per_particle):.2f}")
+        print(f"traffic_bytes_per_collision: {tot:.0f}")
+
+
+if __name__ == "__main__":
+    main()
