@@ -1,6 +1,6 @@
 """Build recipe: libspmd.so (HIP, gfx950) in-tree, plus the CPU oracle.
 
-    python -m starpolymersmd_amd.build          # library + oracle
+    python -m starpolymersmd_amd.build          # library, C++ facade driver, oracle
 """
 from __future__ import annotations
 
@@ -47,6 +47,17 @@ def build_library():
     return out
 
 
+def build_facade():
+    """The reference-style C++ driver over the facade, linked to libspmd.so."""
+    fdir = os.path.join(HERE, "facade")
+    out = os.path.join(fdir, "star_polymers_run")
+    gxx = shutil.which("g++") or "g++"
+    _run([gxx, "-O2", "-std=c++17", "-Wall", os.path.join(fdir, "star_polymers_run.cpp"), "-o", out + ".tmp",
+          "-L" + HERE, "-lspmd", "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle():
     """The FP64 oracle and, when the reference tree is present, its harness."""
     _run(["make", "-C", os.path.join(ROOT, "oracle"), "all"])
@@ -56,6 +67,7 @@ def build_oracle():
 
 def build_all():
     build_library()
+    build_facade()
     build_oracle()
 
 

@@ -1,0 +1,225 @@
+/*
+ * spmd_facade.h -- the reference's C++ hot-path classes, re-declared on top
+ * of the C ABI of libspmd.so (include/spmd.h).
+ *
+ * Names, constructor arguments and call order follow
+ *   Box              star-polymers/lib/Box.h:30-121      (Box.cpp:4-46)
+ *   MPC              star-polymers/lib/Hydrodynamics/MPC.h:25-122
+ *   Thermostat_None  star-polymers/lib/Thermostats/Thermostat_None.h:14
+ * so a driver written like star-polymers/src/star-polymers.cpp:351-405
+ * compiles against it:
+ *
+ *   Box box(Lx, Ly, Lz, Temperature, Lambda);
+ *   MPC hydrodynamics{box, Temperature, N_c, Shear, 100, false};
+ *   box.add_star(A, B, Arms, Mass);
+ *   Thermostat_None thermostat{box, dt};
+ *   hydrodynamics.initialize();
+ *   for (long n = 0; n <= Steps; ++n) { thermostat.propagate(); hydrodynamics.step(n, 100 * dt); }
+ *
+ * Differences, each forced by the device owning the state:
+ *  - Box::Molecules and MPC::Fluid are host mirrors refreshed by
+ *    Box::sync_molecules() / MPC::sync_fluid() (one device-to-host copy);
+ *    writes go back through Box::upload_molecules() / MPC::upload_fluid().
+ *  - Errors the reference turns into raise(SIGSEGV) (MPC.cpp:216-219,
+ *    Box.cpp:133-142) throw spmd::Error with the library's status code.
+ *  - The random stream is Philox4x32-10 keyed by Box::Seed instead of the
+ *    global mt19937_64(1) of Rand.cpp:6.
+ */
+#ifndef SPMD_FACADE_H
+#define SPMD_FACADE_H
+
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/spmd.h"
+
+namespace spmd {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int rc, const spmd_sys* h) {
+    if (rc == SPMD_OK) return;
+    std::string m = spmd_error_string(rc);
+    if (h) m += std::string(": ") + spmd_last_error(h);
+    throw Error(rc, m);
+}
+
+using Vector3d = std::array<double, 3>;
+
+/* Particle.h:17-82 without the Eigen types */
+struct MDParticle {
+    Vector3d Position{}, Velocity{}, Force{};
+    double Mass{};
+};
+struct Molecule {
+    std::vector<MDParticle> Monomers;
+    unsigned NumberOfMonomers{};
+};
+struct MPCParticle {
+    std::array<float, 3> Position{}, Velocity{};
+};
+
+class Box {
+    friend class MPC;
+    friend class Thermostat_None;
+    std::shared_ptr<spmd_sys> h_;
+    spmd_sys* handle() const {
+        if (!h_) throw Error(SPMD_ERR_STATE, "construct the MPC on this Box first (star-polymers.cpp:353)");
+        return h_.get();
+    }
+
+public:
+    double Temperature;
+    double Lambda;
+    std::array<int, 3> BoxSize;
+    uint64_t Seed = 1;
+    double DeltaT = 0.01;
+    std::vector<Molecule> Molecules;
+
+    /* Box::Box(Lx, Ly, Lz, T, Lambda) Box.cpp:4-27 (odd sides are rounded
+     * up to even by the library, Box.cpp:14-16) */
+    Box(int Lx, int Ly, int Lz, double temperature, double lambda)
+        : Temperature(temperature), Lambda(lambda), BoxSize{Lx + (Lx & 1), Ly + (Ly & 1), Lz + (Lz & 1)} {}
+
+    /* Box::add_star Box.cpp:39-46 */
+    void add_star(unsigned A, unsigned B, unsigned Arms, double Mass, double Bond = 1.01, double AnchorBond = 2.0) {
+        check(spmd_add_star(handle(), 0, A, B, Arms, Mass, Bond, AnchorBond), h_.get());
+        sync_molecules();
+    }
+    /* seeded random-coil arms (no reference counterpart) */
+    void add_star_coil(unsigned A, unsigned B, unsigned Arms, double Mass, double Bond = 1.01,
+                       double AnchorBond = 2.0) {
+        check(spmd_add_star_coil(handle(), 0, A, B, Arms, Mass, Bond, AnchorBond), h_.get());
+        sync_molecules();
+    }
+    unsigned numberOfMonomers() const {
+        uint32_t n = 0;
+        check(spmd_num_monomers(handle(), 0, &n), h_.get());
+        return n;
+    }
+    /* Molecules[0] from the device */
+    void sync_molecules() {
+        const unsigned n = numberOfMonomers();
+        std::vector<double> p(3 * n), v(3 * n), f(3 * n);
+        if (n) check(spmd_get_monomers(handle(), 0, p.data(), v.data(), f.data()), h_.get());
+        Molecules.assign(n ? 1 : 0, Molecule{});
+        if (!n) return;
+        Molecule& m = Molecules[0];
+        m.NumberOfMonomers = n;
+        m.Monomers.resize(n);
+        for (unsigned i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) {
+                m.Monomers[i].Position[c] = p[3 * i + c];
+                m.Monomers[i].Velocity[c] = v[3 * i + c];
+                m.Monomers[i].Force[c] = f[3 * i + c];
+            }
+    }
+    /* Molecules[0] positions and velocities to the device */
+    void upload_molecules() {
+        if (Molecules.empty()) return;
+        const Molecule& m = Molecules[0];
+        std::vector<double> p(3 * m.Monomers.size()), v(3 * m.Monomers.size());
+        for (size_t i = 0; i < m.Monomers.size(); ++i)
+            for (int c = 0; c < 3; ++c) { p[3 * i + c] = m.Monomers[i].Position[c]; v[3 * i + c] = m.Monomers[i].Velocity[c]; }
+        check(spmd_set_monomers(handle(), 0, p.data(), v.data()), h_.get());
+    }
+    /* Molecule::Epot after the last propagate(true) */
+    double epot() const {
+        double e = 0.0;
+        check(spmd_epot(handle(), 0, &e), h_.get());
+        return e;
+    }
+};
+
+/* Hydrodynamics.h:7-22 interface subset used by the drivers */
+class MPC {
+    Box& SimBox;
+
+public:
+    std::vector<MPCParticle> Fluid;
+
+    /* MPC::MPC(Box&, T, N_c, aShear, step, angular_mom) MPC.h:44 */
+    MPC(Box& box, double T, int N_c, double aShear = 0., unsigned step = 1, bool angular_mom = false) : SimBox(box) {
+        spmd_params p{};
+        for (int c = 0; c < 3; ++c) p.L[c] = box.BoxSize[c];
+        p.kT = T;
+        p.n_c = N_c;
+        p.shear = aShear;
+        p.step_update = step;
+        p.alpha_deg = 130.0;
+        p.lambda = box.Lambda;
+        p.md_dt = box.DeltaT;
+        p.seed = box.Seed;
+        p.angular_momentum = angular_mom ? 1 : 0;
+        spmd_sys* h = nullptr;
+        check(spmd_create(&p, &h), nullptr);
+        box.h_ = std::shared_ptr<spmd_sys>(h, [](spmd_sys* q) { spmd_destroy(q); });
+    }
+    /* MPC::initialize() MPC.cpp:34-55 */
+    void initialize() { check(spmd_fluid_initialize(SimBox.handle()), SimBox.handle()); }
+    /* MPC::step(t, dt) MPC.cpp:116-142 */
+    void step(const long int& t, const double& dt) { check(spmd_mpc_step(SimBox.handle(), t, dt), SimBox.handle()); }
+    /* MPC::calculateCurrentTemperature MPC.cpp:405-409 */
+    double calculateCurrentTemperature() {
+        double T = 0.0;
+        check(spmd_temperature(SimBox.handle(), 0, &T), SimBox.handle());
+        return T;
+    }
+    /* MPC::NumberOfParticles MPC.h:91 */
+    unsigned NumberOfParticles() const {
+        uint64_t n = 0;
+        check(spmd_num_fluid(SimBox.handle(), 0, &n), SimBox.handle());
+        return (unsigned)n;
+    }
+    void sync_fluid() {
+        const unsigned n = NumberOfParticles();
+        std::vector<float> p(3ull * n), v(3ull * n);
+        check(spmd_get_fluid(SimBox.handle(), 0, p.data(), v.data(), nullptr), SimBox.handle());
+        Fluid.resize(n);
+        for (unsigned i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) { Fluid[i].Position[c] = p[3ull * i + c]; Fluid[i].Velocity[c] = v[3ull * i + c]; }
+    }
+    void upload_fluid() {
+        std::vector<float> p(3 * Fluid.size()), v(3 * Fluid.size());
+        for (size_t i = 0; i < Fluid.size(); ++i)
+            for (int c = 0; c < 3; ++c) { p[3 * i + c] = Fluid[i].Position[c]; v[3 * i + c] = Fluid[i].Velocity[c]; }
+        check(spmd_set_fluid(SimBox.handle(), 0, p.data(), v.data(), Fluid.size()), SimBox.handle());
+    }
+    /* the star-polymers.cpp:412-536 loop body n times from t0, replayed from a HIP graph */
+    void run(long t0, long n, double dt) { check(spmd_run(SimBox.handle(), t0, n, dt), SimBox.handle()); }
+    void sync() { check(spmd_sync(SimBox.handle()), SimBox.handle()); }
+};
+
+/* Thermostat_None (Thermostat_None.h:14, Thermostat_None.cpp:5-38) */
+class Thermostat_None {
+    Box& SimBox;
+    double DeltaT;
+
+public:
+    Thermostat_None(Box& box, double dt, bool gaussian = false) : SimBox(box), DeltaT(dt) {
+        if (gaussian) throw Error(SPMD_ERR_UNSUPPORTED, "Gaussian chains are not on this path");
+        check(spmd_set_md_dt(box.handle(), dt), box.handle());
+        check(spmd_md_init(box.handle()), box.handle());
+    }
+    void propagate(bool calc_epot = false) {
+        check(spmd_md_propagate(SimBox.handle(), calc_epot ? 1 : 0), SimBox.handle());
+    }
+    double dtime() const { return DeltaT; }
+    void dtime(double& delta_time) {
+        DeltaT = delta_time;
+        check(spmd_set_md_dt(SimBox.handle(), delta_time), SimBox.handle());
+    }
+    std::string name() const { return "None"; }
+    std::string info() const { return "velocity Verlet without thermostat (solvent: MPC)"; }
+};
+
+}  // namespace spmd
+
+#endif
