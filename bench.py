@@ -126,9 +126,9 @@ def main():
     g.run(t, a.profile_steps * S, h)
     g.sync()
     t += a.profile_steps * S
-    prof = {k: g.profile_read(k)[0] for k in ("md", "bin", "scan", "scatter", "collide")}
+    prof = {k: g.profile_read(k)[0] for k in ("md", "bin", "scan", "perm", "collide")}
     g.profile(False)
-    pipe_ms = prof["bin"] + prof["scan"] + prof["scatter"] + prof["collide"]
+    pipe_ms = prof["bin"] + prof["scan"] + prof["perm"] + prof["collide"]
     achieved = B_ALG * n_f / (pipe_ms * 1e-3) / 1e9
     T = g.temperature()
 
@@ -141,7 +141,7 @@ def main():
                    "solvent_particles": n_f, "step": "one collision interval", "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "collision pipeline k_bin+k_scan+k_scatter+k_collide",
+                     "kernel": "collision pipeline k_bin+k_scan+k_perm+k_collide",
                      "alg_bytes_per_launch": B_ALG * n_f, "avg_ms": pipe_ms,
                      "per_kernel_ms": prof},
         "md_us_per_interval": prof["md"] * 1e3,

@@ -151,14 +151,14 @@ int spmd_epot(spmd_sys* s, uint32_t replica, double* epot);
 int spmd_md_stats(spmd_sys* s, uint32_t replica, uint32_t* n_rebuild, uint32_t* vl_entries);
 
 /* ---- tuning (no reference counterpart) ---- */
-/* particles per thread in the bin/scatter kernels (1,2,4,8) and cells per
+/* particles per thread in the bin/perm kernels (1,2,4,8) and cells per
  * collide tile (64,128,256); chosen by starpolymersmd_amd.autotune. */
 int spmd_set_tuning(spmd_sys* s, int ppt, int tile_cells);
 int spmd_get_tuning(const spmd_sys* s, int* ppt, int* tile_cells);
 
 /* ---- timing (device events on the handle's stream) ---- */
 /* average duration in ms of each kernel class over the launches since the
- * last reset: names "md", "bin", "scan", "scatter", "collide". */
+ * last reset: names "md", "bin", "scan", "perm", "collide", "prologue". */
 int spmd_profile_enable(spmd_sys* s, int on);
 int spmd_profile_read(spmd_sys* s, const char* name, double* avg_ms, int64_t* count);
 int spmd_profile_reset(spmd_sys* s);

@@ -30,7 +30,7 @@ struct ProfEvents {
     std::vector<hipEvent_t> ev;  /* start/stop pairs */
 };
 
-const char* kProfNames[6] = {"md", "bin", "scan", "scatter", "collide", "prologue"};
+const char* kProfNames[6] = {"md", "bin", "scan", "perm", "collide", "prologue"};
 
 }  // namespace
 
@@ -241,7 +241,7 @@ hipError_t enqueue_collision_ops(spmd_sys* s, int n_md, int epot_last, int n_del
         if ((e = launch_scan(d, s->stream)) != hipSuccess) return e;
         prof_end(s, 2, s->stream);
         prof_begin(s, 3, s->stream);
-        if ((e = launch_scatter(d, s->stream, h, kSortNormal)) != hipSuccess) return e;
+        if ((e = launch_perm(d, s->stream, kSortNormal)) != hipSuccess) return e;
         prof_end(s, 3, s->stream);
     } else {
         if ((e = hipEventRecord(s->ev_fork, s->stream)) != hipSuccess) return e;
@@ -257,15 +257,15 @@ hipError_t enqueue_collision_ops(spmd_sys* s, int n_md, int epot_last, int n_del
         if ((e = launch_scan(d, s->stream)) != hipSuccess) return e;
         prof_end(s, 2, s->stream);
         prof_begin(s, 3, s->stream);
-        if ((e = launch_scatter(d, s->stream, h, kSortSpec)) != hipSuccess) return e;
+        if ((e = launch_perm(d, s->stream, kSortSpec)) != hipSuccess) return e;
         prof_end(s, 3, s->stream);
         if ((e = hipStreamWaitEvent(s->stream, s->ev_join, 0)) != hipSuccess) return e;
         if ((e = launch_bin(d, s->stream, h, kSortRedo)) != hipSuccess) return e;
         if ((e = launch_scan_redo(d, s->stream)) != hipSuccess) return e;
-        if ((e = launch_scatter(d, s->stream, h, kSortRedo)) != hipSuccess) return e;
+        if ((e = launch_perm(d, s->stream, kSortRedo)) != hipSuccess) return e;
     }
     prof_begin(s, 4, s->stream);
-    if ((e = launch_collide(d, s->stream)) != hipSuccess) return e;
+    if ((e = launch_collide(d, s->stream, h)) != hipSuccess) return e;
     prof_end(s, 4, s->stream);
     return hipSuccess;
 }
@@ -302,6 +302,8 @@ int check_status(spmd_sys* s) {
     if (st & ST_CELL_OOB) return fail(s, SPMD_ERR_CELL_OOB, std::string("cell index out of range; ") + buf);
     if (st & ST_VERLET_OOB) return fail(s, SPMD_ERR_VERLET_OOB, std::string("monomer outside Verlet grid; ") + buf);
     if (st & ST_VERLET_OVERFLOW) return fail(s, SPMD_ERR_VERLET_OVERFLOW, std::string("Verlet list overflow; ") + buf);
+    if (st & ST_SORT_MISMATCH)
+        return fail(s, SPMD_ERR_SCAN, std::string("collide found a particle outside its sorted cell; ") + buf);
     if (st & ST_TILE_MONO)
         return fail(s, SPMD_ERR_RANK_OVERFLOW, std::string("more monomers in one collide tile than it holds; ") + buf);
     return fail(s, SPMD_ERR_RANK_OVERFLOW, std::string("cell occupancy overflow; ") + buf);
@@ -554,6 +556,8 @@ int spmd_create_batch(const spmd_params* p, uint32_t n_rep, spmd_sys** out) {
         rc = rc ? rc : dalloc(s, &d.vz[b], nt);
     }
     rc = rc ? rc : dalloc(s, &d.rank, nt);
+    rc = rc ? rc : dalloc(s, &d.pcell, nt);
+    rc = rc ? rc : dalloc(s, &d.perm, nt);
     rc = rc ? rc : dalloc(s, &d.counts, ct + 16);
     rc = rc ? rc : dalloc(s, &d.offsets, ct + 16);
     rc = rc ? rc : dalloc(s, &d.scan_flags, d.n_scan_tiles);
@@ -595,7 +599,7 @@ int spmd_destroy(spmd_sys* s) {
         dfree(d.x[b]); dfree(d.y[b]); dfree(d.z[b]); dfree(d.vx[b]); dfree(d.vy[b]); dfree(d.vz[b]);
         dfree(d.ids[b]);
     }
-    dfree(d.rank); dfree(d.counts); dfree(d.offsets); dfree(d.scan_flags); dfree(d.scan_ticket);
+    dfree(d.rank); dfree(d.pcell); dfree(d.perm); dfree(d.counts); dfree(d.offsets); dfree(d.scan_flags); dfree(d.scan_ticket);
     dfree(d.mpos); dfree(d.mvel); dfree(d.mforce); dfree(d.mvpos); dfree(d.mmass); dfree(d.mbtype);
     dfree(d.vl); dfree(d.bonds); dfree(d.mcell);
     dfree(d.inj_axis); dfree(d.inj_gamma); dfree(d.reps); dfree(d.ctl); dfree(d.pblk_base);

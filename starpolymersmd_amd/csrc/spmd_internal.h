@@ -41,6 +41,7 @@ enum : uint32_t {
     ST_RANK_OVERFLOW = 1u << 3,
     ST_SCAN_TIMEOUT = 1u << 4,
     ST_TILE_MONO = 1u << 5,      /* more monomers in a collide tile than its list holds */
+    ST_SORT_MISMATCH = 1u << 6,  /* collide recomputed a cell outside its tile (internal error) */
 };
 
 /* bond entry: partner index | anchor-FENE flag in bit 31 */
@@ -86,7 +87,9 @@ struct DevState {
     float *x[2], *y[2], *z[2], *vx[2], *vy[2], *vz[2];
     uint32_t* ids[2];      /* null unless tracking */
     int cur;
-    uint16_t* rank;
+    uint16_t* rank;        /* slot of a particle inside its new cell (k_bin) */
+    uint32_t* pcell;       /* global new cell of a particle, ~0u if none (k_bin) */
+    uint32_t* perm;        /* sorted slot -> particle index in the current buffer (k_perm) */
     uint32_t* counts;
     uint32_t* offsets;
     uint64_t* scan_flags;  /* per scan tile: status<<62 | value */
@@ -125,8 +128,8 @@ hipError_t launch_md(const DevState& d, hipStream_t st, int n_md, int epot_last,
 hipError_t launch_bin(const DevState& d, hipStream_t st, double h, int mode);
 hipError_t launch_scan(const DevState& d, hipStream_t st);
 hipError_t launch_scan_redo(const DevState& d, hipStream_t st);
-hipError_t launch_scatter(const DevState& d, hipStream_t st, double h, int mode);
-hipError_t launch_collide(const DevState& d, hipStream_t st);
+hipError_t launch_perm(const DevState& d, hipStream_t st, int mode);
+hipError_t launch_collide(const DevState& d, hipStream_t st, double h);
 hipError_t launch_fluid_init(const DevState& d, hipStream_t st);
 hipError_t launch_temperature(const DevState& d, hipStream_t st, int replica, uint32_t ntiles);
 

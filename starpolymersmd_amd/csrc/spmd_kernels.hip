@@ -9,18 +9,20 @@
  *                  the collision's Philox counter and grid shift
  *   k_md           (second stream) one workgroup per replica: the pending
  *                  velocity-Verlet substeps (Thermostat_None::propagate),
- *                  check_bounds, sorted monomer cell keys
+ *                  check_bounds, the monomers' cells
  *   k_bin          per solvent particle: recentre, stream, LE wrap, shifted
  *                  cell index, slot in the cell via run-aggregated atomics
  *   k_scan         single-pass exclusive scan of the cell counts
- *   k_scatter      per solvent particle: recompute the streamed state and
- *                  its cell, write it to its slot of the cell-sorted arrays
- *   (redo)         k_bin/k_scan_redo/k_scatter again for replicas whose MD
- *                  step recentred (bin and scatter ran beside k_md assuming
- *                  it would not); they return at once otherwise
- *   k_collide      per tile of cells: momentum and energy sums in LDS
+ *   k_perm         per solvent particle: perm[offset[cell] + slot] = index
+ *                  (4-byte scattered stores; the particle data stays put)
+ *   (redo)         k_bin/k_scan_redo/k_perm again for replicas whose MD
+ *                  step recentred (bin and perm ran beside k_md assuming it
+ *                  would not); they return at once otherwise
+ *   k_collide      per tile of cells: gather the tile's particles through
+ *                  perm, stream them again, momentum and energy sums in LDS
  *                  (64-bit fixed point, order independent), MBS thermostat,
- *                  SRD rotation, write-back
+ *                  SRD rotation; positions and velocities are written to
+ *                  the other buffer in cell order (coalesced)
  */
 #pragma clang fp contract(off)
 
@@ -732,9 +734,13 @@ __global__ __launch_bounds__(256) void k_bin(DevState d, double h, int mode) {
             ok = cell < rep.ncells;
             bad |= !ok;
         }
-        const uint32_t rank = wave_rank(d.counts, (uint32_t)rep.cell_base + cell, ok);
+        const uint32_t gcell = (uint32_t)rep.cell_base + cell;
+        const uint32_t rank = wave_rank(d.counts, gcell, ok);
         big |= rank > 0xFFFFu;
-        if (valid) d.rank[i] = (uint16_t)(rank > 0xFFFFu ? 0xFFFFu : rank);
+        if (valid) {
+            d.rank[i] = (uint16_t)(rank > 0xFFFFu ? 0xFFFFu : rank);
+            d.pcell[i] = ok ? gcell : ~0u;
+        }
     }
     if (bad) atomicOr(&d.ctl[r].status, ST_CELL_OOB);
     if (big) atomicOr(&d.ctl[r].status, ST_RANK_OVERFLOW);
@@ -861,19 +867,68 @@ __global__ __launch_bounds__(1024) void k_scan_redo(DevState d) {
     }
 }
 
-/* =========================================================== scatter */
+/* ============================================================== perm */
 
+/* Sorted slot of every particle: perm[offsets[cell] + rank] = index.  Only
+ * 4 bytes per particle move here; k_collide gathers the particle data. */
 template <int PPT>
-__global__ __launch_bounds__(256) void k_scatter(DevState d, double h, int mode) {
+__global__ __launch_bounds__(256) void k_perm(DevState d, int mode) {
     const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
     const int r = find_rep(d.pblk_base, d.n_rep, lb);
     const RepDev& rep = d.reps[r];
-    RepCtl c;
-    if (!sort_ctl(d, r, mode, c)) return;
-    Box b = rep.box;
-    b.delrx = c.delrx;
+    if (mode == kSortRedo && !d.ctl[r].redo) return;
     const uint64_t p0 = rep.part_base + (uint64_t)(lb - d.pblk_base[r]) * (256 * PPT);
     const uint64_t pend = rep.part_base + rep.n_fluid;
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const uint64_t i = p0 + (uint64_t)j * 256 + threadIdx.x;
+        if (i >= pend) continue;
+        const uint32_t cell = d.pcell[i];
+        if (cell == ~0u) continue;
+        const uint32_t dst = d.offsets[cell] + d.rank[i];
+        if (dst >= pend || dst < rep.part_base) continue;  /* cannot happen when bin and scan agree */
+        d.perm[dst] = (uint32_t)i;
+    }
+}
+
+/* =========================================================== collide */
+
+/* One workgroup per tile of TC consecutive cells of one replica.  The
+ * tile's particles are gathered from the current buffer through perm (their
+ * sorted slots are contiguous), streamed again exactly as k_bin streamed
+ * them, and written to the other buffer in cell order.  Up to CAP of them
+ * stay in registers between the passes (larger tiles gather again).  Per
+ * cell, in this order (MPC.cpp:130-137): CMV over fluid and monomers,
+ * relative kinetic energy, MBS scale sqrt(Gamma/E) (only n >= 2), then
+ * v <- CMV + s R (v - CMV).  Sums are 64-bit fixed point in LDS, so their
+ * value does not depend on the order the particles arrive in. */
+template <int TC, int CAP>
+__global__ __launch_bounds__(256) void k_collide(DevState d, double h) {
+    constexpr int NT = 256;
+    constexpr int KM = CAP / NT;       /* particles a thread keeps in registers */
+    constexpr int MCAP = 2 * TC + 64;  /* monomers per tile (unit cells hold ~1 at most) */
+    __shared__ uint32_t s_off[TC + 1];
+    __shared__ unsigned long long s_px[TC], s_py[TC], s_pz[TC], s_e[TC], s_mm[TC];
+    __shared__ uint32_t s_nm[TC];
+    __shared__ double s_cmv[TC][3];
+    __shared__ double s_g[TC];
+    __shared__ float s_R[TC][9];
+    __shared__ uint32_t s_ml[MCAP];    /* local cell << 16 | monomer */
+    __shared__ uint32_t s_mn;
+    __shared__ uint32_t s_bad;
+
+    const int tid = threadIdx.x;
+    const uint32_t lt = xcd_remap(blockIdx.x, gridDim.x);
+    const int r = find_rep(d.tile_base, d.n_rep, lt);
+    const RepDev& rep = d.reps[r];
+    RepCtl* ctl = d.ctl + r;
+    RepCtl cc;
+    sort_ctl(d, r, kSortNormal, cc);
+    Box b = rep.box;
+    b.delrx = cc.delrx;
+    const uint32_t c_lo = (lt - d.tile_base[r]) * TC;
+    const uint32_t nc = min((uint32_t)TC, rep.ncells - c_lo);
+    const uint32_t gc0 = (uint32_t)rep.cell_base + c_lo;
     const int cur = d.cur, alt = cur ^ 1;
     const float *X = d.x[cur], *Y = d.y[cur], *Z = d.z[cur];
     const float *VX = d.vx[cur], *VY = d.vy[cur], *VZ = d.vz[cur];
@@ -881,60 +936,6 @@ __global__ __launch_bounds__(256) void k_scatter(DevState d, double h, int mode)
     float *OVX = d.vx[alt], *OVY = d.vy[alt], *OVZ = d.vz[alt];
     const uint32_t* ids = d.ids[cur];
     uint32_t* oids = d.ids[alt];
-#pragma unroll
-    for (int j = 0; j < PPT; ++j) {
-        const uint64_t i = p0 + (uint64_t)j * 256 + threadIdx.x;
-        if (i >= pend) continue;
-        double x = X[i], y = Y[i], z = Z[i], vx = VX[i], vy = VY[i], vz = VZ[i];
-        recenter_stream(b, h, c, x, y, z, vx, vy, vz);
-        const uint32_t cell = shifted_cell(b, c, x, y, z, vx);
-        if (cell >= rep.ncells) continue;
-        const uint32_t dst = d.offsets[(uint32_t)rep.cell_base + cell] + d.rank[i];
-        if (dst >= pend) continue;  /* cannot happen when bin and scatter agree */
-        OX[dst] = (float)x; OY[dst] = (float)y; OZ[dst] = (float)z;
-        OVX[dst] = (float)vx; OVY[dst] = (float)vy; OVZ[dst] = (float)vz;
-        if (ids) oids[dst] = ids[i];
-    }
-}
-
-/* =========================================================== collide */
-
-/* One workgroup per tile of TC consecutive cells of one replica.  The
- * tile's particles are contiguous in the sorted arrays; up to CAP of them
- * are staged in LDS (larger tiles re-read global memory on each pass).
- * Per cell, in this order (MPC.cpp:130-137): CMV over fluid and monomers,
- * relative kinetic energy, MBS scale sqrt(Gamma/E) (only n >= 2), then
- * v <- CMV + s R (v - CMV).  Sums are 64-bit fixed point in LDS, so their
- * value does not depend on the order the particles arrive in. */
-template <int TC, int CAP>
-__global__ __launch_bounds__(256) void k_collide(DevState d) {
-    constexpr int NT = 256;
-    __shared__ uint32_t s_off[TC + 1];
-    __shared__ unsigned long long s_px[TC], s_py[TC], s_pz[TC], s_e[TC], s_mm[TC];
-    __shared__ uint32_t s_nm[TC];
-    __shared__ double s_cmv[TC][3];
-    __shared__ double s_g[TC];
-    __shared__ float s_R[TC][9];
-    __shared__ float s_vx[CAP], s_vy[CAP], s_vz[CAP];
-    __shared__ uint8_t s_lc[CAP];
-    __shared__ int8_t s_cy[CAP];
-    constexpr int MCAP = 2 * TC + 64;  /* monomers per tile (unit cells hold ~1 at most) */
-    __shared__ uint32_t s_ml[MCAP];    /* local cell << 16 | monomer */
-    __shared__ uint32_t s_mn;
-
-    const int tid = threadIdx.x;
-    const uint32_t lt = xcd_remap(blockIdx.x, gridDim.x);
-    const int r = find_rep(d.tile_base, d.n_rep, lt);
-    const RepDev& rep = d.reps[r];
-    RepCtl* ctl = d.ctl + r;
-    const uint32_t c_lo = (lt - d.tile_base[r]) * TC;
-    const uint32_t nc = min((uint32_t)TC, rep.ncells - c_lo);
-    const uint32_t gc0 = (uint32_t)rep.cell_base + c_lo;
-    const int buf = d.cur ^ 1; /* the scatter wrote here */
-    float *VX = d.vx[buf], *VY = d.vy[buf], *VZ = d.vz[buf];
-    const float* Y = d.y[buf];
-    const uint32_t Ly = (uint32_t)rep.box.iL[1], Lx = (uint32_t)rep.box.iL[0];
-    const bool shear_on = rep.box.shear_on != 0;
     const double shear = rep.box.shear, LyD = rep.box.L[1];
 
     for (uint32_t i = tid; i <= nc; i += NT) s_off[i] = d.offsets[gc0 + i];
@@ -942,7 +943,7 @@ __global__ __launch_bounds__(256) void k_collide(DevState d) {
         s_px[i] = s_py[i] = s_pz[i] = s_e[i] = s_mm[i] = 0ull;
         s_nm[i] = 0u;
     }
-    if (tid == 0) s_mn = 0u;
+    if (tid == 0) { s_mn = 0u; s_bad = 0u; }
     __syncthreads();
     /* the replica's monomers in this tile's cells (any order: the sums
      * are fixed point) */
@@ -955,7 +956,6 @@ __global__ __launch_bounds__(256) void k_collide(DevState d) {
     }
     __syncthreads();
     const uint32_t p0 = s_off[0], np = s_off[nc] - p0;
-    const bool fast = np <= (uint32_t)CAP;
     const uint32_t nmt = s_mn < (uint32_t)MCAP ? s_mn : (uint32_t)MCAP;
     if (tid == 0 && s_mn > (uint32_t)MCAP) atomicOr(&ctl->status, ST_TILE_MONO);
     const uint32_t k = ctl->k_cur;
@@ -985,45 +985,56 @@ __global__ __launch_bounds__(256) void k_collide(DevState d) {
         const double a0 = sqrt(1 - th * th) * cos(phi);
         const double a1 = sqrt(1 - th * th) * sin(phi);
         const double a2 = th;
-        const double cc = rep.c, ss = rep.s;
+        const double cs = rep.c, ss = rep.s;
         float* R = s_R[ci];
-        R[0] = (float)(a0 * a0 + (1 - a0 * a0) * cc);
-        R[1] = (float)(a0 * a1 * (1 - cc) - a2 * ss);
-        R[2] = (float)(a0 * a2 * (1 - cc) + a1 * ss);
-        R[3] = (float)(a0 * a1 * (1 - cc) + a2 * ss);
-        R[4] = (float)(a1 * a1 + (1 - a1 * a1) * cc);
-        R[5] = (float)(a1 * a2 * (1 - cc) - a0 * ss);
-        R[6] = (float)(a0 * a2 * (1 - cc) - a1 * ss);
-        R[7] = (float)(a1 * a2 * (1 - cc) + a0 * ss);
-        R[8] = (float)(a2 * a2 + (1 - a2 * a2) * cc);
+        R[0] = (float)(a0 * a0 + (1 - a0 * a0) * cs);
+        R[1] = (float)(a0 * a1 * (1 - cs) - a2 * ss);
+        R[2] = (float)(a0 * a2 * (1 - cs) + a1 * ss);
+        R[3] = (float)(a0 * a1 * (1 - cs) + a2 * ss);
+        R[4] = (float)(a1 * a1 + (1 - a1 * a1) * cs);
+        R[5] = (float)(a1 * a2 * (1 - cs) - a0 * ss);
+        R[6] = (float)(a0 * a2 * (1 - cs) - a1 * ss);
+        R[7] = (float)(a1 * a2 * (1 - cs) + a0 * ss);
+        R[8] = (float)(a2 * a2 + (1 - a2 * a2) * cs);
     }
 
-    /* last cell whose first particle is <= p0+q */
-    auto cell_of = [&](uint32_t q) -> uint32_t {
-        const uint32_t p = p0 + q;
-        uint32_t lo = 0, hi = nc;
-        while (hi - lo > 1) { const uint32_t mid = (lo + hi) >> 1; if (s_off[mid] <= p) lo = mid; else hi = mid; }
-        return lo;
+    /* slot q of the tile: its particle streamed (MPC::streaming + LEBC, as
+     * in k_bin), FP32 velocity in the unshifted frame, y image index cy of
+     * the shifted position (MPC::shiftParticles' wrap) and local cell; with
+     * write_pos the new position (and id) go to the other buffer */
+    auto gather = [&](uint32_t q, bool write_pos, uint32_t& lc, int& cy, float& fvx, float& fvy, float& fvz) {
+        const uint32_t src = d.perm[p0 + q];
+        double x = X[src], y = Y[src], z = Z[src], vx = VX[src], vy = VY[src], vz = VZ[src];
+        recenter_stream(b, h, cc, x, y, z, vx, vy, vz);
+        if (write_pos) {
+            OX[p0 + q] = (float)x; OY[p0 + q] = (float)y; OZ[p0 + q] = (float)z;
+            if (ids) oids[p0 + q] = ids[src];
+        }
+        fvx = (float)vx; fvy = (float)vy; fvz = (float)vz;
+        double px = x + cc.shift[0], py = y + cc.shift[1], pz = z + cc.shift[2], vimg = vx;
+        cy = (int)le_wrap(b, px, py, pz, vimg);
+        lc = cell_index(b, px, py, pz) - c_lo;
+        if (lc >= nc) { s_bad = 1u; lc = 0; }
     };
-    /* y image index of the shifted image: only cells of the bottom/top y
-     * layer can hold particles the +shift wrap moved across the face, and
-     * their unshifted y sits on the opposite side (valid for Ly >= 4) */
-    auto image_cy = [&](uint32_t q, uint32_t lc) -> int {
-        if (!shear_on) return 0;
-        const uint32_t iy = ((c_lo + lc) / Lx) % Ly;
-        if (iy == 0 && Y[p0 + q] > 0.0f) return 1;
-        if (iy == Ly - 1 && Y[p0 + q] < 0.0f) return -1;
-        return 0;
-    };
+    uint32_t rlc[KM];
+    int rcy[KM];
+    float rvx[KM], rvy[KM], rvz[KM];
 
     /* MPC::calculateCMV (MPC.cpp:314-322) */
-    for (uint32_t q = tid; q < np; q += NT) {
-        const uint32_t lc = cell_of(q);
-        const int cy = image_cy(q, lc);
-        const float fx = VX[p0 + q], fy = VY[p0 + q], fz = VZ[p0 + q];
-        if (fast) { s_vx[q] = fx; s_vy[q] = fy; s_vz[q] = fz; s_lc[q] = (uint8_t)lc; s_cy[q] = (int8_t)cy; }
-        const double vx = (double)fx - (double)cy * shear * LyD;
-        atomicAdd(&s_px[lc], to_fix(vx));
+#pragma unroll
+    for (int kk = 0; kk < KM; ++kk) {
+        const uint32_t q = tid + kk * NT;
+        if (q >= np) continue;
+        gather(q, true, rlc[kk], rcy[kk], rvx[kk], rvy[kk], rvz[kk]);
+        const uint32_t lc = rlc[kk];
+        atomicAdd(&s_px[lc], to_fix((double)rvx[kk] - (double)rcy[kk] * shear * LyD));
+        atomicAdd(&s_py[lc], to_fix((double)rvy[kk]));
+        atomicAdd(&s_pz[lc], to_fix((double)rvz[kk]));
+    }
+    for (uint32_t q = tid + CAP; q < np; q += NT) {
+        uint32_t lc; int cy; float fx, fy, fz;
+        gather(q, true, lc, cy, fx, fy, fz);
+        atomicAdd(&s_px[lc], to_fix((double)fx - (double)cy * shear * LyD));
         atomicAdd(&s_py[lc], to_fix((double)fy));
         atomicAdd(&s_pz[lc], to_fix((double)fz));
     }
@@ -1038,6 +1049,7 @@ __global__ __launch_bounds__(256) void k_collide(DevState d) {
         atomicAdd(&s_mm[lc], to_fix(mm));
     }
     __syncthreads();
+    if (tid == 0 && s_bad) atomicOr(&ctl->status, ST_SORT_MISMATCH);
     for (uint32_t ci = tid; ci < nc; ci += NT) {
         const double M = (double)(s_off[ci + 1] - s_off[ci]) + from_fix(s_mm[ci]);
         double c0 = from_fix(s_px[ci]), c1 = from_fix(s_py[ci]), c2 = from_fix(s_pz[ci]);
@@ -1046,23 +1058,19 @@ __global__ __launch_bounds__(256) void k_collide(DevState d) {
     }
     __syncthreads();
 
-    auto load = [&](uint32_t q, uint32_t& lc, int& cy, double& vx, double& vy, double& vz) {
-        if (fast) {
-            lc = s_lc[q]; cy = s_cy[q];
-            vx = s_vx[q]; vy = s_vy[q]; vz = s_vz[q];
-        } else {
-            lc = cell_of(q); cy = image_cy(q, lc);
-            vx = VX[p0 + q]; vy = VY[p0 + q]; vz = VZ[p0 + q];
-        }
-    };
-
     /* MPC::thermostat relative kinetic energy (MPC.cpp:277-279) */
-    for (uint32_t q = tid; q < np; q += NT) {
-        uint32_t lc; int cy; double vx, vy, vz;
-        load(q, lc, cy, vx, vy, vz);
-        vx -= (double)cy * shear * LyD;
-        const double dx = vx - s_cmv[lc][0], dy = vy - s_cmv[lc][1], dz = vz - s_cmv[lc][2];
+    auto add_e = [&](uint32_t lc, int cy, float fx, float fy, float fz) {
+        const double vx = (double)fx - (double)cy * shear * LyD;
+        const double dx = vx - s_cmv[lc][0], dy = (double)fy - s_cmv[lc][1], dz = (double)fz - s_cmv[lc][2];
         atomicAdd(&s_e[lc], to_fix((dx * dx + dy * dy) + dz * dz));
+    };
+#pragma unroll
+    for (int kk = 0; kk < KM; ++kk)
+        if (tid + kk * NT < np) add_e(rlc[kk], rcy[kk], rvx[kk], rvy[kk], rvz[kk]);
+    for (uint32_t q = tid + CAP; q < np; q += NT) {
+        uint32_t lc; int cy; float fx, fy, fz;
+        gather(q, false, lc, cy, fx, fy, fz);
+        add_e(lc, cy, fx, fy, fz);
     }
     for (uint32_t j = tid; j < nmt; j += NT) {
         const uint32_t e = s_ml[j];
@@ -1098,14 +1106,22 @@ __global__ __launch_bounds__(256) void k_collide(DevState d) {
         vy = c1 + (((double)R[3] * d0 + (double)R[4] * d1) + (double)R[5] * d2);
         vz = c2 + (((double)R[6] * d0 + (double)R[7] * d1) + (double)R[8] * d2);
     };
-    for (uint32_t q = tid; q < np; q += NT) {
-        uint32_t lc; int cy; double vx, vy, vz;
-        load(q, lc, cy, vx, vy, vz);
+    auto finish = [&](uint32_t q, uint32_t lc, int cy, float fx, float fy, float fz) {
         const double off = (double)cy * shear * LyD;
-        vx -= off;
+        double vx = (double)fx - off, vy = fy, vz = fz;
         apply(lc, vx, vy, vz);
         vx += off;
-        VX[p0 + q] = (float)vx; VY[p0 + q] = (float)vy; VZ[p0 + q] = (float)vz;
+        OVX[p0 + q] = (float)vx; OVY[p0 + q] = (float)vy; OVZ[p0 + q] = (float)vz;
+    };
+#pragma unroll
+    for (int kk = 0; kk < KM; ++kk) {
+        const uint32_t q = tid + kk * NT;
+        if (q < np) finish(q, rlc[kk], rcy[kk], rvx[kk], rvy[kk], rvz[kk]);
+    }
+    for (uint32_t q = tid + CAP; q < np; q += NT) {
+        uint32_t lc; int cy; float fx, fy, fz;
+        gather(q, false, lc, cy, fx, fy, fz);
+        finish(q, lc, cy, fx, fy, fz);
     }
     for (uint32_t j = tid; j < nmt; j += NT) {
         const uint32_t e = s_ml[j];
@@ -1279,25 +1295,25 @@ hipError_t launch_scan_redo(const DevState& d, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_scatter(const DevState& d, hipStream_t st, double h, int mode) {
+hipError_t launch_perm(const DevState& d, hipStream_t st, int mode) {
     if (d.n_pblk == 0) return hipSuccess;
     const dim3 g(d.n_pblk), b(256);
     switch (d.ppt) {
-    case 1: hipLaunchKernelGGL(k_scatter<1>, g, b, 0, st, d, h, mode); break;
-    case 2: hipLaunchKernelGGL(k_scatter<2>, g, b, 0, st, d, h, mode); break;
-    case 4: hipLaunchKernelGGL(k_scatter<4>, g, b, 0, st, d, h, mode); break;
-    case 8: hipLaunchKernelGGL(k_scatter<8>, g, b, 0, st, d, h, mode); break;
+    case 1: hipLaunchKernelGGL(k_perm<1>, g, b, 0, st, d, mode); break;
+    case 2: hipLaunchKernelGGL(k_perm<2>, g, b, 0, st, d, mode); break;
+    case 4: hipLaunchKernelGGL(k_perm<4>, g, b, 0, st, d, mode); break;
+    case 8: hipLaunchKernelGGL(k_perm<8>, g, b, 0, st, d, mode); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_collide(const DevState& d, hipStream_t st) {
+hipError_t launch_collide(const DevState& d, hipStream_t st, double h) {
     const dim3 g(d.n_tiles), b(256);
     switch (d.tile_cells) {
-    case 64: hipLaunchKernelGGL((k_collide<64, 896>), g, b, 0, st, d); break;
-    case 128: hipLaunchKernelGGL((k_collide<128, 1536>), g, b, 0, st, d); break;
-    case 256: hipLaunchKernelGGL((k_collide<256, 3072>), g, b, 0, st, d); break;
+    case 64: hipLaunchKernelGGL((k_collide<64, 768>), g, b, 0, st, d, h); break;
+    case 128: hipLaunchKernelGGL((k_collide<128, 1536>), g, b, 0, st, d, h); break;
+    case 256: hipLaunchKernelGGL((k_collide<256, 3072>), g, b, 0, st, d, h); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -1,4 +1,4 @@
-"""Parity of the HIP MPC step (bin/scan/scatter/collide + prologue) with the
+"""Parity of the HIP MPC step (bin/scan/perm/collide + prologue) with the
 FP64 oracle on identical inputs.
 
 Bar: cell indices bit-exact; positions and velocities after the step within
