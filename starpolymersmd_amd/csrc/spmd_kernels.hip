@@ -201,16 +201,18 @@ __device__ double block_sum_all(double v, double* scratch) {
     return t;
 }
 
-/* Forces of kMdBatch independent list entries per lane: half-list pairs
- * (Newton's third law) or bonds, added to both monomers' 64-bit fixed-point
- * LDS accumulators (the sum does not depend on the order of the entries). */
-template <bool BONDS>
-__device__ __forceinline__ void md_forces_flat(const MdLds& S, const uint32_t* list, uint32_t n, double lambda) {
+/* Forces of all list entries -- the npl half-list pairs (Newton's third
+ * law) followed by the nb bonds -- kMdBatch independent entries per lane in
+ * flight, added to both monomers' 64-bit fixed-point LDS accumulators (the
+ * sum does not depend on the order of the entries). */
+__device__ __forceinline__ void md_forces_flat(const MdLds& S, const uint32_t* pl, uint32_t npl, uint32_t nb,
+                                               double lambda) {
 #pragma clang fp contract(fast)
     constexpr int NT = kMdThreads;
     unsigned long long* qx = reinterpret_cast<unsigned long long*>(S.fx);
     unsigned long long* qy = reinterpret_cast<unsigned long long*>(S.fy);
     unsigned long long* qz = reinterpret_cast<unsigned long long*>(S.fz);
+    const uint32_t n = npl + nb;
     for (uint32_t q0 = threadIdx.x; q0 < n; q0 += NT * kMdBatch) {
         int ii[kMdBatch], jj[kMdBatch];
         unsigned long long gx[kMdBatch], gy[kMdBatch], gz[kMdBatch];
@@ -219,17 +221,15 @@ __device__ __forceinline__ void md_forces_flat(const MdLds& S, const uint32_t* l
         for (int k = 0; k < kMdBatch; ++k) {
             const uint32_t q = q0 + k * NT;
             ok[k] = q < n;
-            const uint32_t e = ok[k] ? list[q] : 0u;
-            int i, j;
-            bool anchor = false;
-            if (BONDS) { i = e & 0xFFF; j = (e >> 12) & 0xFFF; anchor = (e >> 24) & 1; }
-            else { i = e & 0xFFFF; j = e >> 16; }
+            const bool bond = q >= npl;
+            const uint32_t e = !ok[k] ? 0u : (bond ? S.bl[q - npl] : pl[q]);
+            const int i = bond ? (int)(e & 0xFFF) : (int)(e & 0xFFFF);
+            const int j = bond ? (int)((e >> 12) & 0xFFF) : (int)(e >> 16);
             ii[k] = i; jj[k] = j;
             const double dx = S.x[j] - S.x[i], dy = S.y[j] - S.y[i], dz = S.z[j] - S.z[i];
             const double r2 = (dx * dx + dy * dy) + dz * dz;
-            double fa;
-            if (BONDS) fa = fene_force_fast(r2, anchor ? 20.25 : 2.25);
-            else fa = pair_force_fast(r2, S.bt[i] && S.bt[j], lambda);
+            double fa = bond ? fene_force_fast(r2, ((e >> 24) & 1) ? 20.25 : 2.25)
+                             : pair_force_fast(r2, S.bt[i] && S.bt[j], lambda);
             if (!ok[k]) fa = 0.0;
             ok[k] = ok[k] && fa != 0.0;
             gx[k] = to_fix(dx * fa); gy[k] = to_fix(dy * fa); gz[k] = to_fix(dz * fa);
@@ -575,20 +575,14 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
         lds_vl = in_lds;
         rebuilt = true;
     };
-    /* Box::calculate_forces_verlet (Box.cpp:243-286); accumulators zeroed
-     * by the caller */
+    /* Box::calculate_forces_verlet (Box.cpp:243-286) into the fixed-point
+     * accumulators (zeroed by the caller); Epot of the same positions.  The
+     * caller converts the accumulators. */
     auto forces = [&](bool calc_epot) {
         const uint32_t* pl = lds_vl ? S.pl : gpl;
-        md_forces_flat<false>(S, pl, npl, rep.lambda);
-        md_forces_flat<true>(S, S.bl, nbond, rep.lambda);
+        md_forces_flat(S, pl, npl, nbond, rep.lambda);
         __syncthreads();
-        for (int i = tid; i < N; i += NT) {
-            S.fx[i] = from_fix(qx[i]);
-            S.fy[i] = from_fix(qy[i]);
-            S.fz[i] = from_fix(qz[i]);
-        }
-        if (calc_epot) epot_val = md_epot_flat(S, pl, npl, nbond, rep.lambda);
-        else { epot_val = 0.0; __syncthreads(); }
+        epot_val = calc_epot ? md_epot_flat(S, pl, npl, nbond, rep.lambda) : 0.0;
         epot_set = true;
     };
 
@@ -596,30 +590,43 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
         if (md_init) {
             rebuild();  /* leaves the accumulators zeroed */
             forces(false);
-        }
-        for (int s = 0; s < n_md; ++s) {
-            /* first half kick, drift, Box::check_VerletLists (Box.cpp:182-194) */
-            bool need = false;
             for (int i = tid; i < N; i += NT) {
-                const double h = S.im[i];
-                double vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
-                vx += S.fx[i] * h; vy += S.fy[i] * h; vz += S.fz[i] * h;
-                qx[i] = 0ull; qy[i] = 0ull; qz[i] = 0ull;
-                const double x = S.x[i] + vx * dt, y = S.y[i] + vy * dt, z = S.z[i] + vz * dt;
-                S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz;
-                S.x[i] = x; S.y[i] = y; S.z[i] = z;
-                const double dx = x - S.px[i], dy = y - S.py[i], dz = z - S.pz[i];
-                need |= sqrt((dx * dx + dy * dy) + dz * dz) > (2.0 - 1.5) * 0.5;
-            }
-            if (__syncthreads_or(need)) rebuild();
-            forces(epot_last && s == n_md - 1);
-            /* second half kick */
-            for (int i = tid; i < N; i += NT) {
-                const double h = S.im[i];
-                S.vx[i] += S.fx[i] * h; S.vy[i] += S.fy[i] * h; S.vz[i] += S.fz[i] * h;
+                S.fx[i] = from_fix(qx[i]); S.fy[i] = from_fix(qy[i]); S.fz[i] = from_fix(qz[i]);
             }
             __syncthreads();
         }
+        /* first half kick and drift of substep s, Box::check_VerletLists
+         * (Box.cpp:182-194); the accumulators are cleared for the forces */
+        auto kick_drift = [&](int i) -> bool {
+            const double h = S.im[i];
+            double vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
+            vx += S.fx[i] * h; vy += S.fy[i] * h; vz += S.fz[i] * h;
+            qx[i] = 0ull; qy[i] = 0ull; qz[i] = 0ull;
+            const double x = S.x[i] + vx * dt, y = S.y[i] + vy * dt, z = S.z[i] + vz * dt;
+            S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz;
+            S.x[i] = x; S.y[i] = y; S.z[i] = z;
+            const double dx = x - S.px[i], dy = y - S.py[i], dz = z - S.pz[i];
+            return sqrt((dx * dx + dy * dy) + dz * dz) > (2.0 - 1.5) * 0.5;
+        };
+        bool need = false;
+        if (n_md > 0)
+            for (int i = tid; i < N; i += NT) need |= kick_drift(i);
+        for (int s = 0; s < n_md; ++s) {
+            if (__syncthreads_or(need)) rebuild();
+            forces(epot_last && s == n_md - 1);
+            /* forces out of fixed point, second half kick of substep s and,
+             * fused, the first half kick and drift of substep s + 1 */
+            need = false;
+            const bool more = s + 1 < n_md;
+            for (int i = tid; i < N; i += NT) {
+                const double fx = from_fix(qx[i]), fy = from_fix(qy[i]), fz = from_fix(qz[i]);
+                S.fx[i] = fx; S.fy[i] = fy; S.fz[i] = fz;
+                const double h = S.im[i];
+                S.vx[i] += fx * h; S.vy[i] += fy * h; S.vz[i] += fz * h;
+                if (more) need |= kick_drift(i);
+            }
+        }
+        __syncthreads();
         if (rebuilt && lds_vl)
             for (uint32_t q = tid; q < npl && q < gcap; q += NT) gpl[q] = S.pl[q];
         if (rebuilt && tid == 0) ctl->vl_count = npl;
