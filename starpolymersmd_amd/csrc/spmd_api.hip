@@ -260,9 +260,7 @@ hipError_t enqueue_collision_ops(spmd_sys* s, int n_md, int epot_last, int n_del
         if ((e = launch_perm(d, s->stream, kSortSpec)) != hipSuccess) return e;
         prof_end(s, 3, s->stream);
         if ((e = hipStreamWaitEvent(s->stream, s->ev_join, 0)) != hipSuccess) return e;
-        if ((e = launch_bin(d, s->stream, h, kSortRedo)) != hipSuccess) return e;
-        if ((e = launch_scan_redo(d, s->stream)) != hipSuccess) return e;
-        if ((e = launch_perm(d, s->stream, kSortRedo)) != hipSuccess) return e;
+        if ((e = launch_sort_redo(d, s->stream, h)) != hipSuccess) return e;
     }
     prof_begin(s, 4, s->stream);
     if ((e = launch_collide(d, s->stream, h)) != hipSuccess) return e;

@@ -120,14 +120,14 @@ size_t md_lds_bytes(int ns, uint32_t cap);
 
 /* launchers (spmd_kernels.hip).  Sort modes: kSortNormal applies the
  * recentring flag, kSortSpec assumes none (runs beside the MD kernel),
- * kSortRedo redoes the replicas whose MD found a recentring. */
-enum { kSortNormal = 0, kSortSpec = 1, kSortRedo = 2 };
+ * (k_sort_redo redoes the replicas whose MD found a recentring). */
+enum { kSortNormal = 0, kSortSpec = 1 };
 hipError_t launch_prologue(const DevState& d, hipStream_t st, int do_collision, int n_delrx, double delrx_dt);
 hipError_t launch_md(const DevState& d, hipStream_t st, int n_md, int epot_last, int md_init, int do_collision,
                      double md_dt);
 hipError_t launch_bin(const DevState& d, hipStream_t st, double h, int mode);
 hipError_t launch_scan(const DevState& d, hipStream_t st);
-hipError_t launch_scan_redo(const DevState& d, hipStream_t st);
+hipError_t launch_sort_redo(const DevState& d, hipStream_t st, double h);
 hipError_t launch_perm(const DevState& d, hipStream_t st, int mode);
 hipError_t launch_collide(const DevState& d, hipStream_t st, double h);
 hipError_t launch_fluid_init(const DevState& d, hipStream_t st);

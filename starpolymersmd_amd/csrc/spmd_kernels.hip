@@ -15,9 +15,10 @@
  *   k_scan         single-pass exclusive scan of the cell counts
  *   k_perm         per solvent particle: perm[offset[cell] + slot] = index
  *                  (4-byte scattered stores; the particle data stays put)
- *   (redo)         k_bin/k_scan_redo/k_perm again for replicas whose MD
- *                  step recentred (bin and perm ran beside k_md assuming it
- *                  would not); they return at once otherwise
+ *   k_sort_redo    one workgroup per replica: bin, scan and perm again for
+ *                  a replica whose MD step recentred (bin and perm ran
+ *                  beside k_md assuming it would not); returns at once
+ *                  otherwise
  *   k_collide      per tile of cells: gather the tile's particles through
  *                  perm, stream them again, momentum and energy sums in LDS
  *                  (64-bit fixed point, order independent), MBS thermostat,
@@ -62,31 +63,6 @@ __device__ __forceinline__ int find_rep(const uint32_t* base, int n, uint32_t b)
 __device__ __forceinline__ uint64_t lanemask_lt() {
     const uint32_t lane = __lane_id();
     return lane ? (~0ull >> (64 - lane)) : 0ull;
-}
-
-/* Slot of each lane's particle inside its cell.  Lanes are grouped into
- * runs of equal consecutive keys (the particles arrive in the previous
- * collision's cell order, so runs are long); every run head reserves its
- * whole run with one atomicAdd, and all heads of the wave issue theirs in
- * the same instruction -- one round trip per wave. */
-__device__ __forceinline__ uint32_t wave_rank(uint32_t* counts, uint32_t key, bool ok) {
-    const uint32_t lane = __lane_id();
-    const uint32_t prev_key = __shfl_up(key, 1);
-    const int prev_ok = __shfl_up((int)ok, 1);
-    const bool head = ok && (lane == 0 || !prev_ok || prev_key != key);
-    const uint64_t heads = __ballot(head);
-    const uint64_t stops = heads | __ballot(!ok);
-    const uint64_t le = (2ull << lane) - 1ull;   /* lanes <= this one (lane 63: all) */
-    uint32_t base = 0;
-    if (head) {
-        const uint64_t above = stops & ~le;
-        const uint32_t next = above ? (uint32_t)(__ffsll((unsigned long long)above) - 1) : 64u;
-        base = atomicAdd(&counts[key], next - lane);
-    }
-    const uint64_t hm = heads & le;
-    const int h = hm ? 63 - __clzll((long long)hm) : (int)lane;
-    const uint32_t b = __shfl(base, h);
-    return ok ? b + (lane - (uint32_t)h) : 0u;
 }
 
 /* MPC::check_bounds fluid part (MPC.cpp:183-188) when this collision
@@ -531,6 +507,9 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
                                                   double dt, int ns, uint32_t cap) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NT = kMdThreads;
+    /* claim all 256 VGPRs: two waves per SIMD then fill the register file,
+     * so no wave of the concurrently running sort kernels shares this CU */
+    asm volatile("v_mov_b32 v255, 0" ::: "v255");
     MdLds S = md_carve(smem, ns, cap);
     const int r = blockIdx.x;
     const int tid = threadIdx.x;
@@ -704,7 +683,6 @@ __global__ __launch_bounds__(64) void k_prologue(DevState d, int do_coll, int n_
  * recentring fields (written concurrently by k_md) are not read. */
 __device__ __forceinline__ bool sort_ctl(const DevState& d, int r, int mode, RepCtl& c) {
     const RepCtl* g = d.ctl + r;
-    if (mode == kSortRedo && !g->redo) return false;
     c.delrx = g->delrx;
     c.shift[0] = g->shift[0]; c.shift[1] = g->shift[1]; c.shift[2] = g->shift[2];
     c.recenter = mode == kSortSpec ? 0 : g->recenter;
@@ -713,8 +691,18 @@ __device__ __forceinline__ bool sort_ctl(const DevState& d, int r, int mode, Rep
     return true;
 }
 
+/* Cells touched by one workgroup's particles are aggregated in an LDS hash
+ * table first (open addressing, HS >= 2 x particles, so an insert always
+ * finds a slot): a particle's slot in its cell is its order among the
+ * workgroup's particles of that cell (LDS atomic) plus the workgroup's base
+ * in the cell (one global atomic per distinct cell).  The particles arrive
+ * in the previous collision's cell order, so a workgroup touches a few
+ * hundred cells. */
 template <int PPT>
 __global__ __launch_bounds__(256) void k_bin(DevState d, double h, int mode) {
+    constexpr int HS = 2 * 256 * PPT;
+    constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+    __shared__ uint32_t h_key[HS], h_cnt[HS];
     const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
     const int r = find_rep(d.pblk_base, d.n_rep, lb);
     const RepDev& rep = d.reps[r];
@@ -722,32 +710,54 @@ __global__ __launch_bounds__(256) void k_bin(DevState d, double h, int mode) {
     if (!sort_ctl(d, r, mode, c)) return;
     Box b = rep.box;
     b.delrx = c.delrx;
+    for (int q = threadIdx.x; q < HS; q += 256) { h_key[q] = kEmpty; h_cnt[q] = 0u; }
+    __syncthreads();
     const uint64_t p0 = rep.part_base + (uint64_t)(lb - d.pblk_base[r]) * (256 * PPT);
     const uint64_t pend = rep.part_base + rep.n_fluid;
     const int cur = d.cur;
     const float *X = d.x[cur], *Y = d.y[cur], *Z = d.z[cur];
     const float *VX = d.vx[cur], *VY = d.vy[cur], *VZ = d.vz[cur];
     bool bad = false, big = false;
+    uint32_t slot[PPT], loc[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         const uint64_t i = p0 + (uint64_t)j * 256 + threadIdx.x;
-        const bool valid = i < pend;
-        uint32_t cell = 0;
-        bool ok = false;
-        if (valid) {
-            double x = X[i], y = Y[i], z = Z[i], vx = VX[i], vy = VY[i], vz = VZ[i];
-            recenter_stream(b, h, c, x, y, z, vx, vy, vz);
-            cell = shifted_cell(b, c, x, y, z, vx);
-            ok = cell < rep.ncells;
-            bad |= !ok;
+        slot[j] = kEmpty;
+        loc[j] = 0u;
+        if (i >= pend) continue;
+        double x = X[i], y = Y[i], z = Z[i], vx = VX[i], vy = VY[i], vz = VZ[i];
+        recenter_stream(b, h, c, x, y, z, vx, vy, vz);
+        const uint32_t cell = shifted_cell(b, c, x, y, z, vx);
+        if (cell >= rep.ncells) {
+            bad = true;
+            d.pcell[i] = ~0u;
+            continue;
         }
-        const uint32_t gcell = (uint32_t)rep.cell_base + cell;
-        const uint32_t rank = wave_rank(d.counts, gcell, ok);
+        const uint32_t g = (uint32_t)rep.cell_base + cell;
+        d.pcell[i] = g;
+        uint32_t sl = (g * 0x9E3779B1u) >> (32 - __builtin_ctz(HS));
+        for (int probe = 0; probe < HS; ++probe) {
+            const uint32_t k = atomicCAS(&h_key[sl], kEmpty, g);
+            if (k == kEmpty || k == g) break;
+            sl = (sl + 1) & (HS - 1);
+        }
+        slot[j] = sl;
+        loc[j] = atomicAdd(&h_cnt[sl], 1u);
+    }
+    __syncthreads();
+    /* one global atomic per distinct cell: the count becomes the base */
+    for (int q = threadIdx.x; q < HS; q += 256) {
+        const uint32_t k = h_key[q];
+        if (k != kEmpty) h_cnt[q] = atomicAdd(&d.counts[k], h_cnt[q]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        if (slot[j] == kEmpty) continue;
+        const uint64_t i = p0 + (uint64_t)j * 256 + threadIdx.x;
+        const uint32_t rank = h_cnt[slot[j]] + loc[j];
         big |= rank > 0xFFFFu;
-        if (valid) {
-            d.rank[i] = (uint16_t)(rank > 0xFFFFu ? 0xFFFFu : rank);
-            d.pcell[i] = ok ? gcell : ~0u;
-        }
+        d.rank[i] = (uint16_t)(rank > 0xFFFFu ? 0xFFFFu : rank);
     }
     if (bad) atomicOr(&d.ctl[r].status, ST_CELL_OOB);
     if (big) atomicOr(&d.ctl[r].status, ST_RANK_OVERFLOW);
@@ -843,19 +853,44 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(DevState d) {
 
 /* Exclusive scan of one replica's cell counts when its speculative sort is
  * redone (rare: only after a recentring); one workgroup per replica. */
-__global__ __launch_bounds__(1024) void k_scan_redo(DevState d) {
+/* The whole sort again (bin, scan, perm) for a replica whose MD step
+ * recentred it, by one workgroup: recentring is rare, and one launch that
+ * returns at once otherwise keeps the common path short.  The counts were
+ * left zero by the speculative scan. */
+__global__ __launch_bounds__(1024) void k_sort_redo(DevState d, double h) {
     __shared__ uint32_t s_w[16];
     __shared__ uint32_t s_carry;
     const int r = blockIdx.x;
     if (!d.ctl[r].redo) return;
     const RepDev& rep = d.reps[r];
+    RepCtl c;
+    sort_ctl(d, r, kSortNormal, c);
+    Box b = rep.box;
+    b.delrx = c.delrx;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint64_t p0 = rep.part_base, pend = rep.part_base + rep.n_fluid;
+    const int cur = d.cur;
+    bool bad = false, big = false;
+    for (uint64_t i = p0 + tid; i < pend; i += 1024) {
+        double x = d.x[cur][i], y = d.y[cur][i], z = d.z[cur][i];
+        double vx = d.vx[cur][i], vy = d.vy[cur][i], vz = d.vz[cur][i];
+        recenter_stream(b, h, c, x, y, z, vx, vy, vz);
+        const uint32_t cell = shifted_cell(b, c, x, y, z, vx);
+        if (cell >= rep.ncells) { bad = true; d.pcell[i] = ~0u; continue; }
+        const uint32_t g = (uint32_t)rep.cell_base + cell;
+        d.pcell[i] = g;
+        const uint32_t rank = atomicAdd(&d.counts[g], 1u);
+        big |= rank > 0xFFFFu;
+        d.rank[i] = (uint16_t)(rank > 0xFFFFu ? 0xFFFFu : rank);
+    }
+    if (bad) atomicOr(&d.ctl[r].status, ST_CELL_OOB);
+    if (big) atomicOr(&d.ctl[r].status, ST_RANK_OVERFLOW);
     if (tid == 0) s_carry = (uint32_t)rep.part_base;
     __syncthreads();
     for (uint32_t c0 = 0; c0 < rep.ncells; c0 += 1024) {
-        const uint32_t c = c0 + tid;
-        const uint32_t g = (uint32_t)rep.cell_base + c;
-        const uint32_t v = c < rep.ncells ? d.counts[g] : 0u;
+        const uint32_t cc = c0 + tid;
+        const uint32_t g = (uint32_t)rep.cell_base + cc;
+        const uint32_t v = cc < rep.ncells ? atomicExch(&d.counts[g], 0u) : 0u;  /* read and re-zero */
         uint32_t inc = v;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
@@ -867,10 +902,19 @@ __global__ __launch_bounds__(1024) void k_scan_redo(DevState d) {
         uint32_t pre = 0, tot = 0;
         for (int q = 0; q < 16; ++q) { pre += q < w ? s_w[q] : 0u; tot += s_w[q]; }
         const uint32_t base = s_carry;
-        if (c < rep.ncells) { d.offsets[g] = base + pre + inc - v; d.counts[g] = 0u; }
+        if (cc < rep.ncells) d.offsets[g] = base + pre + inc - v;
         __syncthreads();
         if (tid == 0) s_carry = base + tot;
         __syncthreads();
+    }
+    __threadfence();
+    __syncthreads();
+    for (uint64_t i = p0 + tid; i < pend; i += 1024) {
+        const uint32_t cell = d.pcell[i];
+        if (cell == ~0u) continue;
+        const uint32_t dst = d.offsets[cell] + d.rank[i];
+        if (dst >= pend || dst < p0) continue;
+        d.perm[dst] = (uint32_t)i;
     }
 }
 
@@ -883,7 +927,6 @@ __global__ __launch_bounds__(256) void k_perm(DevState d, int mode) {
     const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
     const int r = find_rep(d.pblk_base, d.n_rep, lb);
     const RepDev& rep = d.reps[r];
-    if (mode == kSortRedo && !d.ctl[r].redo) return;
     const uint64_t p0 = rep.part_base + (uint64_t)(lb - d.pblk_base[r]) * (256 * PPT);
     const uint64_t pend = rep.part_base + rep.n_fluid;
 #pragma unroll
@@ -1297,8 +1340,8 @@ hipError_t launch_scan(const DevState& d, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_scan_redo(const DevState& d, hipStream_t st) {
-    hipLaunchKernelGGL(k_scan_redo, dim3(d.n_rep), dim3(1024), 0, st, d);
+hipError_t launch_sort_redo(const DevState& d, hipStream_t st, double h) {
+    hipLaunchKernelGGL(k_sort_redo, dim3(d.n_rep), dim3(1024), 0, st, d, h);
     return hipGetLastError();
 }
 

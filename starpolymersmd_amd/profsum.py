@@ -20,7 +20,7 @@ PIPELINE = ("k_bin", "k_scan", "k_perm", "k_collide")
 
 def short(name: str) -> str:
     n = name.split("(")[0]
-    for k in ("k_md", "k_prologue", "k_bin", "k_scan_redo", "k_scan", "k_scatter", "k_perm", "k_collide",
+    for k in ("k_md", "k_prologue", "k_bin", "k_sort_redo", "k_scan_redo", "k_scan", "k_scatter", "k_perm", "k_collide",
               "k_fluid_init", "k_fluid_mean", "k_temperature"):
         if k in n:
             return k
@@ -68,14 +68,13 @@ def main():
     print("|---|---:|---:|---:|")
     for k, (n, tot) in sorted(merged.items(), key=lambda kv: -kv[1][1]):
         print(f"| {k} | {n} | {tot / 1e6:.3f} | {tot / n / 1e3:.2f} |")
-    # per collision: k_collide runs once per collision; k_bin and k_perm
-    # run twice (speculative pass and the redo pass, which exits at once
-    # unless a recentring forced it), so totals are divided by collisions
+    # per collision: every kernel's total is divided by the number of
+    # collisions (k_collide runs once per collision)
     ncoll = merged["k_collide"][0] if "k_collide" in merged else 1
     print(f"\ncollisions profiled: {ncoll}")
     print("\n| kernel | us per collision |")
     print("|---|---:|")
-    for k in ("k_prologue", "k_md", "k_bin", "k_scan", "k_scan_redo", "k_scatter", "k_perm", "k_collide"):
+    for k in ("k_prologue", "k_md", "k_bin", "k_scan", "k_scan_redo", "k_scatter", "k_perm", "k_sort_redo", "k_collide"):
         if k in merged:
             print(f"| {k} | {merged[k][1] / ncoll / 1e3:.2f} |")
     pipe_us = sum(merged[k][1] / ncoll / 1e3 for k in PIPELINE if k in merged)
