@@ -82,6 +82,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=100)
     ap.add_argument("--cpu-intervals", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (ppt 4, 128-cell tiles)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run N MiB device-to-device copies (counter calibration)")
     a = ap.parse_args()
@@ -109,6 +110,12 @@ def main():
     t = 0
     g.run(t, 1, h)  # t=0: first collision
     t += 1
+    tune = None
+    if not a.no_tune:  # launch-shape sweep inside the (untimed) warm-up
+        from starpolymersmd_amd.autotune import autotune
+
+        tune = autotune(g, t, h, cache=None)
+        t = tune["t"]
     g.run(t, a.warmup * S, h)
     t += a.warmup * S
     g.sync()
@@ -138,7 +145,9 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "plain MPC-MD L=64 rho=10 (2,621,440 solvent), star f=20 15A+15B "
                                "(601 monomers), shear 0.05, h=0.1, dt=0.01, 10 MD substeps/collision",
-                   "solvent_particles": n_f, "step": "one collision interval", "parallelism": "1 GPU"},
+                   "solvent_particles": n_f, "step": "one collision interval", "parallelism": "1 GPU",
+                   "tuning": {"ppt": g.get_tuning()[0], "tile_cells": g.get_tuning()[1],
+                              "sweep_ms": None if tune is None else tune["timings"]}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "collision pipeline k_bin+k_scan+k_perm+k_collide",

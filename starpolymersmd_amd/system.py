@@ -23,10 +23,12 @@ class MPCSystem:
         if isinstance(params, Params):
             rc = lib.spmd_create(C.byref(params), C.byref(h))
             self.n_replicas = 1
+            self.params = [params]
         else:
             arr = (Params * len(params))(*params)
             rc = lib.spmd_create_batch(arr, len(params), C.byref(h))
             self.n_replicas = len(params)
+            self.params = list(params)
         if rc:
             raise SpmdError(rc, lib.spmd_error_string(rc).decode())
         self.h = h
