@@ -127,7 +127,10 @@ constexpr int kGridMax = 4096;         /* cells of the rebuild grid */
 /* Dynamic LDS of k_md for a stride of ns monomers (the host computes the
  * same size with md_lds_bytes), with cap pair-list entries. */
 struct MdLds {
-    double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *px, *py, *pz, *im, *red;
+    /* positions, and the force accumulators (64-bit fixed point while the
+     * forces are summed, FP64 forces after conversion); velocities, Verlet
+     * reference positions and masses live in the owning threads' registers */
+    double *x, *y, *z, *fx, *fy, *fz, *red;
     /* rebuild scratch over the force accumulators (zeroed again after it):
      * FP32 positions sorted by grid cell, w = bits of (i | B << 16), and
      * each monomer's grid cell */
@@ -144,11 +147,8 @@ __device__ __forceinline__ MdLds md_carve(unsigned char* base, int ns, uint32_t 
     MdLds L;
     double* p = reinterpret_cast<double*>(base);
     L.x = p; L.y = p + ns; L.z = p + 2 * ns;
-    L.vx = p + 3 * ns; L.vy = p + 4 * ns; L.vz = p + 5 * ns;
-    L.fx = p + 6 * ns; L.fy = p + 7 * ns; L.fz = p + 8 * ns;
-    L.px = p + 9 * ns; L.py = p + 10 * ns; L.pz = p + 11 * ns;   /* Verlet reference positions */
-    L.im = p + 12 * ns;                                           /* dt/(2m) */
-    L.red = p + 13 * ns;                                          /* 80 doubles */
+    L.fx = p + 3 * ns; L.fy = p + 4 * ns; L.fz = p + 5 * ns;
+    L.red = p + 6 * ns;                                           /* 80 doubles */
     L.flags = reinterpret_cast<int*>(L.red + 80);                 /* 16 ints */
     L.sp4 = reinterpret_cast<float4*>(L.fx);                      /* 16 ns <= 24 ns bytes */
     L.gcid = reinterpret_cast<uint32_t*>(L.sp4 + ns);             /* + 4 ns */
@@ -160,7 +160,7 @@ __device__ __forceinline__ MdLds md_carve(unsigned char* base, int ns, uint32_t 
 }
 
 size_t md_lds_bytes(int ns, uint32_t cap) {
-    return (size_t)13 * ns * 8 + 80 * 8 + 16 * 4 + (size_t)(kGridMax + 1) * 4 + (size_t)ns * 4 + (size_t)cap * 4 +
+    return (size_t)6 * ns * 8 + 80 * 8 + 16 * 4 + (size_t)(kGridMax + 1) * 4 + (size_t)ns * 4 + (size_t)cap * 4 +
            (size_t)ns + 64;
 }
 
@@ -350,7 +350,8 @@ __device__ __forceinline__ uint32_t md_scan_cands(const MdLds& S, int p, const i
  * the monomers at its sorted slots, counts their pairs, and after a block
  * scan writes them at its own offset -- into LDS when the list fits there,
  * else into the replica's global region.  Leaves the force accumulators
- * zeroed.  Returns the entry count; *in_lds tells where the list is. */
+ * zeroed; the owning threads reset their Verlet reference positions.
+ * Returns the entry count; *in_lds tells where the list is. */
 __device__ uint32_t md_rebuild(const MdLds& S, uint32_t* gpl, uint32_t gcap, const RepDev& rep, RepCtl* ctl, int N,
                                uint32_t cap, bool* in_lds) {
     constexpr int NT = kMdThreads;
@@ -401,7 +402,6 @@ __device__ uint32_t md_rebuild(const MdLds& S, uint32_t* gpl, uint32_t gcap, con
             const int cn = (int)((v[c] - cs[c] + rep.box.L[c] * 0.5) / rep.vside[c]);
             oob |= cn < 0 || cn >= rep.vgrid[c];
         }
-        S.px[i] = v[0]; S.py[i] = v[1]; S.pz[i] = v[2];
     }
     if (oob) atomicOr(&ctl->status, ST_VERLET_OOB);
     /* grid: side >= 2 (the largest list radius), grown until the cells fit */
@@ -461,48 +461,13 @@ __device__ uint32_t md_rebuild(const MdLds& S, uint32_t* gpl, uint32_t gcap, con
     return total < ocap ? total : ocap;
 }
 
-/* MPC::check_bounds (MPC.cpp:160-190) on the monomers; a recentring also
- * marks the speculative solvent sort of this collision for redo. */
-__device__ __forceinline__ void md_check_bounds(const MdLds& S, const RepDev& rep, RepCtl* ctl, int N) {
-    constexpr int NT = kMdThreads;
-    const int tid = threadIdx.x;
-    bool out = false;
-    for (int i = tid; i < N; i += NT) {
-        out |= fabs(S.x[i]) >= (rep.box.L[0] * 0.5 - 0.5);
-        out |= fabs(S.y[i]) >= (rep.box.L[1] * 0.5 - 0.5);
-        out |= fabs(S.z[i]) >= (rep.box.L[2] * 0.5 - 0.5);
-    }
-    const bool rec = __syncthreads_or(out) != 0;
-    if (rec) {
-        double cp[3], cv[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const double* a = c == 0 ? S.x : (c == 1 ? S.y : S.z);
-            const double* b = c == 0 ? S.vx : (c == 1 ? S.vy : S.vz);
-            double ap = 0.0, av = 0.0;
-            for (int i = tid; i < N; i += NT) { ap += a[i]; av += b[i]; }
-            cp[c] = block_sum_all<NT>(ap, S.red) / (double)N;
-            cv[c] = block_sum_all<NT>(av, S.red) / (double)N;
-        }
-        __syncthreads();
-        for (int i = tid; i < N; i += NT) {
-            S.x[i] -= cp[0]; S.y[i] -= cp[1]; S.z[i] -= cp[2];
-            S.vx[i] -= cv[0]; S.vy[i] -= cv[1]; S.vz[i] -= cv[2];
-        }
-        if (tid == 0)
-            for (int c = 0; c < 3; ++c) { ctl->com[c] = cp[c]; ctl->com_vel[c] = cv[c]; }
-    }
-    if (tid == 0) {
-        ctl->recenter = rec ? 1 : 0;
-        ctl->redo = rec ? 1 : 0;
-    }
-    __syncthreads();
-}
-
 /* One workgroup per replica (its star): the pending velocity-Verlet
- * substeps (Thermostat_None::propagate, Thermostat_None.cpp:21-38) with all
- * per-monomer state in LDS; then, for a collision, check_bounds and the
- * monomers' cells. */
+ * substeps (Thermostat_None::propagate, Thermostat_None.cpp:21-38); then,
+ * for a collision, check_bounds and the monomers' cells.  Thread t owns
+ * monomers t + 512 k (k < R): their velocities, Verlet reference positions
+ * and masses stay in its registers, their positions in registers and LDS
+ * (the force loops read any monomer's position). */
+template <int R>
 __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epot_last, int md_init, int do_coll,
                                                   double dt, int ns, uint32_t cap) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -526,13 +491,19 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
     const uint32_t gcap = (uint32_t)N * kVlPerMono;
     const uint32_t nbond = rep.n_bond < (uint32_t)ns ? rep.n_bond : (uint32_t)ns;
 
-    for (int i = tid; i < N; i += NT) {
+    double rx[R], ry[R], rz[R], rvx[R], rvy[R], rvz[R], rpx[R], rpy[R], rpz[R], rim[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int i = tid + k * NT;
+        rx[k] = ry[k] = rz[k] = rvx[k] = rvy[k] = rvz[k] = rpx[k] = rpy[k] = rpz[k] = rim[k] = 0.0;
+        if (i >= N) continue;
         const size_t g = 3 * (size_t)(mb + i);
-        S.x[i] = d.mpos[g]; S.y[i] = d.mpos[g + 1]; S.z[i] = d.mpos[g + 2];
-        S.vx[i] = d.mvel[g]; S.vy[i] = d.mvel[g + 1]; S.vz[i] = d.mvel[g + 2];
+        rx[k] = d.mpos[g]; ry[k] = d.mpos[g + 1]; rz[k] = d.mpos[g + 2];
+        rvx[k] = d.mvel[g]; rvy[k] = d.mvel[g + 1]; rvz[k] = d.mvel[g + 2];
+        rpx[k] = d.mvpos[g]; rpy[k] = d.mvpos[g + 1]; rpz[k] = d.mvpos[g + 2];
+        rim[k] = dth / d.mmass[mb + i];  /* half-kick factor dt/(2m) */
+        S.x[i] = rx[k]; S.y[i] = ry[k]; S.z[i] = rz[k];
         S.fx[i] = d.mforce[g]; S.fy[i] = d.mforce[g + 1]; S.fz[i] = d.mforce[g + 2];
-        S.px[i] = d.mvpos[g]; S.py[i] = d.mvpos[g + 1]; S.pz[i] = d.mvpos[g + 2];
-        S.im[i] = dth / d.mmass[mb + i];  /* half-kick factor dt/(2m) */
         S.bt[i] = d.mbtype[mb + i];
     }
     for (uint32_t q = tid; q < nbond; q += NT) S.bl[q] = d.bonds[rep.bond_base + q];
@@ -553,6 +524,8 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
         npl = md_rebuild(S, gpl, gcap, rep, ctl, N, cap, &in_lds);
         lds_vl = in_lds;
         rebuilt = true;
+#pragma unroll
+        for (int k = 0; k < R; ++k) { rpx[k] = rx[k]; rpy[k] = ry[k]; rpz[k] = rz[k]; }
     };
     /* Box::calculate_forces_verlet (Box.cpp:243-286) into the fixed-point
      * accumulators (zeroed by the caller); Epot of the same positions.  The
@@ -564,32 +537,40 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
         epot_val = calc_epot ? md_epot_flat(S, pl, npl, nbond, rep.lambda) : 0.0;
         epot_set = true;
     };
+    /* first half kick and drift of a substep with force f, then
+     * Box::check_VerletLists (Box.cpp:182-194); the accumulators are
+     * cleared for the next forces */
+    auto kick_drift = [&](int k, double fx, double fy, double fz) -> bool {
+        const int i = tid + k * NT;
+        const double h = rim[k];
+        rvx[k] += fx * h; rvy[k] += fy * h; rvz[k] += fz * h;
+        rx[k] += rvx[k] * dt; ry[k] += rvy[k] * dt; rz[k] += rvz[k] * dt;
+        S.x[i] = rx[k]; S.y[i] = ry[k]; S.z[i] = rz[k];
+        qx[i] = 0ull; qy[i] = 0ull; qz[i] = 0ull;
+        const double dx = rx[k] - rpx[k], dy = ry[k] - rpy[k], dz = rz[k] - rpz[k];
+        return sqrt((dx * dx + dy * dy) + dz * dz) > (2.0 - 1.5) * 0.5;
+    };
 
     if (N > 0) {
         if (md_init) {
             rebuild();  /* leaves the accumulators zeroed */
             forces(false);
-            for (int i = tid; i < N; i += NT) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const int i = tid + k * NT;
+                if (i >= N) continue;
                 S.fx[i] = from_fix(qx[i]); S.fy[i] = from_fix(qy[i]); S.fz[i] = from_fix(qz[i]);
             }
             __syncthreads();
         }
-        /* first half kick and drift of substep s, Box::check_VerletLists
-         * (Box.cpp:182-194); the accumulators are cleared for the forces */
-        auto kick_drift = [&](int i) -> bool {
-            const double h = S.im[i];
-            double vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
-            vx += S.fx[i] * h; vy += S.fy[i] * h; vz += S.fz[i] * h;
-            qx[i] = 0ull; qy[i] = 0ull; qz[i] = 0ull;
-            const double x = S.x[i] + vx * dt, y = S.y[i] + vy * dt, z = S.z[i] + vz * dt;
-            S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz;
-            S.x[i] = x; S.y[i] = y; S.z[i] = z;
-            const double dx = x - S.px[i], dy = y - S.py[i], dz = z - S.pz[i];
-            return sqrt((dx * dx + dy * dy) + dz * dz) > (2.0 - 1.5) * 0.5;
-        };
         bool need = false;
         if (n_md > 0)
-            for (int i = tid; i < N; i += NT) need |= kick_drift(i);
+#pragma unroll
+            for (int k = 0; k < R; ++k)
+                if (tid + k * NT < N) {
+                    const int i = tid + k * NT;
+                    need |= kick_drift(k, S.fx[i], S.fy[i], S.fz[i]);
+                }
         for (int s = 0; s < n_md; ++s) {
             if (__syncthreads_or(need)) rebuild();
             forces(epot_last && s == n_md - 1);
@@ -597,12 +578,15 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
              * fused, the first half kick and drift of substep s + 1 */
             need = false;
             const bool more = s + 1 < n_md;
-            for (int i = tid; i < N; i += NT) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const int i = tid + k * NT;
+                if (i >= N) continue;
                 const double fx = from_fix(qx[i]), fy = from_fix(qy[i]), fz = from_fix(qz[i]);
-                S.fx[i] = fx; S.fy[i] = fy; S.fz[i] = fz;
-                const double h = S.im[i];
-                S.vx[i] += fx * h; S.vy[i] += fy * h; S.vz[i] += fz * h;
-                if (more) need |= kick_drift(i);
+                const double h = rim[k];
+                rvx[k] += fx * h; rvy[k] += fy * h; rvz[k] += fz * h;
+                if (more) need |= kick_drift(k, fx, fy, fz);
+                else { S.fx[i] = fx; S.fy[i] = fy; S.fz[i] = fz; }
             }
         }
         __syncthreads();
@@ -611,23 +595,59 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
         if (rebuilt && tid == 0) ctl->vl_count = npl;
     }
     if (tid == 0 && epot_set) ctl->epot = epot_val;
-    if (do_coll) md_check_bounds(S, rep, ctl, N);
-
-    /* write the monomer state back */
-    for (int i = tid; i < N; i += NT) {
-        const size_t g = 3 * (size_t)(mb + i);
-        d.mpos[g] = S.x[i]; d.mpos[g + 1] = S.y[i]; d.mpos[g + 2] = S.z[i];
-        d.mvel[g] = S.vx[i]; d.mvel[g + 1] = S.vy[i]; d.mvel[g + 2] = S.vz[i];
-        d.mforce[g] = S.fx[i]; d.mforce[g + 1] = S.fy[i]; d.mforce[g + 2] = S.fz[i];
-        d.mvpos[g] = S.px[i]; d.mvpos[g + 1] = S.py[i]; d.mvpos[g + 2] = S.pz[i];
-    }
 
     if (do_coll) {
-        /* monomer cells (MPC::shiftParticles MPC.cpp:306-310 + sort
-         * MPC.cpp:221-235: shifted, never wrapped) */
-        const double sh[3] = {ctl->shift[0], ctl->shift[1], ctl->shift[2]};
-        for (int i = tid; i < N; i += NT) {
-            const uint32_t ci = cell_index(rep.box, S.x[i] + sh[0], S.y[i] + sh[1], S.z[i] + sh[2]);
+        /* MPC::check_bounds (MPC.cpp:160-190) on the monomers; a recentring
+         * also marks the speculative solvent sort of this collision for redo */
+        bool out = false;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (tid + k * NT >= N) continue;
+            out |= fabs(rx[k]) >= (rep.box.L[0] * 0.5 - 0.5);
+            out |= fabs(ry[k]) >= (rep.box.L[1] * 0.5 - 0.5);
+            out |= fabs(rz[k]) >= (rep.box.L[2] * 0.5 - 0.5);
+        }
+        const bool rec = __syncthreads_or(out) != 0;
+        if (rec) {
+            double sp[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                if (tid + k * NT >= N) continue;
+                sp[0] += rx[k]; sp[1] += ry[k]; sp[2] += rz[k];
+                sp[3] += rvx[k]; sp[4] += rvy[k]; sp[5] += rvz[k];
+            }
+            double cm[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) cm[c] = block_sum_all<NT>(sp[c], S.red) / (double)N;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                rx[k] -= cm[0]; ry[k] -= cm[1]; rz[k] -= cm[2];
+                rvx[k] -= cm[3]; rvy[k] -= cm[4]; rvz[k] -= cm[5];
+            }
+            if (tid == 0)
+                for (int c = 0; c < 3; ++c) { ctl->com[c] = cm[c]; ctl->com_vel[c] = cm[3 + c]; }
+        }
+        if (tid == 0) {
+            ctl->recenter = rec ? 1 : 0;
+            ctl->redo = rec ? 1 : 0;
+        }
+    }
+
+    /* write the monomer state back; for a collision also the monomers'
+     * cells (MPC::shiftParticles MPC.cpp:306-310 + sort MPC.cpp:221-235:
+     * shifted, never wrapped) */
+    const double sh[3] = {ctl->shift[0], ctl->shift[1], ctl->shift[2]};
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int i = tid + k * NT;
+        if (i >= N) continue;
+        const size_t g = 3 * (size_t)(mb + i);
+        d.mpos[g] = rx[k]; d.mpos[g + 1] = ry[k]; d.mpos[g + 2] = rz[k];
+        d.mvel[g] = rvx[k]; d.mvel[g + 1] = rvy[k]; d.mvel[g + 2] = rvz[k];
+        d.mforce[g] = S.fx[i]; d.mforce[g + 1] = S.fy[i]; d.mforce[g + 2] = S.fz[i];
+        d.mvpos[g] = rpx[k]; d.mvpos[g + 1] = rpy[k]; d.mvpos[g + 2] = rpz[k];
+        if (do_coll) {
+            const uint32_t ci = cell_index(rep.box, rx[k] + sh[0], ry[k] + sh[1], rz[k] + sh[2]);
             if (ci >= rep.ncells) atomicOr(&ctl->status, ST_CELL_OOB);
             d.mcell[mb + i] = ci < rep.ncells ? (uint32_t)rep.cell_base + ci : ~0u;
         }
@@ -1304,22 +1324,32 @@ hipError_t launch_prologue(const DevState& d, hipStream_t st, int do_collision, 
     return hipGetLastError();
 }
 
+template <int R>
+static hipError_t launch_md_r(const DevState& d, hipStream_t st, int n_md, int epot_last, int md_init,
+                              int do_collision, double md_dt, int ns, uint32_t cap, size_t lds) {
+    static size_t attr_set = 0;
+    if (lds > attr_set) {
+        const hipError_t e = hipFuncSetAttribute((const void*)k_md<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)lds);
+        if (e != hipSuccess) return e;
+        attr_set = lds;
+    }
+    hipLaunchKernelGGL(k_md<R>, dim3(d.n_rep), dim3(kMdThreads), lds, st, d, n_md, epot_last, md_init,
+                       do_collision, md_dt, ns, cap);
+    return hipGetLastError();
+}
+
 hipError_t launch_md(const DevState& d, hipStream_t st, int n_md, int epot_last, int md_init, int do_collision,
                      double md_dt) {
     if (d.m_tot == 0) return hipSuccess;
     const int ns = d.md_ns;
     const uint32_t cap = d.md_vl_cap;
     const size_t lds = md_lds_bytes(ns, cap);
-    static size_t attr_set = 0;
-    if (lds > attr_set) {
-        const hipError_t e = hipFuncSetAttribute((const void*)k_md, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                 (int)lds);
-        if (e != hipSuccess) return e;
-        attr_set = lds;
-    }
-    hipLaunchKernelGGL(k_md, dim3(d.n_rep), dim3(kMdThreads), lds, st, d, n_md, epot_last, md_init, do_collision,
-                       md_dt, ns, cap);
-    return hipGetLastError();
+    if (ns <= kMdThreads)
+        return launch_md_r<1>(d, st, n_md, epot_last, md_init, do_collision, md_dt, ns, cap, lds);
+    if (ns <= 2 * kMdThreads)
+        return launch_md_r<2>(d, st, n_md, epot_last, md_init, do_collision, md_dt, ns, cap, lds);
+    return launch_md_r<4>(d, st, n_md, epot_last, md_init, do_collision, md_dt, ns, cap, lds);
 }
 
 hipError_t launch_bin(const DevState& d, hipStream_t st, double h, int mode) {
