@@ -161,7 +161,7 @@ int build_monomers(spmd_sys* s) {
     if (int rc = dalloc(s, &d.mvpos, 3ull * mt)) return rc;
     if (int rc = dalloc(s, &d.mmass, mt)) return rc;
     if (int rc = dalloc(s, &d.mbtype, mt)) return rc;
-    if (int rc = dalloc(s, &d.vl, (size_t)mt * kVlPerMono)) return rc;
+    if (int rc = dalloc(s, &d.vl, (size_t)mt * kVlPerMono * 2)) return rc;  /* lists + rebuild scratch */
     if (int rc = dalloc(s, &d.bonds, std::max<size_t>(bonds.size(), 1))) return rc;
     if (int rc = dalloc(s, &d.mcell, mt)) return rc;
     if (mt) {

@@ -97,7 +97,8 @@ struct DevState {
     uint32_t n_scan_tiles;
     double *mpos, *mvel, *mforce, *mvpos, *mmass;
     uint8_t* mbtype;
-    uint32_t* vl;          /* Verlet half lists: i | j << 16, kVlPerMono per monomer */
+    uint32_t* vl;          /* Verlet half lists: i | j << 16, kVlPerMono per monomer; then as
+                              many entries of rebuild scratch rows */
     uint32_t* bonds;
     uint32_t* mcell;       /* global cell of each monomer at the last collision (~0u: none) */
     double* inj_axis;      /* 2 per cell, all replicas */

@@ -137,6 +137,7 @@ struct MdLds {
     float4* sp4;       /* ns */
     uint32_t* gcid;    /* ns */
     uint32_t* gcell;   /* grid cell ends after the fill (kGridMax + 1) */
+    uint32_t* cpre;    /* rebuild: pairs found per sorted slot, then their prefix (ns + 1) */
     uint32_t* bl;      /* bonds: i | j << 12 | anchor << 24 (ns) */
     uint32_t* pl;      /* half pair list: i | j << 16 (cap) */
     uint8_t* bt;       /* monomer is B-type (ns) */
@@ -153,15 +154,16 @@ __device__ __forceinline__ MdLds md_carve(unsigned char* base, int ns, uint32_t 
     L.sp4 = reinterpret_cast<float4*>(L.fx);                      /* 16 ns <= 24 ns bytes */
     L.gcid = reinterpret_cast<uint32_t*>(L.sp4 + ns);             /* + 4 ns */
     L.gcell = reinterpret_cast<uint32_t*>(L.flags + 16);
-    L.bl = L.gcell + kGridMax + 1;
+    L.cpre = L.gcell + kGridMax + 1;
+    L.bl = L.cpre + ns + 1;
     L.pl = L.bl + ns;
     L.bt = reinterpret_cast<uint8_t*>(L.pl + cap);
     return L;
 }
 
 size_t md_lds_bytes(int ns, uint32_t cap) {
-    return (size_t)6 * ns * 8 + 80 * 8 + 16 * 4 + (size_t)(kGridMax + 1) * 4 + (size_t)ns * 4 + (size_t)cap * 4 +
-           (size_t)ns + 64;
+    return (size_t)6 * ns * 8 + 80 * 8 + 16 * 4 + (size_t)(kGridMax + 1) * 4 + (size_t)(ns + 1) * 4 +
+           (size_t)ns * 4 + (size_t)cap * 4 + (size_t)ns + 64;
 }
 
 template <int NT>
@@ -297,13 +299,12 @@ __device__ __forceinline__ uint32_t md_thread_scan(uint32_t v, double* scratch_d
     return pre + inc - v;
 }
 
-/* The list candidates of the monomer at sorted slot p: the 3x3 rows of
- * three neighbouring cells (x-fastest cells, so each row is one contiguous
- * slot range).  WRITE=false counts the pairs j > i inside the list radius,
- * WRITE=true stores them from out[o]. */
-template <bool WRITE>
-__device__ __forceinline__ uint32_t md_scan_cands(const MdLds& S, int p, const int gd[3], uint32_t* out,
-                                                  uint32_t o, uint32_t ocap) {
+/* The list pairs of the monomer at sorted slot p: its candidates are the
+ * slots of the 3x3 rows of three neighbouring cells (x-fastest cells, so
+ * each row is one contiguous slot range; all 18 row bounds are read before
+ * the first candidate).  Pairs j > i inside the list radius go to
+ * scr[0..kVlPerMono); returns their number (more means overflow). */
+__device__ __forceinline__ uint32_t md_scan_cands(const MdLds& S, int p, const int gd[3], uint32_t* scr) {
     const float4 me = S.sp4[p];
     const uint32_t im = __float_as_uint(me.w);
     const int i = (int)(im & 0xFFFF);
@@ -313,30 +314,32 @@ __device__ __forceinline__ uint32_t md_scan_cands(const MdLds& S, int p, const i
     const int cy = (int)((cid / (uint32_t)gd[0]) % (uint32_t)gd[1]);
     const int cz = (int)(cid / ((uint32_t)gd[0] * (uint32_t)gd[1]));
     const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < gd[0] ? cx + 1 : gd[0] - 1;
+    uint32_t ra0[9], ra1[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const int y = cy + r % 3 - 1, z = cz + r / 3 - 1;
+        const bool ok = y >= 0 && y < gd[1] && z >= 0 && z < gd[2];
+        const int rowc = ok ? gd[0] * (y + gd[1] * z) : 0;
+        ra0[r] = ok && rowc + x0 ? S.gcell[rowc + x0 - 1] : 0u;
+        ra1[r] = ok ? S.gcell[rowc + x1] : 0u;
+    }
     uint32_t n = 0;
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int z = cz + dz;
-        if (z < 0 || z >= gd[2]) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int y = cy + dy;
-            if (y < 0 || y >= gd[1]) continue;
-            const int rowc = gd[0] * (y + gd[1] * z);
-            const uint32_t a0 = rowc + x0 ? S.gcell[rowc + x0 - 1] : 0u;
-            const uint32_t a1 = S.gcell[rowc + x1];
-            for (uint32_t q = a0; q < a1; q += 4) {
-                float4 c[4];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) c[k] = S.sp4[q + k < a1 ? q + k : a1 - 1];
+    for (int r = 0; r < 9; ++r) {
+        const uint32_t a1 = ra1[r];
+        for (uint32_t q = ra0[r]; q < a1; q += 4) {
+            float4 c[4];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const uint32_t jm = __float_as_uint(c[k].w);
-                    const int j = (int)(jm & 0xFFFF);
-                    const float dx = c[k].x - me.x, dy2 = c[k].y - me.y, dz2 = c[k].z - me.z;
-                    const float r2 = __builtin_fmaf(dz2, dz2, __builtin_fmaf(dy2, dy2, dx * dx));
-                    const bool hit = q + k < a1 && j > i && r2 <= ((bi && (jm >> 16)) ? kListBB2f : kListAA2f);
-                    if (WRITE && hit && o + n < ocap) out[o + n] = (uint32_t)i | ((uint32_t)j << 16);
-                    n += hit ? 1u : 0u;
-                }
+            for (int k = 0; k < 4; ++k) c[k] = S.sp4[q + k < a1 ? q + k : a1 - 1];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t jm = __float_as_uint(c[k].w);
+                const int j = (int)(jm & 0xFFFF);
+                const float dx = c[k].x - me.x, dy2 = c[k].y - me.y, dz2 = c[k].z - me.z;
+                const float r2 = __builtin_fmaf(dz2, dz2, __builtin_fmaf(dy2, dy2, dx * dx));
+                const bool hit = q + k < a1 && j > i && r2 <= ((bi && (jm >> 16)) ? kListBB2f : kListAA2f);
+                if (hit && n < (uint32_t)kVlPerMono) scr[n] = (uint32_t)i | ((uint32_t)j << 16);
+                n += hit ? 1u : 0u;
             }
         }
     }
@@ -347,13 +350,14 @@ __device__ __forceinline__ uint32_t md_scan_cands(const MdLds& S, int p, const i
  * inside the list radius, plus the COM-centred grid bounds check that
  * raises SIGSEGV in the reference.  The monomers are binned on a grid of
  * side >= 2 over their bounding box and sorted by cell; each lane takes
- * the monomers at its sorted slots, counts their pairs, and after a block
- * scan writes them at its own offset -- into LDS when the list fits there,
- * else into the replica's global region.  Leaves the force accumulators
+ * the monomers at its sorted slots and writes their pairs to a global
+ * scratch row of kVlPerMono entries per slot; after a block scan of the
+ * counts the rows are packed into the list -- in LDS when it fits there,
+ * else in the replica's global region.  Leaves the force accumulators
  * zeroed; the owning threads reset their Verlet reference positions.
  * Returns the entry count; *in_lds tells where the list is. */
-__device__ uint32_t md_rebuild(const MdLds& S, uint32_t* gpl, uint32_t gcap, const RepDev& rep, RepCtl* ctl, int N,
-                               uint32_t cap, bool* in_lds) {
+__device__ uint32_t md_rebuild(const MdLds& S, uint32_t* gpl, uint32_t gcap, uint32_t* gscr, const RepDev& rep,
+                               RepCtl* ctl, int N, uint32_t cap, bool* in_lds) {
     constexpr int NT = kMdThreads;
     const int tid = threadIdx.x;
     /* COM of the molecule (Box::center_of_mass_reference_frame) and the
@@ -438,17 +442,27 @@ __device__ uint32_t md_rebuild(const MdLds& S, uint32_t* gpl, uint32_t gcap, con
                                   __uint_as_float((uint32_t)i | (S.bt[i] ? 1u << 16 : 0u)));
     }
     __syncthreads();  /* gcell[c] now ends cell c */
-    /* count, scan, write */
-    uint32_t cnt = 0;
-    for (int p = tid; p < N; p += NT) cnt += md_scan_cands<false>(S, p, gd, nullptr, 0, 0);
-    uint32_t total;
-    const uint32_t off = md_thread_scan(cnt, S.red, &total);
+    /* pairs per sorted slot into scratch rows, prefix of the counts, pack */
+    bool over = false;
+    for (int p = tid; p < N; p += NT) {
+        uint32_t n = md_scan_cands(S, p, gd, gscr + (size_t)p * kVlPerMono);
+        over |= n > (uint32_t)kVlPerMono;
+        S.cpre[p] = n < (uint32_t)kVlPerMono ? n : (uint32_t)kVlPerMono;
+    }
+    if (over) atomicOr(&ctl->status, ST_VERLET_OVERFLOW);
+    __threadfence_block();
+    __syncthreads();
+    const uint32_t total = md_block_scan(S.cpre, N, S.red);
     const uint32_t lcap = cap < gcap ? cap : gcap;  /* an LDS list must also fit its global copy */
     const bool lds = total <= lcap;
     uint32_t* out = lds ? S.pl : gpl;
     const uint32_t ocap = lds ? lcap : gcap;
-    uint32_t o = off;
-    for (int p = tid; p < N; p += NT) o += md_scan_cands<true>(S, p, gd, out, o, ocap);
+    for (int p = tid; p < N; p += NT) {
+        const uint32_t o = S.cpre[p], n = S.cpre[p + 1] - o;
+        const uint32_t* src = gscr + (size_t)p * kVlPerMono;
+        for (uint32_t k = 0; k < n; ++k)
+            if (o + k < ocap) out[o + k] = src[k];
+    }
     if (!lds && total > gcap && tid == 0) atomicOr(&ctl->status, ST_VERLET_OVERFLOW);
     __syncthreads();
     /* the scratch lived in the force accumulators */
@@ -489,6 +503,7 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
     /* the replica's global pair-list region */
     uint32_t* const gpl = d.vl + (size_t)mb * kVlPerMono;
     const uint32_t gcap = (uint32_t)N * kVlPerMono;
+    uint32_t* const gscr = d.vl + ((size_t)d.m_tot + mb) * kVlPerMono;  /* rebuild scratch rows */
     const uint32_t nbond = rep.n_bond < (uint32_t)ns ? rep.n_bond : (uint32_t)ns;
 
     double rx[R], ry[R], rz[R], rvx[R], rvy[R], rvz[R], rpx[R], rpy[R], rpz[R], rim[R];
@@ -521,7 +536,7 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
 
     auto rebuild = [&]() {
         bool in_lds = false;
-        npl = md_rebuild(S, gpl, gcap, rep, ctl, N, cap, &in_lds);
+        npl = md_rebuild(S, gpl, gcap, gscr, rep, ctl, N, cap, &in_lds);
         lds_vl = in_lds;
         rebuilt = true;
 #pragma unroll
