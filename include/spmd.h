@@ -98,8 +98,10 @@ int spmd_add_star(spmd_sys* s, uint32_t replica, uint32_t A, uint32_t B, uint32_
                   double mass, double bond, double anchor_bond);
 /* Same star with seeded random-coil arms (no reference counterpart: the
  * reference builds straight arms only, which for 30-bead arms leave the
- * Verlet grid of Box.cpp:129-142).  Bead k>=2 sits at distance `bond` from
- * bead k-1 with r^2 >= 1 to all other beads. */
+ * Verlet grid of Box.cpp:129-142).  The first beads sit on a Fibonacci
+ * sphere of radius `anchor_bond`; bead k>=2 at distance `bond` from bead
+ * k-1 with r^2 >= 1 to all other beads (Philox draws; where the core is too
+ * crowded for that, the bond is lengthened by up to 30%). */
 int spmd_add_star_coil(spmd_sys* s, uint32_t replica, uint32_t A, uint32_t B, uint32_t arms,
                        double mass, double bond, double anchor_bond);
 int spmd_num_monomers(const spmd_sys* s, uint32_t replica, uint32_t* n);
@@ -162,6 +164,64 @@ int spmd_get_tuning(const spmd_sys* s, int* ppt, int* tile_cells);
 int spmd_profile_enable(spmd_sys* s, int on);
 int spmd_profile_read(spmd_sys* s, const char* name, double* avg_ms, int64_t* count);
 int spmd_profile_reset(spmd_sys* s);
+
+/* ---- learned implicit-solvent closure (no reference counterpart) ----
+ * A per-monomer closure of the MPC solvent: a network predicts, from 24
+ * features of a monomer's state, the mean and diagonal variance of the
+ * velocity change the solvent imparts at one collision; the star then
+ * advances by its own velocity-Verlet MD plus that stochastic kick, with no
+ * solvent particles.  Explicit MPC stays the fallback and the judge. */
+
+/* The network: MLP 24-128-128-128-6, SiLU, BF16 matrix-core kernels with
+ * FP32 accumulation, Adam.  Pointers named *_dev are device pointers; every
+ * call returns after its device work is complete. */
+typedef struct spmd_mlp spmd_mlp;
+int spmd_mlp_create(uint64_t seed, spmd_mlp** out);
+int spmd_mlp_destroy(spmd_mlp* m);
+const char* spmd_mlp_last_error(const spmd_mlp* m);
+/* 36,998: W0[128][24] b0[128] W1[128][128] b1 W2[128][128] b2 W3[6][128] b3[6] */
+uint32_t spmd_mlp_num_params(void);
+int spmd_mlp_get_params(spmd_mlp* m, float* host);
+int spmd_mlp_set_params(spmd_mlp* m, const float* host);
+/* feature standardisation (x - mu_x) / sd_x (24 each) and target scale sd_y (3) */
+int spmd_mlp_set_norm(spmd_mlp* m, const float* mu_x, const float* sd_x, const float* sd_y);
+int spmd_mlp_get_norm(const spmd_mlp* m, float* mu_x, float* sd_x, float* sd_y);
+/* out_dev[M][6] = (mean[3], log-variance[3]) in units of sd_y for x_dev[M][24] */
+int spmd_mlp_forward(spmd_mlp* m, const float* x_dev, float* out_dev, uint64_t M);
+/* gradient of the mean Gaussian negative log-likelihood of y_dev[M][3]
+ * (FP32, parameter order as above) and the loss */
+int spmd_mlp_gradient(spmd_mlp* m, const float* x_dev, const float* y_dev, uint64_t M, float* grad_host,
+                      double* loss);
+/* one Adam step (beta 0.9/0.999, eps 1e-8) on the mean loss of the batch */
+int spmd_mlp_train_step(spmd_mlp* m, const float* x_dev, const float* y_dev, uint64_t M, double lr, double* loss);
+/* `steps` Adam steps on minibatches of `batch` rows drawn with replacement
+ * (Philox keyed by seed) from x_dev[N][24], y_dev[N][3]; loss_hist[steps]
+ * optional (each entry costs a device round trip) */
+int spmd_mlp_fit(spmd_mlp* m, const float* x_dev, const float* y_dev, uint64_t N, uint32_t batch, uint32_t steps,
+                 double lr, uint64_t seed, double* loss_hist);
+/* plain device buffers for callers without their own device allocator */
+int spmd_dev_alloc(uint64_t bytes, void** p);
+int spmd_dev_free(void* p);
+int spmd_dev_upload(void* dst, const void* src, uint64_t bytes);
+int spmd_dev_download(void* dst, const void* src, uint64_t bytes);
+
+/* The 24 closure features of every monomer of every replica, global
+ * monomer order (replica r from its first monomer; see spmd_num_monomers),
+ * of the current state (pending MD substeps are run first). */
+int spmd_closure_features(spmd_sys* s, float* feat_dev);
+/* n_intervals closure intervals for all replicas: step_update velocity-
+ * Verlet substeps, check_bounds, then the network's kick on every monomer.
+ * With fmin/fmax (host arrays of 24, the trained feature range), monomers
+ * whose features leave it (by more than 25% of the range) are counted per
+ * replica into oor_host[n_replicas] (optional). */
+int spmd_closure_run(spmd_sys* s, spmd_mlp* m, int64_t n_intervals, const float* fmin, const float* fmax,
+                     uint32_t* oor_host);
+/* gyration tensor of one replica's star about its centre of mass
+ * (Box::calculate_gyration_tensor, Box.cpp:330-352), row-major 3x3 */
+int spmd_gyration_tensor(spmd_sys* s, uint32_t replica, double* g9);
+/* rotation frequency omega = I^-1 L of one replica's star in its centre-of-
+ * mass frame (Box::calculate_rotation_frequency, Box.cpp:354-379) */
+int spmd_rotation_frequency(spmd_sys* s, uint32_t replica, double* w3);
 
 #ifdef __cplusplus
 }

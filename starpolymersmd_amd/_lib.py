@@ -92,6 +92,27 @@ _SIGS = {
     "spmd_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "spmd_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, _d, _P(C.c_int64)]),
     "spmd_profile_reset": (C.c_int, [C.c_void_p]),
+    "spmd_mlp_create": (C.c_int, [C.c_uint64, _P(C.c_void_p)]),
+    "spmd_mlp_destroy": (C.c_int, [C.c_void_p]),
+    "spmd_mlp_last_error": (C.c_char_p, [C.c_void_p]),
+    "spmd_mlp_num_params": (C.c_uint32, []),
+    "spmd_mlp_get_params": (C.c_int, [C.c_void_p, _f]),
+    "spmd_mlp_set_params": (C.c_int, [C.c_void_p, _f]),
+    "spmd_mlp_set_norm": (C.c_int, [C.c_void_p, _f, _f, _f]),
+    "spmd_mlp_get_norm": (C.c_int, [C.c_void_p, _f, _f, _f]),
+    "spmd_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "spmd_mlp_gradient": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, _f, _d]),
+    "spmd_mlp_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_double, _d]),
+    "spmd_mlp_fit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double,
+                               C.c_uint64, _d]),
+    "spmd_dev_alloc": (C.c_int, [C.c_uint64, _P(C.c_void_p)]),
+    "spmd_dev_free": (C.c_int, [C.c_void_p]),
+    "spmd_dev_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "spmd_dev_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "spmd_closure_features": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "spmd_closure_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, _f, _f, _u32]),
+    "spmd_gyration_tensor": (C.c_int, [C.c_void_p, C.c_uint32, _d]),
+    "spmd_rotation_frequency": (C.c_int, [C.c_void_p, C.c_uint32, _d]),
 }
 
 _lib = None

@@ -1,0 +1,312 @@
+"""Learned implicit-solvent closure of the MPC solvent.
+
+The star keeps its own velocity-Verlet MD (FENE + LJ forces, exactly as in
+explicit runs); what the solvent does to it at each collision -- the
+per-monomer velocity change of MPC::collide/thermostat (MPC.cpp:239-299)
+with the monomers in the cells -- is replaced by a stochastic kick
+v += mu(f) + diag(sigma(f)) xi drawn from a network of the monomer's 24
+features f (starpolymersmd_amd/csrc/spmd_kernels.hip, k_closure_features).
+The network (MLP 24-128-128-128-6, SiLU) is trained on explicit MPC-MD
+runs of the library itself: mean (drift) and log-variance (noise) of the
+velocity change, Gaussian negative log-likelihood, Adam, with hand-written
+BF16 matrix-core forward/backward kernels (spmd_mlp.hip).
+
+Explicit MPC stays the fallback: features outside the trained range (by
+more than 25% of it) send the affected replica through explicit intervals.
+
+    from starpolymersmd_amd import closure
+    data = closure.generate(shears=[0.01, 0.05], L=32, arms=10, A=5, B=5, intervals=400)
+    net = closure.ClosureNet(seed=1)
+    net.fit(data, steps=500, batch=65536)
+    run = closure.ClosureRunner(net, data.fmin, data.fmax, shear=0.05, arms=10, A=5, B=5)
+    run.advance(1000)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import SpmdError, load, ptr
+from .system import MPCSystem, make_params
+
+N_FEAT = 24
+
+
+class DeviceBuffer:
+    """A device allocation of the library (spmd_dev_*)."""
+
+    def __init__(self, nbytes: int):
+        self._lib = load()
+        p = C.c_void_p()
+        rc = self._lib.spmd_dev_alloc(int(nbytes), C.byref(p))
+        if rc:
+            raise SpmdError(rc, f"device allocation of {nbytes} bytes")
+        self.p, self.nbytes = p, int(nbytes)
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    @property
+    def ptr(self) -> int:
+        return self.p.value
+
+    def upload(self, a: np.ndarray, offset: int = 0):
+        a = np.ascontiguousarray(a)
+        assert offset + a.nbytes <= self.nbytes
+        rc = self._lib.spmd_dev_upload(C.c_void_p(self.ptr + offset), a.ctypes.data_as(C.c_void_p), a.nbytes)
+        if rc:
+            raise SpmdError(rc, "upload")
+
+    def download(self, shape, dtype=np.float32, offset: int = 0) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        assert offset + out.nbytes <= self.nbytes
+        rc = self._lib.spmd_dev_download(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr + offset), out.nbytes)
+        if rc:
+            raise SpmdError(rc, "download")
+        return out
+
+    def close(self):
+        if getattr(self, "p", None) is not None and self.p.value:
+            self._lib.spmd_dev_free(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ClosureNet:
+    """The network behind spmd_mlp_* (C ABI)."""
+
+    def __init__(self, seed: int = 1):
+        self._lib = load()
+        h = C.c_void_p()
+        rc = self._lib.spmd_mlp_create(seed, C.byref(h))
+        if rc:
+            raise SpmdError(rc, "spmd_mlp_create")
+        self.h = h
+        self.seed = seed
+        self.n_params = int(self._lib.spmd_mlp_num_params())
+        self.norm = None
+        self._fit_calls = 0
+
+    def _chk(self, rc):
+        if rc:
+            raise SpmdError(rc, self._lib.spmd_mlp_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.spmd_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # parameters and normalisation
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.n_params, np.float32)
+        self._chk(self._lib.spmd_mlp_get_params(self.h, ptr(p, C.c_float)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        self._chk(self._lib.spmd_mlp_set_params(self.h, ptr(p, C.c_float)))
+
+    def set_norm(self, mu_x, sd_x, sd_y):
+        mu_x, sd_x, sd_y = (np.ascontiguousarray(a, np.float32) for a in (mu_x, sd_x, sd_y))
+        self._chk(self._lib.spmd_mlp_set_norm(self.h, ptr(mu_x, C.c_float), ptr(sd_x, C.c_float),
+                                              ptr(sd_y, C.c_float)))
+        self.norm = (mu_x, sd_x, sd_y)
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        """(M, 24) features -> (M, 6): mean and log-variance in units of sd_y."""
+        x = np.ascontiguousarray(x, np.float32)
+        xb = DeviceBuffer.from_numpy(x)
+        ob = DeviceBuffer(x.shape[0] * 6 * 4)
+        self._chk(self._lib.spmd_mlp_forward(self.h, C.c_void_p(xb.ptr), C.c_void_p(ob.ptr), x.shape[0]))
+        return ob.download((x.shape[0], 6))
+
+    def gradient(self, x: np.ndarray, y: np.ndarray):
+        xb = DeviceBuffer.from_numpy(np.asarray(x, np.float32))
+        yb = DeviceBuffer.from_numpy(np.asarray(y, np.float32))
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_double()
+        self._chk(self._lib.spmd_mlp_gradient(self.h, C.c_void_p(xb.ptr), C.c_void_p(yb.ptr), len(x),
+                                              ptr(g, C.c_float), C.byref(loss)))
+        return loss.value, g
+
+    def fit(self, data: "ClosureData", steps: int = 1000, batch: int = 65536, lr: float = 1e-3,
+            record_loss: bool = True) -> np.ndarray:
+        """Adam on minibatches drawn with replacement on the device; sets the
+        normalisation from the data first.  Returns the per-step loss (NaN
+        where not recorded)."""
+        self.set_norm(data.mu_x, data.sd_x, data.sd_y)
+        hist = np.full(steps, np.nan)
+        self._fit_calls += 1
+        self._chk(self._lib.spmd_mlp_fit(self.h, C.c_void_p(data.X.ptr), C.c_void_p(data.Y.ptr), data.n,
+                                         min(batch, data.n), steps, lr, self.seed * 1000003 + self._fit_calls,
+                                         ptr(hist, C.c_double) if record_loss else None))
+        return hist
+
+    def save(self, path):
+        mu, sd, sy = self.norm
+        np.savez(path, params=self.get_params(), mu_x=mu, sd_x=sd, sd_y=sy)
+
+    def load(self, path):
+        z = np.load(path)
+        self.set_params(z["params"])
+        self.set_norm(z["mu_x"], z["sd_x"], z["sd_y"])
+
+
+@dataclass
+class ClosureData:
+    X: DeviceBuffer    # (n, 24) float32 features
+    Y: DeviceBuffer    # (n, 3)  float32 velocity change at the collision
+    n: int
+    mu_x: np.ndarray
+    sd_x: np.ndarray
+    sd_y: np.ndarray
+    fmin: np.ndarray
+    fmax: np.ndarray
+    intervals: int
+    seconds: float
+
+    @classmethod
+    def from_numpy(cls, X, Y, intervals=0, seconds=0.0) -> "ClosureData":
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        Xd = X.astype(np.float64)
+        mu, sd = Xd.mean(0), Xd.std(0)
+        sd = np.where(sd > 1e-3 * np.maximum(1.0, np.abs(mu)), sd, np.maximum(1.0, np.abs(mu)))
+        return cls(X=DeviceBuffer.from_numpy(X), Y=DeviceBuffer.from_numpy(Y), n=len(X),
+                   mu_x=mu.astype(np.float32), sd_x=sd.astype(np.float32),
+                   sd_y=Y.astype(np.float64).std(0).astype(np.float32), fmin=X.min(0), fmax=X.max(0),
+                   intervals=intervals, seconds=seconds)
+
+
+def _star_system(L, n_c, shear, seed, arms, A, B, mass, step_update, md_dt):
+    g = MPCSystem(make_params(L=(L, L, L), n_c=n_c, shear=shear, step_update=step_update, md_dt=md_dt, seed=seed))
+    g.add_star(A, B, arms, mass, coil=True)
+    g.md_init()
+    return g
+
+
+def generate(shears, L=32, arms=10, A=5, B=5, mass=10.0, intervals=400, warmup=50, n_c=10, step_update=10,
+             md_dt=0.01, seed=11) -> ClosureData:
+    """Explicit MPC-MD runs, one per shear rate: after `warmup` intervals,
+    each interval records every monomer's features just before the
+    collision and its velocity change by the collision (intervals whose
+    collision recentred the star are skipped: the recentring subtracts the
+    star's COM velocity)."""
+    h = md_dt * step_update
+    feats, targs = [], []
+    t_start = time.perf_counter()
+    n_int = 0
+    for si, shear in enumerate(shears):
+        g = _star_system(L, n_c, shear, seed + 101 * si, arms, A, B, mass, step_update, md_dt)
+        g.initialize()
+        t = 0
+        g.run(t, 1 + warmup * step_update, h)
+        t += 1 + warmup * step_update
+        nm = g.num_monomers()
+        row = nm * N_FEAT * 4
+        buf = DeviceBuffer(intervals * row)
+        dv = np.empty((intervals, nm, 3))
+        keep = np.ones(intervals, bool)
+        for k in range(intervals):
+            g.run(t, step_update - 1, h)  # the interval's substeps and MPC::step calls without collision
+            t += step_update - 1
+            g.propagate()
+            g.closure_features(buf.ptr + k * row)
+            _, v0, _ = g.get_monomers()
+            g.step(t, h)                  # the collision
+            t += 1
+            _, v1, _ = g.get_monomers()
+            dv[k] = v1 - v0
+            keep[k] = not g.last_collision()[1]
+            n_int += 1
+        g.close()
+        X = buf.download((intervals, nm, N_FEAT))
+        buf.close()
+        feats.append(X[keep].reshape(-1, N_FEAT))
+        targs.append(dv[keep].reshape(-1, 3))
+    return ClosureData.from_numpy(np.concatenate(feats), np.concatenate(targs), intervals=n_int,
+                                  seconds=time.perf_counter() - t_start)
+
+
+def shape_stats(G):
+    """Rg^2 = tr G, alignment angle chi = 1/2 atan2(2 G_xy, G_xx - G_yy) (rad)
+    of a gyration tensor."""
+    return float(np.trace(G)), 0.5 * math.atan2(2.0 * G[0, 1], G[0, 0] - G[1, 1])
+
+
+class ClosureRunner:
+    """Replicas of one star advanced by MD + the learned solvent kick, with
+    explicit MPC fallback for replicas whose features leave the trained
+    range."""
+
+    def __init__(self, net: ClosureNet, fmin, fmax, shear, arms, A, B, mass=10.0, replicas=1, L=48,
+                 step_update=10, md_dt=0.01, seed=5, shears=None, fallback_intervals=20, n_c_fallback=10,
+                 box=64):
+        """L: box of the explicit fallback runs; box: the closure systems'
+        box, which without solvent only frames the Verlet grid (it must
+        hold the stretched star: monomers within box/2 of its centre)."""
+        self.net, self.fmin, self.fmax = net, np.asarray(fmin, np.float32), np.asarray(fmax, np.float32)
+        self.shears = list(shears) if shears is not None else [shear] * replicas
+        self.L, self.S, self.dt = L, step_update, md_dt
+        self.star = (arms, A, B, mass)
+        self.seed = seed
+        self.sys = MPCSystem([make_params(L=(box, box, box), n_c=0, shear=s, step_update=step_update, md_dt=md_dt,
+                                          seed=seed + r) for r, s in enumerate(self.shears)])
+        for r in range(len(self.shears)):
+            self.sys.add_star(A, B, arms, mass, r=r, coil=True)
+        self.sys.md_init()
+        self.fallback_intervals = fallback_intervals
+        self.n_c_fallback = n_c_fallback
+        self.fallbacks = 0
+        self.t = 0
+
+    def advance(self, n_intervals: int, chunk: int = 10):
+        done = 0
+        while done < n_intervals:
+            k = min(chunk, n_intervals - done)
+            oor = self.sys.closure_run(self.net, k, self.fmin, self.fmax)
+            done += k
+            for r in np.nonzero(oor)[0]:
+                self._explicit(int(r))
+        self.t += n_intervals
+
+    def _explicit(self, r):
+        """Run the replica's star through explicit MPC intervals, then resume."""
+        arms, A, B, mass = self.star
+        g = MPCSystem(make_params(L=(self.L,) * 3, n_c=self.n_c_fallback, shear=self.shears[r],
+                                  step_update=self.S, md_dt=self.dt, seed=self.seed + 1000 + self.fallbacks))
+        g.add_star(A, B, arms, mass, coil=True)
+        pos, vel, _ = self.sys.get_monomers(r)
+        g.set_monomers(pos - pos.mean(axis=0), vel)
+        g.md_init()
+        g.initialize()
+        h = self.S * self.dt
+        g.run(0, 1 + self.fallback_intervals * self.S, h)
+        p2, v2, _ = g.get_monomers()
+        g.close()
+        self.sys.set_monomers(p2, v2, r=r)
+        self.sys.md_init()
+        self.fallbacks += 1
+
+    def gyration(self, r=0):
+        return self.sys.gyration_tensor(r)

@@ -61,6 +61,13 @@ struct spmd_sys {
     bool prof = false;
     ProfEvents pev[6];
     std::vector<size_t> alloc_bytes;
+    /* learned closure workspace */
+    float* cl_feat = nullptr;      /* m_tot x 24 */
+    float* cl_out = nullptr;       /* m_tot x 6 */
+    float* cl_range = nullptr;     /* fmin[24], fmax[24] */
+    uint32_t cl_cap = 0;
+    uint32_t cl_k = 0;             /* closure interval counter (Philox) */
+    double* mom = nullptr;         /* star moments: 12 doubles */
 };
 
 namespace {
@@ -392,20 +399,17 @@ void coil_positions(StarHost& st, const RepDev& rep, const spmd_params& p, uint3
     std::vector<double> P(3ull * N, 0.0);
     std::vector<uint8_t> placed(N, 0);
     placed[0] = 1;
-    const unsigned n = (unsigned)std::ceil(std::sqrt(arms));
-    /* first beads on the reference's (phi, theta) grid, all before any arm grows */
-    unsigned arm_count = 0;
-    for (unsigned i = 0; i < n; i++) {
-        const double phi = 2. * kPi * i / n;
-        for (unsigned j = 1; j < n + 1; j++) {
-            if (arm_count >= arms) break;
-            const double theta = kPi * j / (n + 1);
-            const double dir[3] = {std::cos(phi) * std::sin(theta), std::sin(phi) * std::sin(theta), std::cos(theta)};
-            const unsigned idx = arm_count * per_arm + 1;
-            for (int q = 0; q < 3; ++q) P[3 * idx + q] = anchor_bond * dir[q];
-            placed[idx] = 1;
-            arm_count++;
-        }
+    /* first beads on a Fibonacci sphere of radius anchor_bond (even spacing:
+     * the reference's (phi, theta) grid crowds them near the poles for many
+     * arms), all placed before any arm grows */
+    const double golden = kPi * (3.0 - std::sqrt(5.0));
+    for (unsigned a = 0; a < arms; a++) {
+        const double z = 1.0 - (2.0 * a + 1.0) / arms;
+        const double rxy = std::sqrt(1.0 - z * z), phi = golden * a;
+        const double dir[3] = {rxy * std::cos(phi), rxy * std::sin(phi), z};
+        const unsigned idx = a * per_arm + 1;
+        for (int q = 0; q < 3; ++q) P[3 * idx + q] = anchor_bond * dir[q];
+        placed[idx] = 1;
     }
     auto min_d2 = [&](const double* c, uint32_t skip) {
         double m = 1e300;
@@ -417,27 +421,44 @@ void coil_positions(StarHost& st, const RepDev& rep, const spmd_params& p, uint3
         }
         return m;
     };
-    /* grow all arms one bead at a time */
+    /* Bead idx next to bead idx-1, r^2 >= 1 to every other placed bead:
+     * 4096 Philox draws that must not step inward by more than 0.5 (1024 at
+     * the nominal bond, then 10% longer bonds per 1024, at most 1.3x), then
+     * 4096 without that rule; the best draw is kept.  `stream` selects an
+     * independent set of draws. */
+    auto place = [&](unsigned idx, unsigned stream) -> bool {
+        placed[idx] = 0;
+        const double* prev = &P[3 * (idx - 1)];
+        const double rprev = std::sqrt((prev[0] * prev[0] + prev[1] * prev[1]) + prev[2] * prev[2]);
+        double best[3] = {0, 0, 0}, best_m = -1.0;
+        for (uint32_t att = 0; att < 8192; ++att) {
+            uint32_t w[4];
+            draw4(p.seed, rep.replica, TAG_STAR_INIT, idx, 1 + stream, att, w);
+            const double z = 2.0 * u01(w[0]) - 1.0, ph = 2. * kPi * u01(w[1]);
+            const double sz = std::sqrt(1.0 - z * z);
+            const double bl = bond * (1.0 + 0.1 * ((att % 4096) / 1024));
+            const double c[3] = {prev[0] + bl * sz * std::cos(ph), prev[1] + bl * sz * std::sin(ph),
+                                 prev[2] + bl * z};
+            const double rc = std::sqrt((c[0] * c[0] + c[1] * c[1]) + c[2] * c[2]);
+            const double m = (rc >= rprev - 0.5 || att >= 4096) ? min_d2(c, idx - 1) : -0.5;
+            if (m > best_m) { best_m = m; best[0] = c[0]; best[1] = c[1]; best[2] = c[2]; }
+            if (m >= 1.0) break;
+        }
+        for (int q = 0; q < 3; ++q) P[3 * idx + q] = best[q];
+        placed[idx] = 1;
+        return best_m >= 1.0;
+    };
+    /* grow all arms one bead at a time; a bead that finds no room moves its
+     * predecessor (new draws) and tries again */
     for (unsigned k = 2; k <= per_arm; k++) {
         for (unsigned a = 0; a < arms; a++) {
             const unsigned idx = a * per_arm + k;
-            const double* prev = &P[3 * (idx - 1)];
-            const double rprev = std::sqrt((prev[0] * prev[0] + prev[1] * prev[1]) + prev[2] * prev[2]);
-            double best[3] = {0, 0, 0}, best_m = -1.0;
-            for (uint32_t att = 0; att < 512; ++att) {
-                uint32_t w[4];
-                draw4(p.seed, rep.replica, TAG_STAR_INIT, idx, 1, att, w);
-                const double z = 2.0 * u01(w[0]) - 1.0, ph = 2. * kPi * u01(w[1]);
-                const double sz = std::sqrt(1.0 - z * z);
-                const double c[3] = {prev[0] + bond * sz * std::cos(ph), prev[1] + bond * sz * std::sin(ph),
-                                     prev[2] + bond * z};
-                const double rc = std::sqrt((c[0] * c[0] + c[1] * c[1]) + c[2] * c[2]);
-                const double m = rc >= rprev - 0.5 ? min_d2(c, idx - 1) : -0.5;
-                if (m > best_m) { best_m = m; best[0] = c[0]; best[1] = c[1]; best[2] = c[2]; }
-                if (m >= 1.0) break;
+            bool ok = place(idx, 0);
+            for (unsigned s2 = 1; !ok && s2 <= 16 && k > 2; ++s2) {
+                placed[idx] = 0;
+                place(idx - 1, s2);
+                ok = place(idx, s2);
             }
-            for (int q = 0; q < 3; ++q) P[3 * idx + q] = best[q];
-            placed[idx] = 1;
         }
     }
     double com[3] = {0, 0, 0};
@@ -602,6 +623,7 @@ int spmd_destroy(spmd_sys* s) {
     dfree(d.vl); dfree(d.bonds); dfree(d.mcell);
     dfree(d.inj_axis); dfree(d.inj_gamma); dfree(d.reps); dfree(d.ctl); dfree(d.pblk_base);
     dfree(d.tile_base); dfree(d.red);
+    dfree(s->cl_feat); dfree(s->cl_out); dfree(s->cl_range); dfree(s->mom);
     if (s->side) (void)hipStreamSynchronize(s->side);
     if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
     if (s->ev_join) (void)hipEventDestroy(s->ev_join);
@@ -964,6 +986,94 @@ int spmd_epot(spmd_sys* s, uint32_t r, double* epot) {
     if (int rc = flush_md(s)) return rc;
     HIPCHK(s, hipStreamSynchronize(s->stream));
     HIPCHK(s, hipMemcpy(epot, &s->d.ctl[r].epot, 8, hipMemcpyDeviceToHost));
+    return SPMD_OK;
+}
+
+/* ------------------------------------------------------ learned closure */
+
+static int closure_ws(spmd_sys* s) {
+    if (int rc = ensure_monomers(s)) return rc;
+    const uint32_t m = s->d.m_tot;
+    if (m > s->cl_cap) {
+        dfree(s->cl_feat); dfree(s->cl_out);
+        if (int rc = dalloc(s, &s->cl_feat, (size_t)m * kClosureFeatures)) return rc;
+        if (int rc = dalloc(s, &s->cl_out, (size_t)m * 6)) return rc;
+        s->cl_cap = m;
+    }
+    if (!s->cl_range)
+        if (int rc = dalloc(s, &s->cl_range, 2 * kClosureFeatures)) return rc;
+    return SPMD_OK;
+}
+
+int spmd_closure_features(spmd_sys* s, float* feat_dev) {
+    if (!s || !feat_dev) return SPMD_ERR_ARG;
+    if (int rc = flush_md(s)) return rc;
+    if (int rc = ensure_monomers(s)) return rc;
+    LAUNCH(s, 0, s->stream, launch_closure_features(s->d, s->stream, feat_dev, nullptr, nullptr));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return check_status(s);
+}
+
+int spmd_closure_run(spmd_sys* s, spmd_mlp* m, int64_t n_intervals, const float* fmin, const float* fmax,
+                     uint32_t* oor_host) {
+    if (!s || !m || n_intervals < 0 || (!fmin) != (!fmax)) return SPMD_ERR_ARG;
+    if (int rc = flush_md(s)) return rc;
+    if (int rc = closure_ws(s)) return rc;
+    if (s->d.m_tot == 0) return fail(s, SPMD_ERR_STATE, "no monomers");
+    float mu[24], sd[24], sy[3];
+    if (int rc = spmd_mlp_get_norm(m, mu, sd, sy)) return fail(s, rc, "reading the network's target scale");
+    const float* dmin = nullptr;
+    const float* dmax = nullptr;
+    if (fmin) {
+        HIPCHK(s, hipMemcpy(s->cl_range, fmin, kClosureFeatures * 4ull, hipMemcpyHostToDevice));
+        HIPCHK(s, hipMemcpy(s->cl_range + kClosureFeatures, fmax, kClosureFeatures * 4ull, hipMemcpyHostToDevice));
+        dmin = s->cl_range;
+        dmax = s->cl_range + kClosureFeatures;
+        for (int r = 0; r < s->d.n_rep; ++r) HIPCHK(s, hipMemset(&s->d.ctl[r].closure_oor, 0, 4));
+    }
+    const int S = (int)s->step_update;
+    for (int64_t it = 0; it < n_intervals; ++it) {
+        LAUNCH(s, 0, s->stream, launch_md(s->d, s->stream, S, 0, 0, 1, s->md_dt));
+        LAUNCH(s, 0, s->stream, launch_closure_features(s->d, s->stream, s->cl_feat, dmin, dmax));
+        if (int rc = spmd_mlp_forward_on(m, s->cl_feat, s->cl_out, s->d.m_tot, s->stream))
+            return fail(s, rc, std::string("closure network: ") + spmd_mlp_last_error(m));
+        LAUNCH(s, 0, s->stream, launch_closure_kick(s->d, s->stream, s->cl_out, sy[0], sy[1], sy[2], s->cl_k));
+        s->cl_k++;
+    }
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    if (oor_host) {
+        std::vector<RepCtl> c(s->d.n_rep);
+        HIPCHK(s, hipMemcpy(c.data(), s->d.ctl, c.size() * sizeof(RepCtl), hipMemcpyDeviceToHost));
+        for (int r = 0; r < s->d.n_rep; ++r) oor_host[r] = fmin ? c[r].closure_oor : 0u;
+    }
+    return check_status(s);
+}
+
+int spmd_gyration_tensor(spmd_sys* s, uint32_t r, double* g9) {
+    if (!valid_rep(s, r) || !g9) return SPMD_ERR_ARG;
+    if (int rc = flush_md(s)) return rc;
+    if (int rc = ensure_monomers(s)) return rc;
+    if (!s->mom)
+        if (int rc = dalloc(s, &s->mom, 12)) return rc;
+    LAUNCH(s, 0, s->stream, launch_star_moments(s->d, s->stream, (int)r, s->mom));
+    double out[12];
+    HIPCHK(s, hipMemcpyAsync(out, s->mom, sizeof out, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    for (int q = 0; q < 9; ++q) g9[q] = out[q];
+    return SPMD_OK;
+}
+
+int spmd_rotation_frequency(spmd_sys* s, uint32_t r, double* w3) {
+    if (!valid_rep(s, r) || !w3) return SPMD_ERR_ARG;
+    if (int rc = flush_md(s)) return rc;
+    if (int rc = ensure_monomers(s)) return rc;
+    if (!s->mom)
+        if (int rc = dalloc(s, &s->mom, 12)) return rc;
+    LAUNCH(s, 0, s->stream, launch_star_moments(s->d, s->stream, (int)r, s->mom));
+    double out[12];
+    HIPCHK(s, hipMemcpyAsync(out, s->mom, sizeof out, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    for (int q = 0; q < 3; ++q) w3[q] = out[9 + q];
     return SPMD_OK;
 }
 

@@ -76,7 +76,7 @@ struct RepCtl {
     uint32_t n_rebuild;
     int32_t redo;                /* the speculative sort (no recentring) must be redone */
     uint32_t vl_count;           /* entries of the star's Verlet half list */
-    uint32_t pad2;
+    uint32_t closure_oor;        /* monomers whose closure features left the trained range */
 };
 
 struct DevState {
@@ -132,8 +132,23 @@ hipError_t launch_sort_redo(const DevState& d, hipStream_t st, double h);
 hipError_t launch_perm(const DevState& d, hipStream_t st, int mode);
 hipError_t launch_collide(const DevState& d, hipStream_t st, double h);
 hipError_t launch_fluid_init(const DevState& d, hipStream_t st);
+/* learned closure: 24 features per monomer (all replicas, global monomer
+ * order) with an optional range check against [fmin, fmax] (device arrays
+ * of 24, null for none); the drift + noise kick from the network output
+ * (m_tot x 6: mean and log-variance in units of sdy) */
+constexpr int kClosureFeatures = 24;
+hipError_t launch_closure_features(const DevState& d, hipStream_t st, float* feat, const float* fmin,
+                                   const float* fmax);
+/* gyration tensor (9) and rotation frequency (3) of one replica's star */
+hipError_t launch_star_moments(const DevState& d, hipStream_t st, int replica, double* out);
+hipError_t launch_closure_kick(const DevState& d, hipStream_t st, const float* out, float sy0, float sy1, float sy2,
+                               uint32_t k);
 hipError_t launch_temperature(const DevState& d, hipStream_t st, int replica, uint32_t ntiles);
 
 }  // namespace spmd
+
+/* the closure network's forward pass enqueued on a caller's stream (spmd_mlp.hip) */
+struct spmd_mlp;
+int spmd_mlp_forward_on(spmd_mlp* m, const float* x_dev, float* out_dev, uint64_t M, hipStream_t st);
 
 #endif

@@ -1332,6 +1332,188 @@ __global__ __launch_bounds__(256) void k_temperature(DevState d, int replica) {
     if (threadIdx.x == 0) { d.red[2ull * lt] = te; d.red[2ull * lt + 1] = tf; }
 }
 
+/* =================================================== learned closure */
+
+/* Features of the closure network, one workgroup per replica (its star),
+ * written in global monomer order, 24 per monomer:
+ *   0-2  r = x - x_com                 3-5  v - v_com
+ *   6-8  u = v - (shear y, 0, 0)       (velocity relative to the imposed flow)
+ *   9-11 bond vector to the neighbour nearer the core, 12-14 to the one
+ *        farther out (zero where absent; the core's are zero)
+ *   15   monomers within r < 1.5       16 B type   17 core (degree > 2)
+ *   18   shear rate
+ *   19-21 mean velocity of the monomers within r < 1.5 minus v (zero if none)
+ *   22   1/m           23 bonds of the monomer
+ * With fmin/fmax, a monomer with any feature outside its trained range
+ * widened by 25% counts in RepCtl::closure_oor. */
+__global__ __launch_bounds__(512) void k_closure_features(DevState d, float* feat, const float* fmin,
+                                                           const float* fmax) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int NT = 512;
+    const int r = blockIdx.x, tid = threadIdx.x;
+    const RepDev& rep = d.reps[r];
+    const int N = (int)rep.n_mono;
+    const uint32_t mb = rep.mono_base;
+    float4* sp = reinterpret_cast<float4*>(smem);                 /* N: position, B flag */
+    float4* sv = sp + kMaxMono;                                   /* N: velocity */
+    uint32_t* prv = reinterpret_cast<uint32_t*>(sv + kMaxMono);  /* N */
+    uint32_t* nxt = prv + kMaxMono;                               /* N */
+    uint32_t* deg = nxt + kMaxMono;                               /* N */
+    __shared__ double sred[8];
+    double cs[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = tid; i < N; i += NT) {
+        const size_t g = 3 * (size_t)(mb + i);
+        const double x = d.mpos[g], y = d.mpos[g + 1], z = d.mpos[g + 2];
+        cs[0] += x; cs[1] += y; cs[2] += z;
+        cs[3] += d.mvel[g]; cs[4] += d.mvel[g + 1]; cs[5] += d.mvel[g + 2];
+        sp[i] = make_float4((float)x, (float)y, (float)z, d.mbtype[mb + i] ? 1.0f : 0.0f);
+        sv[i] = make_float4((float)d.mvel[g], (float)d.mvel[g + 1], (float)d.mvel[g + 2], 0.0f);
+        prv[i] = (uint32_t)i; nxt[i] = (uint32_t)i; deg[i] = 0u;
+    }
+    double com[6];
+    for (int c = 0; c < 6; ++c) com[c] = block_sum<NT>(cs[c], sred) / (double)(N > 0 ? N : 1);
+    __syncthreads();
+    for (uint32_t q = tid; q < rep.n_bond; q += NT) {
+        const uint32_t e = d.bonds[rep.bond_base + q];
+        const uint32_t i = e & 0xFFF, j = (e >> 12) & 0xFFF;  /* i < j: i nearer the core */
+        atomicAdd(&deg[i], 1u); atomicAdd(&deg[j], 1u);
+        prv[j] = i;
+        nxt[i] = j;
+    }
+    __syncthreads();
+    const double shear = rep.box.shear;
+    uint32_t oor = 0;
+    for (int i = tid; i < N; i += NT) {
+        const size_t g = 3 * (size_t)(mb + i);
+        const double x = d.mpos[g], y = d.mpos[g + 1], z = d.mpos[g + 2];
+        const double vx = d.mvel[g], vy = d.mvel[g + 1], vz = d.mvel[g + 2];
+        const bool core = deg[i] > 2u;
+        const float4 me = sp[i];
+        int cnt = 0;
+        float nvx = 0.0f, nvy = 0.0f, nvz = 0.0f;
+        for (int j = 0; j < N; ++j) {
+            const float4 o = sp[j];
+            const float dx = o.x - me.x, dy = o.y - me.y, dz = o.z - me.z;
+            if (j != i && __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < 2.25f) {
+                const float4 ov = sv[j];
+                ++cnt;
+                nvx += ov.x; nvy += ov.y; nvz += ov.z;
+            }
+        }
+        float f[kClosureFeatures];
+        f[0] = (float)(x - com[0]); f[1] = (float)(y - com[1]); f[2] = (float)(z - com[2]);
+        f[3] = (float)(vx - com[3]); f[4] = (float)(vy - com[4]); f[5] = (float)(vz - com[5]);
+        const double ux = vx - shear * y;
+        f[6] = (float)ux; f[7] = (float)vy; f[8] = (float)vz;
+        const int ip = core ? i : (int)prv[i], in = core ? i : (int)nxt[i];
+        f[9] = sp[ip].x - me.x; f[10] = sp[ip].y - me.y; f[11] = sp[ip].z - me.z;
+        f[12] = sp[in].x - me.x; f[13] = sp[in].y - me.y; f[14] = sp[in].z - me.z;
+        f[15] = (float)cnt;
+        f[16] = me.w;
+        f[17] = core ? 1.0f : 0.0f;
+        f[18] = (float)shear;
+        const float4 mv = sv[i];
+        const float ic = cnt ? 1.0f / (float)cnt : 0.0f;
+        f[19] = cnt ? nvx * ic - mv.x : 0.0f;
+        f[20] = cnt ? nvy * ic - mv.y : 0.0f;
+        f[21] = cnt ? nvz * ic - mv.z : 0.0f;
+        f[22] = (float)(1.0 / d.mmass[mb + i]);
+        f[23] = (float)deg[i];
+        float* o = feat + (size_t)(mb + i) * kClosureFeatures;
+        bool out = false;
+#pragma unroll
+        for (int c = 0; c < kClosureFeatures; ++c) {
+            o[c] = f[c];
+            if (fmin) {
+                const float lo = fmin[c], hi = fmax[c], pad = 0.25f * (hi - lo);
+                out |= f[c] < lo - pad || f[c] > hi + pad;
+            }
+        }
+        oor += out ? 1u : 0u;
+    }
+    if (fmin && oor) atomicAdd(&d.ctl[r].closure_oor, oor);
+}
+
+/* replica of global monomer m (per lane: a wave may span replicas) */
+__device__ __forceinline__ int find_rep_mono(const DevState& d, uint32_t m) {
+    int lo = 0, hi = d.n_rep;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (d.reps[mid].mono_base <= m) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+/* The closure's solvent kick in place of the MPC collision: per monomer
+ * v += mu + sigma xi, mu = out[0..2] sdy, sigma = exp(out[3..5] / 2) sdy,
+ * xi standard normal from Philox (index = global monomer, counter k =
+ * interval, tag TAG_CLOSURE) through Box-Muller. */
+__global__ __launch_bounds__(256) void k_closure_kick(DevState d, const float* out, float sy0, float sy1, float sy2,
+                                                       uint32_t k) {
+    const uint32_t m = blockIdx.x * 256 + threadIdx.x;
+    if (m >= d.m_tot) return;
+    const int r = find_rep_mono(d, m);
+    const RepDev& rep = d.reps[r];
+    uint32_t w[4];
+    draw4(rep.seed, rep.replica, TAG_CLOSURE, m - rep.mono_base, k, 0, w);
+    const double r0 = sqrt(-2.0 * log(u01(w[0]))), r1 = sqrt(-2.0 * log(u01(w[2])));
+    const double xi[3] = {r0 * cos(2.0 * kPi * u01(w[1])), r0 * sin(2.0 * kPi * u01(w[1])),
+                          r1 * cos(2.0 * kPi * u01(w[3]))};
+    const float* o = out + (size_t)m * 6;
+    const float sy[3] = {sy0, sy1, sy2};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double s = fmin(fmax((double)o[3 + c], -12.0), 12.0);
+        d.mvel[3 * (size_t)m + c] += ((double)o[c] + exp(0.5 * s) * xi[c]) * sy[c];
+    }
+}
+
+/* ======================================================= observables */
+
+/* Box::calculate_gyration_tensor (Box.cpp:330-352) and
+ * Box::calculate_rotation_frequency (Box.cpp:354-379) of one replica's
+ * star in its centre-of-mass frame (Box::center_of_mass_reference_frame,
+ * Box.cpp:81-102): out[0..8] = G (row-major), out[9..11] = omega = I^-1 L. */
+__global__ __launch_bounds__(256) void k_star_moments(DevState d, int replica, double* out) {
+    __shared__ double sred[4];
+    const RepDev& rep = d.reps[replica];
+    const int N = (int)rep.n_mono;
+    const uint32_t mb = rep.mono_base;
+    double c[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < N; i += 256) {
+        const size_t g = 3 * (size_t)(mb + i);
+        for (int q = 0; q < 3; ++q) { c[q] += d.mpos[g + q]; c[3 + q] += d.mvel[g + q]; }
+    }
+    double com[6];
+    for (int q = 0; q < 6; ++q) com[q] = block_sum<256>(c[q], sred) / (double)(N > 0 ? N : 1);
+    double m[15];  /* G (6: xx xy xz yy yz zz), I (6), L (3) */
+    for (int q = 0; q < 15; ++q) m[q] = 0.0;
+    for (int i = threadIdx.x; i < N; i += 256) {
+        const size_t g = 3 * (size_t)(mb + i);
+        const double x = d.mpos[g] - com[0], y = d.mpos[g + 1] - com[1], z = d.mpos[g + 2] - com[2];
+        const double vx = d.mvel[g] - com[3], vy = d.mvel[g + 1] - com[4], vz = d.mvel[g + 2] - com[5];
+        m[0] += x * x; m[1] += x * y; m[2] += x * z; m[3] += y * y; m[4] += y * z; m[5] += z * z;
+        const double r2 = (x * x + y * y) + z * z;
+        m[6] += r2 - x * x; m[7] -= x * y; m[8] -= x * z; m[9] += r2 - y * y; m[10] -= y * z; m[11] += r2 - z * z;
+        m[12] += y * vz - z * vy; m[13] += z * vx - x * vz; m[14] += x * vy - y * vx;
+    }
+    for (int q = 0; q < 15; ++q) m[q] = block_sum<256>(m[q], sred);
+    if (threadIdx.x == 0) {
+        const double inv = 1.0 / (double)(N > 0 ? N : 1);
+        const double G[9] = {m[0], m[1], m[2], m[1], m[3], m[4], m[2], m[4], m[5]};
+        for (int q = 0; q < 9; ++q) out[q] = G[q] * inv;
+        /* omega = I^-1 L by Cramer's rule (I symmetric positive definite for
+         * a non-collinear star) */
+        const double a = m[6], b = m[7], cc = m[8], e = m[9], f = m[10], k = m[11];
+        const double det = a * (e * k - f * f) - b * (b * k - f * cc) + cc * (b * f - e * cc);
+        const double L0 = m[12], L1 = m[13], L2 = m[14];
+        const double w0 = (L0 * (e * k - f * f) - b * (L1 * k - f * L2) + cc * (L1 * f - e * L2)) / det;
+        const double w1 = (a * (L1 * k - f * L2) - L0 * (b * k - f * cc) + cc * (b * L2 - L1 * cc)) / det;
+        const double w2 = (a * (e * L2 - L1 * f) - b * (b * L2 - L1 * cc) + L0 * (b * f - e * cc)) / det;
+        out[9] = w0; out[10] = w1; out[11] = w2;
+    }
+}
+
 /* ========================================================= launchers */
 
 hipError_t launch_prologue(const DevState& d, hipStream_t st, int do_collision, int n_delrx, double delrx_dt) {
@@ -1429,6 +1611,33 @@ hipError_t launch_temperature(const DevState& d, hipStream_t st, int replica, ui
     case 256: hipLaunchKernelGGL(k_temperature<256>, dim3(ntiles), dim3(256), 0, st, d, replica); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_closure_features(const DevState& d, hipStream_t st, float* feat, const float* fmin,
+                                   const float* fmax) {
+    if (d.m_tot == 0) return hipSuccess;
+    const size_t lds = (size_t)kMaxMono * (16 + 16 + 12);
+    static bool attr = false;
+    if (!attr) {
+        const hipError_t e = hipFuncSetAttribute((const void*)k_closure_features,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL(k_closure_features, dim3(d.n_rep), dim3(512), lds, st, d, feat, fmin, fmax);
+    return hipGetLastError();
+}
+
+hipError_t launch_closure_kick(const DevState& d, hipStream_t st, const float* out, float sy0, float sy1, float sy2,
+                               uint32_t k) {
+    if (d.m_tot == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_closure_kick, dim3((d.m_tot + 255) / 256), dim3(256), 0, st, d, out, sy0, sy1, sy2, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_star_moments(const DevState& d, hipStream_t st, int replica, double* out) {
+    hipLaunchKernelGGL(k_star_moments, dim3(1), dim3(256), 0, st, d, replica, out);
     return hipGetLastError();
 }
 

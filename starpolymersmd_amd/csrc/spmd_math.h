@@ -40,7 +40,7 @@ SPMD_HD void philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32
 
 /* Stream layout: counter = (index, collision number, replica<<8 | tag,
  * block), key = seed. */
-enum : uint32_t { TAG_SHIFT = 1, TAG_AXIS = 2, TAG_GAMMA = 3, TAG_FLUID_INIT = 4, TAG_STAR_INIT = 5 };
+enum : uint32_t { TAG_SHIFT = 1, TAG_AXIS = 2, TAG_GAMMA = 3, TAG_FLUID_INIT = 4, TAG_STAR_INIT = 5, TAG_CLOSURE = 6 };
 
 SPMD_HD void draw4(uint64_t seed, uint32_t replica, uint32_t tag, uint32_t index, uint32_t k,
                    uint32_t block, uint32_t out[4]) {

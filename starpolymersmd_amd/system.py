@@ -170,6 +170,33 @@ class MPCSystem:
         self._chk(self._lib.spmd_epot(self.h, r, C.byref(v)))
         return v.value
 
+    def gyration_tensor(self, r: int = 0) -> np.ndarray:
+        """Box::calculate_gyration_tensor (Box.cpp:330-352), 3x3."""
+        g = np.empty(9)
+        self._chk(self._lib.spmd_gyration_tensor(self.h, r, ptr(g, C.c_double)))
+        return g.reshape(3, 3)
+
+    def rotation_frequency(self, r: int = 0) -> np.ndarray:
+        """Box::calculate_rotation_frequency (Box.cpp:354-379)."""
+        w = np.empty(3)
+        self._chk(self._lib.spmd_rotation_frequency(self.h, r, ptr(w, C.c_double)))
+        return w
+
+    def closure_features(self, feat_dev_ptr: int):
+        """24 closure features of every monomer into device memory
+        (float32, total monomers x 24)."""
+        self._chk(self._lib.spmd_closure_features(self.h, C.c_void_p(feat_dev_ptr)))
+
+    def closure_run(self, mlp, n_intervals: int, fmin=None, fmax=None):
+        """n_intervals of MD + the network's solvent kick; returns the number
+        of monomers per replica whose features left [fmin, fmax]."""
+        oor = np.zeros(self.n_replicas, np.uint32)
+        lo = None if fmin is None else np.ascontiguousarray(fmin, np.float32)
+        hi = None if fmax is None else np.ascontiguousarray(fmax, np.float32)
+        self._chk(self._lib.spmd_closure_run(self.h, mlp.h, int(n_intervals), ptr(lo, C.c_float),
+                                             ptr(hi, C.c_float), ptr(oor, C.c_uint32)))
+        return oor
+
     def md_stats(self, r: int = 0):
         """(Verlet rebuilds so far, entries of the current half lists)."""
         a, b = C.c_uint32(), C.c_uint32()
