@@ -10,7 +10,8 @@ velocity-Verlet substeps per collision.  A "step" is one collision interval
 
 Prints one JSON line: particle-updates/s (solvent particles x collisions per
 second), the roofline of the collision pipeline measured with HIP events on
-the library's stream, and the FP64 CPU oracle timed on a bounded sample.
+the library's stream, the learned closure's star-steps/s against explicit
+MPC on the same GPU, and the FP64 CPU oracle timed on a bounded sample.
 
     python bench.py [--gpus 1] [--steps 1000] [--warmup 100]
 """
@@ -74,6 +75,83 @@ def cpu_baseline(w, intervals, gpu_star):
                       f"(10 MD substeps each), 1 thread, {el:.1f} s"}
 
 
+def closure_bench(replicas=256, explicit_replicas=16, check_intervals=200):
+    """Learned closure at batch (256 stars, f=30 arms of 20 A + 20 B beads,
+    1,201 monomers, shear rates uniform in [5e-3, 5e-2]) against explicit
+    MPC-MD of the same star at L=48, rho=10 on the same GPU: star-steps/s
+    (MD steps x stars per second), the network trained here on explicit
+    runs of the library (8 shear rates), fallbacks included."""
+    import numpy as np
+
+    from starpolymersmd_amd import MPCSystem, make_params
+    from starpolymersmd_amd import closure as cl
+
+    arms, A, B, S, dt = 30, 20, 20, 10, 0.01
+    shears_train = list(np.geomspace(5e-3, 5e-2, 8))
+    data = cl.generate(shears_train, L=64, arms=arms, A=A, B=B, intervals=40, warmup=10)
+    net = cl.ClosureNet(1)
+    t0 = time.perf_counter()
+    hist = net.fit(data, steps=300, batch=65536, lr=2e-3)
+    t_fit = time.perf_counter() - t0
+    rng = np.random.default_rng(0)
+    shears = list(rng.uniform(5e-3, 5e-2, replicas))
+    run = cl.ClosureRunner(net, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=48, shears=shears)
+    run.advance(2)
+    run.sys.sync()
+    n = 20
+    t0 = time.perf_counter()
+    run.advance(n, chunk=n)
+    run.sys.sync()
+    el = time.perf_counter() - t0
+    closure_rate = replicas * n * S / el
+    fallbacks = run.fallbacks
+    del run
+    # explicit MPC-MD of the same star, batched replicas at L=48
+    g = MPCSystem([make_params(L=(48, 48, 48), n_c=10, shear=float(shears[r]), step_update=S, md_dt=dt, seed=3 + r)
+                   for r in range(explicit_replicas)])
+    for r in range(explicit_replicas):
+        g.add_star(A, B, arms, 10.0, r=r, coil=True)
+    g.md_init()
+    g.initialize()
+    g.run(0, 1 + 5 * S, 0.1)
+    g.sync()
+    t0 = time.perf_counter()
+    g.run(1 + 5 * S, n * S, 0.1)
+    g.sync()
+    el = time.perf_counter() - t0
+    explicit_rate = explicit_replicas * n * S / el
+    g.close()
+    # accuracy cross-check on one replica: explicit vs closure, same start
+    shear = 0.02
+    e = MPCSystem(make_params(L=(48, 48, 48), n_c=10, shear=shear, step_update=S, md_dt=dt, seed=21))
+    e.add_star(A, B, arms, 10.0, coil=True)
+    e.md_init()
+    e.initialize()
+    c = cl.ClosureRunner(net, data.fmin, data.fmax, shear=shear, arms=arms, A=A, B=B, L=48, seed=21)
+    ge, gc = [], []
+    t = 0
+    for k in range(check_intervals // 10):
+        e.run(t, 10 * S + (1 if t == 0 else 0), 0.1)
+        t += 10 * S + (1 if t == 0 else 0)
+        c.advance(10)
+        if k >= check_intervals // 40:
+            ge.append(cl.shape_stats(e.gyration_tensor()))
+            gc.append(cl.shape_stats(c.gyration()))
+    ge, gc = np.array(ge), np.array(gc)
+    return {
+        "star_steps_per_s": closure_rate, "explicit_star_steps_per_s": explicit_rate,
+        "speedup": closure_rate / explicit_rate,
+        "workload": f"{replicas} stars f={arms} {A}A+{B}B (1,201 monomers) closure vs {explicit_replicas} stars "
+                    f"explicit MPC L=48 rho=10, shear U[5e-3,5e-2], {n} intervals of {S} MD steps",
+        "fallbacks": fallbacks,
+        "train": {"samples": data.n, "shear_rates": 8, "steps": 300, "batch": 65536, "seconds": t_fit,
+                  "loss_first": float(hist[0]), "loss_last": float(np.mean(hist[-10:]))},
+        "check": {"shear": shear, "intervals": check_intervals,
+                  "explicit_Rg2": float(ge[:, 0].mean()), "closure_Rg2": float(gc[:, 0].mean()),
+                  "explicit_chi": float(ge[:, 1].mean()), "closure_chi": float(gc[:, 1].mean())},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +161,7 @@ def main():
     ap.add_argument("--cpu-intervals", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (ppt 4, 128-cell tiles)")
+    ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure throughput section")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run N MiB device-to-device copies (counter calibration)")
     a = ap.parse_args()
@@ -156,6 +235,9 @@ def main():
         "md_us_per_interval": prof["md"] * 1e3,
         "temperature": T,
     }
+    if not a.no_closure:
+        g.close()
+        out["closure"] = closure_bench()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_intervals, star0)
     print(json.dumps(out), flush=True)

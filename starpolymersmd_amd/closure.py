@@ -11,8 +11,9 @@ runs of the library itself: mean (drift) and log-variance (noise) of the
 velocity change, Gaussian negative log-likelihood, Adam, with hand-written
 BF16 matrix-core forward/backward kernels (spmd_mlp.hip).
 
-Explicit MPC stays the fallback: features outside the trained range (by
-more than 25% of it) send the affected replica through explicit intervals.
+Explicit MPC stays the fallback: a monomer whose regime features (star
+extent, bonds, crowding, shear rate; RANGE_CHECKED) leave the trained range
+by more than 25% of it sends its replica through explicit intervals.
 
     from starpolymersmd_amd import closure
     data = closure.generate(shears=[0.01, 0.05], L=32, arms=10, A=5, B=5, intervals=400)
@@ -254,6 +255,22 @@ def shape_stats(G):
     return float(np.trace(G)), 0.5 * math.atan2(2.0 * G[0, 1], G[0, 0] - G[1, 1])
 
 
+# features whose trained range defines the closure's regime: the star's
+# extent and bond geometry, crowding, shear rate, mass and connectivity.
+# Velocity-like features (3-8, 19-21) have Gaussian tails and the type
+# flags are categorical; they are not range-checked.
+RANGE_CHECKED = (0, 1, 2, 9, 10, 11, 12, 13, 14, 15, 18, 22, 23)
+
+
+def regime_range(fmin, fmax, checked=RANGE_CHECKED):
+    """fmin/fmax for spmd_closure_run with the unchecked features open."""
+    lo = np.full(N_FEAT, -np.inf, np.float32)
+    hi = np.full(N_FEAT, np.inf, np.float32)
+    for c in checked:
+        lo[c], hi[c] = fmin[c], fmax[c]
+    return lo, hi
+
+
 class ClosureRunner:
     """Replicas of one star advanced by MD + the learned solvent kick, with
     explicit MPC fallback for replicas whose features leave the trained
@@ -265,7 +282,8 @@ class ClosureRunner:
         """L: box of the explicit fallback runs; box: the closure systems'
         box, which without solvent only frames the Verlet grid (it must
         hold the stretched star: monomers within box/2 of its centre)."""
-        self.net, self.fmin, self.fmax = net, np.asarray(fmin, np.float32), np.asarray(fmax, np.float32)
+        self.net = net
+        self.fmin, self.fmax = regime_range(np.asarray(fmin, np.float32), np.asarray(fmax, np.float32))
         self.shears = list(shears) if shears is not None else [shear] * replicas
         self.L, self.S, self.dt = L, step_update, md_dt
         self.star = (arms, A, B, mass)
