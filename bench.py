@@ -45,6 +45,23 @@ def make_system(w):
     return g
 
 
+TRAFFIC_PROFILE = "profiles/collision_pipeline_L64_rho10.md"
+
+
+def measured_traffic():
+    """HBM bytes per collision of the pipeline from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (scaled by a device copy of known size);
+    None when the summary is absent."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_collision:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(w, intervals, gpu_star):
     """FP64 oracle (faithful single-threaded restatement of the reference
     CPU path) on the same workload shape for a bounded number of intervals."""
@@ -215,6 +232,7 @@ def main():
     prof = {k: g.profile_read(k)[0] for k in ("md", "bin", "scan", "perm", "collide")}
     g.profile(False)
     pipe_ms = prof["bin"] + prof["scan"] + prof["perm"] + prof["collide"]
+    traffic = measured_traffic()
     achieved = B_ALG * n_f / (pipe_ms * 1e-3) / 1e9
     T = g.temperature()
 
@@ -228,7 +246,8 @@ def main():
                    "tuning": {"ppt": g.get_tuning()[0], "tile_cells": g.get_tuning()[1],
                               "sweep_ms": None if tune is None else tune["timings"]}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": TRAFFIC_PROFILE if traffic else None,
                      "kernel": "collision pipeline k_bin+k_scan+k_perm+k_collide",
                      "alg_bytes_per_launch": B_ALG * n_f, "avg_ms": pipe_ms,
                      "per_kernel_ms": prof},
