@@ -147,6 +147,43 @@ int spmd_get_cell_index(spmd_sys* s, uint32_t replica, uint32_t* fluid_cells, ui
 int spmd_temperature(spmd_sys* s, uint32_t replica, double* T);
 /* Molecule::Epot after the last propagate (Box.cpp:243-286). */
 int spmd_epot(spmd_sys* s, uint32_t replica, double* epot);
+/* gyration tensor of one replica's star about its centre of mass
+ * (Box::calculate_gyration_tensor, Box.cpp:330-352), row-major 3x3 */
+int spmd_gyration_tensor(spmd_sys* s, uint32_t replica, double* g9);
+/* rotation frequency omega = I^-1 L of one replica's star in its centre-of-
+ * mass frame (Box::calculate_rotation_frequency, Box.cpp:354-379) */
+int spmd_rotation_frequency(spmd_sys* s, uint32_t replica, double* w3);
+/* Box::calculate_patches_new (Box.cpp:434-507): groups of arms joined by a
+ * B-B pair of negative energy; number of groups of more than one arm and
+ * their mean size in arms (stars of at most 128 arms). */
+int spmd_patches(spmd_sys* s, uint32_t replica, double* n_patch, double* patch_size);
+/* One row of statistics*.dat (star-polymers.cpp:307, 470-482) without the
+ * step: row[0] Epot per monomer (Box::print_Epot Box.cpp:605-613, from the
+ * last propagate that computed it), row[1] Ekin and row[2] 2 Ekin/(3(N-1))
+ * (Box::print_Ekin_and_Temperature Box.cpp:635-646), row[3..4] patches,
+ * row[5] R_gyr = sqrt(tr G), row[6..14] G, row[15..17] omega. */
+int spmd_statistics(spmd_sys* s, uint32_t replica, double* row);
+
+/* ---- solvent observers (Analysis functors applied over MPC::Fluid,
+ * MPC.h:110-116); one pass over a replica's solvent on the device ---- */
+/* VelocityX::operator() Velocity_x.h:76-96: for every solvent particle,
+ * position and velocity relative to the star's centre of mass (Box::COM_Pos
+ * and COM_Vel as Box::center_of_mass_reference_frame Box.cpp:85-102 sets
+ * them; zero without a star), wrapped by MPC::wrap(pos, vel) MPC.cpp:492-495,
+ * binned by floor(x/width) (map 0) and floor(y/width) (map 1).  Bin b of
+ * map a has key (b0[a] + b) * width; sums[(a * max_bins + b) * 3 + c] =
+ * sum vx, sum vy, count.  nbins[a] = floor(L_a/2/width) - b0[a] + 1 <= 1024. */
+int spmd_velocity_profile(spmd_sys* s, uint32_t replica, double width, uint32_t max_bins, int32_t* b0,
+                          uint32_t* nbins, double* sums);
+/* Fourier_Autocorr::operator() Fourier_Autocorr.h:55-72: V_j = (1/N) sum_i
+ * v_i exp(i 2 pi x_ij / L_j) over the solvent for j = x, y, z, with the
+ * component along e_j set to zero.  modes[(j * 3 + c) * 2 + {0,1}] =
+ * (Re, Im) of component c of V_j. */
+int spmd_fourier_modes(spmd_sys* s, uint32_t replica, double* modes);
+/* VelocityDistribution::operator() Vel_distribution.h:35-40: counts of
+ * floor(|v|/width) over the solvent, bins 0..nbins-1 (nbins <= 1024);
+ * counts[nbins] holds the speeds beyond. */
+int spmd_speed_histogram(spmd_sys* s, uint32_t replica, double width, uint32_t nbins, uint64_t* counts);
 
 /* Verlet rebuilds so far and entries of the current (half) lists of one
  * replica's star -- diagnostics, no reference counterpart. */
@@ -216,12 +253,6 @@ int spmd_closure_features(spmd_sys* s, float* feat_dev);
  * replica into oor_host[n_replicas] (optional). */
 int spmd_closure_run(spmd_sys* s, spmd_mlp* m, int64_t n_intervals, const float* fmin, const float* fmax,
                      uint32_t* oor_host);
-/* gyration tensor of one replica's star about its centre of mass
- * (Box::calculate_gyration_tensor, Box.cpp:330-352), row-major 3x3 */
-int spmd_gyration_tensor(spmd_sys* s, uint32_t replica, double* g9);
-/* rotation frequency omega = I^-1 L of one replica's star in its centre-of-
- * mass frame (Box::calculate_rotation_frequency, Box.cpp:354-379) */
-int spmd_rotation_frequency(spmd_sys* s, uint32_t replica, double* w3);
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,9 @@
 #include <cstdio>
 #include <cstring>
 #include <forward_list>
+#include <map>
+#include <array>
+#include <complex>
 #include <random>
 #include <vector>
 
@@ -854,6 +857,127 @@ void orc_rotation_frequency(orc_sys* s, double* omega3) {
 }
 
 double orc_ekin_monomers(orc_sys* s) { return mol_ekin(*s); }
+
+/* VelocityX::operator() (Velocity_x.h:76-96) over every fluid particle, with
+ * Box::COM_Pos/COM_Vel refreshed first as the driver does before applying
+ * it (star-polymers.cpp:467-486; zero without a molecule).  The two
+ * std::map's are returned in key order: n_out[a] entries of map a (0: keyed
+ * by x, 1: by y), keys[a*cap+i], sums[(a*cap+i)*3+c] = sum vx, sum vy,
+ * count. */
+void orc_velocity_profile(orc_sys* s, double width, uint32_t cap, uint32_t* n_out, double* keys, double* sums) {
+    com_reference_frame(*s);
+    std::map<double, std::array<double, 3>> maps[2];
+    for (uint64_t i = 0; i < s->n_fluid; ++i) {
+        double x[3], v[3];
+        for (int c = 0; c < 3; ++c) {
+            x[c] = s->pos[3 * i + c] - s->com_pos[c];
+            v[c] = s->vel[3 * i + c] - s->com_vel[c];
+        }
+        wrap(*s, x, v);
+        for (int a = 0; a < 2; ++a) {
+            auto& e = maps[a][std::floor(x[a] / width) * width];
+            e[0] += v[0];
+            e[1] += v[1];
+            e[2] += 1.0;
+        }
+    }
+    for (int a = 0; a < 2; ++a) {
+        uint32_t k = 0;
+        for (auto& kv : maps[a]) {
+            if (k < cap) {
+                keys[a * cap + k] = kv.first;
+                for (int c = 0; c < 3; ++c) sums[3 * ((size_t)a * cap + k) + c] = kv.second[c];
+            }
+            ++k;
+        }
+        n_out[a] = k;
+    }
+}
+
+/* Fourier_Autocorr::operator() (Fourier_Autocorr.h:55-72) over the fluid:
+ * V_j = (1/N) sum_i v_i exp(i 2 pi x_ij / L), then V_j -= e_j (e_j . V_j);
+ * modes[(3j+c)*2 + {0,1}] = Re, Im. */
+void orc_fourier_modes(orc_sys* s, double* modes) {
+    std::complex<double> V[3][3] = {};
+    const std::complex<double> I(0., 1.);
+    for (int j = 0; j < 3; ++j) {
+        const double nk = 2 * M_PI / s->L[j];
+        for (uint64_t i = 0; i < s->n_fluid; ++i) {
+            const std::complex<double> ph = std::exp(I * (nk * s->pos[3 * i + j]));
+            for (int c = 0; c < 3; ++c) V[j][c] += s->vel[3 * i + c] * ph;
+        }
+        for (int c = 0; c < 3; ++c) V[j][c] /= (double)s->n_fluid;
+        V[j][j] = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            modes[(3 * j + c) * 2] = V[j][c].real();
+            modes[(3 * j + c) * 2 + 1] = V[j][c].imag();
+        }
+    }
+}
+
+/* VelocityDistribution::operator() (Vel_distribution.h:35-40): the map
+ * floor(|v|/width)*width -> count, returned as counts of bins 0..nbins-1
+ * and the rest in counts[nbins]. */
+void orc_speed_histogram(orc_sys* s, double width, uint32_t nbins, uint64_t* counts) {
+    std::map<double, double> dist;
+    for (uint64_t i = 0; i < s->n_fluid; ++i) {
+        const V3 v{{s->vel[3 * i], s->vel[3 * i + 1], s->vel[3 * i + 2]}};
+        dist[std::floor(std::sqrt(dot3(v, v)) / width) * width] += 1.0;
+    }
+    for (uint32_t b = 0; b <= nbins; ++b) counts[b] = 0;
+    for (auto& kv : dist) {
+        const double b = std::round(kv.first / width);
+        counts[b < nbins ? (uint32_t)b : nbins] += (uint64_t)kv.second;
+    }
+}
+
+/* Box::calculate_patches_new, Box.cpp:434-507 (one molecule): arm i and arm
+ * k are bonded if some B bead of i and B bead of k have negative pair
+ * energy (Box::calculate_epot, Box.cpp:386-399); clusters are merged in the
+ * reference's order (cluster i absorbs every later cluster that holds an
+ * arm bonded to one of its arms, while walking its growing member list). */
+void orc_patches(orc_sys* s, double* n_patch, double* patch_size) {
+    const unsigned f = s->arms, per = s->A + s->B;
+    std::vector<std::vector<bool>> bond(f, std::vector<bool>(f, false));
+    auto epot = [&](unsigned a, unsigned b) {
+        const Mono& p1 = s->mono[a];
+        const Mono& p2 = s->mono[b];
+        V3 d{{p2.pos[0] - p1.pos[0], p2.pos[1] - p1.pos[1], p2.pos[2] - p1.pos[2]}};
+        const double r2 = dot3(d, d);
+        double e = (p1.type && p2.type) ? bb_potential(r2, s->p.lambda) : aa_potential(r2);
+        for (int n : p1.neighbors)
+            if (n == (int)b) e += (p1.anchor || p2.anchor) ? fene_anchor_potential(r2) : fene_potential(r2);
+        return e;
+    };
+    for (unsigned i = 0; i < f; ++i)
+        for (unsigned k = 0; k < f; ++k) {
+            if (i == k || bond[i][k]) continue;
+            for (unsigned j = s->A + 1; j <= per && !bond[i][k]; ++j)
+                for (unsigned l = s->A + 1; l <= per; ++l)
+                    if (epot(i * per + j, k * per + l) < 0.0) { bond[i][k] = true; break; }
+        }
+    std::vector<std::vector<unsigned>> cl(f);
+    for (unsigned i = 0; i < f; ++i) cl[i].push_back(i);
+    for (unsigned i = 0; i < f; ++i) {
+        for (size_t pos = 0; pos < cl[i].size(); ++pos) {
+            const unsigned first = cl[i][pos];
+            for (unsigned j = i + 1; j < f; ++j) {
+                bool hit = false;
+                for (unsigned arm : cl[j]) hit = hit || bond[first][arm];
+                if (hit) {
+                    cl[i].insert(cl[i].end(), cl[j].begin(), cl[j].end());
+                    cl[j].clear();
+                }
+            }
+        }
+    }
+    double np = 0.0, sz = 0.0;
+    for (auto& c : cl)
+        if (c.size() > 1) { np += 1.0; sz += (double)c.size(); }
+    if (np > 1) sz /= np;
+    *n_patch = np;
+    *patch_size = sz;
+}
 
 double orc_fene_force(double r2) { return fene_force(r2); }
 double orc_fene_anchor_force(double r2) { return fene_anchor_force(r2); }

@@ -93,6 +93,12 @@ void     orc_com_frame(orc_sys* s);
 double   orc_gyration_tensor(orc_sys* s, double* G9);
 void     orc_rotation_frequency(orc_sys* s, double* omega3);
 double   orc_ekin_monomers(orc_sys* s);
+/* solvent observers and patches (lib/Analysis/Velocity_x.h:76-96,
+ * Fourier_Autocorr.h:55-72, Vel_distribution.h:35-40; Box.cpp:434-507) */
+void     orc_velocity_profile(orc_sys* s, double width, uint32_t cap, uint32_t* n_out, double* keys, double* sums);
+void     orc_fourier_modes(orc_sys* s, double* modes);
+void     orc_speed_histogram(orc_sys* s, double width, uint32_t nbins, uint64_t* counts);
+void     orc_patches(orc_sys* s, double* n_patch, double* patch_size);
 
 /* Potentials (Potentials.h:14-120), exposed for known-answer tests. */
 double   orc_fene_force(double r2);

@@ -113,6 +113,11 @@ _SIGS = {
     "spmd_closure_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, _f, _f, _u32]),
     "spmd_gyration_tensor": (C.c_int, [C.c_void_p, C.c_uint32, _d]),
     "spmd_rotation_frequency": (C.c_int, [C.c_void_p, C.c_uint32, _d]),
+    "spmd_patches": (C.c_int, [C.c_void_p, C.c_uint32, _d, _d]),
+    "spmd_statistics": (C.c_int, [C.c_void_p, C.c_uint32, _d]),
+    "spmd_velocity_profile": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_uint32, _P(C.c_int32), _u32, _d]),
+    "spmd_fourier_modes": (C.c_int, [C.c_void_p, C.c_uint32, _d]),
+    "spmd_speed_histogram": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_uint32, _P(C.c_uint64)]),
 }
 
 _lib = None

@@ -21,6 +21,7 @@ namespace {
 
 struct StarHost {
     uint32_t n = 0;
+    uint32_t arms = 0, A = 0, B = 0;
     std::vector<double> pos, vel, force, mass;
     std::vector<uint8_t> btype, anchor;
     std::vector<std::pair<uint32_t, uint32_t>> bonds;  /* directed (owner, other) */
@@ -67,7 +68,8 @@ struct spmd_sys {
     float* cl_range = nullptr;     /* fmin[24], fmax[24] */
     uint32_t cl_cap = 0;
     uint32_t cl_k = 0;             /* closure interval counter (Philox) */
-    double* mom = nullptr;         /* star moments: 12 doubles */
+    double* mom = nullptr;         /* star moments (kStarMoments) and patches (2) */
+    unsigned long long* obs = nullptr;  /* solvent observer sums */
 };
 
 namespace {
@@ -323,6 +325,9 @@ void build_star(StarHost& st, const RepDev& rep, const spmd_params& p, uint32_t 
     const double kPi = 3.14159265358979323846;
     const uint32_t per_arm = A + B, N = per_arm * arms + 1;
     st.n = N;
+    st.arms = arms;
+    st.A = A;
+    st.B = B;
     st.pos.assign(3ull * N, 0.0);
     st.vel.assign(3ull * N, 0.0);
     st.force.assign(3ull * N, 0.0);
@@ -623,7 +628,7 @@ int spmd_destroy(spmd_sys* s) {
     dfree(d.vl); dfree(d.bonds); dfree(d.mcell);
     dfree(d.inj_axis); dfree(d.inj_gamma); dfree(d.reps); dfree(d.ctl); dfree(d.pblk_base);
     dfree(d.tile_base); dfree(d.red);
-    dfree(s->cl_feat); dfree(s->cl_out); dfree(s->cl_range); dfree(s->mom);
+    dfree(s->cl_feat); dfree(s->cl_out); dfree(s->cl_range); dfree(s->mom); dfree(s->obs);
     if (s->side) (void)hipStreamSynchronize(s->side);
     if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
     if (s->ev_join) (void)hipEventDestroy(s->ev_join);
@@ -1049,31 +1054,167 @@ int spmd_closure_run(spmd_sys* s, spmd_mlp* m, int64_t n_intervals, const float*
     return check_status(s);
 }
 
-int spmd_gyration_tensor(spmd_sys* s, uint32_t r, double* g9) {
-    if (!valid_rep(s, r) || !g9) return SPMD_ERR_ARG;
+/* star moments (kStarMoments doubles, see launch_star_moments) of replica r
+ * into s->mom and the host array out */
+static int star_moments(spmd_sys* s, uint32_t r, double* out) {
     if (int rc = flush_md(s)) return rc;
     if (int rc = ensure_monomers(s)) return rc;
     if (!s->mom)
-        if (int rc = dalloc(s, &s->mom, 12)) return rc;
-    LAUNCH(s, 0, s->stream, launch_star_moments(s->d, s->stream, (int)r, s->mom));
-    double out[12];
-    HIPCHK(s, hipMemcpyAsync(out, s->mom, sizeof out, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(s, hipStreamSynchronize(s->stream));
+        if (int rc = dalloc(s, &s->mom, kStarMoments + 2)) return rc;
+    HIPCHK(s, launch_star_moments(s->d, s->stream, (int)r, s->mom));
+    if (out) {
+        HIPCHK(s, hipMemcpyAsync(out, s->mom, kStarMoments * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+    }
+    return SPMD_OK;
+}
+
+int spmd_gyration_tensor(spmd_sys* s, uint32_t r, double* g9) {
+    if (!valid_rep(s, r) || !g9) return SPMD_ERR_ARG;
+    double out[kStarMoments];
+    if (int rc = star_moments(s, r, out)) return rc;
     for (int q = 0; q < 9; ++q) g9[q] = out[q];
     return SPMD_OK;
 }
 
 int spmd_rotation_frequency(spmd_sys* s, uint32_t r, double* w3) {
     if (!valid_rep(s, r) || !w3) return SPMD_ERR_ARG;
+    double out[kStarMoments];
+    if (int rc = star_moments(s, r, out)) return rc;
+    for (int q = 0; q < 3; ++q) w3[q] = out[9 + q];
+    return SPMD_OK;
+}
+
+int spmd_patches(spmd_sys* s, uint32_t r, double* n_patch, double* patch_size) {
+    if (!valid_rep(s, r)) return SPMD_ERR_ARG;
+    const StarHost& st = s->stars[r];
+    if (st.arms == 0) return fail(s, SPMD_ERR_STATE, "replica has no star");
+    if (st.arms > (uint32_t)kPatchArms) return fail(s, SPMD_ERR_ARG, "patch analysis supports at most 128 arms");
     if (int rc = flush_md(s)) return rc;
     if (int rc = ensure_monomers(s)) return rc;
     if (!s->mom)
-        if (int rc = dalloc(s, &s->mom, 12)) return rc;
-    LAUNCH(s, 0, s->stream, launch_star_moments(s->d, s->stream, (int)r, s->mom));
-    double out[12];
-    HIPCHK(s, hipMemcpyAsync(out, s->mom, sizeof out, hipMemcpyDeviceToHost, s->stream));
+        if (int rc = dalloc(s, &s->mom, kStarMoments + 2)) return rc;
+    HIPCHK(s, launch_star_patches(s->d, s->stream, (int)r, (int)st.arms, (int)st.A, (int)st.B,
+                                  s->mom + kStarMoments));
+    double out[2];
+    HIPCHK(s, hipMemcpyAsync(out, s->mom + kStarMoments, sizeof out, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    for (int q = 0; q < 3; ++q) w3[q] = out[9 + q];
+    if (n_patch) *n_patch = out[0];
+    if (patch_size) *patch_size = out[1];
+    return SPMD_OK;
+}
+
+int spmd_statistics(spmd_sys* s, uint32_t r, double* row) {
+    if (!valid_rep(s, r) || !row) return SPMD_ERR_ARG;
+    const StarHost& st = s->stars[r];
+    if (st.n == 0) return fail(s, SPMD_ERR_STATE, "replica has no star");
+    double m[kStarMoments];
+    if (int rc = star_moments(s, r, m)) return rc;
+    double np = 0.0, sp = 0.0;
+    if (int rc = spmd_patches(s, r, &np, &sp)) return rc;
+    double epot = 0.0;
+    HIPCHK(s, hipMemcpy(&epot, &s->d.ctl[r].epot, 8, hipMemcpyDeviceToHost));
+    const double N = (double)st.n;
+    row[0] = epot / N;                                   /* Box::print_Epot */
+    row[1] = m[18];                                      /* Box::print_Ekin_and_Temperature */
+    row[2] = m[18] * 2. / (3. * (N - 1.));
+    row[3] = np;
+    row[4] = sp;
+    row[5] = std::sqrt((m[0] + m[4]) + m[8]);            /* R_gyr of calculate_gyration_tensor */
+    for (int q = 0; q < 9; ++q) row[6 + q] = m[q];       /* symmetric: row- and column-major agree */
+    for (int q = 0; q < 3; ++q) row[15 + q] = m[9 + q];
+    return SPMD_OK;
+}
+
+/* observer workspace: [2][kObsBins][3] profile, 18 Fourier, kObsSpeedBins+1 speed */
+static int obs_ws(spmd_sys* s) {
+    if (!s->obs)
+        if (int rc = dalloc(s, &s->obs, (size_t)6 * kObsBins + 18 + kObsSpeedBins + 1)) return rc;
+    return SPMD_OK;
+}
+
+static void profile_bins(const RepDev& rp, double width, int axis, int* b0, int* nb) {
+    const double h = rp.box.half[axis];
+    *b0 = (int)std::floor(-h / width);
+    *nb = (int)std::floor(h / width) - *b0 + 1;
+}
+
+int spmd_velocity_profile(spmd_sys* s, uint32_t r, double width, uint32_t max_bins, int32_t* b0, uint32_t* nbins,
+                          double* sums) {
+    if (!valid_rep(s, r) || !(width > 0) || !b0 || !nbins || !sums) return SPMD_ERR_ARG;
+    const RepDev& rp = s->reps[r];
+    ObsArgs a{};
+    a.replica = (int)r;
+    a.flags = kObsProfile;
+    a.w = width;
+    for (int ax = 0; ax < 2; ++ax) {
+        profile_bins(rp, width, ax, &a.b0[ax], &a.nb[ax]);
+        if (a.nb[ax] > kObsBins) return fail(s, SPMD_ERR_ARG, "profile bin width too small (1024 bins per axis at most)");
+        if ((uint32_t)a.nb[ax] > max_bins) return fail(s, SPMD_ERR_ARG, "max_bins smaller than the number of bins");
+        b0[ax] = a.b0[ax];
+        nbins[ax] = (uint32_t)a.nb[ax];
+    }
+    if (int rc = obs_ws(s)) return rc;
+    if (s->stars[r].n > 0) {
+        if (int rc = star_moments(s, r, nullptr)) return rc;
+        a.com = s->mom + 12;
+    }
+    a.prof = s->obs;
+    HIPCHK(s, hipMemsetAsync(s->obs, 0, 6ull * kObsBins * 8, s->stream));
+    HIPCHK(s, launch_fluid_observe(s->d, s->stream, a, rp.n_fluid));
+    std::vector<unsigned long long> h(6ull * kObsBins);
+    HIPCHK(s, hipMemcpyAsync(h.data(), s->obs, h.size() * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    const double unfix = 2.3283064365386962890625e-10;
+    for (int ax = 0; ax < 2; ++ax)
+        for (int b = 0; b < a.nb[ax]; ++b) {
+            const unsigned long long* q = h.data() + 3ull * (ax * kObsBins + b);
+            double* o = sums + 3ull * ((size_t)ax * max_bins + b);
+            o[0] = (double)(long long)q[0] * unfix;
+            o[1] = (double)(long long)q[1] * unfix;
+            o[2] = (double)q[2];
+        }
+    return SPMD_OK;
+}
+
+int spmd_fourier_modes(spmd_sys* s, uint32_t r, double* modes) {
+    if (!valid_rep(s, r) || !modes) return SPMD_ERR_ARG;
+    const RepDev& rp = s->reps[r];
+    if (rp.n_fluid == 0) return fail(s, SPMD_ERR_STATE, "replica has no solvent");
+    if (int rc = obs_ws(s)) return rc;
+    ObsArgs a{};
+    a.replica = (int)r;
+    a.flags = kObsFourier;
+    a.nb[0] = a.nb[1] = 1;
+    a.four = s->obs + 6ull * kObsBins;
+    HIPCHK(s, hipMemsetAsync(a.four, 0, 18 * 8, s->stream));
+    HIPCHK(s, launch_fluid_observe(s->d, s->stream, a, rp.n_fluid));
+    unsigned long long h[18];
+    HIPCHK(s, hipMemcpyAsync(h, a.four, sizeof h, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    const double unfix = 2.3283064365386962890625e-10, n = (double)rp.n_fluid;
+    for (int q = 0; q < 18; ++q) modes[q] = (double)(long long)h[q] * unfix / n;
+    /* Fourier_Autocorr.h:68: drop the component along k_j */
+    for (int j = 0; j < 3; ++j) modes[6 * j + 2 * j] = modes[6 * j + 2 * j + 1] = 0.0;
+    return SPMD_OK;
+}
+
+int spmd_speed_histogram(spmd_sys* s, uint32_t r, double width, uint32_t nbins, uint64_t* counts) {
+    if (!valid_rep(s, r) || !(width > 0) || nbins == 0 || !counts) return SPMD_ERR_ARG;
+    if (nbins > (uint32_t)kObsSpeedBins) return fail(s, SPMD_ERR_ARG, "at most 1024 speed bins");
+    const RepDev& rp = s->reps[r];
+    if (int rc = obs_ws(s)) return rc;
+    ObsArgs a{};
+    a.replica = (int)r;
+    a.flags = kObsSpeed;
+    a.vw = width;
+    a.nb[0] = a.nb[1] = 1;
+    a.nsb = (int)nbins;
+    a.speed = s->obs + 6ull * kObsBins + 18;
+    HIPCHK(s, hipMemsetAsync(a.speed, 0, (nbins + 1) * 8ull, s->stream));
+    HIPCHK(s, launch_fluid_observe(s->d, s->stream, a, rp.n_fluid));
+    HIPCHK(s, hipMemcpyAsync(counts, a.speed, (nbins + 1) * 8ull, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
     return SPMD_OK;
 }
 

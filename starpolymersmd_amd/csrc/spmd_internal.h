@@ -139,8 +139,32 @@ hipError_t launch_fluid_init(const DevState& d, hipStream_t st);
 constexpr int kClosureFeatures = 24;
 hipError_t launch_closure_features(const DevState& d, hipStream_t st, float* feat, const float* fmin,
                                    const float* fmax);
-/* gyration tensor (9) and rotation frequency (3) of one replica's star */
+/* one replica's star: gyration tensor (out[0..8]), rotation frequency
+ * (out[9..11]), COM position and velocity (out[12..17]), kinetic energy
+ * relative to the COM velocity (out[18]) */
+constexpr int kStarMoments = 19;
 hipError_t launch_star_moments(const DevState& d, hipStream_t st, int replica, double* out);
+/* patch number and mean patch size of one replica's star (arms <= kPatchArms) */
+constexpr int kPatchArms = 128;
+hipError_t launch_star_patches(const DevState& d, hipStream_t st, int replica, int arms, int A, int B, double* out);
+/* solvent observers of one replica (flags kObs*); sums accumulate into the
+ * device arrays, which the caller clears */
+constexpr int kObsBins = 1024;        /* bins per axis of the velocity profile */
+constexpr int kObsSpeedBins = 1024;   /* speed histogram bins (+1 for the rest) */
+constexpr int kObsPerThread = 8;
+enum { kObsProfile = 1, kObsFourier = 2, kObsSpeed = 4 };
+struct ObsArgs {
+    int replica;
+    int flags;
+    double w, vw;                 /* profile and speed bin widths */
+    int b0[2], nb[2];             /* first bin index floor(x/w) and bins, x and y maps */
+    int nsb;                      /* speed bins; counts[nsb] takes the speeds beyond */
+    const double* com;            /* 6: COM position and velocity subtracted, or null */
+    unsigned long long* prof;     /* [2][kObsBins][3]: sum vx, sum vy (2^32 fixed), count */
+    unsigned long long* four;     /* [3][3][2] (2^32 fixed) */
+    unsigned long long* speed;    /* [nsb + 1] */
+};
+hipError_t launch_fluid_observe(const DevState& d, hipStream_t st, const ObsArgs& a, uint32_t n_fluid);
 hipError_t launch_closure_kick(const DevState& d, hipStream_t st, const float* out, float sy0, float sy1, float sy2,
                                uint32_t k);
 hipError_t launch_temperature(const DevState& d, hipStream_t st, int replica, uint32_t ntiles);

@@ -1473,7 +1473,9 @@ __global__ __launch_bounds__(256) void k_closure_kick(DevState d, const float* o
 /* Box::calculate_gyration_tensor (Box.cpp:330-352) and
  * Box::calculate_rotation_frequency (Box.cpp:354-379) of one replica's
  * star in its centre-of-mass frame (Box::center_of_mass_reference_frame,
- * Box.cpp:81-102): out[0..8] = G (row-major), out[9..11] = omega = I^-1 L. */
+ * Box.cpp:81-102): out[0..8] = G (row-major), out[9..11] = omega = I^-1 L,
+ * out[12..17] = COM position and velocity (Box::COM_Pos, COM_Vel), out[18]
+ * = kinetic energy in the COM frame. */
 __global__ __launch_bounds__(256) void k_star_moments(DevState d, int replica, double* out) {
     __shared__ double sred[4];
     const RepDev& rep = d.reps[replica];
@@ -1511,7 +1513,161 @@ __global__ __launch_bounds__(256) void k_star_moments(DevState d, int replica, d
         const double w1 = (a * (L1 * k - f * L2) - L0 * (b * k - f * cc) + cc * (b * L2 - L1 * cc)) / det;
         const double w2 = (a * (e * L2 - L1 * f) - b * (b * L2 - L1 * cc) + L0 * (b * f - e * cc)) / det;
         out[9] = w0; out[10] = w1; out[11] = w2;
+        for (int q = 0; q < 6; ++q) out[12 + q] = com[q];
     }
+    /* Molecule::calculate_Ekin Molecule.cpp:60-74: sum m |v - <v>|^2 / 2 */
+    double ek = 0.0;
+    for (int i = threadIdx.x; i < N; i += 256) {
+        const size_t g = 3 * (size_t)(mb + i);
+        const double vx = d.mvel[g] - com[3], vy = d.mvel[g + 1] - com[4], vz = d.mvel[g + 2] - com[5];
+        ek += d.mmass[mb + i] * ((vx * vx + vy * vy) + vz * vz);
+    }
+    ek = block_sum<256>(ek, sred);
+    if (threadIdx.x == 0) out[18] = ek / 2.0;
+}
+
+/* Box::calculate_patches_new Box.cpp:434-507 for one replica's star (arm a,
+ * bead k at a*(A+B)+k, beads k > A solvophobic): two arms are bonded when
+ * some pair of their B beads has a negative pair energy (Box::calculate_epot
+ * Box.cpp:386-399 -- arms share no FENE bond); patches are the connected
+ * groups of more than one arm.  out[0] = number of patches, out[1] = mean
+ * arms per patch (0 without patches). */
+__global__ __launch_bounds__(256) void k_star_patches(DevState d, int replica, int arms, int A, int B,
+                                                      double* out) {
+    __shared__ uint32_t s_bond[kPatchArms][kPatchArms / 32];
+    __shared__ int s_lab[kPatchArms];
+    __shared__ int s_stack[kPatchArms];
+    const RepDev& rep = d.reps[replica];
+    const double* P = d.mpos + 3ull * rep.mono_base;
+    const double lambda = rep.lambda;
+    for (int i = threadIdx.x; i < kPatchArms * (kPatchArms / 32); i += 256) (&s_bond[0][0])[i] = 0u;
+    __syncthreads();
+    const int per = A + B, npair = arms * (arms - 1) / 2;
+    for (int q = threadIdx.x; q < npair; q += 256) {
+        int i = 0, rem = q;
+        while (rem >= arms - 1 - i) { rem -= arms - 1 - i; ++i; }
+        const int k = i + 1 + rem;
+        bool hit = false;
+        for (int j = A + 1; j <= per && !hit; ++j) {
+            const double* pj = P + 3ull * (uint32_t)(i * per + j);
+            for (int l = A + 1; l <= per; ++l) {
+                const double* pl = P + 3ull * (uint32_t)(k * per + l);
+                const double dx = pl[0] - pj[0], dy = pl[1] - pj[1], dz = pl[2] - pj[2];
+                if (pair_potential((dx * dx + dy * dy) + dz * dz, true, lambda) < 0.0) { hit = true; break; }
+            }
+        }
+        if (hit) {
+            atomicOr(&s_bond[i][k >> 5], 1u << (k & 31));
+            atomicOr(&s_bond[k][i >> 5], 1u << (i & 31));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < arms; ++a) s_lab[a] = -1;
+        int n_patch = 0, in_patch = 0;
+        for (int a = 0; a < arms; ++a) {
+            if (s_lab[a] >= 0) continue;
+            int top = 0, size = 0;
+            s_lab[a] = a;
+            s_stack[top++] = a;
+            while (top > 0) {
+                const int u = s_stack[--top];
+                ++size;
+                for (int v = 0; v < arms; ++v)
+                    if (s_lab[v] < 0 && ((s_bond[u][v >> 5] >> (v & 31)) & 1u)) { s_lab[v] = a; s_stack[top++] = v; }
+            }
+            if (size > 1) { ++n_patch; in_patch += size; }
+        }
+        out[0] = (double)n_patch;
+        out[1] = n_patch > 0 ? (double)in_patch / (double)n_patch : 0.0;
+    }
+}
+
+/* Solvent observers of one replica in one pass over its particles (current
+ * buffer), summed in 64-bit fixed point so that results do not depend on
+ * the order of the atomics:
+ *   kObsProfile  VelocityX::operator() Velocity_x.h:76-96: r - COM_Pos and
+ *                v - COM_Vel through MPC::wrap(pos, vel) (MPC.cpp:492-495),
+ *                binned by floor(x/w) and floor(y/w): sum vx, sum vy, count
+ *   kObsFourier  Fourier_Autocorr::operator() Fourier_Autocorr.h:55-63:
+ *                sum_i v_i cos(k_j x_ij) and sum_i v_i sin(k_j x_ij) with
+ *                k_j = 2 pi / L_j, j = x, y, z
+ *   kObsSpeed    VelocityDistribution::operator() Vel_distribution.h:35-40:
+ *                counts of floor(|v|/w_v), the last bin taking the rest */
+__global__ __launch_bounds__(256) void k_fluid_observe(DevState d, ObsArgs a) {
+    __shared__ unsigned long long s_prof[2][kObsBins][3];
+    __shared__ unsigned long long s_four[18];
+    __shared__ uint32_t s_speed[kObsSpeedBins + 1];
+    const RepDev& rep = d.reps[a.replica];
+    Box b = rep.box;
+    b.delrx = d.ctl[a.replica].delrx;
+    const int nb0 = a.nb[0], nb1 = a.nb[1];
+    for (int i = threadIdx.x; i < 3 * nb0; i += 256) (&s_prof[0][0][0])[i] = 0ull;
+    for (int i = threadIdx.x; i < 3 * nb1; i += 256) (&s_prof[1][0][0])[i] = 0ull;
+    for (int i = threadIdx.x; i < 18; i += 256) s_four[i] = 0ull;
+    for (int i = threadIdx.x; i <= a.nsb; i += 256) s_speed[i] = 0u;
+    __syncthreads();
+    double com[6] = {0, 0, 0, 0, 0, 0};
+    if (a.com)
+        for (int q = 0; q < 6; ++q) com[q] = a.com[q];
+    const int cur = d.cur;
+    const size_t base = rep.part_base;
+    const float *X = d.x[cur] + base, *Y = d.y[cur] + base, *Z = d.z[cur] + base;
+    const float *VX = d.vx[cur] + base, *VY = d.vy[cur] + base, *VZ = d.vz[cur] + base;
+    const double kk[3] = {2.0 * kPi / b.L[0], 2.0 * kPi / b.L[1], 2.0 * kPi / b.L[2]};
+    double f[18];
+    for (int q = 0; q < 18; ++q) f[q] = 0.0;
+    const uint32_t n = rep.n_fluid;
+    for (int t = 0; t < kObsPerThread; ++t) {
+        const uint32_t i = (blockIdx.x * kObsPerThread + t) * 256u + threadIdx.x;
+        if (i >= n) break;
+        const double p[3] = {X[i], Y[i], Z[i]};
+        const double v[3] = {VX[i], VY[i], VZ[i]};
+        if (a.flags & kObsProfile) {
+            double x = p[0] - com[0], y = p[1] - com[1], z = p[2] - com[2], ux = v[0] - com[3];
+            const double uy = v[1] - com[4];
+            le_wrap(b, x, y, z, ux);
+            const int bx = min(max((int)floor(x / a.w) - a.b0[0], 0), nb0 - 1);
+            const int by = min(max((int)floor(y / a.w) - a.b0[1], 0), nb1 - 1);
+            const unsigned long long qx = to_fix(ux), qy = to_fix(uy);
+            atomicAdd(&s_prof[0][bx][0], qx);
+            atomicAdd(&s_prof[0][bx][1], qy);
+            atomicAdd(&s_prof[0][bx][2], 1ull);
+            atomicAdd(&s_prof[1][by][0], qx);
+            atomicAdd(&s_prof[1][by][1], qy);
+            atomicAdd(&s_prof[1][by][2], 1ull);
+        }
+        if (a.flags & kObsFourier) {
+            for (int j = 0; j < 3; ++j) {
+                double sn, cs;
+                sincos(kk[j] * p[j], &sn, &cs);
+                for (int c = 0; c < 3; ++c) { f[6 * j + 2 * c] += v[c] * cs; f[6 * j + 2 * c + 1] += v[c] * sn; }
+            }
+        }
+        if (a.flags & kObsSpeed) {
+            const double sp = sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+            const double q = floor(sp / a.vw);
+            atomicAdd(&s_speed[q < (double)a.nsb ? (int)q : a.nsb], 1u);
+        }
+    }
+    if (a.flags & kObsFourier)
+        for (int q = 0; q < 18; ++q) atomicAdd(&s_four[q], to_fix(f[q]));
+    __syncthreads();
+    if (a.flags & kObsProfile) {
+        for (int i = threadIdx.x; i < 3 * nb0; i += 256) {
+            const unsigned long long v = (&s_prof[0][0][0])[i];
+            if (v) atomicAdd(a.prof + i, v);
+        }
+        for (int i = threadIdx.x; i < 3 * nb1; i += 256) {
+            const unsigned long long v = (&s_prof[1][0][0])[i];
+            if (v) atomicAdd(a.prof + 3 * kObsBins + i, v);
+        }
+    }
+    if (a.flags & kObsFourier)
+        for (int i = threadIdx.x; i < 18; i += 256) atomicAdd(a.four + i, s_four[i]);
+    if (a.flags & kObsSpeed)
+        for (int i = threadIdx.x; i <= a.nsb; i += 256)
+            if (s_speed[i]) atomicAdd(a.speed + i, (unsigned long long)s_speed[i]);
 }
 
 /* ========================================================= launchers */
@@ -1638,6 +1794,23 @@ hipError_t launch_closure_kick(const DevState& d, hipStream_t st, const float* o
 
 hipError_t launch_star_moments(const DevState& d, hipStream_t st, int replica, double* out) {
     hipLaunchKernelGGL(k_star_moments, dim3(1), dim3(256), 0, st, d, replica, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_star_patches(const DevState& d, hipStream_t st, int replica, int arms, int A, int B, double* out) {
+    if (arms < 1 || arms > kPatchArms) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_star_patches, dim3(1), dim3(256), 0, st, d, replica, arms, A, B, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fluid_observe(const DevState& d, hipStream_t st, const ObsArgs& a, uint32_t n_fluid) {
+    if (a.nb[0] < 1 || a.nb[0] > kObsBins || a.nb[1] < 1 || a.nb[1] > kObsBins || a.nsb < 0 ||
+        a.nsb > kObsSpeedBins)
+        return hipErrorInvalidValue;
+    const uint32_t per = 256u * kObsPerThread;
+    const uint32_t nblk = (n_fluid + per - 1) / per;
+    if (nblk == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fluid_observe, dim3(nblk), dim3(256), 0, st, d, a);
     return hipGetLastError();
 }
 

@@ -1,0 +1,231 @@
+"""Analysis observers of the reference and its statistics file, on top of
+the device samplers of MPCSystem.
+
+The reference applies functors to every solvent particle on the CPU
+(`MPC::operator()`, star-polymers/lib/Hydrodynamics/MPC.h:110-116) and keeps
+their running results in std::map's.  Here one device pass per sample
+returns the per-bin sums (MPCSystem.velocity_profile / fourier_modes /
+speed_histogram, kernels in csrc/spmd_kernels.hip), and these classes keep
+the running results and print them in the reference's formats.  They also
+accept raw samples, so the same accumulation serves any source of samples.
+
+  VelocityProfile   VelocityX         lib/Analysis/Velocity_x.h:22-148
+  FourierAutocorr   Fourier_Autocorr  lib/Analysis/Fourier_Autocorr.h:19-127
+  SpeedDistribution VelocityDistribution lib/Analysis/Vel_distribution.h:17-57
+  StatisticsFile    statistics*.dat   src/star-polymers.cpp:307, 466-486
+  run_with_output   the driver loop   src/star-polymers.cpp:412-536
+"""
+from __future__ import annotations
+
+from collections import deque
+from typing import Callable, TextIO
+
+import numpy as np
+
+# the SRD kinematic viscosity at rho = 10, h = 0.1, alpha = 130 deg, kT = 1
+# (kinetic + collisional parts), the curve results/plot_MPC_tests.py:107
+# draws through the transverse autocorrelation
+def srd_viscosity(n: float = 10.0, h: float = 0.1, alpha_deg: float = 130.0, kT: float = 1.0) -> float:
+    """Kinematic shear viscosity of SRD with random grid shift, unit cells
+    and unit particle mass: nu_kin = kT h / 2 [5 n / ((n - 1 + e^-n)(2 -
+    cos a - cos 2a)) - 1], nu_coll = (1 - cos a) / (18 h) (n - 1 + e^-n) / n."""
+    a = np.deg2rad(alpha_deg)
+    z = n - 1.0 + np.exp(-n)
+    nu_kin = kT * h / 2.0 * (5.0 * n / (z * (2.0 - np.cos(a) - np.cos(2.0 * a))) - 1.0)
+    nu_coll = (1.0 - np.cos(a)) / (18.0 * h) * z / n
+    return float(nu_kin + nu_coll)
+
+
+class VelocityProfile:
+    """VelocityX (Velocity_x.h:22-148): mean solvent velocity relative to
+    the star's centre of mass, binned by floor(x/width) and floor(y/width)
+    after the Lees-Edwards wrap; map 0 is keyed by x, map 1 by y."""
+
+    def __init__(self, width: float = 0.5):
+        self.width = float(width)
+        self.maps: list[dict[float, np.ndarray]] = [{}, {}]
+
+    def add(self, sample):
+        """sample: [(keys, sums)] per map, sums[:, 0:3] = sum vx, sum vy,
+        count.  Bins with no particle do not enter the maps, as in the
+        reference's std::map."""
+        for a in range(2):
+            keys, sums = sample[a]
+            m = self.maps[a]
+            for k, s in zip(np.asarray(keys, float), np.asarray(sums, float)):
+                if s[2] == 0:
+                    continue
+                k = float(k)
+                if k in m:
+                    m[k] += s
+                else:
+                    m[k] = s.copy()
+
+    def sample(self, system, r: int = 0):
+        """One application of the functor to the current solvent."""
+        self.add(system.velocity_profile(self.width, r))
+
+    def table(self, axis: int):
+        """keys, mean vx, mean vy, count of one map in key order."""
+        m = self.maps[axis]
+        keys = np.array(sorted(m))
+        if keys.size == 0:
+            return keys, keys.copy(), keys.copy(), keys.copy()
+        s = np.array([m[k] for k in keys])
+        return keys, s[:, 0] / s[:, 2], s[:, 1] / s[:, 2], s[:, 2]
+
+    def print_result(self, os: TextIO):
+        """VelocityX::print_result (Velocity_x.h:98-144): both maps, one bin
+        per line (key, mean vx, mean vy, count; %.8e), a blank line between."""
+        for a in range(2):
+            keys, vx, vy, n = self.table(a)
+            for row in zip(keys, vx, vy, n):
+                os.write("%.8e %.8e %.8e %.8e\n" % row)
+            if a == 0:
+                os.write("\n")
+
+    def shear_rate(self, y_max: float | None = None) -> float:
+        """Count-weighted least-squares slope of mean vx against the bin
+        centre of the y map (the analytic profile is vx = shear * y,
+        results/plot_MPC_tests.py:61), over |centre| <= y_max if given."""
+        keys, vx, _, n = self.table(1)
+        y = keys + 0.5 * self.width
+        sel = np.ones_like(y, bool) if y_max is None else np.abs(y) <= y_max
+        return weighted_slope(y[sel], vx[sel], n[sel])
+
+
+def weighted_slope(x, y, w) -> float:
+    x, y, w = (np.asarray(v, float) for v in (x, y, w))
+    xm = np.sum(w * x) / np.sum(w)
+    ym = np.sum(w * y) / np.sum(w)
+    return float(np.sum(w * (x - xm) * (y - ym)) / np.sum(w * (x - xm) ** 2))
+
+
+class FourierAutocorr:
+    """Fourier_Autocorr (Fourier_Autocorr.h:19-127): autocorrelation of the
+    transverse velocity modes V_j at k = 2 pi / L e_j over a sliding window
+    of `entries` samples, `dt` apart.  The window starts filled with zeros;
+    from the entries-th sample on, every sample adds Re(V_j(oldest)^H
+    V_j(oldest + i)) to lag i (Fourier_Autocorr.h:80-93)."""
+
+    def __init__(self, entries: int = 50, dt: float = 1.0):
+        self.entries = int(entries)
+        self.dt = float(dt)
+        self.window = deque([np.zeros((3, 3), complex)] * self.entries, maxlen=self.entries)
+        self.filled = 0
+        self.sums = np.zeros((3, self.entries))
+        self.count = 0
+
+    def add(self, modes):
+        """modes: complex (3, 3), row j = V_j."""
+        self.window.append(np.asarray(modes, complex).copy())
+        if self.filled < self.entries:
+            self.filled += 1
+        if self.filled == self.entries:
+            w = list(self.window)
+            first = w[0]
+            for i in range(self.entries):
+                self.sums[:, i] += np.real(np.sum(np.conj(first) * w[i], axis=1))
+            self.count += 1
+
+    def sample(self, system, r: int = 0):
+        self.add(system.fourier_modes(r))
+
+    def correlation(self) -> np.ndarray:
+        """(3, entries): the averaged autocorrelation for k along x, y, z."""
+        return self.sums / max(self.count, 1)
+
+    def print_result(self, os: TextIO):
+        """Fourier_Autocorr::print_result (Fourier_Autocorr.h:113-119)."""
+        c = self.correlation()
+        for i in range(self.entries):
+            os.write("%g %g %g %g\n" % (i * self.dt, c[0, i], c[1, i], c[2, i]))
+
+    def viscosity(self, L: float, t_fit: float = 15.0) -> float:
+        """nu from C(k, t) / C(k, 0) = exp(-nu k^2 t) (the curve of
+        results/plot_MPC_tests.py:107): least-squares line through
+        ln(C/C(0)) of the three directions' mean over 0 <= t <= t_fit."""
+        c = self.correlation()
+        c = (c / c[:, :1]).mean(axis=0)
+        t = np.arange(self.entries) * self.dt
+        sel = (t <= t_fit) & (c > 0)
+        slope = np.polyfit(t[sel], np.log(c[sel]), 1)[0]
+        return float(-slope / (2.0 * np.pi / L) ** 2)
+
+
+class SpeedDistribution:
+    """VelocityDistribution (Vel_distribution.h:17-57): histogram of |v|
+    over the solvent with bins floor(|v|/width)."""
+
+    def __init__(self, width: float = 0.5, nbins: int = 1024):
+        self.width = float(width)
+        self.nbins = int(nbins)
+        self.counts = np.zeros(self.nbins + 1, np.uint64)
+
+    def add(self, counts):
+        self.counts += np.asarray(counts, np.uint64)
+
+    def sample(self, system, r: int = 0):
+        self.add(system.speed_histogram(self.width, self.nbins, r))
+
+    def density(self):
+        """bin keys and probability density count / (total * width) of the
+        non-empty bins (Vel_distribution.h:42-55)."""
+        total = float(self.counts.sum())
+        b = np.nonzero(self.counts[: self.nbins])[0]
+        return b * self.width, self.counts[b] / (total * self.width)
+
+    def print_result(self, os: TextIO):
+        for k, p in zip(*self.density()):
+            os.write("%.8e %.8e\n" % (k, p))
+        os.write("\n")
+
+
+STAT_HEADER = ("Step   Epot     Ekin     Temp   n_p s_p  R_gyr   G_xx      G_xy      G_xz       G_yx       "
+               "G_yy      G_yz      G_zx     Gzy     Gzz      wx      wy     wz\n")
+
+
+def statistics_row(step: int, row) -> str:
+    """One line of statistics*.dat as star-polymers.cpp:470-482 writes it
+    (each value by operator<< with the default 6 significant digits and a
+    trailing space)."""
+    return f"{int(step)} " + "".join("%g " % float(v) for v in row) + "\n"
+
+
+class StatisticsFile:
+    """statistics*.dat (star-polymers.cpp:300-307, 470-482): the header on
+    an empty file, then one row per output step."""
+
+    def __init__(self, path: str):
+        import os
+        self.f = open(path, "a")
+        if os.path.getsize(path) == 0:
+            self.f.write(STAT_HEADER)
+
+    def write(self, step: int, row):
+        self.f.write(statistics_row(step, row))
+        self.f.flush()
+
+    def close(self):
+        self.f.close()
+
+
+def run_with_output(system, t0: int, n: int, mpc_dt: float, every: int,
+                    on_output: Callable[[int], None], equil: int = -1):
+    """The loop of star-polymers.cpp:412-536 over steps t0 .. t0+n-1: at an
+    output step t (t > equil and t % every == 0) Thermostat::propagate(true),
+    then on_output(t) (the observers), then MPC::step(t, mpc_dt); between
+    output steps the graph-replayed MPCSystem.run."""
+    t, end = t0, t0 + n
+    while t < end:
+        nxt = max(t, equil + 1)
+        nxt = ((nxt + every - 1) // every) * every
+        if nxt >= end:
+            system.run(t, end - t, mpc_dt)
+            return
+        if nxt > t:
+            system.run(t, nxt - t, mpc_dt)
+        system.propagate(True)
+        on_output(nxt)
+        system.step(nxt, mpc_dt)
+        t = nxt + 1

@@ -182,6 +182,51 @@ class MPCSystem:
         self._chk(self._lib.spmd_rotation_frequency(self.h, r, ptr(w, C.c_double)))
         return w
 
+    def patches(self, r: int = 0):
+        """Box::calculate_patches_new (Box.cpp:434-507): (number, mean size)."""
+        n, sz = C.c_double(), C.c_double()
+        self._chk(self._lib.spmd_patches(self.h, r, C.byref(n), C.byref(sz)))
+        return n.value, sz.value
+
+    def statistics(self, r: int = 0) -> np.ndarray:
+        """The 18 values of a statistics*.dat row after the step number
+        (star-polymers.cpp:470-482): Epot, Ekin, Temp, n_p, s_p, R_gyr,
+        G (9), omega (3)."""
+        row = np.empty(18)
+        self._chk(self._lib.spmd_statistics(self.h, r, ptr(row, C.c_double)))
+        return row
+
+    def velocity_profile(self, width: float, r: int = 0):
+        """VelocityX sample (Velocity_x.h:76-96) of the current solvent:
+        [(keys, sums)] for the x- and the y-binned map, sums[:, 0:3] = sum
+        vx, sum vy, count."""
+        L = [int(l) + int(l) % 2 for l in self.params[r].L]   # even-rounded box (Box.cpp:14-19)
+        nmax = max(int(np.floor(L[a] / 2 / width)) - int(np.floor(-L[a] / 2 / width)) + 1 for a in range(2))
+        b0 = np.empty(2, np.int32)
+        nb = np.empty(2, np.uint32)
+        sums = np.zeros((2, nmax, 3))
+        self._chk(self._lib.spmd_velocity_profile(self.h, r, width, nmax, ptr(b0, C.c_int32), ptr(nb, C.c_uint32),
+                                                  ptr(sums, C.c_double)))
+        out = []
+        for a in range(2):
+            keys = (b0[a] + np.arange(nb[a])) * width
+            out.append((keys, sums[a, : nb[a]].copy()))
+        return out
+
+    def fourier_modes(self, r: int = 0) -> np.ndarray:
+        """Fourier_Autocorr sample (Fourier_Autocorr.h:55-72): complex (3, 3),
+        row j = transverse velocity mode at k = 2 pi / L e_j."""
+        m = np.empty(18)
+        self._chk(self._lib.spmd_fourier_modes(self.h, r, ptr(m, C.c_double)))
+        return m[0::2].reshape(3, 3) + 1j * m[1::2].reshape(3, 3)
+
+    def speed_histogram(self, width: float, nbins: int, r: int = 0) -> np.ndarray:
+        """VelocityDistribution counts (Vel_distribution.h:35-40); the last
+        entry counts the speeds beyond nbins * width."""
+        c = np.empty(nbins + 1, np.uint64)
+        self._chk(self._lib.spmd_speed_histogram(self.h, r, width, nbins, ptr(c, C.c_uint64)))
+        return c
+
     def closure_features(self, feat_dev_ptr: int):
         """24 closure features of every monomer into device memory
         (float32, total monomers x 24)."""

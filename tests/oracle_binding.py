@@ -53,6 +53,10 @@ _SIGS = {
     "orc_gyration_tensor": (C.c_double, [C.c_void_p, _d]),
     "orc_rotation_frequency": (None, [C.c_void_p, _d]),
     "orc_ekin_monomers": (C.c_double, [C.c_void_p]),
+    "orc_velocity_profile": (None, [C.c_void_p, C.c_double, C.c_uint32, _u32, _d, _d]),
+    "orc_fourier_modes": (None, [C.c_void_p, _d]),
+    "orc_speed_histogram": (None, [C.c_void_p, C.c_double, C.c_uint32, C.POINTER(C.c_uint64)]),
+    "orc_patches": (None, [C.c_void_p, _d, _d]),
     "orc_fene_force": (C.c_double, [C.c_double]),
     "orc_fene_anchor_force": (C.c_double, [C.c_double]),
     "orc_aa_force": (C.c_double, [C.c_double]),
@@ -211,6 +215,32 @@ class Oracle:
 
     def ekin_monomers(self):
         return lib().orc_ekin_monomers(self.h)
+
+    def velocity_profile(self, width, cap=4096):
+        n = np.zeros(2, np.uint32)
+        keys = np.zeros(2 * cap)
+        sums = np.zeros(2 * cap * 3)
+        lib().orc_velocity_profile(self.h, width, cap, p(n, C.c_uint32), p(keys), p(sums))
+        out = []
+        for a in range(2):
+            k = int(n[a])
+            out.append((keys[a * cap:a * cap + k].copy(), sums.reshape(2, cap, 3)[a, :k].copy()))
+        return out
+
+    def fourier_modes(self):
+        m = np.empty(18)
+        lib().orc_fourier_modes(self.h, p(m))
+        return m[0::2].reshape(3, 3) + 1j * m[1::2].reshape(3, 3)
+
+    def speed_histogram(self, width, nbins):
+        c = np.empty(nbins + 1, np.uint64)
+        lib().orc_speed_histogram(self.h, width, nbins, p(c, C.c_uint64))
+        return c
+
+    def patches(self):
+        n, sz = C.c_double(), C.c_double()
+        lib().orc_patches(self.h, C.byref(n), C.byref(sz))
+        return n.value, sz.value
 
 
 def philox(ctr, key):

@@ -1,0 +1,103 @@
+"""Device observers against the FP64 oracle on identical states: the
+VelocityX profile, the Fourier_Autocorr modes, the VelocityDistribution
+histogram, Box::calculate_patches_new and the statistics row.
+
+Bars: bin keys and counts exact (both sides bin the same FP64 values with
+the same operations); velocity sums within 1e-6 absolute (the device sums
+in 2^-32 fixed point); Fourier modes within 1e-10 (normalised by N);
+patches exact; statistics row within 1e-7 relative after one velocity-Verlet
+step from the same state (MD parity is 1e-9)."""
+import numpy as np
+import pytest
+
+import oracle_binding as ob
+from starpolymersmd_amd import MPCSystem, SpmdError, make_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _state(L=20, shear=0.02, star=True, seed=3, collisions=3):
+    g = MPCSystem(make_params(L=(L,) * 3, n_c=10, shear=shear, step_update=10, seed=seed))
+    if star:
+        g.add_star(5, 5, 10, 10.0, coil=True)
+    g.md_init()
+    g.initialize()
+    g.run(0, 10 * collisions + 1, 0.1)
+    g.sync()
+    pf, vf = g.get_fluid()
+    o = ob.Oracle(L=(L,) * 3, n_c=10, shear=shear)
+    o.set_fluid(pf.astype(np.float64), vf.astype(np.float64))
+    o.delrx = g.get_delrx()
+    if star:
+        o.add_star(5, 5, 10, 10.0)
+        pos, vel, _ = g.get_monomers()
+        o.set_monomers(pos, vel)
+    return g, o
+
+
+@pytest.mark.parametrize("star,shear,width", [(True, 0.02, 0.5), (False, 0.0, 0.2), (True, 0.05, 0.25)])
+def test_velocity_profile_matches_oracle(star, shear, width):
+    g, o = _state(shear=shear, star=star)
+    dev = g.velocity_profile(width)
+    ref = o.velocity_profile(width)
+    for a in range(2):
+        keys, sums = dev[a]
+        nz = sums[:, 2] > 0
+        rk, rs = ref[a]
+        assert np.array_equal(keys[nz], rk), a
+        assert np.array_equal(sums[nz, 2], rs[:, 2]), a
+        assert np.abs(sums[nz, :2] - rs[:, :2]).max() < 1e-6, a
+        assert sums[:, 2].sum() == g.num_fluid()
+
+
+def test_fourier_modes_match_oracle():
+    g, o = _state(shear=0.0, star=False)
+    m = g.fourier_modes()
+    mo = o.fourier_modes()
+    assert np.abs(m - mo).max() < 1e-10
+    assert np.abs(mo).max() > 1e-4          # not trivially zero
+
+
+@pytest.mark.parametrize("width,nbins", [(0.01, 1000), (0.1, 40)])
+def test_speed_histogram_matches_oracle(width, nbins):
+    g, o = _state(shear=0.05, star=True)
+    c = g.speed_histogram(width, nbins)
+    co = o.speed_histogram(width, nbins)
+    assert int(c.sum()) == g.num_fluid()
+    assert np.array_equal(c, co)
+
+
+@pytest.mark.parametrize("side", [3.0, 5.0, 7.0, 10.0])
+def test_patches_match_oracle(side):
+    rng = np.random.default_rng(int(side * 10))
+    g = MPCSystem(make_params(L=(64,) * 3, n_c=1, seed=1))
+    g.add_star(5, 5, 10, 10.0)
+    o = ob.Oracle(L=(64,) * 3, n_c=1)
+    o.add_star(5, 5, 10, 10.0)
+    for trial in range(5):
+        pos = (rng.random((g.num_monomers(), 3)) - 0.5) * side
+        g.set_monomers(pos=pos)
+        o.set_monomers(pos=pos)
+        assert g.patches() == o.patches(), (side, trial)
+
+
+def test_patches_reject_more_than_128_arms():
+    g = MPCSystem(make_params(L=(64,) * 3, n_c=1, seed=1))
+    g.add_star(1, 1, 130, 1.0)
+    with pytest.raises(SpmdError):
+        g.patches()
+
+
+def test_statistics_row_matches_oracle():
+    g, o = _state(shear=0.02, star=True, collisions=5)
+    o.md_init()
+    g.propagate(True)
+    assert o.propagate(True) == 0
+    row = g.statistics()
+    n = o.n_mono
+    rg, G = o.gyration()
+    ek = o.ekin_monomers()
+    want = [o.epot() / n, ek, ek * 2 / (3 * (n - 1)), *o.patches(), rg, *G.ravel(order="F"),
+            *o.rotation_frequency()]
+    np.testing.assert_allclose(row, want, rtol=1e-7, atol=1e-9)
+    np.testing.assert_allclose(row[6:15].reshape(3, 3), g.gyration_tensor(), rtol=0, atol=0)
