@@ -37,8 +37,14 @@ constexpr double kFix = 4294967296.0;          /* 2^32 */
 constexpr double kUnfix = 2.3283064365386962890625e-10;
 constexpr double kPi = 3.14159265358979323846;
 
+/* round(v * 2^32) (ties to even) as a 64-bit two's complement value.  One
+ * FMA against 1.5 * 2^52 leaves the integer in the low mantissa bits, which
+ * is exact for |v| < 2^19; larger |v| (no physical sum comes near) is
+ * clamped there. */
 __device__ __forceinline__ unsigned long long to_fix(double v) {
-    return (unsigned long long)__double2ll_rn(v * kFix);
+    v = fmin(fmax(v, -524287.0), 524287.0);
+    const double t = __builtin_fma(v, kFix, 6755399441055744.0);
+    return (unsigned long long)(__double_as_longlong(t) - 0x4338000000000000LL);
 }
 __device__ __forceinline__ double from_fix(unsigned long long v) { return (double)(long long)v * kUnfix; }
 
@@ -183,41 +189,61 @@ __device__ double block_sum_all(double v, double* scratch) {
  * law) followed by the nb bonds -- kMdBatch independent entries per lane in
  * flight, added to both monomers' 64-bit fixed-point LDS accumulators (the
  * sum does not depend on the order of the entries). */
-__device__ __forceinline__ void md_forces_flat(const MdLds& S, const uint32_t* pl, uint32_t npl, uint32_t nb,
-                                               double lambda) {
-#pragma clang fp contract(fast)
-    constexpr int NT = kMdThreads;
+__device__ __forceinline__ void md_add_force(const MdLds& S, int i, int j, unsigned long long gx,
+                                             unsigned long long gy, unsigned long long gz) {
     unsigned long long* qx = reinterpret_cast<unsigned long long*>(S.fx);
     unsigned long long* qy = reinterpret_cast<unsigned long long*>(S.fy);
     unsigned long long* qz = reinterpret_cast<unsigned long long*>(S.fz);
-    const uint32_t n = npl + nb;
-    for (uint32_t q0 = threadIdx.x; q0 < n; q0 += NT * kMdBatch) {
-        int ii[kMdBatch], jj[kMdBatch];
+    atomicAdd(&qx[j], gx); atomicAdd(&qy[j], gy); atomicAdd(&qz[j], gz);
+    atomicAdd(&qx[i], 0ull - gx); atomicAdd(&qy[i], 0ull - gy); atomicAdd(&qz[i], 0ull - gz);
+}
+
+/* Non-bonded part of Box::calculate_forces_verlet (Box.cpp:243-256) over the
+ * Verlet half list pl (entries i | j << 16; an LDS or a global array, known
+ * at the call site so the loads are not generic): each pair once, Newton's
+ * third law, into the fixed-point accumulators.  The force magnitude is
+ * evaluated without branches (pairs beyond their cutoff get weight 0) and
+ * kMdBatch list entries are loaded before any is used. */
+template <typename PL>
+__device__ __forceinline__ void md_pair_forces(const MdLds& S, PL pl, uint32_t npl, double lambda) {
+#pragma clang fp contract(fast)
+    constexpr int NT = kMdThreads;
+    for (uint32_t q0 = threadIdx.x; q0 < npl; q0 += NT * kMdBatch) {
+        uint32_t e[kMdBatch];
+#pragma unroll
+        for (int k = 0; k < kMdBatch; ++k) {
+            const uint32_t q = q0 + k * NT;
+            e[k] = q < npl ? pl[q] : 0xFFFFFFFFu;
+        }
         unsigned long long gx[kMdBatch], gy[kMdBatch], gz[kMdBatch];
         bool ok[kMdBatch];
 #pragma unroll
         for (int k = 0; k < kMdBatch; ++k) {
-            const uint32_t q = q0 + k * NT;
-            ok[k] = q < n;
-            const bool bond = q >= npl;
-            const uint32_t e = !ok[k] ? 0u : (bond ? S.bl[q - npl] : pl[q]);
-            const int i = bond ? (int)(e & 0xFFF) : (int)(e & 0xFFFF);
-            const int j = bond ? (int)((e >> 12) & 0xFFF) : (int)(e >> 16);
-            ii[k] = i; jj[k] = j;
+            const bool valid = e[k] != 0xFFFFFFFFu;
+            const int i = valid ? (int)(e[k] & 0xFFFF) : 0, j = valid ? (int)(e[k] >> 16) : 0;
             const double dx = S.x[j] - S.x[i], dy = S.y[j] - S.y[i], dz = S.z[j] - S.z[i];
-            const double r2 = (dx * dx + dy * dy) + dz * dz;
-            double fa = bond ? fene_force_fast(r2, ((e >> 24) & 1) ? 20.25 : 2.25)
-                             : pair_force_fast(r2, S.bt[i] && S.bt[j], lambda);
-            if (!ok[k]) fa = 0.0;
-            ok[k] = ok[k] && fa != 0.0;
+            const double r2 = valid ? (dx * dx + dy * dy) + dz * dz : 4.0;
+            const double fa = pair_force_fast(r2, (S.bt[i] & S.bt[j]) != 0, lambda);
+            ok[k] = fa != 0.0;
             gx[k] = to_fix(dx * fa); gy[k] = to_fix(dy * fa); gz[k] = to_fix(dz * fa);
         }
 #pragma unroll
-        for (int k = 0; k < kMdBatch; ++k) {
-            if (!ok[k]) continue;
-            atomicAdd(&qx[jj[k]], gx[k]); atomicAdd(&qy[jj[k]], gy[k]); atomicAdd(&qz[jj[k]], gz[k]);
-            atomicAdd(&qx[ii[k]], 0ull - gx[k]); atomicAdd(&qy[ii[k]], 0ull - gy[k]); atomicAdd(&qz[ii[k]], 0ull - gz[k]);
-        }
+        for (int k = 0; k < kMdBatch; ++k)
+            if (ok[k]) md_add_force(S, (int)(e[k] & 0xFFFF), (int)(e[k] >> 16), gx[k], gy[k], gz[k]);
+    }
+}
+
+/* Bonded part of Box::calculate_forces_verlet (Box.cpp:257-285): every bond
+ * once (entries i | j << 12 | anchor << 24), FENE or anchor FENE. */
+__device__ __forceinline__ void md_bond_forces(const MdLds& S, uint32_t nb) {
+#pragma clang fp contract(fast)
+    for (uint32_t q = threadIdx.x; q < nb; q += kMdThreads) {
+        const uint32_t e = S.bl[q];
+        const int i = (int)(e & 0xFFF), j = (int)((e >> 12) & 0xFFF);
+        const double dx = S.x[j] - S.x[i], dy = S.y[j] - S.y[i], dz = S.z[j] - S.z[i];
+        const double r2 = (dx * dx + dy * dy) + dz * dz;
+        const double fa = fene_force_fast(r2, ((e >> 24) & 1) ? 20.25 : 2.25);
+        if (fa != 0.0) md_add_force(S, i, j, to_fix(dx * fa), to_fix(dy * fa), to_fix(dz * fa));
     }
 }
 
@@ -546,9 +572,11 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
      * accumulators (zeroed by the caller); Epot of the same positions.  The
      * caller converts the accumulators. */
     auto forces = [&](bool calc_epot) {
-        const uint32_t* pl = lds_vl ? S.pl : gpl;
-        md_forces_flat(S, pl, npl, nbond, rep.lambda);
+        if (lds_vl) md_pair_forces(S, S.pl, npl, rep.lambda);
+        else md_pair_forces(S, (const uint32_t*)gpl, npl, rep.lambda);
+        md_bond_forces(S, nbond);
         __syncthreads();
+        const uint32_t* pl = lds_vl ? S.pl : gpl;
         epot_val = calc_epot ? md_epot_flat(S, pl, npl, nbond, rep.lambda) : 0.0;
         epot_set = true;
     };

@@ -157,13 +157,13 @@ SPMD_HD double rcp_nr(double x) {
  * rounding (the 48-24 core and B-B tail share 192 r^-2 r^-24 (r^-24 - 1/2),
  * the tail scaled by lambda). */
 SPMD_HD double pair_force_fast(double r2, bool both_b, double lambda) {
-    if (!(r2 < 2.25)) return 0.0;
+    /* no early exit: beyond both cutoffs the weight below is 0 (r2 > 0) */
     const double rm2 = rcp_nr(r2);
     double rm24 = rm2 * rm2 * rm2;
     rm24 *= rm24;
     rm24 *= rm24;
     const double f = 192.0 * rm2 * rm24 * (rm24 - 0.5);
-    const double w = r2 < 1.0594631 ? 1.0 : (both_b ? lambda : 0.0);
+    const double w = r2 < 1.0594631 ? 1.0 : (both_b && r2 < 2.25 ? lambda : 0.0);
     return f * w;
 }
 SPMD_HD double fene_force_fast(double r2, double R02) {

@@ -162,6 +162,29 @@ def test_coil_star_builder_and_md():
     assert np.abs(pg - po).max() < 1e-9 and np.abs(vg - vo).max() < 1e-9
 
 
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_forces_of_compressed_coil_match_oracle(seed):
+    """Forces of Box::calculate_forces_verlet on a coil star squeezed to 95%
+    (many non-bonded pairs at every separation, B-B pairs on both sides of
+    the 1.5 cutoff): within 1e-10 of max|F| of the FP64 oracle."""
+    g = MPCSystem(make_params(L=(64, 64, 64), n_c=0, step_update=10, seed=seed, md_dt=0.01))
+    g.add_star(15, 15, 20, 10.0, coil=True)
+    pos, vel, _ = g.get_monomers()
+    pos = pos * 0.95
+    g.set_monomers(pos=pos)
+    g.md_init()
+    o = ob.Oracle(L=(64, 64, 64), n_c=0, step_update=10, rng_mode=ob.RNG_PHILOX, seed=seed, md_dt=0.01)
+    o.add_star(15, 15, 20, 10.0)
+    o.set_monomers(pos, vel)
+    assert o.md_init() == 0
+    _, _, fg = g.get_monomers()
+    _, _, fo = o.get_monomers()
+    d = np.linalg.norm(pos[:, None] - pos[None], axis=2)
+    bb = np.outer(o.monomer_props()[1], o.monomer_props()[1]).astype(bool)
+    assert np.sum(bb & (d > 1.5) & (d < 2.0)) > 0 and np.sum(bb & (d > 1.0) & (d < 1.5)) > 0
+    assert np.abs(fg - fo).max() <= 1e-10 * np.abs(fo).max()
+
+
 @pytest.mark.parametrize("arms,A,B", [(16, 10, 10), (21, 13, 12), (30, 20, 20)])
 def test_md_large_stars_match_oracle(arms, A, B):
     """Stars of 321, 526 and 1,201 monomers run with one, two and four
