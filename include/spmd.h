@@ -189,6 +189,38 @@ int spmd_speed_histogram(spmd_sys* s, uint32_t replica, double width, uint32_t n
  * replica's star -- diagnostics, no reference counterpart. */
 int spmd_md_stats(spmd_sys* s, uint32_t replica, uint32_t* n_rebuild, uint32_t* vl_entries);
 
+/* ---- restart and configuration files ---- */
+/* spmd_params of a replica as created */
+int spmd_get_params(const spmd_sys* s, uint32_t replica, spmd_params* p);
+/* a replica's star: monomers, arms and beads per arm (zeros without a star) */
+int spmd_star_layout(const spmd_sys* s, uint32_t replica, uint32_t* n, uint32_t* arms, uint32_t* A, uint32_t* B);
+/* MPC::print_fluid_complete (MPC.cpp:570-579): "%f %f %f %f %f %f \n" =
+ * position, velocity of every solvent particle. */
+int spmd_write_fluid(spmd_sys* s, uint32_t replica, const char* path);
+/* MPC::initialize(filename, Box_old) (MPC.cpp:57-113): solvent from such a
+ * file written for a box old_box[3]; particles outside the present box,
+ * and those the file lacks, are placed at random in the region the box
+ * gained with U(-1/2,1/2) velocities (Philox, keyed by particle); all are
+ * wrapped and the mean velocity removed.  *n_from_file (optional) counts
+ * the particles taken from the file. */
+int spmd_read_fluid(spmd_sys* s, uint32_t replica, const char* path, const int32_t* old_box, uint64_t* n_from_file);
+/* Box::print_PDB_with_velocity (Box.cpp:563-602): one MODEL of the star,
+ * appended to the file when append != 0. */
+int spmd_write_pdb(spmd_sys* s, uint32_t replica, const char* path, int64_t step, int append);
+/* Box::add_star(filename, A, B, Arms, Mass, set_zero) (Box.cpp:48-54,
+ * Molecule::star_from_file Molecule.cpp:178-224): the star topology with
+ * positions and velocities from the first MODEL of such a file; set_zero
+ * moves its centre of mass and COM velocity to zero. */
+int spmd_add_star_pdb(spmd_sys* s, uint32_t replica, const char* path, uint32_t A, uint32_t B, uint32_t arms,
+                      double mass, int set_zero);
+/* Lossless binary snapshot of a handle (no reference counterpart: the
+ * reference restarts from the text files above, star-polymers.cpp:381-405):
+ * parameters, solvent (and particle ids when tracking), stars, delrx and the
+ * Philox collision counters, plus a caller's step number.  A handle loaded
+ * from it continues the run bit for bit. */
+int spmd_snapshot_save(spmd_sys* s, const char* path, int64_t step);
+int spmd_snapshot_load(const char* path, spmd_sys** out, int64_t* step);
+
 /* ---- tuning (no reference counterpart) ---- */
 /* particles per thread in the bin/perm kernels (1,2,4,8) and cells per
  * collide tile (64,128,256); chosen by starpolymersmd_amd.autotune. */

@@ -118,6 +118,15 @@ _SIGS = {
     "spmd_velocity_profile": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_uint32, _P(C.c_int32), _u32, _d]),
     "spmd_fourier_modes": (C.c_int, [C.c_void_p, C.c_uint32, _d]),
     "spmd_speed_histogram": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_uint32, _P(C.c_uint64)]),
+    "spmd_get_params": (C.c_int, [C.c_void_p, C.c_uint32, _P(Params)]),
+    "spmd_star_layout": (C.c_int, [C.c_void_p, C.c_uint32, _u32, _u32, _u32, _u32]),
+    "spmd_write_fluid": (C.c_int, [C.c_void_p, C.c_uint32, C.c_char_p]),
+    "spmd_read_fluid": (C.c_int, [C.c_void_p, C.c_uint32, C.c_char_p, _P(C.c_int32), _P(C.c_uint64)]),
+    "spmd_write_pdb": (C.c_int, [C.c_void_p, C.c_uint32, C.c_char_p, C.c_int64, C.c_int]),
+    "spmd_add_star_pdb": (C.c_int, [C.c_void_p, C.c_uint32, C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                    C.c_double, C.c_int]),
+    "spmd_snapshot_save": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    "spmd_snapshot_load": (C.c_int, [C.c_char_p, _P(C.c_void_p), _P(C.c_int64)]),
 }
 
 _lib = None

@@ -19,7 +19,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-LIB_SOURCES = ["spmd_kernels.hip", "spmd_api.hip", "spmd_mlp.hip"]
+LIB_SOURCES = ["spmd_kernels.hip", "spmd_api.hip", "spmd_mlp.hip", "spmd_io.hip"]
 
 
 def _run(cmd, cwd=None):

@@ -239,6 +239,16 @@ void prof_end(spmd_sys* s, int k, hipStream_t st) {
 hipError_t enqueue_collision_ops(spmd_sys* s, int n_md, int epot_last, int n_delrx, double delrx_dt, double h) {
     const DevState& d = s->d;
     hipError_t e;
+    if (d.m_tot > 0) {
+        /* the MD substeps need nothing of this collision's prologue: they
+         * start first, beside the prologue and the speculative sort */
+        if ((e = hipEventRecord(s->ev_fork, s->stream)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(s->side, s->ev_fork, 0)) != hipSuccess) return e;
+        prof_begin(s, 0, s->side);
+        if ((e = launch_md(d, s->side, n_md, epot_last, 0, 1, s->md_dt)) != hipSuccess) return e;
+        prof_end(s, 0, s->side);
+        if ((e = hipEventRecord(s->ev_join, s->side)) != hipSuccess) return e;
+    }
     prof_begin(s, 5, s->stream);
     if ((e = launch_prologue(d, s->stream, 1, n_delrx, delrx_dt)) != hipSuccess) return e;
     prof_end(s, 5, s->stream);
@@ -253,12 +263,6 @@ hipError_t enqueue_collision_ops(spmd_sys* s, int n_md, int epot_last, int n_del
         if ((e = launch_perm(d, s->stream, kSortNormal)) != hipSuccess) return e;
         prof_end(s, 3, s->stream);
     } else {
-        if ((e = hipEventRecord(s->ev_fork, s->stream)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(s->side, s->ev_fork, 0)) != hipSuccess) return e;
-        prof_begin(s, 0, s->side);
-        if ((e = launch_md(d, s->side, n_md, epot_last, 0, 1, s->md_dt)) != hipSuccess) return e;
-        prof_end(s, 0, s->side);
-        if ((e = hipEventRecord(s->ev_join, s->side)) != hipSuccess) return e;
         prof_begin(s, 1, s->stream);
         if ((e = launch_bin(d, s->stream, h, kSortSpec)) != hipSuccess) return e;
         prof_end(s, 1, s->stream);
@@ -1215,6 +1219,208 @@ int spmd_speed_histogram(spmd_sys* s, uint32_t r, double width, uint32_t nbins, 
     HIPCHK(s, launch_fluid_observe(s->d, s->stream, a, rp.n_fluid));
     HIPCHK(s, hipMemcpyAsync(counts, a.speed, (nbins + 1) * 8ull, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(s, hipStreamSynchronize(s->stream));
+    return SPMD_OK;
+}
+
+int spmd_get_params(const spmd_sys* s, uint32_t r, spmd_params* p) {
+    if (!valid_rep(s, r) || !p) return SPMD_ERR_ARG;
+    *p = s->params[r];
+    return SPMD_OK;
+}
+
+int spmd_star_layout(const spmd_sys* s, uint32_t r, uint32_t* n, uint32_t* arms, uint32_t* A, uint32_t* B) {
+    if (!valid_rep(s, r)) return SPMD_ERR_ARG;
+    const StarHost& st = s->stars[r];
+    if (n) *n = st.n;
+    if (arms) *arms = st.arms;
+    if (A) *A = st.A;
+    if (B) *B = st.B;
+    return SPMD_OK;
+}
+
+/* ------------------------------------------------------------ snapshots */
+
+}  // extern "C"
+
+namespace {
+
+constexpr char kSnapMagic[8] = {'S', 'P', 'M', 'D', 'S', 'N', 'P', '1'};
+constexpr uint32_t kSnapVersion = 1;
+
+struct SnapFile {
+    FILE* f;
+    bool ok = true;
+    explicit SnapFile(FILE* fp) : f(fp) {}
+    ~SnapFile() {
+        if (f) std::fclose(f);
+    }
+    void put(const void* p, size_t n) { ok = ok && (n == 0 || std::fwrite(p, 1, n, f) == n); }
+    void get(void* p, size_t n) { ok = ok && (n == 0 || std::fread(p, 1, n, f) == n); }
+    template <typename T> void put(const T& v) { put(&v, sizeof v); }
+    template <typename T> void get(T& v) { get(&v, sizeof v); }
+    template <typename T> void put_vec(const std::vector<T>& v) { put(v.data(), v.size() * sizeof(T)); }
+    template <typename T> void get_vec(std::vector<T>& v, size_t n) { v.resize(n); get(v.data(), n * sizeof(T)); }
+};
+
+}  // namespace
+
+extern "C" {
+
+int spmd_snapshot_save(spmd_sys* s, const char* path, int64_t step) {
+    if (!s || !path) return SPMD_ERR_ARG;
+    if (int rc = flush_md(s)) return rc;
+    if (int rc = ensure_monomers(s)) return rc;
+    if (int rc = download_monomers(s)) return rc;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    if (int rc = check_status(s)) return rc;
+    SnapFile out(std::fopen(path, "wb"));
+    if (!out.f) return fail(s, SPMD_ERR_ARG, std::string("cannot write ") + path);
+    const DevState& d = s->d;
+    const uint32_t nrep = (uint32_t)d.n_rep, tracking = d.ids[0] ? 1u : 0u;
+    out.put(kSnapMagic, sizeof kSnapMagic);
+    out.put(kSnapVersion);
+    out.put(nrep);
+    out.put((uint32_t)sizeof(spmd_params));
+    out.put(step);
+    out.put(s->md_dt);
+    out.put(tracking);
+    out.put(s->cl_k);
+    std::vector<RepCtl> ctl(nrep);
+    HIPCHK(s, hipMemcpy(ctl.data(), d.ctl, nrep * sizeof(RepCtl), hipMemcpyDeviceToHost));
+    for (uint32_t r = 0; r < nrep; ++r) {
+        const RepDev& rp = s->reps[r];
+        out.put(s->params[r]);
+        out.put(ctl[r].delrx);
+        out.put(ctl[r].k_done);
+        const uint64_t n = rp.n_fluid;
+        out.put(n);
+        std::vector<float> t(n);
+        const float* src[6] = {d.x[d.cur], d.y[d.cur], d.z[d.cur], d.vx[d.cur], d.vy[d.cur], d.vz[d.cur]};
+        for (int c = 0; c < 6; ++c) {
+            HIPCHK(s, hipMemcpy(t.data(), src[c] + rp.part_base, n * 4, hipMemcpyDeviceToHost));
+            out.put_vec(t);
+        }
+        if (tracking) {
+            std::vector<uint32_t> id(n);
+            HIPCHK(s, hipMemcpy(id.data(), d.ids[d.cur] + rp.part_base, n * 4, hipMemcpyDeviceToHost));
+            out.put_vec(id);
+        }
+        const StarHost& st = s->stars[r];
+        out.put(st.n);
+        out.put(st.arms);
+        out.put(st.A);
+        out.put(st.B);
+        if (st.n) {
+            out.put_vec(st.pos);
+            out.put_vec(st.vel);
+            out.put_vec(st.force);
+            out.put_vec(st.mass);
+            out.put_vec(st.btype);
+            out.put_vec(st.anchor);
+            const uint32_t nb = (uint32_t)st.bonds.size();
+            out.put(nb);
+            for (auto& b : st.bonds) { out.put(b.first); out.put(b.second); }
+        }
+    }
+    if (!out.ok) return fail(s, SPMD_ERR_ARG, std::string("short write to ") + path);
+    return SPMD_OK;
+}
+
+int spmd_snapshot_load(const char* path, spmd_sys** res, int64_t* step) {
+    if (!path || !res) return SPMD_ERR_ARG;
+    *res = nullptr;
+    SnapFile in(std::fopen(path, "rb"));
+    if (!in.f) return SPMD_ERR_ARG;
+    char magic[8];
+    uint32_t version = 0, nrep = 0, psize = 0, tracking = 0, cl_k = 0;
+    int64_t st0 = 0;
+    double md_dt = 0.0;
+    in.get(magic, sizeof magic);
+    in.get(version);
+    in.get(nrep);
+    in.get(psize);
+    in.get(st0);
+    in.get(md_dt);
+    in.get(tracking);
+    in.get(cl_k);
+    if (!in.ok || std::memcmp(magic, kSnapMagic, 8) || version != kSnapVersion || psize != sizeof(spmd_params) ||
+        nrep == 0 || nrep > (1u << 20))
+        return SPMD_ERR_ARG;
+    struct Rep {
+        spmd_params p;
+        double delrx;
+        uint32_t k_done;
+        std::vector<float> f[6];
+        std::vector<uint32_t> ids;
+        StarHost st;
+    };
+    std::vector<Rep> reps(nrep);
+    for (auto& rp : reps) {
+        in.get(rp.p);
+        in.get(rp.delrx);
+        in.get(rp.k_done);
+        uint64_t n = 0;
+        in.get(n);
+        if (!in.ok || n > (1ull << 34)) return SPMD_ERR_ARG;
+        for (int c = 0; c < 6; ++c) in.get_vec(rp.f[c], n);
+        if (tracking) in.get_vec(rp.ids, n);
+        StarHost& st = rp.st;
+        in.get(st.n);
+        in.get(st.arms);
+        in.get(st.A);
+        in.get(st.B);
+        if (!in.ok || st.n > (uint32_t)kMaxMono) return SPMD_ERR_ARG;
+        if (st.n) {
+            in.get_vec(st.pos, 3ull * st.n);
+            in.get_vec(st.vel, 3ull * st.n);
+            in.get_vec(st.force, 3ull * st.n);
+            in.get_vec(st.mass, st.n);
+            in.get_vec(st.btype, st.n);
+            in.get_vec(st.anchor, st.n);
+            uint32_t nb = 0;
+            in.get(nb);
+            if (!in.ok || nb > 4u * st.n) return SPMD_ERR_ARG;
+            st.bonds.resize(nb);
+            for (auto& b : st.bonds) { in.get(b.first); in.get(b.second); }
+        }
+        if (!in.ok) return SPMD_ERR_ARG;
+    }
+    std::vector<spmd_params> ps(nrep);
+    for (uint32_t r = 0; r < nrep; ++r) ps[r] = reps[r].p;
+    spmd_sys* s = nullptr;
+    if (int rc = spmd_create_batch(ps.data(), nrep, &s)) return rc;
+    auto bail = [&](int rc) { spmd_destroy(s); return rc; };
+    s->md_dt = md_dt;
+    s->cl_k = cl_k;
+    if (tracking)
+        if (int rc = spmd_set_tracking(s, 1)) return bail(rc);
+    DevState& d = s->d;
+    for (uint32_t r = 0; r < nrep; ++r) {
+        const RepDev& rp = s->reps[r];
+        if (reps[r].f[0].size() != rp.n_fluid) return bail(SPMD_ERR_ARG);
+        float* dst[6] = {d.x[d.cur], d.y[d.cur], d.z[d.cur], d.vx[d.cur], d.vy[d.cur], d.vz[d.cur]};
+        for (int c = 0; c < 6; ++c)
+            if (hipMemcpy(dst[c] + rp.part_base, reps[r].f[c].data(), rp.n_fluid * 4, hipMemcpyHostToDevice))
+                return bail(SPMD_ERR_HIP);
+        if (tracking && hipMemcpy(d.ids[d.cur] + rp.part_base, reps[r].ids.data(), rp.n_fluid * 4,
+                                  hipMemcpyHostToDevice))
+            return bail(SPMD_ERR_HIP);
+        if (hipMemcpy(&d.ctl[r].delrx, &reps[r].delrx, 8, hipMemcpyHostToDevice) ||
+            hipMemcpy(&d.ctl[r].k_done, &reps[r].k_done, 4, hipMemcpyHostToDevice))
+            return bail(SPMD_ERR_HIP);
+        s->stars[r] = std::move(reps[r].st);
+    }
+    s->mono_dirty = true;
+    /* Verlet lists and forces from the saved positions (the forces are a
+     * function of the positions alone, so they equal the saved ones) */
+    if (d.n_rep > 0) {
+        bool any = false;
+        for (auto& st : s->stars) any = any || st.n > 0;
+        if (any)
+            if (int rc = spmd_md_init(s)) return bail(rc);
+    }
+    if (step) *step = st0;
+    *res = s;
     return SPMD_OK;
 }
 

@@ -676,10 +676,8 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
         }
     }
 
-    /* write the monomer state back; for a collision also the monomers'
-     * cells (MPC::shiftParticles MPC.cpp:306-310 + sort MPC.cpp:221-235:
-     * shifted, never wrapped) */
-    const double sh[3] = {ctl->shift[0], ctl->shift[1], ctl->shift[2]};
+    /* write the monomer state back (their cells for the collision are
+     * found by k_sort_redo, once the grid shift is drawn) */
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int i = tid + k * NT;
@@ -689,11 +687,6 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
         d.mvel[g] = rvx[k]; d.mvel[g + 1] = rvy[k]; d.mvel[g + 2] = rvz[k];
         d.mforce[g] = S.fx[i]; d.mforce[g + 1] = S.fy[i]; d.mforce[g + 2] = S.fz[i];
         d.mvpos[g] = rpx[k]; d.mvpos[g + 1] = rpy[k]; d.mvpos[g + 2] = rpz[k];
-        if (do_coll) {
-            const uint32_t ci = cell_index(rep.box, rx[k] + sh[0], ry[k] + sh[1], rz[k] + sh[2]);
-            if (ci >= rep.ncells) atomicOr(&ctl->status, ST_CELL_OOB);
-            d.mcell[mb + i] = ci < rep.ncells ? (uint32_t)rep.cell_base + ci : ~0u;
-        }
     }
 }
 
@@ -730,8 +723,10 @@ __global__ __launch_bounds__(64) void k_prologue(DevState d, int do_coll, int n_
             for (int c = 0; c < 3; ++c) ctl->shift[c] = sh[c];
             ctl->k_cur = k;
             ctl->k_done = k + 1;
-            ctl->recenter = 0;
-            ctl->redo = 0;
+            if (d.m_tot == 0) {  /* else k_md, running beside this kernel, sets them */
+                ctl->recenter = 0;
+                ctl->redo = 0;
+            }
         }
     }
     if (do_coll && r == 0) {
@@ -924,8 +919,22 @@ __global__ __launch_bounds__(1024) void k_sort_redo(DevState d, double h) {
     __shared__ uint32_t s_w[16];
     __shared__ uint32_t s_carry;
     const int r = blockIdx.x;
-    if (!d.ctl[r].redo) return;
     const RepDev& rep = d.reps[r];
+    {
+        /* the monomers' cells (MPC::shiftParticles MPC.cpp:306-310 + sort
+         * MPC.cpp:221-235: shifted, never wrapped), from the positions
+         * k_md left and the grid shift k_prologue drew */
+        const double sh0 = d.ctl[r].shift[0], sh1 = d.ctl[r].shift[1], sh2 = d.ctl[r].shift[2];
+        bool oob = false;
+        for (uint32_t i = threadIdx.x; i < rep.n_mono; i += 1024) {
+            const size_t g = 3 * (size_t)(rep.mono_base + i);
+            const uint32_t ci = cell_index(rep.box, d.mpos[g] + sh0, d.mpos[g + 1] + sh1, d.mpos[g + 2] + sh2);
+            oob |= ci >= rep.ncells;
+            d.mcell[rep.mono_base + i] = ci < rep.ncells ? (uint32_t)rep.cell_base + ci : ~0u;
+        }
+        if (oob) atomicOr(&d.ctl[r].status, ST_CELL_OOB);
+    }
+    if (!d.ctl[r].redo) return;
     RepCtl c;
     sort_ctl(d, r, kSortNormal, c);
     Box b = rep.box;

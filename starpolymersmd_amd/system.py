@@ -33,6 +33,61 @@ class MPCSystem:
             raise SpmdError(rc, lib.spmd_error_string(rc).decode())
         self.h = h
 
+    @classmethod
+    def load_snapshot(cls, path: str):
+        """A handle restored from spmd_snapshot_save; returns (system, step)."""
+        lib = load()
+        h = C.c_void_p()
+        step = C.c_int64()
+        rc = lib.spmd_snapshot_load(str(path).encode(), C.byref(h), C.byref(step))
+        if rc:
+            raise SpmdError(rc, lib.spmd_error_string(rc).decode())
+        self = cls.__new__(cls)
+        self._lib = lib
+        self.h = h
+        n = C.c_uint32()
+        lib.spmd_num_replicas(h, C.byref(n))
+        self.n_replicas = n.value
+        self.params = []
+        for r in range(self.n_replicas):
+            p = Params()
+            lib.spmd_get_params(h, r, C.byref(p))
+            self.params.append(p)
+        return self, step.value
+
+    def save_snapshot(self, path: str, step: int = 0):
+        """Lossless binary snapshot (parameters, solvent, stars, delrx,
+        Philox counters); a loaded copy continues bit for bit."""
+        self._chk(self._lib.spmd_snapshot_save(self.h, str(path).encode(), step))
+
+    # ------------------------------------------------------- restart files
+    def write_fluid(self, path: str, r: int = 0):
+        """MPC::print_fluid_complete (MPC.cpp:570-579)."""
+        self._chk(self._lib.spmd_write_fluid(self.h, r, str(path).encode()))
+
+    def read_fluid(self, path: str, old_box=None, r: int = 0) -> int:
+        """MPC::initialize(filename, Box_old) (MPC.cpp:57-113); returns the
+        number of particles taken from the file."""
+        L = [int(l) for l in self.params[r].L] if old_box is None else list(old_box)
+        ob = (C.c_int32 * 3)(*L)
+        n = C.c_uint64()
+        self._chk(self._lib.spmd_read_fluid(self.h, r, str(path).encode(), ob, C.byref(n)))
+        return n.value
+
+    def write_pdb(self, path: str, step: int, r: int = 0, append: bool = False):
+        """Box::print_PDB_with_velocity (Box.cpp:563-602)."""
+        self._chk(self._lib.spmd_write_pdb(self.h, r, str(path).encode(), step, int(append)))
+
+    def add_star_pdb(self, path: str, A: int, B: int, arms: int, mass: float, set_zero: bool = True, r: int = 0):
+        """Box::add_star(filename, A, B, Arms, Mass, set_zero) (Box.cpp:48-54)."""
+        self._chk(self._lib.spmd_add_star_pdb(self.h, r, str(path).encode(), A, B, arms, mass, int(set_zero)))
+
+    def star_layout(self, r: int = 0):
+        """(monomers, arms, A, B) of a replica's star."""
+        v = [C.c_uint32() for _ in range(4)]
+        self._chk(self._lib.spmd_star_layout(self.h, r, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
     # ------------------------------------------------------------ helpers
     def _chk(self, rc: int):
         if rc:
