@@ -37,7 +37,7 @@ enum {
     SPMD_ERR_CELL_OOB = 4,       /* MPC::sort cell index out of range (MPC.cpp:212-219,228-233 raise SIGSEGV) */
     SPMD_ERR_VERLET_OOB = 5,     /* Box::update_VerletLists grid bounds (Box.cpp:133-142 raise SIGSEGV) */
     SPMD_ERR_VERLET_OVERFLOW = 6,/* a monomer has more Verlet neighbours than the list capacity */
-    SPMD_ERR_UNSUPPORTED = 7,    /* e.g. the angular-momentum-conserving collision (MPC.cpp:258-269) */
+    SPMD_ERR_UNSUPPORTED = 7,    /* an option this build does not provide */
     SPMD_ERR_STATE = 8,          /* call out of order */
     SPMD_ERR_RANK_OVERFLOW = 9,  /* more than 65535 particles in one cell */
     SPMD_ERR_NO_DEVICE = 10,     /* no usable gfx950 device */
@@ -56,7 +56,7 @@ typedef struct spmd_params {
     double   lambda;          /* B-B attraction (Box Lambda) */
     double   md_dt;           /* velocity-Verlet step (Thermostat DeltaT) */
     uint64_t seed;            /* Philox4x32-10 key (replaces mt19937_64(1), Rand.cpp:6) */
-    int32_t  angular_momentum;/* MPC angular_mom flag; must be 0 */
+    int32_t  angular_momentum;/* MPC angular_mom flag: angular-momentum-conserving collision (MPC.cpp:258-269) */
 } spmd_params;
 
 typedef struct spmd_sys spmd_sys;

@@ -431,6 +431,36 @@ inline double* member_vel(orc_sys& s, int64_t id) {
     return id >= 0 ? &s.vel[3 * id] : s.mono[-1 - id].vel.v;
 }
 inline double member_mass(const orc_sys& s, int64_t id) { return id >= 0 ? 1.0 : s.mono[-1 - id].mass; }
+inline const double* member_pos(const orc_sys& s, int64_t id) {
+    return id >= 0 ? &s.pos[3 * id] : s.mono[-1 - id].pos.v;
+}
+
+/* The 3x3 symmetric positive-definite solve of Matrix3d::llt().solve
+ * (MPC.cpp:373): Cholesky A = L L^T, then L y = b and L^T x = y. */
+void llt_solve3(const double A[3][3], const double b[3], double x[3]) {
+    double L[3][3] = {};
+    for (int j = 0; j < 3; ++j) {
+        double d = A[j][j];
+        for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
+        L[j][j] = std::sqrt(d);
+        for (int i = j + 1; i < 3; ++i) {
+            double v = A[i][j];
+            for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
+            L[i][j] = v / L[j][j];
+        }
+    }
+    double y[3];
+    for (int i = 0; i < 3; ++i) {
+        double v = b[i];
+        for (int k = 0; k < i; ++k) v -= L[i][k] * y[k];
+        y[i] = v / L[i][i];
+    }
+    for (int i = 2; i >= 0; --i) {
+        double v = y[i];
+        for (int k = i + 1; k < 3; ++k) v -= L[k][i] * x[k];
+        x[i] = v / L[i][i];
+    }
+}
 
 /* MPC::calculateCMV, MPC.cpp:314-322. */
 V3 calc_cmv(orc_sys& s, uint32_t cell) {
@@ -473,7 +503,9 @@ void thermostat(orc_sys& s, uint32_t cell, const V3& cmv) {
     }
 }
 
-/* MPC::collide, MPC.cpp:239-270 (angular-momentum branch off). */
+/* MPC::collide, MPC.cpp:239-270; with the angular-momentum flag and more
+ * than two members, MPC::calculateCMP (MPC.cpp:324-333) and
+ * MPC::calculateAngular (MPC.cpp:344-374) add omega x (r - CMP). */
 void collide(orc_sys& s, uint32_t cell, const V3& cmv) {
     double u1, u2;
     switch (s.p.rng_mode) {
@@ -502,12 +534,54 @@ void collide(orc_sys& s, uint32_t cell, const V3& cmv) {
     R[2][0] = a0 * a2 * (1 - c) - a1 * sn;
     R[2][1] = a1 * a2 * (1 - c) + a0 * sn;
     R[2][2] = a2 * a2 + (1 - a2 * a2) * c;
+    const bool amc = s.p.amc && s.cell_list[cell].size() > 2;
+    double cmp[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+    if (amc) {
+        double tm = 0.0;
+        for (int64_t id : s.cell_list[cell]) {
+            const double* x = member_pos(s, id);
+            const double m = member_mass(s, id);
+            for (int i = 0; i < 3; ++i) cmp[i] += x[i] * m;
+            tm += m;
+        }
+        for (int i = 0; i < 3; ++i) cmp[i] /= tm;
+        double C[3] = {0, 0, 0}, I[3][3] = {};
+        for (int64_t id : s.cell_list[cell]) {
+            const double* x = member_pos(s, id);
+            const double* v = member_vel(s, id);
+            const double m = member_mass(s, id);
+            const double r[3] = {x[0] - cmp[0], x[1] - cmp[1], x[2] - cmp[2]};
+            const double u[3] = {v[0] - cmv[0], v[1] - cmv[1], v[2] - cmv[2]};
+            double q[3];  /* u - R u */
+            for (int i = 0; i < 3; ++i) q[i] = u[i] - ((R[i][0] * u[0] + R[i][1] * u[1]) + R[i][2] * u[2]);
+            C[0] += m * (r[1] * q[2] - r[2] * q[1]);
+            C[1] += m * (r[2] * q[0] - r[0] * q[2]);
+            C[2] += m * (r[0] * q[1] - r[1] * q[0]);
+            I[0][0] += m * (r[1] * r[1] + r[2] * r[2]);
+            I[1][1] += m * (r[0] * r[0] + r[2] * r[2]);
+            I[2][2] += m * (r[1] * r[1] + r[0] * r[0]);
+            I[0][1] -= m * r[0] * r[1];
+            I[0][2] -= m * r[0] * r[2];
+            I[1][2] -= m * r[1] * r[2];
+        }
+        I[1][0] = I[0][1];
+        I[2][0] = I[0][2];
+        I[2][1] = I[1][2];
+        llt_solve3(I, C, w);
+    }
     for (int64_t id : s.cell_list[cell]) {
         double* v = member_vel(s, id);
         double d0 = v[0] - cmv[0], d1 = v[1] - cmv[1], d2 = v[2] - cmv[2];
         double n0 = cmv[0] + ((R[0][0] * d0 + R[0][1] * d1) + R[0][2] * d2);
         double n1 = cmv[1] + ((R[1][0] * d0 + R[1][1] * d1) + R[1][2] * d2);
         double n2 = cmv[2] + ((R[2][0] * d0 + R[2][1] * d1) + R[2][2] * d2);
+        if (amc) {
+            const double* x = member_pos(s, id);
+            const double r[3] = {x[0] - cmp[0], x[1] - cmp[1], x[2] - cmp[2]};
+            n0 += w[1] * r[2] - w[2] * r[1];
+            n1 += w[2] * r[0] - w[0] * r[2];
+            n2 += w[0] * r[1] - w[1] * r[0];
+        }
         v[0] = n0; v[1] = n1; v[2] = n2;
     }
 }

@@ -40,6 +40,7 @@ typedef struct orc_params {
     int32_t  rng_mode;
     uint64_t seed;          /* Philox key (ORC_RNG_PHILOX) */
     uint32_t replica;       /* replica id entering the Philox counter */
+    int32_t  amc;           /* angular-momentum-conserving collision (MPC ctor angular_mom) */
 } orc_params;
 
 typedef struct orc_sys orc_sys;

@@ -530,12 +530,13 @@ int spmd_create_batch(const spmd_params* p, uint32_t n_rep, spmd_sys** out) {
     uint64_t nt = 0, ct = 0;
     for (uint32_t r = 0; r < n_rep; ++r) {
         const spmd_params& q = p[r];
-        if (q.angular_momentum) { delete s; return SPMD_ERR_UNSUPPORTED; }
         if (q.step_update != p[0].step_update || q.md_dt != p[0].md_dt || q.n_c < 0 || q.kT <= 0.0) {
             delete s;
             return SPMD_ERR_ARG;
         }
         RepDev rp{};
+        rp.amc = q.angular_momentum != 0;
+        d.any_amc |= rp.amc;
         for (int i = 0; i < 3; ++i) {
             int l = q.L[i];
             if (l % 2) l++;                      /* Box.cpp:14-16 */

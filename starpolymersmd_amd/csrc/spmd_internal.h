@@ -60,6 +60,7 @@ struct RepDev {
     uint32_t bond_base, n_bond;  /* this replica's entries of the bond list */
     int vgrid[3];          /* Verlet grid cells per side (Box.cpp:20-22) */
     double vside[3];       /* Verlet grid cell side (Box.cpp:23-25) */
+    int amc;               /* angular-momentum-conserving collision (MPC angular_mom) */
 };
 
 struct RepCtl {
@@ -114,6 +115,7 @@ struct DevState {
     uint32_t md_vl_cap;    /* Verlet entries the MD kernel keeps in LDS */
     int ppt;               /* particles per thread in bin/scatter */
     int tile_cells;        /* cells per collide tile */
+    int any_amc;           /* some replica collides with angular-momentum conservation */
 };
 
 /* bytes of dynamic LDS the MD kernel needs for stride ns and cap entries */

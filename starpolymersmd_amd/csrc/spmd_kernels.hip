@@ -1023,12 +1023,20 @@ __global__ __launch_bounds__(256) void k_perm(DevState d, int mode) {
  * cell, in this order (MPC.cpp:130-137): CMV over fluid and monomers,
  * relative kinetic energy, MBS scale sqrt(Gamma/E) (only n >= 2), then
  * v <- CMV + s R (v - CMV).  Sums are 64-bit fixed point in LDS, so their
- * value does not depend on the order the particles arrive in. */
-template <int TC, int CAP>
+ * value does not depend on the order the particles arrive in.
+ * AMC adds, for replicas with the angular-momentum flag and cells of more
+ * than two members (MPC.cpp:258-269, 324-374): the centre of mass CMP, the
+ * inertia tensor I and C = sum m r x (u - R u) about it (r = x - CMP, u = v -
+ * CMV, before the thermostat), omega = s I^-1 C by Cholesky, and v +=
+ * omega x r.  Positions enter as coordinates inside the cell (FP32, exact to
+ * 6e-8), so r carries no cancellation. */
+template <int TC, int CAP, bool AMC>
 __global__ __launch_bounds__(256) void k_collide(DevState d, double h) {
     constexpr int NT = 256;
     constexpr int KM = CAP / NT;       /* particles a thread keeps in registers */
     constexpr int MCAP = 2 * TC + 64;  /* monomers per tile (unit cells hold ~1 at most) */
+    constexpr int TA = AMC ? TC : 1;   /* cells of the angular-momentum sums */
+    constexpr int KA = AMC ? KM : 1;
     __shared__ uint32_t s_off[TC + 1];
     __shared__ unsigned long long s_px[TC], s_py[TC], s_pz[TC], s_e[TC], s_mm[TC];
     __shared__ uint32_t s_nm[TC];
@@ -1038,6 +1046,8 @@ __global__ __launch_bounds__(256) void k_collide(DevState d, double h) {
     __shared__ uint32_t s_ml[MCAP];    /* local cell << 16 | monomer */
     __shared__ uint32_t s_mn;
     __shared__ uint32_t s_bad;
+    __shared__ unsigned long long s_am[12][TA];  /* sum m l (3), I (6: xx yy zz xy xz yz), C (3) */
+    __shared__ double s_cmp[TA][3], s_w[TA][3];
 
     const int tid = threadIdx.x;
     const uint32_t lt = xcd_remap(blockIdx.x, gridDim.x);
@@ -1066,6 +1076,9 @@ __global__ __launch_bounds__(256) void k_collide(DevState d, double h) {
         s_nm[i] = 0u;
     }
     if (tid == 0) { s_mn = 0u; s_bad = 0u; }
+    const bool amc = AMC && rep.amc;
+    if (amc)
+        for (uint32_t i = tid; i < 12u * nc; i += NT) s_am[i / nc][i % nc] = 0ull;
     __syncthreads();
     /* the replica's monomers in this tile's cells (any order: the sums
      * are fixed point) */
@@ -1124,7 +1137,8 @@ __global__ __launch_bounds__(256) void k_collide(DevState d, double h) {
      * in k_bin), FP32 velocity in the unshifted frame, y image index cy of
      * the shifted position (MPC::shiftParticles' wrap) and local cell; with
      * write_pos the new position (and id) go to the other buffer */
-    auto gather = [&](uint32_t q, bool write_pos, uint32_t& lc, int& cy, float& fvx, float& fvy, float& fvz) {
+    auto gather = [&](uint32_t q, bool write_pos, uint32_t& lc, int& cy, float& fvx, float& fvy, float& fvz,
+                      float& lx, float& ly, float& lz) {
         const uint32_t src = d.perm[p0 + q];
         double x = X[src], y = Y[src], z = Z[src], vx = VX[src], vy = VY[src], vz = VZ[src];
         recenter_stream(b, h, cc, x, y, z, vx, vy, vz);
@@ -1137,28 +1151,51 @@ __global__ __launch_bounds__(256) void k_collide(DevState d, double h) {
         cy = (int)le_wrap(b, px, py, pz, vimg);
         lc = cell_index(b, px, py, pz) - c_lo;
         if (lc >= nc) { s_bad = 1u; lc = 0; }
+        if (AMC) {  /* coordinates inside the cell, as cell_index truncates */
+            const double qx = px + b.half[0], qy = py + b.half[1], qz = pz + b.half[2];
+            lx = (float)(qx - (double)(int)qx); ly = (float)(qy - (double)(int)qy); lz = (float)(qz - (double)(int)qz);
+        }
+    };
+    /* a monomer's coordinates inside its cell (shifted, never wrapped) */
+    auto mono_local = [&](uint32_t mi, double& lx, double& ly, double& lz) {
+        const double qx = d.mpos[3 * mi] + cc.shift[0] + b.half[0];
+        const double qy = d.mpos[3 * mi + 1] + cc.shift[1] + b.half[1];
+        const double qz = d.mpos[3 * mi + 2] + cc.shift[2] + b.half[2];
+        lx = qx - (double)(int)qx; ly = qy - (double)(int)qy; lz = qz - (double)(int)qz;
     };
     uint32_t rlc[KM];
     int rcy[KM];
     float rvx[KM], rvy[KM], rvz[KM];
+    float rlx[KA], rly[KA], rlz[KA];
+    auto add_l = [&](uint32_t lc, double m, double lx, double ly, double lz) {
+        atomicAdd(&s_am[0][lc], to_fix(m * lx));
+        atomicAdd(&s_am[1][lc], to_fix(m * ly));
+        atomicAdd(&s_am[2][lc], to_fix(m * lz));
+    };
 
     /* MPC::calculateCMV (MPC.cpp:314-322) */
 #pragma unroll
     for (int kk = 0; kk < KM; ++kk) {
         const uint32_t q = tid + kk * NT;
         if (q >= np) continue;
-        gather(q, true, rlc[kk], rcy[kk], rvx[kk], rvy[kk], rvz[kk]);
+        float lx = 0.f, ly = 0.f, lz = 0.f;
+        gather(q, true, rlc[kk], rcy[kk], rvx[kk], rvy[kk], rvz[kk], lx, ly, lz);
         const uint32_t lc = rlc[kk];
         atomicAdd(&s_px[lc], to_fix((double)rvx[kk] - (double)rcy[kk] * shear * LyD));
         atomicAdd(&s_py[lc], to_fix((double)rvy[kk]));
         atomicAdd(&s_pz[lc], to_fix((double)rvz[kk]));
+        if (AMC) {
+            rlx[AMC ? kk : 0] = lx; rly[AMC ? kk : 0] = ly; rlz[AMC ? kk : 0] = lz;
+            if (amc) add_l(lc, 1.0, lx, ly, lz);
+        }
     }
     for (uint32_t q = tid + CAP; q < np; q += NT) {
-        uint32_t lc; int cy; float fx, fy, fz;
-        gather(q, true, lc, cy, fx, fy, fz);
+        uint32_t lc; int cy; float fx, fy, fz, lx = 0.f, ly = 0.f, lz = 0.f;
+        gather(q, true, lc, cy, fx, fy, fz, lx, ly, lz);
         atomicAdd(&s_px[lc], to_fix((double)fx - (double)cy * shear * LyD));
         atomicAdd(&s_py[lc], to_fix((double)fy));
         atomicAdd(&s_pz[lc], to_fix((double)fz));
+        if (amc) add_l(lc, 1.0, lx, ly, lz);
     }
     for (uint32_t j = tid; j < nmt; j += NT) {
         const uint32_t e = s_ml[j];
@@ -1169,6 +1206,11 @@ __global__ __launch_bounds__(256) void k_collide(DevState d, double h) {
         atomicAdd(&s_py[lc], to_fix(d.mvel[3 * mi + 1] * mm));
         atomicAdd(&s_pz[lc], to_fix(d.mvel[3 * mi + 2] * mm));
         atomicAdd(&s_mm[lc], to_fix(mm));
+        if (amc) {
+            double lx, ly, lz;
+            mono_local(mi, lx, ly, lz);
+            add_l(lc, mm, lx, ly, lz);
+        }
     }
     __syncthreads();
     if (tid == 0 && s_bad) atomicOr(&ctl->status, ST_SORT_MISMATCH);
@@ -1177,22 +1219,49 @@ __global__ __launch_bounds__(256) void k_collide(DevState d, double h) {
         double c0 = from_fix(s_px[ci]), c1 = from_fix(s_py[ci]), c2 = from_fix(s_pz[ci]);
         if (M > 0) { c0 /= M; c1 /= M; c2 /= M; }
         s_cmv[ci][0] = c0; s_cmv[ci][1] = c1; s_cmv[ci][2] = c2;
+        if (amc) {  /* MPC::calculateCMP MPC.cpp:324-333, inside the cell */
+            const double inv = M > 0 ? 1.0 / M : 0.0;
+            s_cmp[ci][0] = from_fix(s_am[0][ci]) * inv;
+            s_cmp[ci][1] = from_fix(s_am[1][ci]) * inv;
+            s_cmp[ci][2] = from_fix(s_am[2][ci]) * inv;
+        }
     }
     __syncthreads();
 
     /* MPC::thermostat relative kinetic energy (MPC.cpp:277-279) */
-    auto add_e = [&](uint32_t lc, int cy, float fx, float fy, float fz) {
+    /* the angular-momentum sums of one member: I about CMP and
+     * C = m r x (u - R u) (MPC::calculateAngular MPC.cpp:344-372) */
+    auto add_amc = [&](uint32_t lc, double m, double lx, double ly, double lz, double ux, double uy, double uz) {
+        const double rx = lx - s_cmp[lc][0], ry = ly - s_cmp[lc][1], rz = lz - s_cmp[lc][2];
+        const float* R = s_R[lc];
+        const double qx = ux - (((double)R[0] * ux + (double)R[1] * uy) + (double)R[2] * uz);
+        const double qy = uy - (((double)R[3] * ux + (double)R[4] * uy) + (double)R[5] * uz);
+        const double qz = uz - (((double)R[6] * ux + (double)R[7] * uy) + (double)R[8] * uz);
+        atomicAdd(&s_am[3][lc], to_fix(m * (ry * ry + rz * rz)));
+        atomicAdd(&s_am[4][lc], to_fix(m * (rx * rx + rz * rz)));
+        atomicAdd(&s_am[5][lc], to_fix(m * (ry * ry + rx * rx)));
+        atomicAdd(&s_am[6][lc], to_fix(-m * rx * ry));
+        atomicAdd(&s_am[7][lc], to_fix(-m * rx * rz));
+        atomicAdd(&s_am[8][lc], to_fix(-m * ry * rz));
+        atomicAdd(&s_am[9][lc], to_fix(m * (ry * qz - rz * qy)));
+        atomicAdd(&s_am[10][lc], to_fix(m * (rz * qx - rx * qz)));
+        atomicAdd(&s_am[11][lc], to_fix(m * (rx * qy - ry * qx)));
+    };
+    auto add_e = [&](uint32_t lc, int cy, float fx, float fy, float fz, float lx, float ly, float lz) {
         const double vx = (double)fx - (double)cy * shear * LyD;
         const double dx = vx - s_cmv[lc][0], dy = (double)fy - s_cmv[lc][1], dz = (double)fz - s_cmv[lc][2];
         atomicAdd(&s_e[lc], to_fix((dx * dx + dy * dy) + dz * dz));
+        if (amc) add_amc(lc, 1.0, lx, ly, lz, dx, dy, dz);
     };
 #pragma unroll
     for (int kk = 0; kk < KM; ++kk)
-        if (tid + kk * NT < np) add_e(rlc[kk], rcy[kk], rvx[kk], rvy[kk], rvz[kk]);
+        if (tid + kk * NT < np)
+            add_e(rlc[kk], rcy[kk], rvx[kk], rvy[kk], rvz[kk], AMC ? rlx[AMC ? kk : 0] : 0.f,
+                  AMC ? rly[AMC ? kk : 0] : 0.f, AMC ? rlz[AMC ? kk : 0] : 0.f);
     for (uint32_t q = tid + CAP; q < np; q += NT) {
-        uint32_t lc; int cy; float fx, fy, fz;
-        gather(q, false, lc, cy, fx, fy, fz);
-        add_e(lc, cy, fx, fy, fz);
+        uint32_t lc; int cy; float fx, fy, fz, lx = 0.f, ly = 0.f, lz = 0.f;
+        gather(q, false, lc, cy, fx, fy, fz, lx, ly, lz);
+        add_e(lc, cy, fx, fy, fz, lx, ly, lz);
     }
     for (uint32_t j = tid; j < nmt; j += NT) {
         const uint32_t e = s_ml[j];
@@ -1202,13 +1271,33 @@ __global__ __launch_bounds__(256) void k_collide(DevState d, double h) {
         const double dy = d.mvel[3 * mi + 1] - s_cmv[lc][1];
         const double dz = d.mvel[3 * mi + 2] - s_cmv[lc][2];
         atomicAdd(&s_e[lc], to_fix(d.mmass[mi] * ((dx * dx + dy * dy) + dz * dz)));
+        if (amc) {
+            double lx, ly, lz;
+            mono_local(mi, lx, ly, lz);
+            add_amc(lc, d.mmass[mi], lx, ly, lz, dx, dy, dz);
+        }
     }
     __syncthreads();
 
-    /* MBS scale sqrt(Gamma / E_rel) for n >= 2 (MPC.cpp:287-289) */
+    /* MBS scale sqrt(Gamma / E_rel) for n >= 2 (MPC.cpp:287-289); with the
+     * angular-momentum flag, omega = s I^-1 C for n > 2 (MPC.cpp:260, 373) */
     for (uint32_t ci = tid; ci < nc; ci += NT) {
         const uint32_t n = (s_off[ci + 1] - s_off[ci]) + s_nm[ci];
         s_g[ci] = n >= 2 ? sqrt(s_g[ci] / (from_fix(s_e[ci]) * 0.5)) : -1.0;
+        if (amc) {
+            double w[3] = {0.0, 0.0, 0.0};
+            if (n > 2) {
+                const double A[3][3] = {{from_fix(s_am[3][ci]), from_fix(s_am[6][ci]), from_fix(s_am[7][ci])},
+                                        {from_fix(s_am[6][ci]), from_fix(s_am[4][ci]), from_fix(s_am[8][ci])},
+                                        {from_fix(s_am[7][ci]), from_fix(s_am[8][ci]), from_fix(s_am[5][ci])}};
+                const double c3[3] = {from_fix(s_am[9][ci]), from_fix(s_am[10][ci]), from_fix(s_am[11][ci])};
+                llt_solve3(A, c3, w);
+                const double sc = s_g[ci] >= 0.0 ? s_g[ci] : 1.0;
+                const bool ok = isfinite(w[0]) && isfinite(w[1]) && isfinite(w[2]);
+                for (int q = 0; q < 3; ++q) w[q] = ok ? w[q] * sc : 0.0;
+            }
+            s_w[ci][0] = w[0]; s_w[ci][1] = w[1]; s_w[ci][2] = w[2];
+        }
     }
     __syncthreads();
 
@@ -1228,22 +1317,33 @@ __global__ __launch_bounds__(256) void k_collide(DevState d, double h) {
         vy = c1 + (((double)R[3] * d0 + (double)R[4] * d1) + (double)R[5] * d2);
         vz = c2 + (((double)R[6] * d0 + (double)R[7] * d1) + (double)R[8] * d2);
     };
-    auto finish = [&](uint32_t q, uint32_t lc, int cy, float fx, float fy, float fz) {
+    /* v += omega x (x - CMP) (MPC.cpp:268) */
+    auto add_w = [&](uint32_t lc, double lx, double ly, double lz, double& vx, double& vy, double& vz) {
+        const double rx = lx - s_cmp[lc][0], ry = ly - s_cmp[lc][1], rz = lz - s_cmp[lc][2];
+        const double w0 = s_w[lc][0], w1 = s_w[lc][1], w2 = s_w[lc][2];
+        vx += w1 * rz - w2 * ry;
+        vy += w2 * rx - w0 * rz;
+        vz += w0 * ry - w1 * rx;
+    };
+    auto finish = [&](uint32_t q, uint32_t lc, int cy, float fx, float fy, float fz, float lx, float ly, float lz) {
         const double off = (double)cy * shear * LyD;
         double vx = (double)fx - off, vy = fy, vz = fz;
         apply(lc, vx, vy, vz);
+        if (amc) add_w(lc, lx, ly, lz, vx, vy, vz);
         vx += off;
         OVX[p0 + q] = (float)vx; OVY[p0 + q] = (float)vy; OVZ[p0 + q] = (float)vz;
     };
 #pragma unroll
     for (int kk = 0; kk < KM; ++kk) {
         const uint32_t q = tid + kk * NT;
-        if (q < np) finish(q, rlc[kk], rcy[kk], rvx[kk], rvy[kk], rvz[kk]);
+        if (q < np)
+            finish(q, rlc[kk], rcy[kk], rvx[kk], rvy[kk], rvz[kk], AMC ? rlx[AMC ? kk : 0] : 0.f,
+                   AMC ? rly[AMC ? kk : 0] : 0.f, AMC ? rlz[AMC ? kk : 0] : 0.f);
     }
     for (uint32_t q = tid + CAP; q < np; q += NT) {
-        uint32_t lc; int cy; float fx, fy, fz;
-        gather(q, false, lc, cy, fx, fy, fz);
-        finish(q, lc, cy, fx, fy, fz);
+        uint32_t lc; int cy; float fx, fy, fz, lx = 0.f, ly = 0.f, lz = 0.f;
+        gather(q, false, lc, cy, fx, fy, fz, lx, ly, lz);
+        finish(q, lc, cy, fx, fy, fz, lx, ly, lz);
     }
     for (uint32_t j = tid; j < nmt; j += NT) {
         const uint32_t e = s_ml[j];
@@ -1251,6 +1351,11 @@ __global__ __launch_bounds__(256) void k_collide(DevState d, double h) {
         const uint32_t mi = rep.mono_base + (e & 0xFFFFu);
         double vx = d.mvel[3 * mi], vy = d.mvel[3 * mi + 1], vz = d.mvel[3 * mi + 2];
         apply(lc, vx, vy, vz);
+        if (amc) {
+            double lx, ly, lz;
+            mono_local(mi, lx, ly, lz);
+            add_w(lc, lx, ly, lz, vx, vy, vz);
+        }
         d.mvel[3 * mi] = vx; d.mvel[3 * mi + 1] = vy; d.mvel[3 * mi + 2] = vz;
     }
 }
@@ -1780,10 +1885,19 @@ hipError_t launch_perm(const DevState& d, hipStream_t st, int mode) {
 
 hipError_t launch_collide(const DevState& d, hipStream_t st, double h) {
     const dim3 g(d.n_tiles), b(256);
+    if (d.any_amc) {
+        switch (d.tile_cells) {
+        case 64: hipLaunchKernelGGL((k_collide<64, 768, true>), g, b, 0, st, d, h); break;
+        case 128: hipLaunchKernelGGL((k_collide<128, 1536, true>), g, b, 0, st, d, h); break;
+        case 256: hipLaunchKernelGGL((k_collide<256, 3072, true>), g, b, 0, st, d, h); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (d.tile_cells) {
-    case 64: hipLaunchKernelGGL((k_collide<64, 768>), g, b, 0, st, d, h); break;
-    case 128: hipLaunchKernelGGL((k_collide<128, 1536>), g, b, 0, st, d, h); break;
-    case 256: hipLaunchKernelGGL((k_collide<256, 3072>), g, b, 0, st, d, h); break;
+    case 64: hipLaunchKernelGGL((k_collide<64, 768, false>), g, b, 0, st, d, h); break;
+    case 128: hipLaunchKernelGGL((k_collide<128, 1536, false>), g, b, 0, st, d, h); break;
+    case 256: hipLaunchKernelGGL((k_collide<256, 3072, false>), g, b, 0, st, d, h); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

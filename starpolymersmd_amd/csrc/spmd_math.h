@@ -152,6 +152,24 @@ SPMD_HD double rcp_nr(double x) {
     return r;
 }
 
+/* x = A^-1 b for a symmetric positive-definite 3x3 A, as
+ * Matrix3d::llt().solve does (MPC.cpp:373): A = L L^T, L y = b, L^T x = y.
+ * A singular cell (all members on a line) gives non-finite x; the caller
+ * keeps only finite results. */
+SPMD_HD void llt_solve3(const double A[3][3], const double b[3], double x[3]) {
+    const double l00 = sqrt(A[0][0]);
+    const double l10 = A[1][0] / l00, l20 = A[2][0] / l00;
+    const double l11 = sqrt(A[1][1] - l10 * l10);
+    const double l21 = (A[2][1] - l20 * l10) / l11;
+    const double l22 = sqrt((A[2][2] - l20 * l20) - l21 * l21);
+    const double y0 = b[0] / l00;
+    const double y1 = (b[1] - l10 * y0) / l11;
+    const double y2 = ((b[2] - l20 * y0) - l21 * y1) / l22;
+    x[2] = y2 / l22;
+    x[1] = (y1 - l21 * x[2]) / l11;
+    x[0] = ((y0 - l10 * x[1]) - l20 * x[2]) / l00;
+}
+
 /* The MD kernels' forms of the force magnitudes: one reciprocal per pair,
  * no divergent division.  Same functions as pair_force / fene_force up to
  * rounding (the 48-24 core and B-B tail share 192 r^-2 r^-24 (r^-24 - 1/2),

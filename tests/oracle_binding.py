@@ -19,6 +19,7 @@ class OParams(C.Structure):
         ("L", C.c_int32 * 3), ("kT", C.c_double), ("n_c", C.c_int32), ("shear", C.c_double),
         ("step_update", C.c_uint32), ("alpha_deg", C.c_double), ("lambda_", C.c_double),
         ("md_dt", C.c_double), ("rng_mode", C.c_int32), ("seed", C.c_uint64), ("replica", C.c_uint32),
+        ("amc", C.c_int32),
     ]
 
 
@@ -93,13 +94,14 @@ def p(a, t=C.c_double):
 
 class Oracle:
     def __init__(self, L=(8, 8, 8), kT=1.0, n_c=10, shear=0.0, step_update=1, alpha_deg=130.0, lam=1.0,
-                 md_dt=0.01, rng_mode=RNG_PHILOX, seed=1, replica=0):
+                 md_dt=0.01, rng_mode=RNG_PHILOX, seed=1, replica=0, amc=False):
         prm = OParams()
         for i in range(3):
             prm.L[i] = int(L[i])
         prm.kT, prm.n_c, prm.shear, prm.step_update = kT, n_c, shear, step_update
         prm.alpha_deg, prm.lambda_, prm.md_dt = alpha_deg, lam, md_dt
         prm.rng_mode, prm.seed, prm.replica = rng_mode, seed, replica
+        prm.amc = int(amc)
         self.L = tuple(int(l) + (int(l) % 2) for l in L)
         self.ncells = self.L[0] * self.L[1] * self.L[2]
         self.h = lib().orc_create(C.byref(prm))

@@ -329,3 +329,49 @@ def test_md_time_reversible():
     p2, v2, _ = o.get_monomers()
     assert np.abs(p2 - p0).max() < 1e-8
     assert np.abs(v2 + v0).max() < 1e-7
+
+
+# ------------------------------------------- angular-momentum-conserving SRD
+
+@pytest.mark.parametrize("amc", [True, False])
+def test_amc_collision_conserves_cell_angular_momentum(amc):
+    """MPC::collide with angular_mom (MPC.cpp:258-269, 344-374): the MBS
+    thermostat scales a cell's relative velocities by s = sqrt(Gamma/E_rel),
+    after which the rotation of every cell of more than two members must
+    keep the cell's momentum and its angular momentum about the centre of
+    mass, s L_0; without the flag the rotation changes it."""
+    L, n_c, h = 4, 5, 0.1
+    rng = np.random.default_rng(11)
+    n = n_c * L ** 3
+    pos = (rng.random((n, 3)) - 0.5) * L
+    vel = rng.normal(size=(n, 3))
+    o = ob.Oracle(L=(L, L, L), n_c=n_c, shear=0.0, rng_mode=ob.RNG_INJECTED, amc=amc)
+    o.set_fluid(pos, vel)
+    shift = rng.random(3) - 0.5
+    gamma = rng.gamma(6.0, 1.0, L ** 3)
+    o.inject(shift, rng.random(2 * L ** 3), gamma)
+    assert o.step(0, h) == 0
+    cells, _ = o.cell_index()
+    p2, v2 = o.get_fluid()
+    xs = p2 + shift                       # positions during the collision
+    xs -= L * np.round(xs / L)
+    dL_max, dp_max, n_checked = 0.0, 0.0, 0
+    for c in np.unique(cells):
+        m = cells == c
+        if m.sum() <= 2:
+            continue
+        n_checked += 1
+        cmv = vel[m].mean(0)
+        r = xs[m] - xs[m].mean(0)
+        u0 = vel[m] - cmv
+        s_mbs = np.sqrt(gamma[c] / (0.5 * (u0 ** 2).sum()))
+        dp_max = max(dp_max, np.abs(vel[m].sum(0) - v2[m].sum(0)).max())
+        l_want = s_mbs * np.cross(r, u0).sum(0)
+        l_got = np.cross(r, v2[m] - cmv).sum(0)
+        dL_max = max(dL_max, np.abs(l_want - l_got).max())
+    assert n_checked > 40
+    assert dp_max < 1e-10
+    if amc:
+        assert dL_max < 1e-9
+    else:
+        assert dL_max > 1e-2
