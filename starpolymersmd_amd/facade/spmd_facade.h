@@ -29,10 +29,15 @@
 #define SPMD_FACADE_H
 
 #include <array>
+#include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <map>
 #include <memory>
+#include <ostream>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/spmd.h"
@@ -52,6 +57,9 @@ inline void check(int rc, const spmd_sys* h) {
 }
 
 using Vector3d = std::array<double, 3>;
+/* Eigen's Matrix3d is column-major; data()[i] of the reference's printing
+ * (star-polymers.cpp:477-479) is element i of this array */
+using Matrix3d = std::array<double, 9>;
 
 /* Particle.h:17-82 without the Eigen types */
 struct MDParticle {
@@ -136,6 +144,59 @@ public:
         check(spmd_epot(handle(), 0, &e), h_.get());
         return e;
     }
+
+    /* ---- analysis of Box.cpp:296-507, 605-660 on the device ---- */
+    Vector3d COM_Pos{}, COM_Vel{};
+    /* Box::center_of_mass_reference_frame Box.cpp:81-102: COM_Pos / COM_Vel
+     * (the device observers compute the frame themselves) */
+    void center_of_mass_reference_frame() {
+        sync_molecules();
+        COM_Pos = COM_Vel = Vector3d{};
+        if (Molecules.empty()) return;
+        for (auto& m : Molecules[0].Monomers)
+            for (int c = 0; c < 3; ++c) { COM_Pos[c] += m.Position[c]; COM_Vel[c] += m.Velocity[c]; }
+        for (int c = 0; c < 3; ++c) { COM_Pos[c] /= Molecules[0].NumberOfMonomers; COM_Vel[c] /= Molecules[0].NumberOfMonomers; }
+    }
+    /* the 18 values of a statistics row (spmd_statistics) */
+    std::array<double, 18> statistics() const {
+        std::array<double, 18> row{};
+        check(spmd_statistics(handle(), 0, row.data()), h_.get());
+        return row;
+    }
+    /* Box::calculate_patches_new Box.cpp:434-507 */
+    std::vector<double> calculate_patches_new() const {
+        std::vector<double> r(2, 0.0);
+        check(spmd_patches(handle(), 0, &r[0], &r[1]), h_.get());
+        return r;
+    }
+    /* Box::calculate_gyration_tensor Box.cpp:330-352: (R_gyr, G) */
+    std::tuple<double, Matrix3d> calculate_gyration_tensor() const {
+        Matrix3d g{};
+        check(spmd_gyration_tensor(handle(), 0, g.data()), h_.get());
+        return std::make_tuple(std::sqrt((g[0] + g[4]) + g[8]), g);
+    }
+    /* Box::calculate_rotation_frequency Box.cpp:354-379 */
+    Vector3d calculate_rotation_frequency() const {
+        Vector3d w{};
+        check(spmd_rotation_frequency(handle(), 0, w.data()), h_.get());
+        return w;
+    }
+    /* Box::print_Epot Box.cpp:605-613 */
+    std::ostream& print_Epot(std::ostream& os) const { return os << statistics()[0] << " "; }
+    /* Box::print_Ekin_and_Temperature Box.cpp:635-646 */
+    std::ostream& print_Ekin_and_Temperature(std::ostream& os) const {
+        const auto r = statistics();
+        return os << r[1] << " " << r[2] << " ";
+    }
+    /* Box::print_PDB_with_velocity Box.cpp:563-602, to a file (appended) */
+    void print_PDB_with_velocity(const std::string& path, int step, bool append = true) {
+        check(spmd_write_pdb(handle(), 0, path.c_str(), step, append ? 1 : 0), h_.get());
+    }
+    /* Box::add_star(filename, A, B, Arms, Mass, set_zero) Box.cpp:48-54 */
+    void add_star(const std::string& filename, unsigned A, unsigned B, unsigned Arms, double Mass, bool set_zero) {
+        check(spmd_add_star_pdb(handle(), 0, filename.c_str(), A, B, Arms, Mass, set_zero ? 1 : 0), h_.get());
+        sync_molecules();
+    }
 };
 
 /* Hydrodynamics.h:7-22 interface subset used by the drivers */
@@ -164,6 +225,21 @@ public:
     }
     /* MPC::initialize() MPC.cpp:34-55 */
     void initialize() { check(spmd_fluid_initialize(SimBox.handle()), SimBox.handle()); }
+    /* MPC::initialize(filename, Box_old) MPC.cpp:57-113 */
+    void initialize(const std::string& filename, std::array<int, 3> Box_old) {
+        const int32_t ob[3] = {Box_old[0], Box_old[1], Box_old[2]};
+        check(spmd_read_fluid(SimBox.handle(), 0, filename.c_str(), ob, nullptr), SimBox.handle());
+    }
+    /* MPC::print_fluid_complete MPC.cpp:570-579, to a file */
+    void print_fluid_complete(const std::string& path) {
+        check(spmd_write_fluid(SimBox.handle(), 0, path.c_str()), SimBox.handle());
+    }
+    /* MPC::operator()(functor) MPC.h:110-116 for the device observers */
+    template <class Observer>
+    Observer& operator()(Observer& f) {
+        f.sample(SimBox.handle());
+        return f;
+    }
     /* MPC::step(t, dt) MPC.cpp:116-142 */
     void step(const long int& t, const double& dt) { check(spmd_mpc_step(SimBox.handle(), t, dt), SimBox.handle()); }
     /* MPC::calculateCurrentTemperature MPC.cpp:405-409 */
@@ -196,6 +272,56 @@ public:
     void run(long t0, long n, double dt) { check(spmd_run(SimBox.handle(), t0, n, dt), SimBox.handle()); }
     void sync() { check(spmd_sync(SimBox.handle()), SimBox.handle()); }
 };
+
+/* VelocityX (Velocity_x.h:22-148): mean solvent velocity relative to the
+ * star's centre of mass, binned by x and by y; applied as
+ * hydrodynamics(velocity_average_x) like the reference's functor. */
+class VelocityX {
+    double width;
+    std::map<double, std::array<double, 3>> maps[2];  /* key -> sum vx, sum vy, count */
+
+public:
+    VelocityX(Box&, MPC&, double a_width = 0.5) : width(a_width) {}
+    void sample(spmd_sys* h) {
+        spmd_params p{};
+        check(spmd_get_params(h, 0, &p), h);
+        uint32_t nmax = 0;
+        for (int a = 0; a < 2; ++a) {
+            const double half = 0.5 * (p.L[a] + (p.L[a] & 1));
+            nmax = std::max<uint32_t>(nmax, (uint32_t)(std::floor(half / width) - std::floor(-half / width) + 1));
+        }
+        int32_t b0[2];
+        uint32_t nb[2];
+        std::vector<double> sums(2ull * nmax * 3);
+        check(spmd_velocity_profile(h, 0, width, nmax, b0, nb, sums.data()), h);
+        for (int a = 0; a < 2; ++a)
+            for (uint32_t b = 0; b < nb[a]; ++b) {
+                const double* q = &sums[3ull * (a * nmax + b)];
+                if (q[2] == 0) continue;
+                auto& e = maps[a][(b0[a] + (int)b) * width];
+                for (int c = 0; c < 3; ++c) e[c] += q[c];
+            }
+    }
+    /* VelocityX::print_result Velocity_x.h:98-144 */
+    std::ostream& print_result(std::ostream& os) const {
+        char buf[128];
+        for (int a = 0; a < 2; ++a) {
+            for (auto& kv : maps[a]) {
+                std::snprintf(buf, sizeof buf, "%.8e %.8e %.8e %.8e\n", kv.first, kv.second[0] / kv.second[2],
+                              kv.second[1] / kv.second[2], kv.second[2]);
+                os << buf;
+            }
+            if (a == 0) os << '\n';
+        }
+        return os;
+    }
+};
+
+/* statistics*.dat header of star-polymers.cpp:307 */
+inline const char* statistics_header() {
+    return "Step   Epot     Ekin     Temp   n_p s_p  R_gyr   G_xx      G_xy      G_xz       G_yx       G_yy      "
+           "G_yz      G_zx     Gzy     Gzz      wx      wy     wz\n";
+}
 
 /* Thermostat_None (Thermostat_None.h:14, Thermostat_None.cpp:5-38) */
 class Thermostat_None {

@@ -33,23 +33,33 @@ def test_driver_built_and_linked():
 
 
 @pytest.mark.gpu
-def test_driver_matches_python_binding():
+def test_driver_matches_python_binding(tmp_path):
+    """Same trajectory, same statistics.dat rows, VelocityX profile and
+    restart files as the Python binding run with the same calls."""
+    from starpolymersmd_amd import observables as ob
+
     L, A, B, arms, shear, steps, out, dt, nc = 16, 3, 3, 6, 0.02, 40, 20, 0.01, 10
     r = subprocess.run([DRIVER, str(L), str(A), str(B), str(arms), str(shear), str(steps), str(out), str(dt),
-                        str(nc), "1"], capture_output=True, text=True, timeout=120)
+                        str(nc), "1", str(tmp_path)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     rows = [l.split() for l in r.stdout.splitlines() if "T=" in l]
     assert [int(x[0]) for x in rows] == [20, 40]
+    stat_lines = (tmp_path / "statistics.dat").read_text().split("\n")
+    assert stat_lines[0] + "\n" == ob.STAT_HEADER
 
     g = MPCSystem(make_params(L=(L, L, L), kT=1.0, n_c=nc, shear=shear, step_update=10, md_dt=dt, seed=1, lam=1.0))
     g.add_star(A, B, arms, 10.0, coil=True)
     g.md_init()
     g.initialize()
+    prof = ob.VelocityProfile(0.5)
     for n in range(steps + 1):
-        g.propagate(calc_epot=(n % out == 0))
-        g.step(n, 10 * dt)
-        if n % out == 0 and n > 0:
-            row = rows[n // out - 1]
+        out_step = n % out == 0 and n > 0
+        g.propagate(calc_epot=out_step)
+        if out_step:
+            k = n // out
+            assert stat_lines[k] + "\n" == ob.statistics_row(n, g.statistics())
+            prof.sample(g)
+            row = rows[k - 1]
             T = float(row[1].split("=")[1])
             E = float(row[2].split("=")[1])
             com = np.array([float(v) for v in row[3].split("=")[1].strip("()").split(",")])
@@ -57,3 +67,15 @@ def test_driver_matches_python_binding():
             assert abs(g.temperature() - T) <= 1e-6
             assert abs(g.epot() - E) <= 1e-6 * max(1.0, abs(E))
             assert np.abs(pos.mean(axis=0) - com).max() <= 1e-4
+        g.step(n, 10 * dt)
+    g.sync()
+    import io
+    buf = io.StringIO()
+    prof.print_result(buf)
+    assert (tmp_path / "fluid_profile.dat").read_text() == buf.getvalue()
+    g.write_fluid(tmp_path / "py_fluid.dat")
+    # the same particles; their order inside a cell follows the atomics of the sort
+    assert sorted((tmp_path / "end_fluid.dat").read_text().split("\n")) == \
+        sorted((tmp_path / "py_fluid.dat").read_text().split("\n"))
+    g.write_pdb(tmp_path / "py_config.pdb", steps)
+    assert (tmp_path / "end_config.pdb").read_text() == (tmp_path / "py_config.pdb").read_text()
