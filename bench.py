@@ -92,6 +92,81 @@ def cpu_baseline(w, intervals, gpu_star):
                       f"(10 MD substeps each), 1 thread, {el:.1f} s"}
 
 
+def batch_workload(n_rep=64, seed=2024):
+    """64 replicas of uneven size: L in {20, 24, 32, 40}, f in 5..30 arms,
+    N_A = N_B in 3..15 beads, shear rates log-spaced 1e-3 .. 1e-1, rho 10
+    (seeded draws).  N_A + N_B is drawn again until the arm fits half the
+    box less two cells: the Verlet grid spans the box around the star's
+    centre (Box.cpp:129-142), so a longer arm stops the run there too."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    shears = np.logspace(-3, -1, n_rep)
+    reps = []
+    for r in range(n_rep):
+        L = int(rng.choice([20, 24, 32, 40]))
+        f = int(rng.integers(5, 31))
+        nab = int(rng.integers(3, 16))
+        while 2 * nab > L // 2 - 2:
+            nab = int(rng.integers(3, 16))
+        reps.append(dict(L=L, arms=f, A=nab, B=nab, shear=float(shears[r])))
+    return reps
+
+
+def batch_bench(intervals=100, seq_intervals=10):
+    """Batched replicas: all 64 advance through one graph launch per
+    collision interval with autotuned launch shapes, against the same
+    replicas advanced one handle at a time on the same GPU."""
+    from starpolymersmd_amd import MPCSystem, make_params
+    from starpolymersmd_amd.autotune import autotune
+
+    reps = batch_workload()
+    S, h = 10, 0.1
+
+    def params(c):
+        return make_params(L=(c["L"],) * 3, n_c=10, shear=c["shear"], step_update=S, md_dt=0.01, seed=7, lam=1.0)
+
+    g = MPCSystem([params(c) for c in reps])
+    for r, c in enumerate(reps):
+        g.add_star(c["A"], c["B"], c["arms"], 10.0, r=r, coil=True)
+    g.md_init()
+    g.initialize()
+    g.run(0, 1, h)
+    tune = autotune(g, 1, h, cache=None)
+    t = tune["t"]
+    g.run(t, 10 * S, h)
+    t += 10 * S
+    g.sync()
+    n_f = sum(g.num_fluid(r) for r in range(len(reps)))
+    t0 = time.perf_counter()
+    g.run(t, intervals * S, h)
+    g.sync()
+    el = time.perf_counter() - t0
+    batched = n_f * intervals / el
+    g.close()
+    # one handle per replica, the same intervals each
+    seq_el = 0.0
+    for c in reps:
+        q = MPCSystem(params(c))
+        q.add_star(c["A"], c["B"], c["arms"], 10.0, coil=True)
+        q.md_init()
+        q.initialize()
+        q.run(0, 1 + 5 * S, h)
+        q.sync()
+        t1 = time.perf_counter()
+        q.run(1 + 5 * S, seq_intervals * S, h)
+        q.sync()
+        seq_el += (time.perf_counter() - t1) * intervals / seq_intervals
+        q.close()
+    return {"workload": "64 replicas, L in {20,24,32,40}, f 5..30, N_A=N_B 3..15, shear 1e-3..1e-1 log-spaced, "
+                        "rho 10, coil stars (seeded draws)",
+            "solvent_particles": n_f, "monomers": sum(c["arms"] * 2 * c["A"] + 1 for c in reps),
+            "tuning": {"ppt": tune["ppt"], "tile_cells": tune["tile_cells"]},
+            "ms_per_interval": el * 1e3 / intervals, "particle_updates_per_s": batched,
+            "one_handle_at_a_time_particle_updates_per_s": n_f * intervals / seq_el,
+            "speedup": seq_el / el}
+
+
 def closure_bench(replicas=256, explicit_replicas=16, check_intervals=200):
     """Learned closure at batch (256 stars, f=30 arms of 20 A + 20 B beads,
     1,201 monomers, shear rates uniform in [5e-3, 5e-2]) against explicit
@@ -179,6 +254,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (ppt 4, 128-cell tiles)")
     ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure throughput section")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-replicas section")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run N MiB device-to-device copies (counter calibration)")
     a = ap.parse_args()
@@ -254,8 +330,10 @@ def main():
         "md_us_per_interval": prof["md"] * 1e3,
         "temperature": T,
     }
+    g.close()
+    if not a.no_batch:
+        out["batch"] = batch_bench()
     if not a.no_closure:
-        g.close()
         out["closure"] = closure_bench()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_intervals, star0)

@@ -105,3 +105,35 @@ def test_autotune_picks_and_applies(tmp_path):
     assert again["cached"] and (again["ppt"], again["tile_cells"]) == (res["ppt"], res["tile_cells"])
     T = g.temperature()
     assert 0.5 < T < 1.5
+
+
+def test_uneven_64_replica_batch_runs_in_one_graph():
+    """The batched workload of bench.py (64 replicas, L in {20,24,32,40},
+    f 5..30, N_A=N_B 3..15, shear 1e-3..1e-1): 60 graph-replayed intervals
+    run clean, every replica's solvent keeps kT and its star its monomers'
+    count; replica 0 equals the same system run alone."""
+    import bench
+
+    reps = bench.batch_workload()
+    ps = [make_params(L=(c["L"],) * 3, n_c=10, shear=c["shear"], step_update=10, md_dt=0.01, seed=7, lam=1.0)
+          for c in reps]
+    g = MPCSystem(ps)
+    for r, c in enumerate(reps):
+        g.add_star(c["A"], c["B"], c["arms"], 10.0, r=r, coil=True)
+    g.md_init()
+    g.initialize()
+    g.run(0, 601, 0.1)
+    g.sync()
+    for r, c in enumerate(reps):
+        assert abs(g.temperature(r) - 1.0) < 0.05, r
+        assert g.num_monomers(r) == c["arms"] * 2 * c["A"] + 1
+    # replica 0 of the batch and a lone system share their Philox streams
+    g0 = MPCSystem(ps[0])
+    g0.add_star(reps[0]["A"], reps[0]["B"], reps[0]["arms"], 10.0, coil=True)
+    g0.md_init()
+    g0.initialize()
+    g0.run(0, 601, 0.1)
+    g0.sync()
+    a = g.get_monomers(0)
+    b = g0.get_monomers()
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
