@@ -255,6 +255,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (ppt 4, 128-cell tiles)")
     ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure throughput section")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-replicas section")
+    ap.add_argument("--closure-only", action="store_true",
+                    help="only run the learned-closure section (for profiling its kernels)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run N MiB device-to-device copies (counter calibration)")
     a = ap.parse_args()
@@ -273,6 +275,10 @@ def main():
               "slab-decomposed along z with per-face halo sums; not built). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+
+    if a.closure_only:
+        print(json.dumps({"closure": closure_bench()}), flush=True)
+        return
 
     w = WORKLOAD
     S, h = w["step_update"], w["mpc_dt"]

@@ -255,8 +255,12 @@ int spmd_mlp_set_params(spmd_mlp* m, const float* host);
 /* feature standardisation (x - mu_x) / sd_x (24 each) and target scale sd_y (3) */
 int spmd_mlp_set_norm(spmd_mlp* m, const float* mu_x, const float* sd_x, const float* sd_y);
 int spmd_mlp_get_norm(const spmd_mlp* m, float* mu_x, float* sd_x, float* sd_y);
-/* out_dev[M][6] = (mean[3], log-variance[3]) in units of sd_y for x_dev[M][24] */
+/* out_dev[M][6] = (mean[3], log-variance[3]) in units of sd_y for x_dev[M][24]
+ * (one fused kernel: weights in LDS, activations in registers) */
 int spmd_mlp_forward(spmd_mlp* m, const float* x_dev, float* out_dev, uint64_t M);
+/* the same outputs through the training path's per-layer products (the
+ * activations kept in the workspace for a backward pass) */
+int spmd_mlp_forward_unfused(spmd_mlp* m, const float* x_dev, float* out_dev, uint64_t M);
 /* gradient of the mean Gaussian negative log-likelihood of y_dev[M][3]
  * (FP32, parameter order as above) and the loss */
 int spmd_mlp_gradient(spmd_mlp* m, const float* x_dev, const float* y_dev, uint64_t M, float* grad_host,
@@ -276,7 +280,8 @@ int spmd_dev_download(void* dst, const void* src, uint64_t bytes);
 
 /* The 24 closure features of every monomer of every replica, global
  * monomer order (replica r from its first monomer; see spmd_num_monomers),
- * of the current state (pending MD substeps are run first). */
+ * of the current state (pending MD substeps are run first).  feat_dev is
+ * 16-byte aligned (rows are written as 16-byte stores). */
 int spmd_closure_features(spmd_sys* s, float* feat_dev);
 /* n_intervals closure intervals for all replicas: step_update velocity-
  * Verlet substeps, check_bounds, then the network's kick on every monomer.

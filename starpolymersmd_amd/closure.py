@@ -133,12 +133,14 @@ class ClosureNet:
                                               ptr(sd_y, C.c_float)))
         self.norm = (mu_x, sd_x, sd_y)
 
-    def forward(self, x: np.ndarray) -> np.ndarray:
-        """(M, 24) features -> (M, 6): mean and log-variance in units of sd_y."""
+    def forward(self, x: np.ndarray, fused: bool = True) -> np.ndarray:
+        """(M, 24) features -> (M, 6): mean and log-variance in units of sd_y
+        (fused=False: through the training path's per-layer products)."""
         x = np.ascontiguousarray(x, np.float32)
         xb = DeviceBuffer.from_numpy(x)
         ob = DeviceBuffer(x.shape[0] * 6 * 4)
-        self._chk(self._lib.spmd_mlp_forward(self.h, C.c_void_p(xb.ptr), C.c_void_p(ob.ptr), x.shape[0]))
+        fn = self._lib.spmd_mlp_forward if fused else self._lib.spmd_mlp_forward_unfused
+        self._chk(fn(self.h, C.c_void_p(xb.ptr), C.c_void_p(ob.ptr), x.shape[0]))
         return ob.download((x.shape[0], 6))
 
     def gradient(self, x: np.ndarray, y: np.ndarray):

@@ -1017,6 +1017,7 @@ static int closure_ws(spmd_sys* s) {
 
 int spmd_closure_features(spmd_sys* s, float* feat_dev) {
     if (!s || !feat_dev) return SPMD_ERR_ARG;
+    if (reinterpret_cast<uintptr_t>(feat_dev) & 15u) return fail(s, SPMD_ERR_ARG, "feature buffer must be 16-byte aligned");
     if (int rc = flush_md(s)) return rc;
     if (int rc = ensure_monomers(s)) return rc;
     LAUNCH(s, 0, s->stream, launch_closure_features(s->d, s->stream, feat_dev, nullptr, nullptr));

@@ -270,10 +270,11 @@ __device__ double md_epot_flat(const MdLds& S, const uint32_t* pl, uint32_t np, 
 
 /* Exclusive scan of a[0..n) in place, total in a[n] and returned; each
  * thread scans a contiguous chunk, deterministic. */
-__device__ uint32_t md_block_scan(uint32_t* a, int n, double* scratch_d) {
+template <int NT>
+__device__ uint32_t block_scan_u32(uint32_t* a, int n, double* scratch_d) {
     uint32_t* wsum = reinterpret_cast<uint32_t*>(scratch_d);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int per = (n + kMdThreads - 1) / kMdThreads;
+    const int per = (n + NT - 1) / NT;
     const int i0 = tid * per;
     uint32_t t = 0;
     for (int q = 0; q < per; ++q) t += i0 + q < n ? a[i0 + q] : 0u;
@@ -288,7 +289,7 @@ __device__ uint32_t md_block_scan(uint32_t* a, int n, double* scratch_d) {
     __syncthreads();
     uint32_t pre = 0, tot = 0;
 #pragma unroll
-    for (int q = 0; q < kMdThreads / 64; ++q) {
+    for (int q = 0; q < NT / 64; ++q) {
         const uint32_t u = wsum[q];
         pre += q < w ? u : 0u;
         tot += u;
@@ -299,6 +300,9 @@ __device__ uint32_t md_block_scan(uint32_t* a, int n, double* scratch_d) {
     if (tid == 0) a[n] = tot;
     __syncthreads();
     return tot;
+}
+__device__ __forceinline__ uint32_t md_block_scan(uint32_t* a, int n, double* scratch_d) {
+    return block_scan_u32<kMdThreads>(a, n, scratch_d);
 }
 
 /* exclusive block scan of one value per thread; *total gets the sum */
@@ -1487,34 +1491,66 @@ __global__ __launch_bounds__(256) void k_temperature(DevState d, int replica) {
  *   19-21 mean velocity of the monomers within r < 1.5 minus v (zero if none)
  *   22   1/m           23 bonds of the monomer
  * With fmin/fmax, a monomer with any feature outside its trained range
- * widened by 25% counts in RepCtl::closure_oor. */
-__global__ __launch_bounds__(512) void k_closure_features(DevState d, float* feat, const float* fmin,
-                                                           const float* fmax) {
+ * widened by 25% counts in RepCtl::closure_oor.
+ *
+ * Neighbours (15, 19-21) come from cells of side kNbSide (1.5 and a
+ * margin that keeps every pair the FP32 distance test accepts in adjacent
+ * cells) around the star's centre, folded onto a periodic 16^3 grid of
+ * buckets: a counting sort of the monomers by bucket into LDS copies of
+ * their positions and velocities, then, per monomer, 9 slot ranges (the
+ * three x-neighbour buckets of a row are adjacent unless the row wraps).
+ * Cells 16 apart share a bucket; their monomers are >= 22.5 apart and fail
+ * the distance test, and the 27 cells around a monomer never share one, so
+ * nothing is counted twice.  The velocity sums are 64-bit fixed point, so
+ * the order inside a bucket (set by LDS atomics) does not change the
+ * result.  Thread t owns monomers t + kClThreads k: their cells and sorted
+ * slots stay in its registers between the phases. */
+constexpr int kNbHash = 4096;
+constexpr float kNbSide = 1.5f * 1.001f;
+constexpr int kClThreads = 1024;
+constexpr int kClPer = kMaxMono / kClThreads;
+constexpr size_t kClLds = (size_t)kMaxMono * (16 + 16 + 12) + (size_t)(kNbHash + 1) * 4;
+
+__device__ __forceinline__ uint32_t nb_bucket(int cx, int cy, int cz) {
+    return (uint32_t)(cx & 15) | ((uint32_t)(cy & 15) << 4) | ((uint32_t)(cz & 15) << 8);
+}
+
+__global__ __launch_bounds__(kClThreads) void k_closure_features(DevState d, float* feat, const float* fmin,
+                                                                  const float* fmax) {
     extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int NT = 512;
+    constexpr int NT = kClThreads;
     const int r = blockIdx.x, tid = threadIdx.x;
     const RepDev& rep = d.reps[r];
     const int N = (int)rep.n_mono;
     const uint32_t mb = rep.mono_base;
-    float4* sp = reinterpret_cast<float4*>(smem);                 /* N: position, B flag */
-    float4* sv = sp + kMaxMono;                                   /* N: velocity */
-    uint32_t* prv = reinterpret_cast<uint32_t*>(sv + kMaxMono);  /* N */
-    uint32_t* nxt = prv + kMaxMono;                               /* N */
-    uint32_t* deg = nxt + kMaxMono;                               /* N */
-    __shared__ double sred[8];
+    float4* spos = reinterpret_cast<float4*>(smem);              /* N, bucket order: position, key */
+    float4* svel = spos + kMaxMono;                              /* N, bucket order: velocity */
+    uint32_t* prv = reinterpret_cast<uint32_t*>(svel + kMaxMono);  /* N */
+    uint32_t* nxt = prv + kMaxMono;                              /* N */
+    uint32_t* deg = nxt + kMaxMono;                              /* N */
+    uint32_t* hb = deg + kMaxMono;                               /* kNbHash + 1: bucket ends */
+    __shared__ double sred[NT / 64];
+    float px[kClPer], py[kClPer], pz[kClPer];
+    int cx[kClPer], cy[kClPer], cz[kClPer];
+    uint32_t slot[kClPer];
     double cs[6] = {0, 0, 0, 0, 0, 0};
-    for (int i = tid; i < N; i += NT) {
-        const size_t g = 3 * (size_t)(mb + i);
-        const double x = d.mpos[g], y = d.mpos[g + 1], z = d.mpos[g + 2];
-        cs[0] += x; cs[1] += y; cs[2] += z;
-        cs[3] += d.mvel[g]; cs[4] += d.mvel[g + 1]; cs[5] += d.mvel[g + 2];
-        sp[i] = make_float4((float)x, (float)y, (float)z, d.mbtype[mb + i] ? 1.0f : 0.0f);
-        sv[i] = make_float4((float)d.mvel[g], (float)d.mvel[g + 1], (float)d.mvel[g + 2], 0.0f);
-        prv[i] = (uint32_t)i; nxt[i] = (uint32_t)i; deg[i] = 0u;
+    for (int c = tid; c <= kNbHash; c += NT) hb[c] = 0u;
+#pragma unroll
+    for (int k = 0; k < kClPer; ++k) {
+        const int i = tid + k * NT;
+        if (i < N) {
+            const size_t g = 3 * (size_t)(mb + i);
+            const double x = d.mpos[g], y = d.mpos[g + 1], z = d.mpos[g + 2];
+            cs[0] += x; cs[1] += y; cs[2] += z;
+            cs[3] += d.mvel[g]; cs[4] += d.mvel[g + 1]; cs[5] += d.mvel[g + 2];
+            px[k] = (float)x; py[k] = (float)y; pz[k] = (float)z;
+            prv[i] = (uint32_t)i; nxt[i] = (uint32_t)i; deg[i] = 0u;
+        }
     }
     double com[6];
     for (int c = 0; c < 6; ++c) com[c] = block_sum<NT>(cs[c], sred) / (double)(N > 0 ? N : 1);
     __syncthreads();
+    const float ox = (float)com[0], oy = (float)com[1], oz = (float)com[2], inv = 1.0f / kNbSide;
     for (uint32_t q = tid; q < rep.n_bond; q += NT) {
         const uint32_t e = d.bonds[rep.bond_base + q];
         const uint32_t i = e & 0xFFF, j = (e >> 12) & 0xFFF;  /* i < j: i nearer the core */
@@ -1522,50 +1558,95 @@ __global__ __launch_bounds__(512) void k_closure_features(DevState d, float* fea
         prv[j] = i;
         nxt[i] = j;
     }
+#pragma unroll
+    for (int k = 0; k < kClPer; ++k)
+        if (tid + k * NT < N) {
+            cx[k] = (int)floorf((px[k] - ox) * inv);
+            cy[k] = (int)floorf((py[k] - oy) * inv);
+            cz[k] = (int)floorf((pz[k] - oz) * inv);
+            atomicAdd(&hb[nb_bucket(cx[k], cy[k], cz[k])], 1u);
+        }
     __syncthreads();
+    block_scan_u32<NT>(hb, kNbHash, sred);
+#pragma unroll
+    for (int k = 0; k < kClPer; ++k) {
+        const int i = tid + k * NT;
+        if (i < N) {
+            const size_t g = 3 * (size_t)(mb + i);
+            slot[k] = atomicAdd(&hb[nb_bucket(cx[k], cy[k], cz[k])], 1u);
+            spos[slot[k]] = make_float4(px[k], py[k], pz[k], 0.0f);
+            svel[slot[k]] = make_float4((float)d.mvel[g], (float)d.mvel[g + 1], (float)d.mvel[g + 2], 0.0f);
+        }
+    }
+    __syncthreads();  /* hb[b] now ends bucket b */
     const double shear = rep.box.shear;
     uint32_t oor = 0;
-    for (int i = tid; i < N; i += NT) {
+    static_assert(kClPer == 2, "two monomers per thread");
+    const uint32_t slot0 = slot[0], slot1 = slot[1];
+#pragma unroll 1
+    for (int k = 0; k < kClPer; ++k) {
+        const int i = tid + k * NT;
+        if (i >= N) continue;
+        const uint32_t mys = k ? slot1 : slot0;
+        const float4 me = spos[mys];  /* own position and cell, as sorted */
+        const float mx = me.x, my = me.y, mz = me.z;
+        const int qx = (int)floorf((mx - ox) * inv), qy = (int)floorf((my - oy) * inv),
+                  qz = (int)floorf((mz - oz) * inv);
         const size_t g = 3 * (size_t)(mb + i);
+        int cnt = 0;
+        unsigned long long nvx = 0ull, nvy = 0ull, nvz = 0ull;
+        const uint32_t x0 = (uint32_t)(qx - 1) & 15u;
+#pragma unroll 1
+        for (int c = 0; c < 18; ++c) {
+            /* row (dy, dz) = (c/2 % 3 - 1, c/6 - 1); part c%2: the run from
+             * x0, then (if the row wraps) the run from bucket 0 */
+            const uint32_t row = ((uint32_t)(qy + (c >> 1) % 3 - 1) & 15u) << 4 |
+                                 ((uint32_t)(qz + c / 6 - 1) & 15u) << 8;
+            uint32_t b0, b1;
+            if (!(c & 1)) { b0 = row + x0; b1 = row + (x0 + 2 < 16u ? x0 + 2 : 15u); }
+            else if (x0 + 2 >= 16u) { b0 = row; b1 = row + x0 + 2 - 16u; }
+            else continue;
+            const uint32_t s1 = hb[b1];
+            for (uint32_t t = b0 ? hb[b0 - 1] : 0u; t < s1; ++t) {
+                const float4 o = spos[t];
+                const float dx = o.x - mx, dy = o.y - my, dz = o.z - mz;
+                if (t != mys && __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < 2.25f) {
+                    const float4 ov = svel[t];
+                    ++cnt;
+                    nvx += to_fix(ov.x); nvy += to_fix(ov.y); nvz += to_fix(ov.z);
+                }
+            }
+        }
         const double x = d.mpos[g], y = d.mpos[g + 1], z = d.mpos[g + 2];
         const double vx = d.mvel[g], vy = d.mvel[g + 1], vz = d.mvel[g + 2];
         const bool core = deg[i] > 2u;
-        const float4 me = sp[i];
-        int cnt = 0;
-        float nvx = 0.0f, nvy = 0.0f, nvz = 0.0f;
-        for (int j = 0; j < N; ++j) {
-            const float4 o = sp[j];
-            const float dx = o.x - me.x, dy = o.y - me.y, dz = o.z - me.z;
-            if (j != i && __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < 2.25f) {
-                const float4 ov = sv[j];
-                ++cnt;
-                nvx += ov.x; nvy += ov.y; nvz += ov.z;
-            }
-        }
         float f[kClosureFeatures];
         f[0] = (float)(x - com[0]); f[1] = (float)(y - com[1]); f[2] = (float)(z - com[2]);
         f[3] = (float)(vx - com[3]); f[4] = (float)(vy - com[4]); f[5] = (float)(vz - com[5]);
         const double ux = vx - shear * y;
         f[6] = (float)ux; f[7] = (float)vy; f[8] = (float)vz;
         const int ip = core ? i : (int)prv[i], in = core ? i : (int)nxt[i];
-        f[9] = sp[ip].x - me.x; f[10] = sp[ip].y - me.y; f[11] = sp[ip].z - me.z;
-        f[12] = sp[in].x - me.x; f[13] = sp[in].y - me.y; f[14] = sp[in].z - me.z;
+        const size_t gp = 3 * (size_t)(mb + ip), gn = 3 * (size_t)(mb + in);
+        f[9] = (float)d.mpos[gp] - mx; f[10] = (float)d.mpos[gp + 1] - my; f[11] = (float)d.mpos[gp + 2] - mz;
+        f[12] = (float)d.mpos[gn] - mx; f[13] = (float)d.mpos[gn + 1] - my; f[14] = (float)d.mpos[gn + 2] - mz;
         f[15] = (float)cnt;
-        f[16] = me.w;
+        f[16] = d.mbtype[mb + i] ? 1.0f : 0.0f;
         f[17] = core ? 1.0f : 0.0f;
         f[18] = (float)shear;
-        const float4 mv = sv[i];
-        const float ic = cnt ? 1.0f / (float)cnt : 0.0f;
-        f[19] = cnt ? nvx * ic - mv.x : 0.0f;
-        f[20] = cnt ? nvy * ic - mv.y : 0.0f;
-        f[21] = cnt ? nvz * ic - mv.z : 0.0f;
+        const float mvx = (float)vx, mvy = (float)vy, mvz = (float)vz;
+        const double ic = cnt ? 1.0 / (double)cnt : 0.0;
+        f[19] = cnt ? (float)(from_fix(nvx) * ic) - mvx : 0.0f;
+        f[20] = cnt ? (float)(from_fix(nvy) * ic) - mvy : 0.0f;
+        f[21] = cnt ? (float)(from_fix(nvz) * ic) - mvz : 0.0f;
         f[22] = (float)(1.0 / d.mmass[mb + i]);
         f[23] = (float)deg[i];
-        float* o = feat + (size_t)(mb + i) * kClosureFeatures;
+        /* six 16-byte stores per row (rows are 96 bytes, 16-byte aligned) */
+        float4* o = reinterpret_cast<float4*>(feat + (size_t)(mb + i) * kClosureFeatures);
+#pragma unroll
+        for (int c = 0; c < kClosureFeatures / 4; ++c) o[c] = make_float4(f[4 * c], f[4 * c + 1], f[4 * c + 2], f[4 * c + 3]);
         bool out = false;
 #pragma unroll
         for (int c = 0; c < kClosureFeatures; ++c) {
-            o[c] = f[c];
             if (fmin) {
                 const float lo = fmin[c], hi = fmax[c], pad = 0.25f * (hi - lo);
                 out |= f[c] < lo - pad || f[c] > hi + pad;
@@ -1924,7 +2005,7 @@ hipError_t launch_temperature(const DevState& d, hipStream_t st, int replica, ui
 hipError_t launch_closure_features(const DevState& d, hipStream_t st, float* feat, const float* fmin,
                                    const float* fmax) {
     if (d.m_tot == 0) return hipSuccess;
-    const size_t lds = (size_t)kMaxMono * (16 + 16 + 12);
+    const size_t lds = kClLds;
     static bool attr = false;
     if (!attr) {
         const hipError_t e = hipFuncSetAttribute((const void*)k_closure_features,
@@ -1932,7 +2013,7 @@ hipError_t launch_closure_features(const DevState& d, hipStream_t st, float* fea
         if (e != hipSuccess) return e;
         attr = true;
     }
-    hipLaunchKernelGGL(k_closure_features, dim3(d.n_rep), dim3(512), lds, st, d, feat, fmin, fmax);
+    hipLaunchKernelGGL(k_closure_features, dim3(d.n_rep), dim3(kClThreads), lds, st, d, feat, fmin, fmax);
     return hipGetLastError();
 }
 

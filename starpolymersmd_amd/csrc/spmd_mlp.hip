@@ -279,6 +279,187 @@ __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ W, int n
     }
 }
 
+/* ---------------------------------------------- fused inference pass */
+
+/* The inference pass in one kernel: the transposed products D = W X^T, with
+ * the weights as the matrix-core A operand (16 output neurons x 32 inputs)
+ * and 16 rows as B.  Lane l then holds neurons 4(l/16)+r (r < 4) of row
+ * l%16 in each 16-neuron tile, which is, up to an order of the inputs, the
+ * B operand the next layer needs from that lane: inputs 8g..8g+7 of
+ * 32-input block kb (g = l/16) are taken as neurons kperm(kb, g, 0..7),
+ * and the next layer's weight columns are packed in that order.  The
+ * activations never leave registers; the weights (BF16, 77,824 bytes)
+ * and biases sit in LDS in the order the lanes read them. */
+constexpr int fL0 = 0;                          /* 8 tiles x 64 lanes x 8 */
+constexpr int fL1 = fL0 + 8 * 64 * 8;           /* 8 tiles x 4 blocks x 64 x 8 */
+constexpr int fL2 = fL1 + 8 * 4 * 64 * 8;
+constexpr int fL3 = fL2 + 8 * 4 * 64 * 8;       /* 4 blocks x 64 x 8 */
+constexpr int fEnd = fL3 + 4 * 64 * 8;          /* 38,912 BF16 */
+constexpr int fBias = 3 * kH + 16;              /* b0, b1, b2, b3 (padded) */
+constexpr int kFwdThreads = 256;
+constexpr size_t kFwdLds = (size_t)fEnd * 2 + (size_t)fBias * 4;
+
+__host__ __device__ __forceinline__ int kperm(int kb, int g, int i) {
+    return 32 * kb + (i < 4 ? 4 * g + i : 16 + 4 * g + (i - 4));
+}
+
+/* FP32 parameters -> the lane-ready BF16 weights of k_forward_fused */
+__global__ __launch_bounds__(256) void k_pack_fused(const float* __restrict__ p, uint16_t* __restrict__ wf) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= fEnd) return;
+    const int i = e & 7, lane = (e >> 3) & 63, n16 = lane & 15, g = lane >> 4;
+    float v = 0.0f;
+    if (e < fL1) {
+        const int n = (e >> 9) * 16 + n16, k = 8 * g + i;
+        v = k < kIn ? p[oW0 + n * kIn + k] : 0.0f;
+    } else if (e < fL3) {
+        const int q = e - (e < fL2 ? fL1 : fL2), off = e < fL2 ? oW1 : oW2;
+        const int n = (q >> 11) * 16 + n16, kb = (q >> 9) & 3;
+        v = p[off + n * kH + kperm(kb, g, i)];
+    } else {
+        const int kb = (e - fL3) >> 9;
+        v = n16 < kOut ? p[oW3 + n16 * kH + kperm(kb, g, i)] : 0.0f;
+    }
+    wf[e] = f2bf(v);
+}
+
+/* BF16 by the gfx950 conversion instruction (round to nearest even, as
+ * f2bf for finite values) and SiLU with the hardware reciprocal: the
+ * result is rounded to BF16 at once, far coarser than either step's error */
+__device__ __forceinline__ short bf16_rne(float f) { return __builtin_bit_cast(short, (__bf16)f); }
+__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+
+/* z + b, SiLU, BF16 into the next layer's B operands */
+template <int NS>
+__device__ __forceinline__ void fwd_epilogue(const f32x4 (&acc)[8][NS], const float* __restrict__ bias, int g,
+                                             bf16x8 (&bh)[4][NS]) {
+#pragma unroll
+    for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float b = bias[mt * 16 + 4 * g + r];
+#pragma unroll
+            for (int ns = 0; ns < NS; ++ns) bh[mt >> 1][ns][(mt & 1) * 4 + r] = bf16_rne(silu_fast(acc[mt][ns][r] + b));
+        }
+}
+
+/* M feature rows (FP32, 24 each) -> M x 6 outputs (FP32).  Each wave takes
+ * 16 NS rows at a time, wave-strided over the grid (no synchronisation
+ * after the weights are staged). */
+template <int NS>
+__global__ __launch_bounds__(kFwdThreads, 2) void k_forward_fused(const float* __restrict__ x,
+                                                                   const float* __restrict__ mu,
+                                                                   const float* __restrict__ sd,
+                                                                   const uint16_t* __restrict__ wf,
+                                                                   const float* __restrict__ p, float* __restrict__ out,
+                                                                   uint64_t M) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint16_t* sw = reinterpret_cast<uint16_t*>(smem);
+    float* sb = reinterpret_cast<float*>(smem + (size_t)fEnd * 2);
+    const int tid = threadIdx.x;
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(wf);
+        uint4* dst = reinterpret_cast<uint4*>(sw);
+        for (int e = tid; e < fEnd / 8; e += kFwdThreads) dst[e] = src[e];
+        for (int e = tid; e < fBias; e += kFwdThreads) {
+            float v = 0.0f;
+            if (e < kH) v = p[oB0 + e];
+            else if (e < 2 * kH) v = p[oB1 + e - kH];
+            else if (e < 3 * kH) v = p[oB2 + e - 2 * kH];
+            else if (e - 3 * kH < kOut) v = p[oB3 + e - 3 * kH];
+            sb[e] = v;
+        }
+    }
+    __syncthreads();
+    const int lane = tid & 63, w = tid >> 6, g = lane >> 4, s = lane & 15;
+    float m8[8], d8[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int c = 8 * g + i;
+        m8[i] = c < kIn ? mu[c] : 0.0f;
+        d8[i] = c < kIn ? sd[c] : 1.0f;
+    }
+    const uint64_t tiles = (M + 16 * NS - 1) / (16 * NS);
+    const uint64_t stride = (uint64_t)gridDim.x * (kFwdThreads / 64);
+    for (uint64_t t = (uint64_t)blockIdx.x * (kFwdThreads / 64) + w; t < tiles; t += stride) {
+        const uint64_t row0 = t * 16 * NS;
+        /* the weight reads are the same in every pass: an opaque base keeps
+         * the compiler from hoisting all 76 of them out of the loop (304
+         * registers, spilled to scratch) */
+        uint32_t wo = 0;  /* an offset, not a pointer: the reads stay LDS reads */
+        asm volatile("" : "+v"(wo));
+        bf16x8 bx[NS];
+#pragma unroll
+        for (int ns = 0; ns < NS; ++ns) {
+            const uint64_t row = row0 + ns * 16 + s;
+            float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (row < M && g < 3) {
+                const float4* xr = reinterpret_cast<const float4*>(x + row * kIn + 8 * g);
+                const float4 u0 = xr[0], u1 = xr[1];
+                v[0] = u0.x; v[1] = u0.y; v[2] = u0.z; v[3] = u0.w;
+                v[4] = u1.x; v[5] = u1.y; v[6] = u1.z; v[7] = u1.w;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) v[i] = (v[i] - m8[i]) / d8[i];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) bx[ns][i] = bf16_rne(v[i]);
+        }
+        f32x4 acc[8][NS];
+        bf16x8 bh[4][NS];
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(sw + wo + fL0 + (mt * 64 + lane) * 8);
+#pragma unroll
+            for (int ns = 0; ns < NS; ++ns)
+                acc[mt][ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bx[ns], f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        }
+        fwd_epilogue<NS>(acc, sb, g, bh);
+#pragma unroll
+        for (int layer = 0; layer < 2; ++layer) {
+#pragma unroll
+            for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+                for (int ns = 0; ns < NS; ++ns) acc[mt][ns] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) {
+                /* one 32-input block at a time: 8 weight operands live */
+                uint32_t ko = (layer ? fL2 : fL1) + wo;
+                asm volatile("" : "+v"(ko));
+                const uint16_t* wl = sw + ko;
+#pragma unroll
+                for (int mt = 0; mt < 8; ++mt) {
+                    const bf16x8 a = *reinterpret_cast<const bf16x8*>(wl + ((mt * 4 + kb) * 64 + lane) * 8);
+#pragma unroll
+                    for (int ns = 0; ns < NS; ++ns)
+                        acc[mt][ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[kb][ns], acc[mt][ns], 0, 0, 0);
+                }
+            }
+            fwd_epilogue<NS>(acc, sb + (layer + 1) * kH, g, bh);
+        }
+        f32x4 o[NS];
+#pragma unroll
+        for (int ns = 0; ns < NS; ++ns) o[ns] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(sw + wo + fL3 + (kb * 64 + lane) * 8);
+#pragma unroll
+            for (int ns = 0; ns < NS; ++ns) o[ns] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[kb][ns], o[ns], 0, 0, 0);
+        }
+        if (g < 2) {
+#pragma unroll
+            for (int ns = 0; ns < NS; ++ns) {
+                const uint64_t row = row0 + ns * 16 + s;
+                if (row < M)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int c = 4 * g + r;
+                        if (c < kOut) out[row * kOut + c] = o[ns][r] + sb[3 * kH + c];
+                    }
+            }
+        }
+    }
+}
+
 }  // namespace mlpk
 
 using namespace mlpk;
@@ -289,6 +470,8 @@ struct spmd_mlp {
     float *mu = nullptr, *sd = nullptr, *sdy = nullptr;
     uint16_t *W0 = nullptr, *W1 = nullptr, *W2 = nullptr, *W3 = nullptr;
     uint16_t *W1t = nullptr, *W2t = nullptr, *W3t = nullptr;
+    uint16_t* Wf = nullptr;  /* lane-ready weights of the fused inference pass */
+    int n_cu = 0;
     /* workspace */
     uint64_t cap = 0;
     uint16_t *X0 = nullptr, *Z1 = nullptr, *A1 = nullptr, *Z2 = nullptr, *A2 = nullptr, *Z3 = nullptr, *A3 = nullptr;
@@ -352,6 +535,26 @@ int pack_all(spmd_mlp* h) {
     hipLaunchKernelGGL(k_pack, g(kH * kH), dim3(256), 0, h->st, p + oW2, kH, kH, kH, kH, h->W2, h->W2t, kH);
     hipLaunchKernelGGL(k_pack, g(kH * kOutK), dim3(256), 0, h->st, p + oW3, kOut, kH, kOutP, kH, h->W3, h->W3t,
                        kOutK);
+    hipLaunchKernelGGL(k_pack_fused, g(fEnd), dim3(256), 0, h->st, p, h->Wf);
+    MCHK(h, hipGetLastError());
+    return SPMD_OK;
+}
+
+/* the fused inference pass of M rows on stream st: x (M x 24) -> out (M x 6) */
+constexpr int kFwdRows = 2;  /* 16-row tiles per wave and pass (168 VGPRs, no spills) */
+int forward_fused(spmd_mlp* h, const float* x, float* out, uint64_t M, hipStream_t st) {
+    static bool attr = false;
+    if (!attr) {
+        MCHK(h, hipFuncSetAttribute((const void*)k_forward_fused<kFwdRows>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFwdLds));
+        attr = true;
+    }
+    const uint64_t tiles = (M + 16 * kFwdRows - 1) / (16 * kFwdRows);
+    const uint64_t want = (tiles + kFwdThreads / 64 - 1) / (kFwdThreads / 64);
+    const uint64_t cap = 2ull * (uint64_t)(h->n_cu > 0 ? h->n_cu : 256);
+    const unsigned nb = (unsigned)(want < cap ? want : cap);
+    hipLaunchKernelGGL(k_forward_fused<kFwdRows>, dim3(nb), dim3(kFwdThreads), kFwdLds, st, x, h->mu, h->sd, h->Wf,
+                       h->p, out, M);
     MCHK(h, hipGetLastError());
     return SPMD_OK;
 }
@@ -439,11 +642,16 @@ int spmd_mlp_create(uint64_t seed, spmd_mlp** out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SPMD_ERR_NO_DEVICE;
     spmd_mlp* h = new spmd_mlp();
     auto bad = [&](hipError_t e) { return e != hipSuccess; };
+    int dev = 0;
+    if (bad(hipGetDevice(&dev)) || bad(hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, dev))) {
+        delete h;
+        return SPMD_ERR_HIP;
+    }
     if (bad(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking)) || bad(dal(&h->p, kParams)) ||
         bad(dal(&h->m, kParams)) || bad(dal(&h->v, kParams)) || bad(dal(&h->g, kParams)) || bad(dal(&h->mu, kIn)) ||
         bad(dal(&h->sd, kIn)) || bad(dal(&h->sdy, 3)) || bad(dal(&h->W0, kH * kInP)) || bad(dal(&h->W1, kH * kH)) ||
         bad(dal(&h->W2, kH * kH)) || bad(dal(&h->W3, kOutP * kH)) || bad(dal(&h->W1t, kH * kH)) ||
-        bad(dal(&h->W2t, kH * kH)) || bad(dal(&h->W3t, kH * kOutK))) {
+        bad(dal(&h->W2t, kH * kH)) || bad(dal(&h->W3t, kH * kOutK)) || bad(dal(&h->Wf, fEnd))) {
         spmd_mlp_destroy(h);
         return SPMD_ERR_OOM;
     }
@@ -483,7 +691,7 @@ int spmd_mlp_destroy(spmd_mlp* h) {
     if (!h) return SPMD_OK;
     if (h->st) (void)hipStreamSynchronize(h->st);
     dfr(h->p); dfr(h->m); dfr(h->v); dfr(h->g); dfr(h->mu); dfr(h->sd); dfr(h->sdy);
-    dfr(h->W0); dfr(h->W1); dfr(h->W2); dfr(h->W3); dfr(h->W1t); dfr(h->W2t); dfr(h->W3t);
+    dfr(h->W0); dfr(h->W1); dfr(h->W2); dfr(h->W3); dfr(h->W1t); dfr(h->W2t); dfr(h->W3t); dfr(h->Wf);
     dfr(h->X0); dfr(h->Z1); dfr(h->A1); dfr(h->Z2); dfr(h->A2); dfr(h->Z3); dfr(h->A3);
     dfr(h->dZ4); dfr(h->dZ); dfr(h->O); dfr(h->dA); dfr(h->P); dfr(h->Pb); dfr(h->lpart);
     dfr(h->Xb); dfr(h->Yb);
@@ -542,29 +750,23 @@ int spmd_mlp_get_norm(const spmd_mlp* h, float* mu_x, float* sd_x, float* sd_y) 
 int spmd_mlp_forward_on(spmd_mlp* h, const float* x_dev, float* out_dev, uint64_t M, hipStream_t st) {
     if (!h || !x_dev || !out_dev) return SPMD_ERR_ARG;
     if (M == 0) return SPMD_OK;
-    if (int rc = ensure(h, M)) return rc;
     MCHK(h, hipStreamSynchronize(h->st));  /* parameters packed by training are complete */
-    const uint64_t Mp = pad64(M);
-    const uint64_t n = Mp * kInP;
-    hipLaunchKernelGGL(k_in, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x_dev, h->mu, h->sd, h->X0, M, Mp);
-    const dim3 g((unsigned)(Mp / kRowTile)), b(256);
-    const float* p = h->p;
-    hipLaunchKernelGGL((k_gemm_nt<kInP, kH, EPI_SILU>), g, b, 0, st, h->X0, h->W0, p + oB0, h->Z1, h->A1,
-                       (float*)nullptr);
-    hipLaunchKernelGGL((k_gemm_nt<kH, kH, EPI_SILU>), g, b, 0, st, h->A1, h->W1, p + oB1, h->Z2, h->A2,
-                       (float*)nullptr);
-    hipLaunchKernelGGL((k_gemm_nt<kH, kH, EPI_SILU>), g, b, 0, st, h->A2, h->W2, p + oB2, h->Z3, h->A3,
-                       (float*)nullptr);
-    hipLaunchKernelGGL((k_gemm_nt<kH, kOutP, EPI_LINEAR>), g, b, 0, st, h->A3, h->W3, p + oB3,
-                       (uint16_t*)nullptr, (uint16_t*)nullptr, h->O);
-    MCHK(h, hipGetLastError());
-    MCHK(h, hipMemcpy2DAsync(out_dev, kOut * 4, h->O, kOutP * 4, kOut * 4, M, hipMemcpyDeviceToDevice, st));
-    return SPMD_OK;
+    return forward_fused(h, x_dev, out_dev, M, st);
 }
 
 extern "C" {
 
 int spmd_mlp_forward(spmd_mlp* h, const float* x_dev, float* out_dev, uint64_t M) {
+    if (!h || !x_dev || !out_dev) return SPMD_ERR_ARG;
+    if (M == 0) return SPMD_OK;
+    if (int rc = forward_fused(h, x_dev, out_dev, M, h->st)) return rc;
+    MCHK(h, hipStreamSynchronize(h->st));
+    return SPMD_OK;
+}
+
+/* the training-path forward (four row-tiled products through the
+ * workspace, the activations kept for the backward pass) */
+int spmd_mlp_forward_unfused(spmd_mlp* h, const float* x_dev, float* out_dev, uint64_t M) {
     if (!h || !x_dev || !out_dev) return SPMD_ERR_ARG;
     if (M == 0) return SPMD_OK;
     uint64_t Mp;

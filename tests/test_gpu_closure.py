@@ -29,7 +29,8 @@ def _net(seed=1):
     return ClosureNet(seed)
 
 
-def test_forward_matches_oracle():
+@pytest.mark.parametrize("fused", [True, False])
+def test_forward_matches_oracle(fused):
     net = _net()
     p = (omlp.init_params(7) * 0.7).astype(np.float32)
     net.set_params(p)
@@ -38,10 +39,34 @@ def test_forward_matches_oracle():
     net.set_norm(mu, sd, np.ones(3, np.float32))
     rng = np.random.default_rng(0)
     x = (rng.normal(size=(1000, 24)) * sd + mu).astype(np.float32)
-    out = net.forward(x)
+    out = net.forward(x, fused=fused)
     ref, _ = omlp.forward(p.astype(np.float64), x.astype(np.float64), mu, sd)
     scale = np.abs(ref).max(axis=0)
     assert np.all(np.abs(out - ref) <= 3e-2 * scale + 1e-3), np.abs(out - ref).max(axis=0) / scale
+
+
+@pytest.mark.parametrize("M", [1, 17, 63, 64, 65, 1000, 307456])
+def test_fused_forward_matches_unfused(M):
+    """The fused pass reorders each layer's inputs inside the matrix-core
+    products; both paths round the same FP32 sums to BF16 between layers,
+    so they agree to a few BF16 units on the outputs.  Ragged row counts
+    cover the partial last tile; 307,456 rows (256 stars of 1,201 monomers)
+    cover the wave-strided loop over the grid."""
+    net = _net()
+    p = (omlp.init_params(11) * 0.7).astype(np.float32)
+    net.set_params(p)
+    mu = np.linspace(-0.5, 0.5, 24).astype(np.float32)
+    sd = np.linspace(0.5, 1.5, 24).astype(np.float32)
+    net.set_norm(mu, sd, np.ones(3, np.float32))
+    rng = np.random.default_rng(M)
+    x = (rng.normal(size=(M, 24)) * sd + mu).astype(np.float32)
+    a = net.forward(x, fused=True)
+    b = net.forward(x, fused=False)
+    assert np.all(np.isfinite(a))
+    scale = np.abs(b).max(axis=0) + 1e-6
+    err = np.abs(a - b) / scale
+    assert err.max() <= 2e-2, err.max(axis=0)
+    assert np.median(err) <= 1e-3
 
 
 def test_gradient_matches_oracle():
@@ -84,15 +109,19 @@ def test_adam_learns_a_known_map():
     assert last < floor + 2.0, (last, floor, hist)
 
 
-def test_features_match_oracle():
-    g = MPCSystem(make_params(L=(20, 20, 20), n_c=10, shear=0.03, step_update=10, seed=8, md_dt=0.01))
-    g.add_star(4, 4, 6, 10.0, coil=True)
+@pytest.mark.parametrize("L,A,B,arms", [(20, 4, 4, 6), (48, 20, 20, 30)])
+def test_features_match_oracle(L, A, B, arms):
+    """The neighbour features come from a spatial hash on the device and from
+    all pairs in the oracle; the second case is the 1,201-monomer star the
+    closure is benchmarked on."""
+    g = MPCSystem(make_params(L=(L,) * 3, n_c=10, shear=0.03, step_update=10, seed=8, md_dt=0.01))
+    g.add_star(A, B, arms, 10.0, coil=True)
     g.md_init()
     g.initialize()
     g.run(0, 31, 0.1)
     g.sync()
-    o = ob.Oracle(L=(20, 20, 20), n_c=10, shear=0.03)
-    o.add_star(4, 4, 6, 10.0)
+    o = ob.Oracle(L=(L,) * 3, n_c=10, shear=0.03)
+    o.add_star(A, B, arms, 10.0)
     bonds = [(int(min(a, b)), int(max(a, b))) for a, b in o.bonds()]
     mass, btype, _ = o.monomer_props()
     pos, vel, _ = g.get_monomers()
