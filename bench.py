@@ -329,6 +329,11 @@ def main():
                               "sweep_ms": None if tune is None else tune["timings"]}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     # measured bytes (FETCH_SIZE + WRITE_SIZE of the pipeline's kernels) over the
+                     # same kernel time: the bandwidth the hardware moved, amplification included
+                     "measured_gbs": traffic / (pipe_ms * 1e-3) / 1e9 if traffic else None,
+                     "measured_frac": traffic / (pipe_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
+                     "amplification": traffic / (B_ALG * n_f) if traffic else None,
                      "traffic_source": TRAFFIC_PROFILE if traffic else None,
                      "kernel": "collision pipeline k_bin+k_scan+k_perm+k_collide",
                      "alg_bytes_per_launch": B_ALG * n_f, "avg_ms": pipe_ms,
