@@ -184,6 +184,18 @@ int spmd_fourier_modes(spmd_sys* s, uint32_t replica, double* modes);
  * floor(|v|/width) over the solvent, bins 0..nbins-1 (nbins <= 1024);
  * counts[nbins] holds the speeds beyond. */
 int spmd_speed_histogram(spmd_sys* s, uint32_t replica, double width, uint32_t nbins, uint64_t* counts);
+/* Vel_Autocorr_Fluid (Vel_Autocorr_Fluid.h:19-116): one call of its
+ * operator() over replica r's solvent -- every particle's velocity enters
+ * its window of `entries` samples (particles followed by id: tracking must
+ * be on); once the window is full, every call adds v(oldest) . v(oldest +
+ * lag) of every particle to the lag sums (calculate(), :68-78).  The
+ * windows live on the device; `entries` (at most 1024) is fixed by the
+ * first call until a reset. */
+int spmd_fluid_vacf_sample(spmd_sys* s, uint32_t r, uint32_t entries);
+/* sums[entries]: the lag sums over particles and calls; *counter: the
+ * calls that made them (the reference's Counter) */
+int spmd_fluid_vacf_read(spmd_sys* s, uint32_t r, double* sums, uint64_t* counter);
+int spmd_fluid_vacf_reset(spmd_sys* s, uint32_t r);
 
 /* Verlet rebuilds so far and entries of the current (half) lists of one
  * replica's star -- diagnostics, no reference counterpart. */

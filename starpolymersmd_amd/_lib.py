@@ -119,6 +119,9 @@ _SIGS = {
     "spmd_velocity_profile": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_uint32, _P(C.c_int32), _u32, _d]),
     "spmd_fourier_modes": (C.c_int, [C.c_void_p, C.c_uint32, _d]),
     "spmd_speed_histogram": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_uint32, _P(C.c_uint64)]),
+    "spmd_fluid_vacf_sample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "spmd_fluid_vacf_read": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_uint64)]),
+    "spmd_fluid_vacf_reset": (C.c_int, [C.c_void_p, C.c_uint32]),
     "spmd_get_params": (C.c_int, [C.c_void_p, C.c_uint32, _P(Params)]),
     "spmd_star_layout": (C.c_int, [C.c_void_p, C.c_uint32, _u32, _u32, _u32, _u32]),
     "spmd_write_fluid": (C.c_int, [C.c_void_p, C.c_uint32, C.c_char_p]),

@@ -70,6 +70,15 @@ struct spmd_sys {
     uint32_t cl_k = 0;             /* closure interval counter (Philox) */
     double* mom = nullptr;         /* star moments (kStarMoments) and patches (2) */
     unsigned long long* obs = nullptr;  /* solvent observer sums */
+    /* fluid velocity autocorrelation per replica: windows by particle id,
+     * lag sums (64-bit fixed point), samples taken and sums made */
+    struct Vacf {
+        float* hist = nullptr;
+        unsigned long long* sums = nullptr;
+        uint32_t entries = 0;
+        uint64_t calls = 0, counter = 0;
+    };
+    std::vector<Vacf> vacf;
 };
 
 namespace {
@@ -634,6 +643,7 @@ int spmd_destroy(spmd_sys* s) {
     dfree(d.inj_axis); dfree(d.inj_gamma); dfree(d.reps); dfree(d.ctl); dfree(d.pblk_base);
     dfree(d.tile_base); dfree(d.red);
     dfree(s->cl_feat); dfree(s->cl_out); dfree(s->cl_range); dfree(s->mom); dfree(s->obs);
+    for (auto& v : s->vacf) { dfree(v.hist); dfree(v.sums); }
     if (s->side) (void)hipStreamSynchronize(s->side);
     if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
     if (s->ev_join) (void)hipEventDestroy(s->ev_join);
@@ -1221,6 +1231,55 @@ int spmd_speed_histogram(spmd_sys* s, uint32_t r, double width, uint32_t nbins, 
     HIPCHK(s, launch_fluid_observe(s->d, s->stream, a, rp.n_fluid));
     HIPCHK(s, hipMemcpyAsync(counts, a.speed, (nbins + 1) * 8ull, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(s, hipStreamSynchronize(s->stream));
+    return SPMD_OK;
+}
+
+int spmd_fluid_vacf_sample(spmd_sys* s, uint32_t r, uint32_t entries) {
+    if (!valid_rep(s, r) || entries == 0) return SPMD_ERR_ARG;
+    if (entries > (uint32_t)kVacfMaxEntries) return fail(s, SPMD_ERR_ARG, "at most 1024 VACF entries");
+    DevState& d = s->d;
+    if (!d.ids[d.cur]) return fail(s, SPMD_ERR_STATE, "the fluid VACF follows particles by id: turn tracking on");
+    if (s->vacf.size() < s->reps.size()) s->vacf.resize(s->reps.size());
+    spmd_sys::Vacf& v = s->vacf[r];
+    const RepDev& rp = s->reps[r];
+    if (!v.hist) {
+        if (int rc = dalloc(s, &v.hist, 3ull * rp.n_fluid * entries)) return rc;
+        if (int rc = dalloc(s, &v.sums, entries)) { dfree(v.hist); return rc; }
+        HIPCHK(s, hipMemsetAsync(v.sums, 0, entries * 8ull, s->stream));
+        v.entries = entries;
+        v.calls = v.counter = 0;
+    } else if (v.entries != entries) {
+        return fail(s, SPMD_ERR_ARG, "entries differ from the running VACF's (reset it first)");
+    }
+    /* the window after this sample holds samples calls-E+1 .. calls */
+    const uint32_t slot = (uint32_t)(v.calls % v.entries);
+    const int calc = v.calls + 1 >= v.entries;
+    const uint32_t oldest = (uint32_t)((v.calls + 1) % v.entries);
+    HIPCHK(s, launch_fluid_vacf(d, s->stream, (int)r, rp.n_fluid, v.hist, v.entries, slot, oldest, calc, v.sums));
+    v.calls++;
+    if (calc) v.counter++;
+    return SPMD_OK;
+}
+
+int spmd_fluid_vacf_read(spmd_sys* s, uint32_t r, double* sums, uint64_t* counter) {
+    if (!valid_rep(s, r) || !sums || !counter) return SPMD_ERR_ARG;
+    if (s->vacf.size() <= r || !s->vacf[r].hist) return fail(s, SPMD_ERR_STATE, "no VACF sample taken");
+    const spmd_sys::Vacf& v = s->vacf[r];
+    std::vector<unsigned long long> q(v.entries);
+    HIPCHK(s, hipMemcpyAsync(q.data(), v.sums, v.entries * 8ull, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    for (uint32_t j = 0; j < v.entries; ++j) sums[j] = (double)(long long)q[j] * (1.0 / 4294967296.0);
+    *counter = v.counter;
+    return SPMD_OK;
+}
+
+int spmd_fluid_vacf_reset(spmd_sys* s, uint32_t r) {
+    if (!valid_rep(s, r)) return SPMD_ERR_ARG;
+    if (s->vacf.size() <= r) return SPMD_OK;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    dfree(s->vacf[r].hist);
+    dfree(s->vacf[r].sums);
+    s->vacf[r] = spmd_sys::Vacf{};
     return SPMD_OK;
 }
 

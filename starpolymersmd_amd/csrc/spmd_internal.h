@@ -167,6 +167,11 @@ struct ObsArgs {
     unsigned long long* speed;    /* [nsb + 1] */
 };
 hipError_t launch_fluid_observe(const DevState& d, hipStream_t st, const ObsArgs& a, uint32_t n_fluid);
+/* Vel_Autocorr_Fluid: window slot `slot` <- velocities by id; with calc,
+ * sums[lag] += sum over particles of v(oldest) . v(oldest + lag) */
+constexpr int kVacfMaxEntries = 1024;
+hipError_t launch_fluid_vacf(const DevState& d, hipStream_t st, int r, uint32_t n_fluid, float* hist, uint32_t E,
+                             uint32_t slot, uint32_t oldest, int calc, unsigned long long* sums);
 hipError_t launch_closure_kick(const DevState& d, hipStream_t st, const float* out, float sy0, float sy1, float sy2,
                                uint32_t k);
 hipError_t launch_temperature(const DevState& d, hipStream_t st, int replica, uint32_t ntiles);

@@ -2035,6 +2035,60 @@ hipError_t launch_star_patches(const DevState& d, hipStream_t st, int replica, i
     return hipGetLastError();
 }
 
+/* Vel_Autocorr_Fluid::operator() and calculate (Vel_Autocorr_Fluid.h:44-78)
+ * for replica r: each solvent particle's velocity goes into slot `slot` of
+ * its window of E samples (hist, by particle id: E x 3 floats); with calc
+ * (the window is full), the products of the oldest sample (slot `oldest`)
+ * with every sample of the window, oldest first, are summed over the
+ * particles into sums[lag] -- 64-bit fixed point (exact, so the order of
+ * particles and workgroups does not matter), one global atomic per lag and
+ * workgroup. */
+__global__ __launch_bounds__(256) void k_fluid_vacf(DevState d, int r, float* hist, uint32_t E, uint32_t slot,
+                                                    uint32_t oldest, int calc, unsigned long long* sums) {
+    __shared__ unsigned long long s_sum[kVacfMaxEntries];
+    const RepDev& rep = d.reps[r];
+    const int tid = threadIdx.x;
+    const uint32_t li = blockIdx.x * 256u + (uint32_t)tid;
+    const bool live = li < rep.n_fluid;
+    if (calc)
+        for (uint32_t j = tid; j < E; j += 256) s_sum[j] = 0ull;
+    float* h = nullptr;
+    if (live) {
+        const int cur = d.cur;
+        const uint64_t i = rep.part_base + li;
+        const uint32_t id = d.ids[cur][i];
+        h = hist + (size_t)id * E * 3;
+        h[3 * slot] = d.vx[cur][i]; h[3 * slot + 1] = d.vy[cur][i]; h[3 * slot + 2] = d.vz[cur][i];
+    }
+    if (!calc) return;
+    __syncthreads();
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    if (live) { ox = h[3 * oldest]; oy = h[3 * oldest + 1]; oz = h[3 * oldest + 2]; }
+    for (uint32_t j = 0; j < E; ++j) {
+        unsigned long long p = 0ull;
+        if (live) {
+            uint32_t sl = oldest + j;
+            if (sl >= E) sl -= E;
+            p = to_fix((ox * (double)h[3 * sl] + oy * (double)h[3 * sl + 1]) + oz * (double)h[3 * sl + 2]);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) p += __shfl_xor(p, o);
+        if ((tid & 63) == 0) atomicAdd(&s_sum[j], p);
+    }
+    __syncthreads();
+    for (uint32_t j = tid; j < E; j += 256)
+        if (s_sum[j]) atomicAdd(&sums[j], s_sum[j]);
+}
+
+hipError_t launch_fluid_vacf(const DevState& d, hipStream_t st, int r, uint32_t n_fluid, float* hist, uint32_t E,
+                             uint32_t slot, uint32_t oldest, int calc, unsigned long long* sums) {
+    if (E == 0 || E > (uint32_t)kVacfMaxEntries || slot >= E || oldest >= E) return hipErrorInvalidValue;
+    const uint32_t nblk = (n_fluid + 255) / 256;
+    if (nblk == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fluid_vacf, dim3(nblk), dim3(256), 0, st, d, r, hist, E, slot, oldest, calc, sums);
+    return hipGetLastError();
+}
+
 hipError_t launch_fluid_observe(const DevState& d, hipStream_t st, const ObsArgs& a, uint32_t n_fluid) {
     if (a.nb[0] < 1 || a.nb[0] > kObsBins || a.nb[1] < 1 || a.nb[1] > kObsBins || a.nsb < 0 ||
         a.nsb > kObsSpeedBins)

@@ -12,8 +12,10 @@ accept raw samples, so the same accumulation serves any source of samples.
   VelocityProfile   VelocityX         lib/Analysis/Velocity_x.h:22-148
   FourierAutocorr   Fourier_Autocorr  lib/Analysis/Fourier_Autocorr.h:19-127
   SpeedDistribution VelocityDistribution lib/Analysis/Vel_distribution.h:17-57
+  FluidVelocityAutocorr Vel_Autocorr_Fluid lib/Analysis/Vel_Autocorr_Fluid.h:19-116
   StatisticsFile    statistics*.dat   src/star-polymers.cpp:307, 466-486
   run_with_output   the driver loop   src/star-polymers.cpp:412-536
+  run_mpc_test      the solvent test  src/mpc_test.cpp:42-146
 """
 from __future__ import annotations
 
@@ -181,6 +183,49 @@ class SpeedDistribution:
         os.write("\n")
 
 
+class FluidVelocityAutocorr:
+    """Vel_Autocorr_Fluid (Vel_Autocorr_Fluid.h:19-116): every solvent
+    particle keeps a window of `entries` velocity samples, `dt` apart; once
+    the windows are full, every sample adds v(oldest) . v(oldest + i) of
+    every particle to lag i (calculate(), :68-78).  The windows (particles
+    followed by id, so tracking must be on) and the lag sums stay on the
+    device (MPCSystem.fluid_vacf_*); this class reads them back."""
+
+    def __init__(self, entries: int = 50, dt: float = 1.0):
+        self.entries = int(entries)
+        self.dt = float(dt)
+        self.system = None
+        self.r = 0
+
+    def sample(self, system, r: int = 0):
+        """One application of the functor to the current solvent."""
+        system.fluid_vacf_sample(self.entries, r)
+        self.system, self.r = system, r
+
+    def average(self) -> np.ndarray:
+        """calculate_average (:80-88): the lag sums / Counter / particles."""
+        sums, counter = self.system.fluid_vacf_read(self.entries, self.r)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return sums / np.float64(counter) / float(self.system.num_fluid(self.r))
+
+    def counter(self) -> int:
+        return self.system.fluid_vacf_read(self.entries, self.r)[1]
+
+    def value(self) -> float:
+        """value() (:90-97) as the reference computes it: the sum of the
+        averaged lags times dt, divided by Counter once more and by 3."""
+        avg, n = self.average(), self.counter()
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return float(np.sum(avg * self.dt / np.float64(n)) / 3.0)
+
+    def print_result(self, os: TextIO):
+        """print_result (:99-106)."""
+        avg = self.average()
+        os.write("#Diffusion coefficient = %g\n" % self.value())
+        for i in range(self.entries):
+            os.write("%g %g\n" % (i * self.dt, avg[i]))
+
+
 STAT_HEADER = ("Step   Epot     Ekin     Temp   n_p s_p  R_gyr   G_xx      G_xy      G_xz       G_yx       "
                "G_yy      G_yz      G_zx     Gzy     Gzz      wx      wy     wz\n")
 
@@ -229,3 +274,47 @@ def run_with_output(system, t0: int, n: int, mpc_dt: float, every: int,
         on_output(nxt)
         system.step(nxt, mpc_dt)
         t = nxt + 1
+
+
+def mpc_test_tag(n_particles: int, shear: float, h: float, amc: bool) -> str:
+    """The file-name suffix of mpc_test.cpp:79-82 (ss_para; operator<< of
+    the doubles)."""
+    return "_N%d_Shear%g_h%g%s" % (n_particles, shear, h, "_AMC" if amc else "")
+
+
+def run_mpc_test(system, steps_equil: int, steps_total: int, steps_output: int, step_size: float,
+                 outdir: str = ".", r: int = 0):
+    """mpc_test.cpp:42-146 on a pure-solvent MPCSystem made with
+    step_update = 1 (and the angular-momentum flag if wanted), tracking on:
+    MPC::step every step; after the equilibration, at every output step the
+    temperature line, VelocityX (bins 0.2) and VelocityDistribution (bins
+    0.01), and every 10th step Vel_Autocorr_Fluid and Fourier_Autocorr (50
+    entries, dt 1.0); then the five files fluid*, temperature*, velocity*,
+    fourier*, maxwell* with the reference's names.  Returns their paths."""
+    import os
+
+    p = system.params[r]
+    # BoxX*BoxY*BoxZ*N of the requested (not even-rounded) box, as :79
+    n_part = int(p.L[0]) * int(p.L[1]) * int(p.L[2]) * int(p.n_c)
+    tag = mpc_test_tag(n_part, p.shear, step_size, bool(p.angular_momentum))
+    names = {k: os.path.join(outdir, k + tag + ".dat") for k in ("fluid", "temperature", "velocity", "fourier",
+                                                                  "maxwell")}
+    velocity_average = VelocityProfile(0.2)
+    autocorr = FluidVelocityAutocorr(50, 1.0)
+    maxwell = SpeedDistribution(0.01, 1024)
+    fourier = FourierAutocorr(50, 1.0)
+    with open(names["temperature"], "w") as temperature_file:
+        for n in range(steps_total):
+            system.step(n, step_size)
+            if n > steps_equil and not n % steps_output:
+                velocity_average.sample(system, r)
+                maxwell.sample(system, r)
+                temperature_file.write("%d %g\n" % (n, system.temperature(r)))
+            if n > steps_equil and not n % 10:
+                autocorr.sample(system, r)
+                fourier.sample(system, r)
+    for key, obs in (("fluid", velocity_average), ("maxwell", maxwell), ("velocity", autocorr),
+                     ("fourier", fourier)):
+        with open(names[key], "w") as f:
+            obs.print_result(f)
+    return names

@@ -282,6 +282,21 @@ class MPCSystem:
         self._chk(self._lib.spmd_speed_histogram(self.h, r, width, nbins, ptr(c, C.c_uint64)))
         return c
 
+    def fluid_vacf_sample(self, entries: int, r: int = 0):
+        """Vel_Autocorr_Fluid::operator() (Vel_Autocorr_Fluid.h:44-66) over
+        the solvent; the windows stay on the device (tracking must be on)."""
+        self._chk(self._lib.spmd_fluid_vacf_sample(self.h, r, entries))
+
+    def fluid_vacf_read(self, entries: int, r: int = 0):
+        """(lag sums over particles and calls, calls that made them)."""
+        s = np.empty(entries, np.float64)
+        n = C.c_uint64()
+        self._chk(self._lib.spmd_fluid_vacf_read(self.h, r, ptr(s, C.c_double), C.byref(n)))
+        return s, int(n.value)
+
+    def fluid_vacf_reset(self, r: int = 0):
+        self._chk(self._lib.spmd_fluid_vacf_reset(self.h, r))
+
     def closure_features(self, feat_dev_ptr: int):
         """24 closure features of every monomer into device memory
         (float32, total monomers x 24)."""

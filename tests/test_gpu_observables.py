@@ -12,6 +12,7 @@ import pytest
 
 import oracle_binding as ob
 from starpolymersmd_amd import MPCSystem, SpmdError, make_params
+from starpolymersmd_amd import observables as obs
 
 pytestmark = pytest.mark.gpu
 
@@ -101,3 +102,77 @@ def test_statistics_row_matches_oracle():
             *o.rotation_frequency()]
     np.testing.assert_allclose(row, want, rtol=1e-7, atol=1e-9)
     np.testing.assert_allclose(row[6:15].reshape(3, 3), g.gyration_tensor(), rtol=0, atol=0)
+
+
+# ---------------------------------------------------------------- fluid VACF
+def _vacf_oracle():
+    import os
+    import sys
+
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+    import vacf
+
+    return vacf
+
+
+def _by_id(g, r=0):
+    _, v, ids = g.get_fluid(r, ids=True)
+    out = np.empty_like(v)
+    out[ids] = v
+    return out
+
+
+def test_fluid_vacf_matches_restatement():
+    """Device windows (by particle id) and lag sums against the reference's
+    functor restated on the CPU, fed the same velocities in id order; 9
+    samples of a 4-entry window, a collision between samples.  Bar: the
+    calls counted equal, every lag sum within 1e-6 (the device rounds each
+    product to 2^-32)."""
+    vacf = _vacf_oracle()
+    g = MPCSystem(make_params(L=(6, 6, 6), n_c=5, shear=0.1, step_update=1, seed=12))
+    g.set_tracking(True)
+    g.initialize()
+    o = vacf.VelAutocorrFluid(g.num_fluid(), 4, 1.0)
+    for t in range(9):
+        g.step(t, 0.1)
+        g.fluid_vacf_sample(4)
+        o.sample(_by_id(g).astype(np.float64))
+    sums, counter = g.fluid_vacf_read(4)
+    assert counter == o.counter == 6
+    assert np.allclose(sums, o.lag_sums(), rtol=0, atol=1e-6), (sums, o.lag_sums())
+    with pytest.raises(SpmdError):
+        g.fluid_vacf_sample(5)  # entries are fixed until a reset
+    g.fluid_vacf_reset()
+    g.fluid_vacf_sample(5)
+    assert g.fluid_vacf_read(5)[1] == 0
+
+
+def test_fluid_vacf_needs_tracking():
+    g = MPCSystem(make_params(L=(6, 6, 6), n_c=5, seed=1))
+    g.initialize()
+    with pytest.raises(SpmdError):
+        g.fluid_vacf_sample(4)
+
+
+def test_run_mpc_test_writes_reference_files(tmp_path):
+    """mpc_test.cpp's run on a 5^3 box (N=10, T=0.5, no shear, h=0.1):
+    the five files under the reference's names and formats; the VACF at lag
+    0 is <v^2> = 3 kT for the thermalised solvent."""
+    g = MPCSystem(make_params(L=(5, 5, 5), n_c=10, kT=0.5, shear=0.0, step_update=1, seed=4))
+    g.set_tracking(True)
+    g.initialize()
+    names = obs.run_mpc_test(g, 100, 1100, 100, 0.1, str(tmp_path))
+    assert sorted(names) == ["fluid", "fourier", "maxwell", "temperature", "velocity"]
+    assert names["velocity"].endswith("velocity_N1250_Shear0_h0.1.dat")
+    temp = [l.split() for l in open(names["temperature"]).read().splitlines()]
+    assert [int(a) for a, _ in temp] == list(range(200, 1100, 100))
+    T = np.mean([float(b) for _, b in temp])
+    assert abs(T - 0.5) < 0.05
+    vel = open(names["velocity"]).read().splitlines()
+    assert vel[0].startswith("#Diffusion coefficient = ") and len(vel) == 51
+    lag = np.array([[float(x) for x in l.split()] for l in vel[1:]])
+    assert np.allclose(lag[:, 0], np.arange(50) * 1.0)
+    assert abs(lag[0, 1] - 3 * 0.5) < 0.1 * 1.5
+    assert abs(lag[1:, 1]).max() < 0.05  # 10 collisions apart: decorrelated
+    assert len(open(names["fourier"]).read().splitlines()) == 50
+    assert open(names["maxwell"]).read().strip() and open(names["fluid"]).read().strip()

@@ -213,3 +213,53 @@ def test_run_with_output_visits_the_reference_output_steps():
             done.append(c[1])
     assert done == list(range(250))
     assert ("prop", True) in fs.calls
+
+
+# ---------------------------------------------------------------- fluid VACF
+def _vacf_oracle():
+    import os
+    import sys
+
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+    import vacf
+
+    return vacf
+
+
+def test_vacf_restatement_hand_worked():
+    """One particle, 3 entries, velocities (1,0,0) .. (4,0,0): the window
+    fills on the third call ([1,2,3]: products 1, 2, 3) and the fourth adds
+    [2,3,4] (4, 6, 8); Counter = 2, so the average is [2.5, 4, 5.5] and
+    value() = (2.5 + 4 + 5.5) dt / 2 / 3 = 2 for dt = 1."""
+    vacf = _vacf_oracle()
+    o = vacf.VelAutocorrFluid(1, 3, 1.0)
+    for k in range(1, 5):
+        o.sample(np.array([[k, 0.0, 0.0]]))
+    assert o.counter == 2
+    assert np.array_equal(o.lag_sums(), [5.0, 8.0, 11.0])
+    o.calculate_average()
+    assert np.allclose(o.average, [2.5, 4.0, 5.5])
+    assert o.value() == pytest.approx(2.0)
+
+
+def test_vacf_restatement_particles_apart_and_signs():
+    """Two particles with constant and alternating velocities: the first's
+    products are |v|^2 at every lag, the second's (-1)^lag |v|^2; nothing
+    mixes between particles, and no sum is made before the windows fill."""
+    vacf = _vacf_oracle()
+    o = vacf.VelAutocorrFluid(2, 4, 0.5)
+    a = np.array([1.0, 2.0, 2.0])
+    for k in range(7):
+        o.sample(np.array([a, (-1) ** k * np.array([0.0, 3.0, 4.0])]))
+        if k < 3:
+            assert o.counter == 0 and not o.autocorrelation.any()
+    assert o.counter == 4
+    assert np.allclose(o.autocorrelation[0], 4 * 9.0)
+    assert np.allclose(o.autocorrelation[1], 4 * 25.0 * np.array([1, -1, 1, -1]))
+
+
+def test_mpc_test_file_tag():
+    from starpolymersmd_amd.observables import mpc_test_tag
+
+    assert mpc_test_tag(1250, 0.5, 0.1, False) == "_N1250_Shear0.5_h0.1"
+    assert mpc_test_tag(80000, 0.0, 0.05, True) == "_N80000_Shear0_h0.05_AMC"
