@@ -167,27 +167,36 @@ def batch_bench(intervals=100, seq_intervals=10):
             "speedup": seq_el / el}
 
 
-def closure_bench(replicas=256, explicit_replicas=16, check_intervals=200):
+def closure_bench(replicas=256, explicit_replicas=16, check_replicas=16, check_shear=0.012, check_warmup=400,
+                  check_intervals=800):
     """Learned closure at batch (256 stars, f=30 arms of 20 A + 20 B beads,
-    1,201 monomers, shear rates uniform in [5e-3, 5e-2]) against explicit
-    MPC-MD of the same star at L=48, rho=10 on the same GPU: star-steps/s
-    (MD steps x stars per second), the network trained here on explicit
-    runs of the library (8 shear rates), fallbacks included."""
+    1,201 monomers, shear rates uniform in [4e-3, 1.6e-2]) against explicit
+    MPC-MD of the same star at L=64, rho=10 on the same GPU (the smallest
+    box that holds the star stretched by these rates): star-steps/s (MD
+    steps x stars per second), the network trained here on explicit runs of
+    the library (8 shear rates, solvent started from the steady flow
+    profile), fallbacks included.  Accuracy: 16 replicas each way at shear
+    0.012 in steady flow, Rg^2 and alignment angle averaged over the run
+    after a warm-up, standard errors from the spread of replica means."""
     import numpy as np
 
     from starpolymersmd_amd import MPCSystem, make_params
     from starpolymersmd_amd import closure as cl
 
-    arms, A, B, S, dt = 30, 20, 20, 10, 0.01
-    shears_train = list(np.geomspace(5e-3, 5e-2, 8))
-    data = cl.generate(shears_train, L=64, arms=arms, A=A, B=B, intervals=40, warmup=10)
+    arms, A, B, S, dt, L = 30, 20, 20, 10, 0.01, 64
+    lo, hi = 4e-3, 1.6e-2
+    shears_train = list(np.geomspace(lo, hi, 8))
+    t0 = time.perf_counter()
+    data = cl.generate(shears_train, L=L, arms=arms, A=A, B=B, intervals=150, warmup=800)
+    t_gen = time.perf_counter() - t0
     net = cl.ClosureNet(1)
     t0 = time.perf_counter()
-    hist = net.fit(data, steps=300, batch=65536, lr=2e-3)
+    steps = 2000
+    hist = net.fit(data, steps=steps, batch=65536, lr=2e-3)
     t_fit = time.perf_counter() - t0
     rng = np.random.default_rng(0)
-    shears = list(rng.uniform(5e-3, 5e-2, replicas))
-    run = cl.ClosureRunner(net, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=48, shears=shears)
+    shears = list(rng.uniform(lo, hi, replicas))
+    run = cl.ClosureRunner(net, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=L, shears=shears)
     run.advance(2)
     run.sys.sync()
     n = 20
@@ -198,13 +207,14 @@ def closure_bench(replicas=256, explicit_replicas=16, check_intervals=200):
     closure_rate = replicas * n * S / el
     fallbacks = run.fallbacks
     del run
-    # explicit MPC-MD of the same star, batched replicas at L=48
-    g = MPCSystem([make_params(L=(48, 48, 48), n_c=10, shear=float(shears[r]), step_update=S, md_dt=dt, seed=3 + r)
+    # explicit MPC-MD of the same star, batched replicas
+    g = MPCSystem([make_params(L=(L, L, L), n_c=10, shear=float(shears[r]), step_update=S, md_dt=dt, seed=3 + r)
                    for r in range(explicit_replicas)])
     for r in range(explicit_replicas):
         g.add_star(A, B, arms, 10.0, r=r, coil=True)
     g.md_init()
     g.initialize()
+    cl.preshear(g)
     g.run(0, 1 + 5 * S, 0.1)
     g.sync()
     t0 = time.perf_counter()
@@ -213,34 +223,53 @@ def closure_bench(replicas=256, explicit_replicas=16, check_intervals=200):
     el = time.perf_counter() - t0
     explicit_rate = explicit_replicas * n * S / el
     g.close()
-    # accuracy cross-check on one replica: explicit vs closure, same start
-    shear = 0.02
-    e = MPCSystem(make_params(L=(48, 48, 48), n_c=10, shear=shear, step_update=S, md_dt=dt, seed=21))
-    e.add_star(A, B, arms, 10.0, coil=True)
+
+    # accuracy in steady shear flow: ensembles of replicas, explicit vs closure
+    def ensemble(advance, gyr, R):
+        advance(check_warmup)
+        acc = []
+        for _ in range(check_intervals // 10):
+            advance(10)
+            acc.append([cl.shape_stats(gyr(r)) for r in range(R)])
+        pm = np.array(acc).mean(axis=0)  # per replica (Rg^2, chi)
+        return pm.mean(axis=0), pm.std(axis=0, ddof=1) / np.sqrt(R)
+
+    R = check_replicas
+    e = MPCSystem([make_params(L=(L, L, L), n_c=10, shear=check_shear, step_update=S, md_dt=dt, seed=100 + r)
+                   for r in range(R)])
+    for r in range(R):
+        e.add_star(A, B, arms, 10.0, r=r, coil=True)
     e.md_init()
     e.initialize()
-    c = cl.ClosureRunner(net, data.fmin, data.fmax, shear=shear, arms=arms, A=A, B=B, L=48, seed=21)
-    ge, gc = [], []
-    t = 0
-    for k in range(check_intervals // 10):
-        e.run(t, 10 * S + (1 if t == 0 else 0), 0.1)
-        t += 10 * S + (1 if t == 0 else 0)
-        c.advance(10)
-        if k >= check_intervals // 40:
-            ge.append(cl.shape_stats(e.gyration_tensor()))
-            gc.append(cl.shape_stats(c.gyration()))
-    ge, gc = np.array(ge), np.array(gc)
+    cl.preshear(e)
+    t = [0]
+
+    def adv_e(k):
+        e.run(t[0], k * S + (1 if t[0] == 0 else 0), 0.1)
+        t[0] += k * S + (1 if t[0] == 0 else 0)
+
+    em, es = ensemble(adv_e, e.gyration_tensor, R)
+    e.close()
+    c = cl.ClosureRunner(net, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=L, shears=[check_shear] * R,
+                         seed=500)
+    cm, cs = ensemble(lambda k: c.advance(k, chunk=k), c.gyration, R)
     return {
         "star_steps_per_s": closure_rate, "explicit_star_steps_per_s": explicit_rate,
         "speedup": closure_rate / explicit_rate,
         "workload": f"{replicas} stars f={arms} {A}A+{B}B (1,201 monomers) closure vs {explicit_replicas} stars "
-                    f"explicit MPC L=48 rho=10, shear U[5e-3,5e-2], {n} intervals of {S} MD steps",
+                    f"explicit MPC L={L} rho=10, shear U[{lo},{hi}], {n} intervals of {S} MD steps",
         "fallbacks": fallbacks,
-        "train": {"samples": data.n, "shear_rates": 8, "steps": 300, "batch": 65536, "seconds": t_fit,
+        "train": {"samples": data.n, "shear_rates": 8, "steps": steps, "batch": 65536,
+                  "generate_seconds": t_gen, "fit_seconds": t_fit,
                   "loss_first": float(hist[0]), "loss_last": float(np.mean(hist[-10:]))},
-        "check": {"shear": shear, "intervals": check_intervals,
-                  "explicit_Rg2": float(ge[:, 0].mean()), "closure_Rg2": float(gc[:, 0].mean()),
-                  "explicit_chi": float(ge[:, 1].mean()), "closure_chi": float(gc[:, 1].mean())},
+        "check": {"shear": check_shear, "replicas": R, "warmup_intervals": check_warmup,
+                  "intervals": check_intervals, "steady_flow": True,
+                  "explicit_Rg2": float(em[0]), "explicit_Rg2_se": float(es[0]),
+                  "closure_Rg2": float(cm[0]), "closure_Rg2_se": float(cs[0]),
+                  "explicit_chi": float(em[1]), "explicit_chi_se": float(es[1]),
+                  "closure_chi": float(cm[1]), "closure_chi_se": float(cs[1]),
+                  "Rg2_ratio": float(cm[0] / em[0]), "chi_ratio": float(cm[1] / em[1]),
+                  "closure_fallbacks": c.fallbacks},
     }
 
 

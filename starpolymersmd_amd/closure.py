@@ -208,13 +208,29 @@ def _star_system(L, n_c, shear, seed, arms, A, B, mass, step_update, md_dt):
     return g
 
 
+def preshear(system):
+    """Add the steady Couette profile v_x += shear y to every replica's
+    solvent.  The reference starts the solvent at rest (MPC::initialize);
+    the Lees-Edwards boundaries then set up the flow by momentum diffusion,
+    which takes about L^2 / (pi^2 nu) to reach the centre of the box (270
+    time units, 2,700 collision intervals at L=48): until then a star at the
+    centre sits in still solvent.  Starting from the profile skips that
+    transient."""
+    for r, p in enumerate(system.params):
+        pos, vel = system.get_fluid(r)
+        vel[:, 0] += np.float32(p.shear) * pos[:, 1]
+        system.set_fluid(pos, vel, r)
+
+
 def generate(shears, L=32, arms=10, A=5, B=5, mass=10.0, intervals=400, warmup=50, n_c=10, step_update=10,
-             md_dt=0.01, seed=11) -> ClosureData:
+             md_dt=0.01, seed=11, presheared=True) -> ClosureData:
     """Explicit MPC-MD runs, one per shear rate: after `warmup` intervals,
     each interval records every monomer's features just before the
     collision and its velocity change by the collision (intervals whose
     collision recentred the star are skipped: the recentring subtracts the
-    star's COM velocity)."""
+    star's COM velocity).  presheared: the solvent starts from the steady
+    flow profile (preshear), so the data show the star in developed shear
+    flow rather than in the start-up transient."""
     h = md_dt * step_update
     feats, targs = [], []
     t_start = time.perf_counter()
@@ -222,6 +238,8 @@ def generate(shears, L=32, arms=10, A=5, B=5, mass=10.0, intervals=400, warmup=5
     for si, shear in enumerate(shears):
         g = _star_system(L, n_c, shear, seed + 101 * si, arms, A, B, mass, step_update, md_dt)
         g.initialize()
+        if presheared:
+            preshear(g)
         t = 0
         g.run(t, 1 + warmup * step_update, h)
         t += 1 + warmup * step_update
@@ -280,7 +298,7 @@ class ClosureRunner:
 
     def __init__(self, net: ClosureNet, fmin, fmax, shear, arms, A, B, mass=10.0, replicas=1, L=48,
                  step_update=10, md_dt=0.01, seed=5, shears=None, fallback_intervals=20, n_c_fallback=10,
-                 box=64):
+                 box=128):
         """L: box of the explicit fallback runs; box: the closure systems'
         box, which without solvent only frames the Verlet grid (it must
         hold the stretched star: monomers within box/2 of its centre)."""
@@ -313,13 +331,22 @@ class ClosureRunner:
     def _explicit(self, r):
         """Run the replica's star through explicit MPC intervals, then resume."""
         arms, A, B, mass = self.star
-        g = MPCSystem(make_params(L=(self.L,) * 3, n_c=self.n_c_fallback, shear=self.shears[r],
+        pos, vel, _ = self.sys.get_monomers(r)
+        com = pos.mean(axis=0)
+        pos = pos - com
+        vel = vel.copy()
+        vel[:, 0] -= self.shears[r] * com[1]  # same velocities relative to the local flow
+        # a box that holds the star with room to move: monomers must stay
+        # inside the Verlet grid (|x| < L/2 - 1/2, MPC::check_bounds)
+        reach = float(np.abs(pos).max()) if len(pos) else 0.0
+        L = max(self.L, 2 * int(math.ceil(reach + 4.0)))
+        g = MPCSystem(make_params(L=(L,) * 3, n_c=self.n_c_fallback, shear=self.shears[r],
                                   step_update=self.S, md_dt=self.dt, seed=self.seed + 1000 + self.fallbacks))
         g.add_star(A, B, arms, mass, coil=True)
-        pos, vel, _ = self.sys.get_monomers(r)
-        g.set_monomers(pos - pos.mean(axis=0), vel)
+        g.set_monomers(pos, vel)
         g.md_init()
         g.initialize()
+        preshear(g)  # the flow the closure stands for
         h = self.S * self.dt
         g.run(0, 1 + self.fallback_intervals * self.S, h)
         p2, v2, _ = g.get_monomers()
