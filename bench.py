@@ -62,12 +62,17 @@ def measured_traffic():
     return None
 
 
-def cpu_baseline(w, intervals, gpu_star):
-    """FP64 oracle (faithful single-threaded restatement of the reference
-    CPU path) on the same workload shape for a bounded number of intervals."""
+def cpu_baseline(w, intervals, gpu_star, threads=1):
+    """FP64 oracle (faithful restatement of the reference CPU path) on the
+    same workload shape for a bounded number of intervals.  threads=1 is
+    the reference's own single-threaded schedule; more threads run the
+    oracle's solvent loops (streaming, shift, cell index, per-cell
+    collision) with OpenMP, the sort's push order and the MD staying
+    sequential."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_binding as ob
 
+    ob.set_threads(threads)
     o = ob.Oracle(L=w["L"], n_c=w["n_c"], shear=w["shear"], step_update=w["step_update"], md_dt=w["md_dt"],
                   rng_mode=ob.RNG_PHILOX, seed=w["seed"])
     o.add_star(w["A"], w["B"], w["arms"], w["mass"])  # topology; positions/velocities from the library
@@ -87,9 +92,10 @@ def cpu_baseline(w, intervals, gpu_star):
             t += 1
     el = time.perf_counter() - t0
     n = o.n_fluid
-    return {"value": n * intervals / el, "unit": "particle-updates/s", "cores": 1, "kind": "port",
+    used = ob.get_threads()
+    return {"value": n * intervals / el, "unit": "particle-updates/s", "cores": used, "kind": "port",
             "sample": f"FP64 oracle, L=64 rho=10 f=20 15A+15B, {intervals} collision intervals "
-                      f"(10 MD substeps each), 1 thread, {el:.1f} s"}
+                      f"(10 MD substeps each), {used} thread{'s' if used > 1 else ''}, {el:.1f} s"}
 
 
 def batch_workload(n_rep=64, seed=2024):
@@ -377,6 +383,9 @@ def main():
         out["closure"] = closure_bench()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_intervals, star0)
+        # all the host threads this process may use (OMP_NUM_THREADS, else the CPUs it sees)
+        nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+        out["cpu_baseline_threads"] = cpu_baseline(w, 3 * a.cpu_intervals, star0, threads=nthr)
     print(json.dumps(out), flush=True)
 
 

@@ -22,6 +22,7 @@
  * correctly rounded IEEE operation, which the bit-exact cell-index tests
  * rely on.
  */
+#include <omp.h>
 #include "oracle.h"
 
 #include <algorithm>
@@ -407,11 +408,14 @@ int sort_cells(orc_sys& s) {
     for (auto& l : s.cell_list) l.clear();
     std::fill(s.cell_fluid_count.begin(), s.cell_fluid_count.end(), 0u);
     const int Lx = s.L[0], Ly = s.L[1], Lz = s.L[2];
+#pragma omp parallel for schedule(static)
     for (uint64_t p = 0; p < s.n_fluid; ++p) {
         const double* x = &s.pos[3 * p];
-        uint32_t ci = (uint32_t)((int)(x[0] + Lx * 0.5) + Lx * (int)(x[1] + Ly * 0.5) +
-                                 Lx * Ly * (int)(x[2] + Lz * 0.5));
-        s.fcell[p] = ci;
+        s.fcell[p] = (uint32_t)((int)(x[0] + Lx * 0.5) + Lx * (int)(x[1] + Ly * 0.5) +
+                                Lx * Ly * (int)(x[2] + Lz * 0.5));
+    }
+    for (uint64_t p = 0; p < s.n_fluid; ++p) {  /* in particle order, as the reference pushes */
+        const uint32_t ci = s.fcell[p];
         if (ci > s.number_of_cells) return -1;
         s.cell_list[ci].push_back((int64_t)p);
         s.cell_fluid_count[ci]++;
@@ -588,6 +592,7 @@ void collide(orc_sys& s, uint32_t cell, const V3& cmv) {
 
 /* MPC::shiftParticles, MPC.cpp:301-311. */
 void shift_particles(orc_sys& s, const V3& sh) {
+#pragma omp parallel for schedule(static)
     for (uint64_t p = 0; p < s.n_fluid; ++p) {
         double* x = &s.pos[3 * p];
         for (int i = 0; i < 3; ++i) x[i] += sh[i];
@@ -657,6 +662,12 @@ void orc_fluid_initialize(orc_sys* s) {
 }
 
 uint64_t orc_num_fluid(const orc_sys* s) { return s->n_fluid; }
+
+/* OpenMP threads of the streaming, shift, cell-index and per-cell loops
+ * (counter-based randomness only; the reference's generator keeps them
+ * sequential); results do not depend on it */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
 
 void orc_set_fluid(orc_sys* s, const double* pos, const double* vel, uint64_t n) {
     s->n_fluid = n;
@@ -795,6 +806,8 @@ int orc_mpc_step(orc_sys* s, int64_t t, double dt) {
     s->delrx -= s->L[0] * std::round(s->delrx / s->L[0]);
     if (t % s->p.step_update) return 0;
     check_bounds(*s);
+    const bool par = s->p.rng_mode != ORC_RNG_REFERENCE;  /* mt19937 draws stay in order */
+#pragma omp parallel for schedule(static) if (par)
     for (uint64_t p = 0; p < s->n_fluid; ++p) {  /* MPC::streaming, MPC.cpp:192-197 */
         double* x = &s->pos[3 * p];
         double* v = &s->vel[3 * p];
@@ -824,6 +837,9 @@ int orc_mpc_step(orc_sys* s, int64_t t, double dt) {
     s->last_shift = sh;
     shift_particles(*s, sh);
     if (sort_cells(*s)) return -1;
+    /* cells are disjoint and, with counter-based randomness, independent:
+     * any thread order gives the same result */
+#pragma omp parallel for schedule(dynamic, 1024) if (par)
     for (uint32_t cell = 0; cell <= s->number_of_cells; ++cell) {
         V3 cmv = calc_cmv(*s, cell);
         thermostat(*s, cell, cmv);

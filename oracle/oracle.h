@@ -51,6 +51,10 @@ void     orc_destroy(orc_sys* s);
 /* MPC::initialize() (MPC.cpp:34-55). */
 void     orc_fluid_initialize(orc_sys* s);
 uint64_t orc_num_fluid(const orc_sys* s);
+/* OpenMP threads of the solvent loops in counter-based random modes (the
+ * results do not depend on the count) */
+void orc_set_threads(int n);
+int orc_get_threads(void);
 void     orc_set_fluid(orc_sys* s, const double* pos, const double* vel, uint64_t n);
 void     orc_get_fluid(const orc_sys* s, double* pos, double* vel);
 

@@ -43,6 +43,8 @@ _SIGS = {
     "orc_md_propagate": (C.c_int, [C.c_void_p, C.c_int]),
     "orc_epot": (C.c_double, [C.c_void_p]),
     "orc_mpc_step": (C.c_int, [C.c_void_p, C.c_int64, C.c_double]),
+    "orc_set_threads": (None, [C.c_int]),
+    "orc_get_threads": (C.c_int, []),
     "orc_inject": (None, [C.c_void_p, _d, _d, _d]),
     "orc_get_delrx": (C.c_double, [C.c_void_p]),
     "orc_set_delrx": (None, [C.c_void_p, C.c_double]),
@@ -292,3 +294,13 @@ def ref_lib():
     l.ref_rng_sequence.restype = None
     l.ref_rng_sequence.argtypes = [C.POINTER(C.c_int), _d, _d, C.c_int, _d]
     return l
+
+
+def set_threads(n: int):
+    """OpenMP threads of the oracle's solvent loops (Philox and injected
+    modes; results do not depend on it)."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().orc_get_threads())

@@ -375,3 +375,25 @@ def test_amc_collision_conserves_cell_angular_momentum(amc):
         assert dL_max < 1e-9
     else:
         assert dL_max > 1e-2
+
+
+def test_oracle_threads_do_not_change_results():
+    """The oracle's solvent loops run on OpenMP threads in the Philox mode;
+    cells and particles are independent there, so one thread and many give
+    the same state bit for bit (star, shear and the angular-momentum
+    collision included)."""
+    states = []
+    for n in (1, 4):
+        ob.set_threads(n)
+        o = ob.Oracle(L=(14, 12, 16), n_c=10, shear=0.05, step_update=5, seed=9, amc=True)
+        o.add_star(2, 2, 6, 10.0)
+        assert o.md_init() == 0
+        o.initialize()
+        for t in range(26):
+            o.propagate()
+            assert o.step(t, 0.1) == 0
+        p, v = o.get_fluid()
+        pm, vm = o.get_monomers()[:2]
+        states.append((p.copy(), v.copy(), pm.copy(), vm.copy()))
+    ob.set_threads(0)
+    assert all(np.array_equal(a, b) for a, b in zip(*states))
