@@ -306,7 +306,7 @@ def main():
         print(json.dumps({"copy_probe_mib": a.copy_probe, "bytes_read": x.numel(), "bytes_written": y.numel()}))
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the MPC box would be "
+        print("bench.py measures one GPU: the library has no multi-GPU path (the MPC box would be "
               "slab-decomposed along z with per-face halo sums; not built). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)

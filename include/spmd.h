@@ -37,7 +37,7 @@ enum {
     SPMD_ERR_CELL_OOB = 4,       /* MPC::sort cell index out of range (MPC.cpp:212-219,228-233 raise SIGSEGV) */
     SPMD_ERR_VERLET_OOB = 5,     /* Box::update_VerletLists grid bounds (Box.cpp:133-142 raise SIGSEGV) */
     SPMD_ERR_VERLET_OVERFLOW = 6,/* a monomer has more Verlet neighbours than the list capacity */
-    SPMD_ERR_UNSUPPORTED = 7,    /* an option this build does not provide */
+    SPMD_ERR_UNSUPPORTED = 7,    /* an option this library does not provide */
     SPMD_ERR_STATE = 8,          /* call out of order */
     SPMD_ERR_RANK_OVERFLOW = 9,  /* more than 65535 particles in one cell */
     SPMD_ERR_NO_DEVICE = 10,     /* no usable gfx950 device */
@@ -251,7 +251,8 @@ int spmd_profile_reset(spmd_sys* s);
  * features of a monomer's state, the mean and diagonal variance of the
  * velocity change the solvent imparts at one collision; the star then
  * advances by its own velocity-Verlet MD plus that stochastic kick, with no
- * solvent particles.  Explicit MPC stays the fallback and the judge. */
+ * solvent particles.  Explicit MPC stays the fallback and the standard the
+ * closure is checked against. */
 
 /* The network: MLP 24-128-128-128-6, SiLU, BF16 matrix-core kernels with
  * FP32 accumulation, Adam.  Pointers named *_dev are device pointers; every
