@@ -215,7 +215,8 @@ def preshear(system):
     which takes about L^2 / (pi^2 nu) to reach the centre of the box (270
     time units, 2,700 collision intervals at L=48): until then a star at the
     centre sits in still solvent.  Starting from the profile skips that
-    transient."""
+    transient.  The solvent goes back through set_fluid, which numbers the
+    particle ids afresh in storage order."""
     for r, p in enumerate(system.params):
         pos, vel = system.get_fluid(r)
         vel[:, 0] += np.float32(p.shear) * pos[:, 1]
