@@ -317,6 +317,10 @@ class ClosureRunner:
         self.fallback_intervals = fallback_intervals
         self.n_c_fallback = n_c_fallback
         self.fallbacks = 0
+        # explicit fallback systems kept for reuse, by (box, shear rate):
+        # their solvent stays in developed flow between fallbacks
+        self._pool = {}
+        self.pool_size = 4
         self.t = 0
 
     def advance(self, n_intervals: int, chunk: int = 10):
@@ -341,20 +345,40 @@ class ClosureRunner:
         # inside the Verlet grid (|x| < L/2 - 1/2, MPC::check_bounds)
         reach = float(np.abs(pos).max()) if len(pos) else 0.0
         L = max(self.L, 2 * int(math.ceil(reach + 4.0)))
-        g = MPCSystem(make_params(L=(L,) * 3, n_c=self.n_c_fallback, shear=self.shears[r],
-                                  step_update=self.S, md_dt=self.dt, seed=self.seed + 1000 + self.fallbacks))
-        g.add_star(A, B, arms, mass, coil=True)
-        g.set_monomers(pos, vel)
-        g.md_init()
-        g.initialize()
-        preshear(g)  # the flow the closure stands for
         h = self.S * self.dt
-        g.run(0, 1 + self.fallback_intervals * self.S, h)
+        key = (L, float(self.shears[r]))
+        entry = self._pool.pop(key, None)
+        if entry is None:
+            g = MPCSystem(make_params(L=(L,) * 3, n_c=self.n_c_fallback, shear=self.shears[r], step_update=self.S,
+                                      md_dt=self.dt, seed=self.seed + 1000 + self.fallbacks))
+            g.add_star(A, B, arms, mass, coil=True)
+            g.set_monomers(pos, vel)
+            g.md_init()
+            g.initialize()
+            preshear(g)  # the flow the closure stands for
+            g.run(0, 1 + self.fallback_intervals * self.S, h)
+            t = 1 + self.fallback_intervals * self.S
+        else:
+            # a solvent already in steady flow: put this star in its centre
+            g, t = entry
+            g.set_monomers(pos, vel)
+            g.md_init()
+            g.run(t, self.fallback_intervals * self.S, h)
+            t += self.fallback_intervals * self.S
         p2, v2, _ = g.get_monomers()
-        g.close()
+        self._pool[key] = (g, t)  # most recently used last
+        while len(self._pool) > self.pool_size:
+            old = next(iter(self._pool))
+            self._pool.pop(old)[0].close()
         self.sys.set_monomers(p2, v2, r=r)
         self.sys.md_init()
         self.fallbacks += 1
 
     def gyration(self, r=0):
         return self.sys.gyration_tensor(r)
+
+    def close(self):
+        """Release the pooled fallback systems."""
+        for g, _ in self._pool.values():
+            g.close()
+        self._pool.clear()
