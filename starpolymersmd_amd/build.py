@@ -48,14 +48,18 @@ def build_library():
 
 
 def build_facade():
-    """The reference-style C++ driver over the facade, linked to libspmd.so."""
+    """The reference-style C++ drivers over the facade (star-polymers and
+    mpc-test), linked to libspmd.so."""
     fdir = os.path.join(HERE, "facade")
-    out = os.path.join(fdir, "star_polymers_run")
     gxx = shutil.which("g++") or "g++"
-    _run([gxx, "-O2", "-std=c++17", "-Wall", os.path.join(fdir, "star_polymers_run.cpp"), "-o", out + ".tmp",
-          "-L" + HERE, "-lspmd", "-Wl,-rpath,$ORIGIN/.."])
-    os.replace(out + ".tmp", out)
-    return out
+    outs = []
+    for name in ("star_polymers_run", "mpc_test_run"):
+        out = os.path.join(fdir, name)
+        _run([gxx, "-O2", "-std=c++17", "-Wall", os.path.join(fdir, name + ".cpp"), "-o", out + ".tmp",
+              "-L" + HERE, "-lspmd", "-Wl,-rpath,$ORIGIN/.."])
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+    return outs
 
 
 def build_oracle():

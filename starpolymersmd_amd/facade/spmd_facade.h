@@ -30,6 +30,8 @@
 
 #include <array>
 #include <cmath>
+#include <complex>
+#include <deque>
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -313,6 +315,111 @@ public:
             }
             if (a == 0) os << '\n';
         }
+        return os;
+    }
+};
+
+/* VelocityDistribution (Vel_distribution.h:17-57): speeds binned by
+ * floor(|v| / width) (the device histogram holds 1024 bins and counts the
+ * speeds beyond them, which print_result leaves out but normalises by). */
+class VelocityDistribution {
+    static constexpr uint32_t kBins = 1024;
+    double width;
+    std::vector<uint64_t> counts;
+
+public:
+    explicit VelocityDistribution(double a_width = 0.5) : width(a_width), counts(kBins + 1, 0) {}
+    void sample(spmd_sys* h) {
+        std::vector<uint64_t> c(kBins + 1);
+        check(spmd_speed_histogram(h, 0, width, kBins, c.data()), h);
+        for (uint32_t b = 0; b <= kBins; ++b) counts[b] += c[b];
+    }
+    /* VelocityDistribution::print_result Vel_distribution.h:43-56 */
+    std::ostream& print_result(std::ostream& os) {
+        uint64_t total = 0;
+        for (uint64_t c : counts) total += c;
+        os.precision(8);
+        os << std::scientific;
+        for (uint32_t b = 0; b < kBins; ++b)
+            if (counts[b]) os << b * width << ' ' << (double)counts[b] / ((double)total * width) << '\n';
+        os << '\n';
+        return os;
+    }
+};
+
+/* Fourier_Autocorr (Fourier_Autocorr.h:19-127): the transverse velocity
+ * modes V_j at k = 2 pi / L e_j (one device pass per sample,
+ * spmd_fourier_modes) in a window of Entries samples; from the window's
+ * filling on, every sample adds Re(V_j(oldest)^H V_j(oldest + i)) to lag i.
+ * Sums in double (the reference's are long double). */
+class Fourier_Autocorr {
+    using Modes = std::array<std::complex<double>, 9>;
+    unsigned Entries;
+    double DeltaT;
+    unsigned EntryCounter = 0, Counter = 0;
+    std::deque<Modes> Velocities;
+    std::array<std::vector<double>, 3> Autocorrelation;
+
+public:
+    Fourier_Autocorr(unsigned /*N_p*/, unsigned N, double dt, double /*L*/)
+        : Entries(N), DeltaT(dt), Velocities(N, Modes{}) {
+        for (auto& a : Autocorrelation) a.assign(N, 0.0);
+    }
+    void sample(spmd_sys* h) {
+        double m[18];
+        check(spmd_fourier_modes(h, 0, m), h);
+        Modes v;
+        for (int q = 0; q < 9; ++q) v[q] = std::complex<double>(m[2 * q], m[2 * q + 1]);
+        Velocities.pop_front();
+        Velocities.push_back(v);
+        if (EntryCounter < Entries) EntryCounter++;
+        if (EntryCounter == Entries) {  /* calculate(), Fourier_Autocorr.h:80-93 */
+            const Modes& first = Velocities.front();
+            for (unsigned i = 0; i < 3; ++i)
+                for (unsigned j = 0; j < Entries; ++j) {
+                    double d = 0.0;
+                    for (int c = 0; c < 3; ++c) d += (std::conj(first[3 * i + c]) * Velocities[j][3 * i + c]).real();
+                    Autocorrelation[i][j] += d;
+                }
+            Counter++;
+        }
+    }
+    /* Fourier_Autocorr::print_result Fourier_Autocorr.h:113-119 */
+    std::ostream& print_result(std::ostream& os) const {
+        for (unsigned i = 0; i < Entries; ++i)
+            os << i * DeltaT << " " << Autocorrelation[0][i] / Counter << " " << Autocorrelation[1][i] / Counter << " "
+               << Autocorrelation[2][i] / Counter << "\n";
+        return os;
+    }
+};
+
+/* Vel_Autocorr_Fluid (Vel_Autocorr_Fluid.h:19-116): every solvent
+ * particle's window of Entries velocities and the lag sums live on the
+ * device (spmd_fluid_vacf_*); particles are followed by id, so the first
+ * sample turns tracking on. */
+class Vel_Autocorr_Fluid {
+    unsigned Entries, NumberOfParticles;
+    double DeltaT;
+    spmd_sys* h_ = nullptr;
+
+public:
+    Vel_Autocorr_Fluid(unsigned N_p, unsigned N, double dt) : Entries(N), NumberOfParticles(N_p), DeltaT(dt) {}
+    void sample(spmd_sys* h) {
+        if (!h_) check(spmd_set_tracking(h, 1), h);
+        h_ = h;
+        check(spmd_fluid_vacf_sample(h, 0, Entries), h);
+    }
+    /* calculate_average (:80-88), value (:90-97), print_result (:99-106) */
+    std::ostream& print_result(std::ostream& os) const {
+        std::vector<double> sums(Entries, 0.0), avg(Entries, 0.0);
+        uint64_t counter = 0;
+        if (h_) check(spmd_fluid_vacf_read(h_, 0, sums.data(), &counter), h_);
+        for (unsigned i = 0; i < Entries; ++i) avg[i] = sums[i] / (double)counter / (double)NumberOfParticles;
+        double diffusion = 0.0;
+        for (double a : avg) diffusion += a * DeltaT / (double)counter;
+        diffusion /= 3.;
+        os << "#Diffusion coefficient = " << diffusion << "\n";
+        for (unsigned i = 0; i < Entries; ++i) os << i * DeltaT << " " << avg[i] << "\n";
         return os;
     }
 };

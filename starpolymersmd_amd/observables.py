@@ -215,8 +215,11 @@ class FluidVelocityAutocorr:
         """value() (:90-97) as the reference computes it: the sum of the
         averaged lags times dt, divided by Counter once more and by 3."""
         avg, n = self.average(), self.counter()
+        d = 0.0
         with np.errstate(divide="ignore", invalid="ignore"):
-            return float(np.sum(avg * self.dt / np.float64(n)) / 3.0)
+            for a in avg:  # in order, as the reference's loop
+                d += float(np.float64(a) * self.dt / np.float64(n))
+        return d / 3.0
 
     def print_result(self, os: TextIO):
         """print_result (:99-106)."""

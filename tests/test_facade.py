@@ -79,3 +79,32 @@ def test_driver_matches_python_binding(tmp_path):
         sorted((tmp_path / "py_fluid.dat").read_text().split("\n"))
     g.write_pdb(tmp_path / "py_config.pdb", steps)
     assert (tmp_path / "end_config.pdb").read_text() == (tmp_path / "py_config.pdb").read_text()
+
+
+MPC_DRIVER = os.path.join(FDIR, "mpc_test_run")
+
+
+@pytest.mark.gpu
+def test_mpc_test_driver_matches_python_driver(tmp_path):
+    """mpc_test.cpp over the facade (C++) and observables.run_mpc_test
+    (Python) with the same parameters write the same five files byte for
+    byte: temperature, VelocityX profile, speed distribution, fluid VACF
+    and transverse-mode autocorrelation."""
+    from starpolymersmd_amd import observables as ob
+
+    cpp = tmp_path / "cpp"
+    py = tmp_path / "py"
+    cpp.mkdir()
+    py.mkdir()
+    args = ["10", "0.5", "6", "6", "6", "0.3", "0.1", "100", "1200", "100", "0"]
+    r = subprocess.run([MPC_DRIVER, *args], capture_output=True, text=True, timeout=120, cwd=cpp)
+    assert r.returncode == 0, r.stderr
+    g = MPCSystem(make_params(L=(6, 6, 6), kT=0.5, n_c=10, shear=0.3, step_update=1, seed=1, lam=1.0, md_dt=0.01))
+    g.set_tracking(True)
+    g.initialize()
+    names = ob.run_mpc_test(g, 100, 1200, 100, 0.1, str(py))
+    assert sorted(os.listdir(cpp)) == sorted(os.listdir(py)) == sorted(os.path.basename(p) for p in names.values())
+    for f in os.listdir(py):
+        a, b = (cpp / f).read_text(), (py / f).read_text()
+        assert a == b, (f, a[:300], b[:300])
+    assert (py / "velocity_N2160_Shear0.3_h0.1.dat").read_text().startswith("#Diffusion coefficient = ")
