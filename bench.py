@@ -174,7 +174,7 @@ def batch_bench(intervals=100, seq_intervals=10):
 
 
 def closure_bench(replicas=256, explicit_replicas=16, check_replicas=16, check_shear=0.012, check_warmup=400,
-                  check_intervals=800):
+                  check_intervals=800, check_box=80):
     """Learned closure at batch (256 stars, f=30 arms of 20 A + 20 B beads,
     1,201 monomers, shear rates uniform in [4e-3, 1.6e-2]) against explicit
     MPC-MD of the same star at L=64, rho=10 on the same GPU (the smallest
@@ -231,17 +231,26 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=16, check_s
     g.close()
 
     # accuracy in steady shear flow: ensembles of replicas, explicit vs closure
-    def ensemble(advance, gyr, R):
+    def ensemble(advance, sysm, R):
+        """per-replica time averages of Rg^2, alignment angle and the
+        rotation frequency about z (Box::calculate_rotation_frequency);
+        ensemble means and standard errors, and the wall time"""
+        t0 = time.perf_counter()
         advance(check_warmup)
         acc = []
         for _ in range(check_intervals // 10):
             advance(10)
-            acc.append([cl.shape_stats(gyr(r)) for r in range(R)])
-        pm = np.array(acc).mean(axis=0)  # per replica (Rg^2, chi)
-        return pm.mean(axis=0), pm.std(axis=0, ddof=1) / np.sqrt(R)
+            acc.append([(*cl.shape_stats(sysm.gyration_tensor(r)), sysm.rotation_frequency(r)[2]) for r in range(R)])
+        sysm.sync()
+        el = time.perf_counter() - t0
+        pm = np.array(acc).mean(axis=0)  # per replica (Rg^2, chi, w_z)
+        return pm.mean(axis=0), pm.std(axis=0, ddof=1) / np.sqrt(R), el
 
     R = check_replicas
-    e = MPCSystem([make_params(L=(L, L, L), n_c=10, shear=check_shear, step_update=S, md_dt=dt, seed=100 + r)
+    # the stars stretch over the check: a box whose Verlet grid holds them
+    # (a monomer leaving it ends the run, as the reference's SIGSEGV does)
+    Le = check_box
+    e = MPCSystem([make_params(L=(Le, Le, Le), n_c=10, shear=check_shear, step_update=S, md_dt=dt, seed=100 + r)
                    for r in range(R)])
     for r in range(R):
         e.add_star(A, B, arms, 10.0, r=r, coil=True)
@@ -254,11 +263,13 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=16, check_s
         e.run(t[0], k * S + (1 if t[0] == 0 else 0), 0.1)
         t[0] += k * S + (1 if t[0] == 0 else 0)
 
-    em, es = ensemble(adv_e, e.gyration_tensor, R)
+    em, es, et = ensemble(adv_e, e, R)
     e.close()
     c = cl.ClosureRunner(net, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=L, shears=[check_shear] * R,
                          seed=500)
-    cm, cs = ensemble(lambda k: c.advance(k, chunk=k), c.gyration, R)
+    cm, cs, ct = ensemble(lambda k: c.advance(k, chunk=k), c.sys, R)
+    # time to the same standard error of Rg^2 (the error falls as 1/sqrt(time))
+    ttc = (et * es[0] ** 2) / (ct * cs[0] ** 2)
     return {
         "star_steps_per_s": closure_rate, "explicit_star_steps_per_s": explicit_rate,
         "speedup": closure_rate / explicit_rate,
@@ -268,13 +279,18 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=16, check_s
         "train": {"samples": data.n, "shear_rates": 8, "steps": steps, "batch": 65536,
                   "generate_seconds": t_gen, "fit_seconds": t_fit,
                   "loss_first": float(hist[0]), "loss_last": float(np.mean(hist[-10:]))},
-        "check": {"shear": check_shear, "replicas": R, "warmup_intervals": check_warmup,
+        "check": {"shear": check_shear, "replicas": R, "explicit_box": Le, "warmup_intervals": check_warmup,
                   "intervals": check_intervals, "steady_flow": True,
                   "explicit_Rg2": float(em[0]), "explicit_Rg2_se": float(es[0]),
                   "closure_Rg2": float(cm[0]), "closure_Rg2_se": float(cs[0]),
                   "explicit_chi": float(em[1]), "explicit_chi_se": float(es[1]),
                   "closure_chi": float(cm[1]), "closure_chi_se": float(cs[1]),
+                  "explicit_wz": float(em[2]), "explicit_wz_se": float(es[2]),
+                  "closure_wz": float(cm[2]), "closure_wz_se": float(cs[2]),
                   "Rg2_ratio": float(cm[0] / em[0]), "chi_ratio": float(cm[1] / em[1]),
+                  "wz_ratio": float(cm[2] / em[2]),
+                  "explicit_seconds": et, "closure_seconds": ct,
+                  "time_to_equal_Rg2_error_speedup": float(ttc),
                   "closure_fallbacks": c.fallbacks},
     }
 

@@ -232,12 +232,13 @@ def test_preshear_sets_the_couette_profile():
 def test_closure_tracks_explicit_in_shear_flow():
     """The 1,201-monomer star (f=30, 20A+20B) in steady shear flow at
     0.012: the network trained on explicit runs started from the flow
-    profile (8 rates in [4e-3, 1.6e-2]); 16 replicas each way from coils,
-    400 intervals of warm-up, then Rg^2 and the alignment angle averaged
-    over 800 intervals.  Bar: the closure's ensemble means within 3
-    combined standard errors (replica spread) of explicit MPC's, and within
-    8% -- the stars are still stretching over this window, and the closure
-    follows the same start-up."""
+    profile (8 rates in [4e-3, 1.6e-2]); 16 replicas each way from coils
+    (explicit at L=80, whose Verlet grid holds the stretched stars), 400
+    intervals of warm-up, then Rg^2, the alignment angle and the rotation
+    frequency about z averaged over 800 intervals.  Bar: the closure's
+    ensemble means within 3 combined standard errors (replica spread) of
+    explicit MPC's, and within 8% -- the stars are still stretching over
+    this window, and the closure follows the same start-up."""
     from starpolymersmd_amd import closure as cl
 
     arms, A, B, S, L, shear, R = 30, 20, 20, 10, 64, 0.012, 16
@@ -245,16 +246,17 @@ def test_closure_tracks_explicit_in_shear_flow():
     net = cl.ClosureNet(1)
     net.fit(data, steps=2000, batch=65536, lr=2e-3)
 
-    def ensemble(advance, gyr):
+    def ensemble(advance, sysm):
         advance(400)
         acc = []
         for _ in range(80):
             advance(10)
-            acc.append([cl.shape_stats(gyr(r)) for r in range(R)])
+            acc.append([(*cl.shape_stats(sysm.gyration_tensor(r)), sysm.rotation_frequency(r)[2]) for r in range(R)])
         pm = np.array(acc).mean(axis=0)
         return pm.mean(axis=0), pm.std(axis=0, ddof=1) / np.sqrt(R)
 
-    e = MPCSystem([make_params(L=(L,) * 3, n_c=10, shear=shear, step_update=S, seed=100 + r) for r in range(R)])
+    # the explicit stars stretch: a box whose Verlet grid holds them (L=80)
+    e = MPCSystem([make_params(L=(80,) * 3, n_c=10, shear=shear, step_update=S, seed=100 + r) for r in range(R)])
     for r in range(R):
         e.add_star(A, B, arms, 10.0, r=r, coil=True)
     e.md_init()
@@ -267,11 +269,13 @@ def test_closure_tracks_explicit_in_shear_flow():
         e.run(t[0], n, 0.1)
         t[0] += n
 
-    em, es = ensemble(adv_e, e.gyration_tensor)
+    em, es = ensemble(adv_e, e)
+    e.sync()  # raises if a monomer left the Verlet grid
     e.close()
     c = cl.ClosureRunner(net, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=L, shears=[shear] * R, seed=500)
-    cm, cs = ensemble(lambda k: c.advance(k, chunk=k), c.gyration)
-    print("\n(Rg^2, chi) explicit", em, "+-", es, "closure", cm, "+-", cs, "fallbacks", c.fallbacks)
+    cm, cs = ensemble(lambda k: c.advance(k, chunk=k), c.sys)
+    c.sys.sync()
+    print("\n(Rg^2, chi, w_z) explicit", em, "+-", es, "closure", cm, "+-", cs, "fallbacks", c.fallbacks)
     se = np.sqrt(es ** 2 + cs ** 2)
     assert np.all(np.abs(cm - em) <= 3 * se), (cm, em, se)
     assert np.all(np.abs(cm / em - 1) < 0.08), (cm, em)
