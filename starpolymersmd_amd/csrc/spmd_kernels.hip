@@ -344,18 +344,26 @@ __device__ __forceinline__ uint32_t md_scan_cands(const MdLds& S, int p, const i
     const int cy = (int)((cid / (uint32_t)gd[0]) % (uint32_t)gd[1]);
     const int cz = (int)(cid / ((uint32_t)gd[0] * (uint32_t)gd[1]));
     const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < gd[0] ? cx + 1 : gd[0] - 1;
-    uint32_t ra0[9], ra1[9];
+    /* half shell: the slots after p up to the end of cell x+1 of its own
+     * row, then the whole x-runs of rows (y+1, z) and (y-1..y+1, z+1):
+     * every pair of the 27-cell neighbourhood once */
+    uint32_t ra0[5], ra1[5];
+    {
+        const int rowc = gd[0] * (cy + gd[1] * cz);
+        ra0[0] = (uint32_t)p + 1u;
+        ra1[0] = S.gcell[rowc + x1];
+    }
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        const int y = cy + r % 3 - 1, z = cz + r / 3 - 1;
-        const bool ok = y >= 0 && y < gd[1] && z >= 0 && z < gd[2];
+    for (int r = 1; r < 5; ++r) {
+        const int y = r == 1 ? cy + 1 : cy + (r - 3), z = r == 1 ? cz : cz + 1;
+        const bool ok = y >= 0 && y < gd[1] && z < gd[2];
         const int rowc = ok ? gd[0] * (y + gd[1] * z) : 0;
         ra0[r] = ok && rowc + x0 ? S.gcell[rowc + x0 - 1] : 0u;
         ra1[r] = ok ? S.gcell[rowc + x1] : 0u;
     }
     uint32_t n = 0;
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
+    for (int r = 0; r < 5; ++r) {
         const uint32_t a1 = ra1[r];
         for (uint32_t q = ra0[r]; q < a1; q += 4) {
             float4 c[4];
@@ -367,8 +375,9 @@ __device__ __forceinline__ uint32_t md_scan_cands(const MdLds& S, int p, const i
                 const int j = (int)(jm & 0xFFFF);
                 const float dx = c[k].x - me.x, dy2 = c[k].y - me.y, dz2 = c[k].z - me.z;
                 const float r2 = __builtin_fmaf(dz2, dz2, __builtin_fmaf(dy2, dy2, dx * dx));
-                const bool hit = q + k < a1 && j > i && r2 <= ((bi && (jm >> 16)) ? kListBB2f : kListAA2f);
-                if (hit && n < (uint32_t)kVlPerMono) scr[n] = (uint32_t)i | ((uint32_t)j << 16);
+                const bool hit = q + k < a1 && r2 <= ((bi && (jm >> 16)) ? kListBB2f : kListAA2f);
+                const uint32_t lo = (uint32_t)(i < j ? i : j), hi = (uint32_t)(i < j ? j : i);
+                if (hit && n < (uint32_t)kVlPerMono) scr[n] = lo | (hi << 16);
                 n += hit ? 1u : 0u;
             }
         }
@@ -380,8 +389,9 @@ __device__ __forceinline__ uint32_t md_scan_cands(const MdLds& S, int p, const i
  * inside the list radius, plus the COM-centred grid bounds check that
  * raises SIGSEGV in the reference.  The monomers are binned on a grid of
  * side >= 2 over their bounding box and sorted by cell; each lane takes
- * the monomers at its sorted slots and writes their pairs to a global
- * scratch row of kVlPerMono entries per slot; after a block scan of the
+ * the monomers at its sorted slots, scans their half shell (md_scan_cands)
+ * and writes their pairs to a global scratch row of kVlPerMono entries per
+ * slot; after a block scan of the
  * counts the rows are packed into the list -- in LDS when it fits there,
  * else in the replica's global region.  Leaves the force accumulators
  * zeroed; the owning threads reset their Verlet reference positions.
