@@ -173,7 +173,7 @@ def batch_bench(intervals=100, seq_intervals=10):
             "speedup": seq_el / el}
 
 
-def closure_bench(replicas=256, explicit_replicas=16, check_replicas=16, check_shear=0.012, check_warmup=400,
+def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_shear=0.012, check_warmup=400,
                   check_intervals=800, check_box=80):
     """Learned closure at batch (256 stars, f=30 arms of 20 A + 20 B beads,
     1,201 monomers, shear rates uniform in [4e-3, 1.6e-2]) against explicit
@@ -181,9 +181,12 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=16, check_s
     box that holds the star stretched by these rates): star-steps/s (MD
     steps x stars per second), the network trained here on explicit runs of
     the library (8 shear rates, solvent started from the steady flow
-    profile), fallbacks included.  Accuracy: 16 replicas each way at shear
-    0.012 in steady flow, Rg^2 and alignment angle averaged over the run
-    after a warm-up, standard errors from the spread of replica means."""
+    profile), fallbacks included.  Accuracy: 64 replicas each way at shear
+    0.012 in steady flow, Rg^2, alignment angle and rotation frequency
+    averaged over the run after a warm-up, standard errors from the spread
+    of replica means; each closure/explicit ratio carries the standard error
+    propagated from both sides (Rg = sqrt(<Rg^2>), so its ratio is the
+    square root of the Rg^2 ratio with half its relative error)."""
     import numpy as np
 
     from starpolymersmd_amd import MPCSystem, make_params
@@ -270,6 +273,8 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=16, check_s
     cm, cs, ct = ensemble(lambda k: c.advance(k, chunk=k), c.sys, R)
     # time to the same standard error of Rg^2 (the error falls as 1/sqrt(time))
     ttc = (et * es[0] ** 2) / (ct * cs[0] ** 2)
+    ratio = cm / em
+    ratio_se = np.abs(ratio) * np.sqrt((cs / cm) ** 2 + (es / em) ** 2)
     return {
         "star_steps_per_s": closure_rate, "explicit_star_steps_per_s": explicit_rate,
         "speedup": closure_rate / explicit_rate,
@@ -287,8 +292,10 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=16, check_s
                   "closure_chi": float(cm[1]), "closure_chi_se": float(cs[1]),
                   "explicit_wz": float(em[2]), "explicit_wz_se": float(es[2]),
                   "closure_wz": float(cm[2]), "closure_wz_se": float(cs[2]),
-                  "Rg2_ratio": float(cm[0] / em[0]), "chi_ratio": float(cm[1] / em[1]),
-                  "wz_ratio": float(cm[2] / em[2]),
+                  "Rg2_ratio": float(ratio[0]), "Rg2_ratio_se": float(ratio_se[0]),
+                  "Rg_ratio": float(np.sqrt(ratio[0])), "Rg_ratio_se": float(0.5 * ratio_se[0] / np.sqrt(ratio[0])),
+                  "chi_ratio": float(ratio[1]), "chi_ratio_se": float(ratio_se[1]),
+                  "wz_ratio": float(ratio[2]), "wz_ratio_se": float(ratio_se[2]),
                   "explicit_seconds": et, "closure_seconds": ct,
                   "time_to_equal_Rg2_error_speedup": float(ttc),
                   "closure_fallbacks": c.fallbacks},
