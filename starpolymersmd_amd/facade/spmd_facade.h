@@ -34,9 +34,11 @@
 #include <deque>
 #include <cstdint>
 #include <cstdio>
+#include <iomanip>
 #include <map>
 #include <memory>
 #include <ostream>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -428,6 +430,76 @@ public:
 inline const char* statistics_header() {
     return "Step   Epot     Ekin     Temp   n_p s_p  R_gyr   G_xx      G_xy      G_xz       G_yx       G_yy      "
            "G_yz      G_zx     Gzy     Gzz      wx      wy     wz\n";
+}
+
+/* ---- run parameters encoded in file names ----
+ * The production driver names every output file by a parameter string
+ * (star-polymers.cpp:270-288), e.g.
+ *   _Star_A20_B20_Arms30_Mass10_Lx64_Ly64_Lz64_Lambda1.00_T1.00_run10000_MPCON_Shear0.05000
+ * (integers as such, Lambda and T with 2 decimals, the run's total step
+ * count, the shear rate with 5 decimals or _MPCOFF), writes the restart
+ * files ./results/end_config<tag>.pdb and end_fluid<tag>.dat, and reads
+ * the parameters of a continued run back from the end_config name
+ * (:118-133, :199-216) with find_parameter (:74-83). */
+struct RunParams {
+    bool chain = false;
+    int A = 0, B = 0, Arms = 0, Mass = 0, Lx = 0, Ly = 0, Lz = 0;
+    double Lambda = 0.0, T = 0.0, Shear = 0.0;
+    long Steps = 0;  /* "run": the step count the name carries */
+    bool MPC_on = false;
+};
+
+inline std::string run_tag(const RunParams& p) {
+    std::ostringstream s;
+    s << (p.chain ? "_Chain" : "_Star") << "_A" << p.A << "_B" << p.B;
+    if (!p.chain) s << "_Arms" << p.Arms;
+    s << "_Mass" << p.Mass << "_Lx" << p.Lx << "_Ly" << p.Ly << "_Lz" << p.Lz;
+    s << std::fixed << std::setprecision(2) << "_Lambda" << p.Lambda << "_T" << p.T;
+    s << "_run" << p.Steps;
+    if (p.MPC_on) s << std::setprecision(5) << "_MPCON_Shear" << p.Shear;
+    else s << "_MPCOFF";
+    return s.str();
+}
+
+/* the text after the first occurrence of name up to the next '_', "0.0"
+ * when name is absent (star-polymers.cpp:74-83) */
+inline std::string find_parameter(const std::string& tag, const std::string& name) {
+    const std::string::size_type i = tag.find(name);
+    if (i == std::string::npos) return "0.0";
+    std::string v = tag.substr(i + name.size());
+    const std::string::size_type j = v.find('_');
+    if (j != std::string::npos) v.erase(j);
+    return v;
+}
+
+/* the parameters of a restart file name ".../end_config<tag>.pdb"
+ * (star-polymers.cpp:118-133, 199-216) */
+inline RunParams parse_restart_name(const std::string& path) {
+    std::string s = path;
+    const std::string::size_type i = s.find("end_config");
+    if (i == std::string::npos) throw Error(SPMD_ERR_ARG, "not an end_config file name: " + path);
+    s.erase(0, i + std::string("end_config").size());
+    const std::string::size_type k = s.find(".pdb");
+    if (k != std::string::npos) s.erase(k, 4);
+    RunParams p;
+    try {
+        p.chain = s.compare(0, 6, "_Chain") == 0;
+        p.Lx = std::stoi(find_parameter(s, "Lx"));
+        p.Ly = std::stoi(find_parameter(s, "Ly"));
+        p.Lz = std::stoi(find_parameter(s, "Lz"));
+        p.A = std::stoi(find_parameter(s, "A"));
+        p.B = std::stoi(find_parameter(s, "B"));
+        p.Arms = std::stoi(find_parameter(s, "Arms"));
+        p.Mass = std::stoi(find_parameter(s, "Mass"));
+        p.Steps = std::stol(find_parameter(s, "run"));
+        p.T = std::stod(find_parameter(s, "T"));
+        p.Shear = std::stod(find_parameter(s, "Shear"));
+        p.Lambda = std::stod(find_parameter(s, "Lambda"));
+        p.MPC_on = find_parameter(s, "MPC") == "ON";
+    } catch (const std::logic_error&) {
+        throw Error(SPMD_ERR_ARG, "unreadable parameters in file name: " + path);
+    }
+    return p;
 }
 
 /* Thermostat_None (Thermostat_None.h:14, Thermostat_None.cpp:5-38) */

@@ -108,3 +108,60 @@ def test_mpc_test_driver_matches_python_driver(tmp_path):
         a, b = (cpp / f).read_text(), (py / f).read_text()
         assert a == b, (f, a[:300], b[:300])
     assert (py / "velocity_N2160_Shear0.3_h0.1.dat").read_text().startswith("#Diffusion coefficient = ")
+
+
+@pytest.mark.gpu
+def test_driver_signal_dump_and_continue(tmp_path):
+    """SIGTERM stops a tagged run at the next step and writes its restart
+    files with the step reached in the name (star-polymers.cpp:413-463);
+    the driver then continues from that end_config file, reading the run's
+    parameters from its name, the star from the PDB and the solvent from
+    the matching end_fluid file (:118-133, 199-225, 361-404)."""
+    import re
+    import signal
+
+    L, A, B, arms, shear, nc = 16, 3, 3, 6, 0.02, 10
+    run1 = tmp_path / "run1"
+    run2 = tmp_path / "run2"
+    run1.mkdir()
+    run2.mkdir()
+    p = subprocess.Popen([DRIVER, str(L), str(A), str(B), str(arms), str(shear), "1000000", "20", "0.01", str(nc),
+                          "1", str(run1), "1"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    try:
+        first = p.stdout.readline()  # the loop has reached its first output step
+        assert "T=" in first, first
+        p.send_signal(signal.SIGTERM)
+        out, err = p.communicate(timeout=120)
+    finally:
+        if p.poll() is None:
+            p.kill()
+            p.wait()
+    assert p.returncode == signal.SIGTERM, err
+    assert "signal 15 caught, data saved" in out
+    ends = sorted(run1.glob("end_config_*.pdb"))
+    assert len(ends) == 1
+    m = re.fullmatch(r"end_config_Star_A3_B3_Arms6_Mass10_Lx16_Ly16_Lz16_Lambda1\.00_T1\.00_run(\d+)"
+                     r"_MPCON_Shear0\.02000\.pdb", ends[0].name)
+    assert m, ends[0].name
+    n = int(m.group(1))
+    assert 20 <= n < 1000000
+    tag = ends[0].name[len("end_config"):-len(".pdb")]
+    fluid = (run1 / f"end_fluid{tag}.dat").read_text().splitlines()
+    assert len([l for l in fluid if l.strip()]) >= L ** 3 * nc
+    stats = (run1 / f"statistics{tag}.dat").read_text().splitlines()
+    steps = [int(l.split()[0]) for l in stats[1:] if l.strip()]
+    assert steps and all(s % 20 == 0 and s < n + 1 for s in steps)
+    assert (run1 / f"fluid_profile{tag}.dat").exists()
+    atoms = [l for l in ends[0].read_text().splitlines() if l.startswith(("ATOM", "HETATM"))]
+    assert len(atoms) == 1 + arms * (A + B)
+
+    r = subprocess.run([DRIVER, str(ends[0]), str(n + 40), "20", "0.01", str(nc), str(run2)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = [l.split() for l in r.stdout.splitlines() if "T=" in l]
+    assert [int(x[0]) for x in rows] == [s for s in range(n + 1, n + 41) if s % 20 == 0]
+    for x in rows:  # the continued run stays at kT
+        assert abs(float(x[1].split("=")[1]) - 1.0) < 0.1
+    tag2 = tag.replace(f"_run{n}_", f"_run{n + 40}_")
+    assert (run2 / f"end_config{tag2}.pdb").exists()
+    assert (run2 / f"end_fluid{tag2}.dat").exists()
