@@ -174,19 +174,23 @@ def batch_bench(intervals=100, seq_intervals=10):
 
 
 def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_shear=0.012, check_warmup=400,
-                  check_intervals=800, check_box=80):
+                  check_intervals=800, check_box=80, train_stars=4):
     """Learned closure at batch (256 stars, f=30 arms of 20 A + 20 B beads,
     1,201 monomers, shear rates uniform in [4e-3, 1.6e-2]) against explicit
     MPC-MD of the same star at L=64, rho=10 on the same GPU (the smallest
     box that holds the star stretched by these rates): star-steps/s (MD
     steps x stars per second), the network trained here on explicit runs of
     the library (8 shear rates, solvent started from the steady flow
-    profile), fallbacks included.  Accuracy: 64 replicas each way at shear
+    profile; 4 independent stars per rate), fallbacks included.  Accuracy:
+    64 replicas each way at shear
     0.012 in steady flow, Rg^2, alignment angle and rotation frequency
     averaged over the run after a warm-up, standard errors from the spread
     of replica means; each closure/explicit ratio carries the standard error
     propagated from both sides (Rg = sqrt(<Rg^2>), so its ratio is the
-    square root of the Rg^2 ratio with half its relative error)."""
+    square root of the Rg^2 ratio with half its relative error).  The same
+    check runs with the network restricted to the flow-relative velocity and
+    the shear rate (free_draining), and both networks' likelihood of held-out
+    collisions is compared per sample."""
     import numpy as np
 
     from starpolymersmd_amd import MPCSystem, make_params
@@ -197,6 +201,17 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
     shears_train = list(np.geomspace(lo, hi, 8))
     t0 = time.perf_counter()
     data = cl.generate(shears_train, L=L, arms=arms, A=A, B=B, intervals=150, warmup=800)
+    if train_stars > 1:
+        # more independent stars per rate, in a box that holds them stretched
+        parts = [data] + [cl.generate(shears_train, L=80, arms=arms, A=A, B=B, intervals=150, warmup=800,
+                                      seed=11 + 7 * k) for k in range(1, train_stars)]
+        xs = [q.X.download((q.n, cl.N_FEAT)) for q in parts]
+        ys = [q.Y.download((q.n, 3)) for q in parts]
+        for q in parts:
+            q.X.close()
+            q.Y.close()
+        data = cl.ClosureData.from_numpy(np.concatenate(xs), np.concatenate(ys))
+        del xs, ys
     t_gen = time.perf_counter() - t0
     net = cl.ClosureNet(1)
     t0 = time.perf_counter()
@@ -271,17 +286,39 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
     c = cl.ClosureRunner(net, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=L, shears=[check_shear] * R,
                          seed=500)
     cm, cs, ct = ensemble(lambda k: c.advance(k, chunk=k), c.sys, R)
+    c.close()
     # time to the same standard error of Rg^2 (the error falls as 1/sqrt(time))
     ttc = (et * es[0] ** 2) / (ct * cs[0] ** 2)
     ratio = cm / em
     ratio_se = np.abs(ratio) * np.sqrt((cs / cm) ** 2 + (es / em) ** 2)
+    # what the star's configuration features gain: the same training on the
+    # flow-relative velocity and the shear rate alone (a learned
+    # free-draining Langevin bath), the same ensemble check, and the
+    # likelihood of one collision's velocity changes held out from training
+    # (an independent run at the check's shear), compared per sample
+    net_fd = cl.ClosureNet(1)
+    hist_fd = net_fd.fit(data, steps=steps, batch=65536, lr=2e-3, features=cl.FREE_DRAINING)
+    c_fd = cl.ClosureRunner(net_fd, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=L,
+                            shears=[check_shear] * R, seed=500)
+    fm, fs, _ = ensemble(lambda k: c_fd.advance(k, chunk=k), c_fd.sys, R)
+    fd_fallbacks = c_fd.fallbacks
+    c_fd.close()
+    held = cl.generate([check_shear], L=80, arms=arms, A=A, B=B, intervals=50, warmup=800, seed=99)
+    xh, yh = held.X.download((held.n, 24)), held.Y.download((held.n, 3))
+    held.X.close()
+    held.Y.close()
+    nll_full, nll_fd = net.nll(xh, yh), net_fd.nll(xh, yh)
+    gain = nll_fd - nll_full
+    gain_blk = gain.reshape(-1, 1 + arms * (A + B)).mean(axis=1)  # per interval: its monomers are correlated
+    fd_ratio = fm / em
+    fd_ratio_se = np.abs(fd_ratio) * np.sqrt((fs / fm) ** 2 + (es / em) ** 2)
     return {
         "star_steps_per_s": closure_rate, "explicit_star_steps_per_s": explicit_rate,
         "speedup": closure_rate / explicit_rate,
         "workload": f"{replicas} stars f={arms} {A}A+{B}B (1,201 monomers) closure vs {explicit_replicas} stars "
                     f"explicit MPC L={L} rho=10, shear U[{lo},{hi}], {n} intervals of {S} MD steps",
         "fallbacks": fallbacks,
-        "train": {"samples": data.n, "shear_rates": 8, "steps": steps, "batch": 65536,
+        "train": {"samples": data.n, "shear_rates": 8, "stars_per_rate": train_stars, "steps": steps, "batch": 65536,
                   "generate_seconds": t_gen, "fit_seconds": t_fit,
                   "loss_first": float(hist[0]), "loss_last": float(np.mean(hist[-10:]))},
         "check": {"shear": check_shear, "replicas": R, "explicit_box": Le, "warmup_intervals": check_warmup,
@@ -299,6 +336,17 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
                   "explicit_seconds": et, "closure_seconds": ct,
                   "time_to_equal_Rg2_error_speedup": float(ttc),
                   "closure_fallbacks": c.fallbacks},
+        "free_draining": {"network": "features " + ",".join(str(f) for f in cl.FREE_DRAINING)
+                                + " only (flow-relative velocity, shear rate): a learned free-draining bath",
+                     "loss_last": float(np.mean(hist_fd[-10:])),
+                     "Rg2_ratio": float(fd_ratio[0]), "Rg2_ratio_se": float(fd_ratio_se[0]),
+                     "chi_ratio": float(fd_ratio[1]), "chi_ratio_se": float(fd_ratio_se[1]),
+                     "wz_ratio": float(fd_ratio[2]), "wz_ratio_se": float(fd_ratio_se[2]),
+                     "fallbacks": fd_fallbacks,
+                     "heldout_samples": int(len(gain)),
+                     "heldout_nll_full": float(nll_full.mean()), "heldout_nll_free_draining": float(nll_fd.mean()),
+                     "heldout_nll_gain": float(gain.mean()),
+                     "heldout_nll_gain_se": float(gain_blk.std(ddof=1) / np.sqrt(len(gain_blk)))},
     }
 
 
@@ -313,6 +361,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (ppt 4, 128-cell tiles)")
     ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure throughput section")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-replicas section")
+    ap.add_argument("--train-stars", type=int, default=4,
+                    help="independent explicit stars per shear rate in the closure's training data")
     ap.add_argument("--closure-only", action="store_true",
                     help="only run the learned-closure section (for profiling its kernels)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -335,7 +385,7 @@ def main():
         sys.exit(2)
 
     if a.closure_only:
-        print(json.dumps({"closure": closure_bench()}), flush=True)
+        print(json.dumps({"closure": closure_bench(train_stars=a.train_stars)}), flush=True)
         return
 
     w = WORKLOAD
@@ -403,7 +453,7 @@ def main():
     if not a.no_batch:
         out["batch"] = batch_bench()
     if not a.no_closure:
-        out["closure"] = closure_bench()
+        out["closure"] = closure_bench(train_stars=a.train_stars)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_intervals, star0)
         # all the host threads this process may use (OMP_NUM_THREADS, else the CPUs it sees)

@@ -153,17 +153,35 @@ class ClosureNet:
         return loss.value, g
 
     def fit(self, data: "ClosureData", steps: int = 1000, batch: int = 65536, lr: float = 1e-3,
-            record_loss: bool = True) -> np.ndarray:
+            record_loss: bool = True, features=None) -> np.ndarray:
         """Adam on minibatches drawn with replacement on the device; sets the
         normalisation from the data first.  Returns the per-step loss (NaN
-        where not recorded)."""
-        self.set_norm(data.mu_x, data.sd_x, data.sd_y)
+        where not recorded).  features: the indices the network may use;
+        the others get a normalisation scale of 1e30, so they enter as zero
+        in training and inference alike (FREE_DRAINING restricts it to a
+        learned Langevin bath)."""
+        sd_x = np.array(data.sd_x, np.float32)
+        if features is not None:
+            off = np.ones(N_FEAT, bool)
+            off[list(features)] = False
+            sd_x[off] = 1e30
+        self.set_norm(data.mu_x, sd_x, data.sd_y)
         hist = np.full(steps, np.nan)
         self._fit_calls += 1
         self._chk(self._lib.spmd_mlp_fit(self.h, C.c_void_p(data.X.ptr), C.c_void_p(data.Y.ptr), data.n,
                                          min(batch, data.n), steps, lr, self.seed * 1000003 + self._fit_calls,
                                          ptr(hist, C.c_double) if record_loss else None))
         return hist
+
+    def nll(self, x: np.ndarray, y: np.ndarray) -> np.ndarray:
+        """Per-sample Gaussian negative log-likelihood (without the constant)
+        of velocity changes y (M, 3) under the network's prediction for
+        features x: sum over components of ((y/sd_y - mu)^2 exp(-s) + s) / 2,
+        the distribution the closure's kick draws from."""
+        o = self.forward(x).astype(np.float64)
+        yn = np.asarray(y, np.float64) / self.norm[2].astype(np.float64)
+        s = np.clip(o[:, 3:], -12.0, 12.0)
+        return 0.5 * np.sum((yn - o[:, :3]) ** 2 * np.exp(-s) + s, axis=1)
 
     def save(self, path):
         mu, sd, sy = self.norm
@@ -281,6 +299,11 @@ def shape_stats(G):
 # Velocity-like features (3-8, 19-21) have Gaussian tails and the type
 # flags are categorical; they are not range-checked.
 RANGE_CHECKED = (0, 1, 2, 9, 10, 11, 12, 13, 14, 15, 18, 22, 23)
+
+# the velocity relative to the imposed flow and the shear rate: a network
+# restricted to these is a learned free-draining Langevin bath (drag toward
+# the local flow plus noise, no dependence on the star's configuration)
+FREE_DRAINING = (6, 7, 8, 18)
 
 
 def regime_range(fmin, fmax, checked=RANGE_CHECKED):
