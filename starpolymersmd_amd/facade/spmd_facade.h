@@ -246,10 +246,15 @@ public:
     }
     /* MPC::step(t, dt) MPC.cpp:116-142 */
     void step(const long int& t, const double& dt) { check(spmd_mpc_step(SimBox.handle(), t, dt), SimBox.handle()); }
-    /* MPC::calculateCurrentTemperature MPC.cpp:405-409 */
+    /* MPC::calculateCurrentTemperature MPC.cpp:405-409.  Before the first
+     * collision after the fluid was set (a continued run, MPC.cpp:57-113,
+     * has not sorted yet) the reference's cell lists are empty and the
+     * result is 0; the library reports that state as SPMD_ERR_STATE. */
     double calculateCurrentTemperature() {
         double T = 0.0;
-        check(spmd_temperature(SimBox.handle(), 0, &T), SimBox.handle());
+        const int rc = spmd_temperature(SimBox.handle(), 0, &T);
+        if (rc == SPMD_ERR_STATE) return 0.0;
+        check(rc, SimBox.handle());
         return T;
     }
     /* MPC::NumberOfParticles MPC.h:91 */

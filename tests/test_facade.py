@@ -160,8 +160,14 @@ def test_driver_signal_dump_and_continue(tmp_path):
     assert r.returncode == 0, r.stderr
     rows = [l.split() for l in r.stdout.splitlines() if "T=" in l]
     assert [int(x[0]) for x in rows] == [s for s in range(n + 1, n + 41) if s % 20 == 0]
-    for x in rows:  # the continued run stays at kT
-        assert abs(float(x[1].split("=")[1]) - 1.0) < 0.1
+    for x in rows:
+        T = float(x[1].split("=")[1])
+        if int(x[0]) <= (n + 1 + 9) // 10 * 10:
+            # an output step before the continued run's first collision:
+            # the cell lists are still empty, as in the reference, T = 0
+            assert T == 0.0 or abs(T - 1.0) < 0.1, x
+        else:  # the continued run stays at kT
+            assert abs(T - 1.0) < 0.1, x
     tag2 = tag.replace(f"_run{n}_", f"_run{n + 40}_")
     assert (run2 / f"end_config{tag2}.pdb").exists()
     assert (run2 / f"end_fluid{tag2}.dat").exists()
