@@ -189,8 +189,9 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
     propagated from both sides (Rg = sqrt(<Rg^2>), so its ratio is the
     square root of the Rg^2 ratio with half its relative error).  The same
     check runs with the network restricted to the flow-relative velocity and
-    the shear rate (free_draining), and both networks' likelihood of held-out
-    collisions is compared per sample."""
+    the shear rate (free_draining) and to those plus the neighbours' mean
+    velocity, crowding and type (local_coupling); the networks' likelihood
+    of held-out collisions is compared per sample."""
     import numpy as np
 
     from starpolymersmd_amd import MPCSystem, make_params
@@ -291,27 +292,44 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
     ttc = (et * es[0] ** 2) / (ct * cs[0] ** 2)
     ratio = cm / em
     ratio_se = np.abs(ratio) * np.sqrt((cs / cm) ** 2 + (es / em) ** 2)
-    # what the star's configuration features gain: the same training on the
-    # flow-relative velocity and the shear rate alone (a learned
-    # free-draining Langevin bath), the same ensemble check, and the
-    # likelihood of one collision's velocity changes held out from training
-    # (an independent run at the check's shear), compared per sample
-    net_fd = cl.ClosureNet(1)
-    hist_fd = net_fd.fit(data, steps=steps, batch=65536, lr=2e-3, features=cl.FREE_DRAINING)
-    c_fd = cl.ClosureRunner(net_fd, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=L,
-                            shears=[check_shear] * R, seed=500)
-    fm, fs, _ = ensemble(lambda k: c_fd.advance(k, chunk=k), c_fd.sys, R)
-    fd_fallbacks = c_fd.fallbacks
-    c_fd.close()
+    # what the inputs gain: the same training restricted to the flow-relative
+    # velocity and the shear rate (a learned free-draining Langevin bath) and
+    # to those plus the neighbours' mean velocity, crowding and type (local
+    # coupling); the same ensemble check, and the likelihood of one
+    # collision's velocity changes held out from training (an independent
+    # run at the check's shear), compared per sample with standard errors
+    # from interval blocks (the monomers of one interval are correlated)
     held = cl.generate([check_shear], L=80, arms=arms, A=A, B=B, intervals=50, warmup=800, seed=99)
     xh, yh = held.X.download((held.n, 24)), held.Y.download((held.n, 3))
     held.X.close()
     held.Y.close()
-    nll_full, nll_fd = net.nll(xh, yh), net_fd.nll(xh, yh)
-    gain = nll_fd - nll_full
-    gain_blk = gain.reshape(-1, 1 + arms * (A + B)).mean(axis=1)  # per interval: its monomers are correlated
-    fd_ratio = fm / em
-    fd_ratio_se = np.abs(fd_ratio) * np.sqrt((fs / fm) ** 2 + (es / em) ** 2)
+    nmono = 1 + arms * (A + B)
+    nll = {"full": net.nll(xh, yh)}
+
+    def gain(a, b):
+        g = (nll[a] - nll[b]).reshape(-1, nmono).mean(axis=1)
+        return float(g.mean()), float(g.std(ddof=1) / np.sqrt(len(g)))
+
+    restricted = {}
+    for key, feats in (("free_draining", cl.FREE_DRAINING), ("local_coupling", cl.LOCAL_COUPLING)):
+        n2 = cl.ClosureNet(1)
+        h2 = n2.fit(data, steps=steps, batch=65536, lr=2e-3, features=feats)
+        r2 = cl.ClosureRunner(n2, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=L,
+                              shears=[check_shear] * R, seed=500)
+        m2, s2, _ = ensemble(lambda k: r2.advance(k, chunk=k), r2.sys, R)
+        fb2 = r2.fallbacks
+        r2.close()
+        nll[key] = n2.nll(xh, yh)
+        rt = m2 / em
+        rse = np.abs(rt) * np.sqrt((s2 / m2) ** 2 + (es / em) ** 2)
+        restricted[key] = {"features": list(feats), "loss_last": float(np.mean(h2[-10:])),
+                           "Rg2_ratio": float(rt[0]), "Rg2_ratio_se": float(rse[0]),
+                           "Rg_ratio": float(np.sqrt(rt[0])), "Rg_ratio_se": float(0.5 * rse[0] / np.sqrt(rt[0])),
+                           "chi_ratio": float(rt[1]), "chi_ratio_se": float(rse[1]),
+                           "wz_ratio": float(rt[2]), "wz_ratio_se": float(rse[2]),
+                           "fallbacks": fb2, "heldout_nll": float(nll[key].mean())}
+    g_full, g_full_se = gain("free_draining", "full")
+    g_loc, g_loc_se = gain("free_draining", "local_coupling")
     return {
         "star_steps_per_s": closure_rate, "explicit_star_steps_per_s": explicit_rate,
         "speedup": closure_rate / explicit_rate,
@@ -336,17 +354,11 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
                   "explicit_seconds": et, "closure_seconds": ct,
                   "time_to_equal_Rg2_error_speedup": float(ttc),
                   "closure_fallbacks": c.fallbacks},
-        "free_draining": {"network": "features " + ",".join(str(f) for f in cl.FREE_DRAINING)
-                                + " only (flow-relative velocity, shear rate): a learned free-draining bath",
-                     "loss_last": float(np.mean(hist_fd[-10:])),
-                     "Rg2_ratio": float(fd_ratio[0]), "Rg2_ratio_se": float(fd_ratio_se[0]),
-                     "chi_ratio": float(fd_ratio[1]), "chi_ratio_se": float(fd_ratio_se[1]),
-                     "wz_ratio": float(fd_ratio[2]), "wz_ratio_se": float(fd_ratio_se[2]),
-                     "fallbacks": fd_fallbacks,
-                     "heldout_samples": int(len(gain)),
-                     "heldout_nll_full": float(nll_full.mean()), "heldout_nll_free_draining": float(nll_fd.mean()),
-                     "heldout_nll_gain": float(gain.mean()),
-                     "heldout_nll_gain_se": float(gain_blk.std(ddof=1) / np.sqrt(len(gain_blk)))},
+        "restricted_networks": restricted,
+        "heldout": {"samples": int(len(yh)), "nll_full": float(nll["full"].mean()),
+                    "nll_gain_full_over_free_draining": g_full, "nll_gain_full_over_free_draining_se": g_full_se,
+                    "nll_gain_local_coupling_over_free_draining": g_loc,
+                    "nll_gain_local_coupling_over_free_draining_se": g_loc_se},
     }
 
 

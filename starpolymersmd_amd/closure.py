@@ -304,6 +304,10 @@ RANGE_CHECKED = (0, 1, 2, 9, 10, 11, 12, 13, 14, 15, 18, 22, 23)
 # restricted to these is a learned free-draining Langevin bath (drag toward
 # the local flow plus noise, no dependence on the star's configuration)
 FREE_DRAINING = (6, 7, 8, 18)
+# the free-draining inputs plus the crowding count, the bead type and the
+# mean velocity of the monomers within 1.5 relative to the monomer's own:
+# drag toward the local flow and a learned coupling to the neighbours
+LOCAL_COUPLING = (6, 7, 8, 15, 16, 18, 19, 20, 21)
 
 
 def regime_range(fmin, fmax, checked=RANGE_CHECKED):
