@@ -257,10 +257,10 @@ def test_closure_tracks_explicit_in_shear_flow():
     (explicit at L=80, whose Verlet grid holds the stretched stars), 400
     intervals of warm-up, then Rg^2, the alignment angle and the rotation
     frequency about z averaged over 800 intervals.  Bar, for the network
-    and for its free-draining restriction: the closure's ensemble means
-    within 3 combined standard errors (replica spread) of explicit MPC's,
-    and within 8% -- the stars are still stretching over this window, and
-    the closure follows the same start-up."""
+    and for its free-draining and local-coupling restrictions: the closure's
+    ensemble means within 3 combined standard errors (replica spread) of
+    explicit MPC's, and within 8% -- the stars are still stretching over
+    this window, and the closure follows the same start-up."""
     from starpolymersmd_amd import closure as cl
 
     arms, A, B, S, L, shear, R = 30, 20, 20, 10, 64, 0.012, 16
@@ -303,17 +303,20 @@ def test_closure_tracks_explicit_in_shear_flow():
     assert np.all(np.abs(cm / em - 1) < 0.08), (cm, em)
     c.close()
     # the same network on the flow-relative velocity and the shear rate alone
-    # (a learned free-draining bath), same bar
-    fd = cl.ClosureNet(1)
-    fd.fit(data, steps=2000, batch=65536, lr=2e-3, features=cl.FREE_DRAINING)
-    f = cl.ClosureRunner(fd, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=L, shears=[shear] * R, seed=500)
-    fm, fs = ensemble(lambda k: f.advance(k, chunk=k), f.sys)
-    f.sys.sync()
-    f.close()
-    print("free-draining", fm, "+-", fs)
-    se = np.sqrt(es ** 2 + fs ** 2)
-    assert np.all(np.abs(fm - em) <= 3 * se), (fm, em, se)
-    assert np.all(np.abs(fm / em - 1) < 0.08), (fm, em)
+    # (a learned free-draining bath), and on those plus the neighbours' mean
+    # velocity, crowding and type (local coupling): same bar
+    for name, feats in (("free-draining", cl.FREE_DRAINING), ("local coupling", cl.LOCAL_COUPLING)):
+        rn = cl.ClosureNet(1)
+        rn.fit(data, steps=2000, batch=65536, lr=2e-3, features=feats)
+        f = cl.ClosureRunner(rn, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=L, shears=[shear] * R,
+                             seed=500)
+        fm, fs = ensemble(lambda k: f.advance(k, chunk=k), f.sys)
+        f.sys.sync()
+        f.close()
+        print(name, fm, "+-", fs)
+        se = np.sqrt(es ** 2 + fs ** 2)
+        assert np.all(np.abs(fm - em) <= 3 * se), (name, fm, em, se)
+        assert np.all(np.abs(fm / em - 1) < 0.08), (name, fm, em)
 
 
 @pytest.mark.gpu
