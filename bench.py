@@ -316,7 +316,7 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
         h2 = n2.fit(data, steps=steps, batch=65536, lr=2e-3, features=feats)
         r2 = cl.ClosureRunner(n2, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=L,
                               shears=[check_shear] * R, seed=500)
-        m2, s2, _ = ensemble(lambda k: r2.advance(k, chunk=k), r2.sys, R)
+        m2, s2, t2 = ensemble(lambda k: r2.advance(k, chunk=k), r2.sys, R)
         fb2 = r2.fallbacks
         r2.close()
         nll[key] = n2.nll(xh, yh)
@@ -327,7 +327,8 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
                            "Rg_ratio": float(np.sqrt(rt[0])), "Rg_ratio_se": float(0.5 * rse[0] / np.sqrt(rt[0])),
                            "chi_ratio": float(rt[1]), "chi_ratio_se": float(rse[1]),
                            "wz_ratio": float(rt[2]), "wz_ratio_se": float(rse[2]),
-                           "fallbacks": fb2, "heldout_nll": float(nll[key].mean())}
+                           "fallbacks": fb2, "heldout_nll": float(nll[key].mean()),
+                           "time_to_equal_Rg2_error_speedup": float((et * es[0] ** 2) / (t2 * s2[0] ** 2))}
     g_full, g_full_se = gain("free_draining", "full")
     g_loc, g_loc_se = gain("free_draining", "local_coupling")
     return {
