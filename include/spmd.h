@@ -93,7 +93,10 @@ int spmd_set_tracking(spmd_sys* s, int on);
 
 /* ---- star (Box::Molecules, Box.h:50) ---- */
 /* Box::add_star(A, B, Arms, Mass, Bond, AnchorBond) Box.cpp:39-46 ->
- * Molecule::initialize_open_star Molecule.cpp:126-176 (Philox draws). */
+ * Molecule::initialize_open_star Molecule.cpp:126-176 (Philox draws).
+ * Adding a star rebuilds the device monomer buffers of every replica; once
+ * spmd_md_init has run, that rebuild also redoes its work (Verlet lists and
+ * forces of all replicas), so a star added later interacts at once. */
 int spmd_add_star(spmd_sys* s, uint32_t replica, uint32_t A, uint32_t B, uint32_t arms,
                   double mass, double bond, double anchor_bond);
 /* Same star with seeded random-coil arms (no reference counterpart: the

@@ -47,6 +47,7 @@ struct spmd_sys {
     std::string err;
     bool tracking = false;
     bool mono_dirty = true;        /* device monomer buffers need (re)building */
+    bool md_inited = false;        /* spmd_md_init ran: a monomer rebuild redoes it */
     uint32_t step_update = 1;
     double md_dt = 0.01;
     /* lazily batched work */
@@ -198,6 +199,14 @@ int build_monomers(spmd_sys* s) {
     HIPCHK(s, hipMemcpy(d.reps, s->reps.data(), s->reps.size() * sizeof(RepDev), hipMemcpyHostToDevice));
     s->mono_dirty = false;
     clear_graphs(s);
+    /* The rebuild emptied every replica's Verlet list and reset its
+     * reference positions; after spmd_md_init (the Thermostat_None ctor,
+     * Thermostat_None.cpp:5-12) the lists and forces are made again here,
+     * so MD never runs on an empty list. */
+    if (s->md_inited && mt) {
+        HIPCHK(s, launch_md(d, s->stream, 0, 0, 1, 0, s->md_dt));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+    }
     return SPMD_OK;
 }
 
@@ -828,6 +837,7 @@ int spmd_md_init(spmd_sys* s) {
     if (int rc = ensure_monomers(s)) return rc;
     LAUNCH(s, 0, s->stream, launch_md(s->d, s->stream, 0, 0, 1, 0, s->md_dt));
     HIPCHK(s, hipStreamSynchronize(s->stream));
+    s->md_inited = true;
     return check_status(s);
 }
 

@@ -653,6 +653,7 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
     }
     if (tid == 0 && epot_set) ctl->epot = epot_val;
 
+    bool forces_stored = false;  /* the recentring wrote the forces already */
     if (do_coll) {
         /* MPC::check_bounds (MPC.cpp:160-190) on the monomers; a recentring
          * also marks the speculative solvent sort of this collision for redo */
@@ -666,16 +667,39 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
         }
         const bool rec = __syncthreads_or(out) != 0;
         if (rec) {
-            double sp[6] = {0, 0, 0, 0, 0, 0};
+            /* Molecule::calculate_center_of_mass(_velocity) (Molecule.cpp:76-92):
+             * summed in monomer order, then divided by N, so the COM -- and
+             * every solvent image the recentring decides with it -- is
+             * bit-identical to the sequential FP64 sum.  The forces go to
+             * HBM first; their LDS rows then carry the velocities. */
 #pragma unroll
             for (int k = 0; k < R; ++k) {
-                if (tid + k * NT >= N) continue;
-                sp[0] += rx[k]; sp[1] += ry[k]; sp[2] += rz[k];
-                sp[3] += rvx[k]; sp[4] += rvy[k]; sp[5] += rvz[k];
+                const int i = tid + k * NT;
+                if (i >= N) continue;
+                const size_t g = 3 * (size_t)(mb + i);
+                d.mforce[g] = S.fx[i]; d.mforce[g + 1] = S.fy[i]; d.mforce[g + 2] = S.fz[i];
             }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const int i = tid + k * NT;
+                if (i >= N) continue;
+                S.x[i] = rx[k]; S.y[i] = ry[k]; S.z[i] = rz[k];
+                S.fx[i] = rvx[k]; S.fy[i] = rvy[k]; S.fz[i] = rvz[k];
+            }
+            __syncthreads();
+            if (tid < 6) {
+                const double* src = tid == 0 ? S.x : tid == 1 ? S.y : tid == 2 ? S.z
+                                  : tid == 3 ? S.fx : tid == 4 ? S.fy : S.fz;
+                double acc = 0.0;
+#pragma unroll 8
+                for (int i = 0; i < N; ++i) acc += src[i];
+                S.red[tid] = acc / (double)N;
+            }
+            __syncthreads();
             double cm[6];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) cm[c] = block_sum_all<NT>(sp[c], S.red) / (double)N;
+            for (int c = 0; c < 6; ++c) cm[c] = S.red[c];
 #pragma unroll
             for (int k = 0; k < R; ++k) {
                 rx[k] -= cm[0]; ry[k] -= cm[1]; rz[k] -= cm[2];
@@ -684,6 +708,7 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
             if (tid == 0)
                 for (int c = 0; c < 3; ++c) { ctl->com[c] = cm[c]; ctl->com_vel[c] = cm[3 + c]; }
         }
+        forces_stored = rec;
         if (tid == 0) {
             ctl->recenter = rec ? 1 : 0;
             ctl->redo = rec ? 1 : 0;
@@ -699,7 +724,7 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
         const size_t g = 3 * (size_t)(mb + i);
         d.mpos[g] = rx[k]; d.mpos[g + 1] = ry[k]; d.mpos[g + 2] = rz[k];
         d.mvel[g] = rvx[k]; d.mvel[g + 1] = rvy[k]; d.mvel[g + 2] = rvz[k];
-        d.mforce[g] = S.fx[i]; d.mforce[g + 1] = S.fy[i]; d.mforce[g + 2] = S.fz[i];
+        if (!forces_stored) { d.mforce[g] = S.fx[i]; d.mforce[g + 1] = S.fy[i]; d.mforce[g + 2] = S.fz[i]; }
         d.mvpos[g] = rpx[k]; d.mvpos[g + 1] = rpy[k]; d.mvpos[g + 2] = rpz[k];
     }
 }
@@ -1104,7 +1129,15 @@ __global__ __launch_bounds__(256) void k_collide(DevState d, double h) {
         }
     }
     __syncthreads();
-    const uint32_t p0 = s_off[0], np = s_off[nc] - p0;
+    /* A scan whose bounded look-back wait expired (reported as
+     * ST_SCAN_TIMEOUT) leaves offsets that need not be monotone: the tile
+     * then touches nothing, and its slot range is clamped to the replica's
+     * particles in any case. */
+    if (d.ctl[0].status & ST_SCAN_TIMEOUT) return;
+    const uint64_t pr0 = rep.part_base, pr1 = rep.part_base + rep.n_fluid;
+    const uint32_t p0 = (uint32_t)min<uint64_t>(max<uint64_t>(s_off[0], pr0), pr1);
+    const uint32_t p1 = (uint32_t)min<uint64_t>(max<uint64_t>(s_off[nc], p0), pr1);
+    const uint32_t np = p1 - p0;
     const uint32_t nmt = s_mn < (uint32_t)MCAP ? s_mn : (uint32_t)MCAP;
     if (tid == 0 && s_mn > (uint32_t)MCAP) atomicOr(&ctl->status, ST_TILE_MONO);
     const uint32_t k = ctl->k_cur;

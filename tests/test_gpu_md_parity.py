@@ -209,3 +209,34 @@ def test_md_large_stars_match_oracle(arms, A, B):
     pg, vg, _ = g.get_monomers()
     po, vo, _ = o.get_monomers()
     assert np.abs(pg - po).max() < 1e-9 and np.abs(vg - vo).max() < 1e-9
+
+
+def test_star_added_after_md_init_gets_lists_and_forces():
+    """Adding a star to one replica after spmd_md_init rebuilds the device
+    monomer buffers of all replicas; the lists and forces are made again at
+    once (the Thermostat_None ctor's work), so the other replica's forces
+    are unchanged and the new star's match the oracle before and after 20
+    substeps."""
+    prm = [make_params(L=(24, 24, 24), n_c=0, step_update=10, seed=s, md_dt=0.01, lam=1.0) for s in (1, 2)]
+    g = MPCSystem(prm)
+    g.add_star(5, 5, 10, 10.0, r=0)
+    g.md_init()
+    _, _, f0 = g.get_monomers(r=0)
+    assert np.abs(f0).max() > 1.0
+    g.add_star(4, 4, 8, 10.0, r=1, coil=True)
+    _, _, f0b = g.get_monomers(r=0)
+    assert np.abs(f0b - f0).max() <= 1e-10 * np.abs(f0).max()
+    pos, vel, f1 = g.get_monomers(r=1)
+    o = ob.Oracle(L=(24, 24, 24), n_c=0, step_update=10, rng_mode=ob.RNG_PHILOX, seed=2, md_dt=0.01, lam=1.0)
+    o.add_star(4, 4, 8, 10.0)
+    o.set_monomers(pos, vel)
+    assert o.md_init() == 0
+    _, _, fo = o.get_monomers()
+    assert np.abs(fo).max() > 1.0
+    assert np.abs(f1 - fo).max() <= 1e-10 * np.abs(fo).max()
+    for _ in range(20):
+        g.propagate()
+        assert o.propagate() == 0
+    pg, vg, _ = g.get_monomers(r=1)
+    po, vo, _ = o.get_monomers()
+    assert np.abs(pg - po).max() < 1e-9 and np.abs(vg - vo).max() < 1e-9
