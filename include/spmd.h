@@ -178,6 +178,20 @@ int spmd_statistics(spmd_sys* s, uint32_t replica, double* row);
  * sum vx, sum vy, count.  nbins[a] = floor(L_a/2/width) - b0[a] + 1 <= 1024. */
 int spmd_velocity_profile(spmd_sys* s, uint32_t replica, double width, uint32_t max_bins, int32_t* b0,
                           uint32_t* nbins, double* sums);
+/* No counterpart in the reference: spmd_velocity_profile for callers that
+ * cannot allocate device memory (code under graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same bins and sums, written as doubles
+ * to the caller's device buffer sums_dev (2 * max_bins * 3 doubles, layout
+ * as above) by work enqueued on the handle's stream; b0 and nbins are set
+ * on return, sums_dev is ready after spmd_sync.  Allocates nothing: the bins
+ * and the star's COM live in a static device array of 49,408 bytes per GPU
+ * (held until the process ends); widths that need more than 1024 bins per
+ * axis are rejected (SPMD_ERR_ARG).  Every call on one GPU shares that array
+ * and leaves its contents behind; calls that overlap on one GPU (handles on
+ * different streams) race.  Returns SPMD_ERR_STATE if a star was added
+ * without a rebuild since (call spmd_md_init first). */
+int spmd_velocity_profile_noalloc(spmd_sys* s, uint32_t replica, double width, uint32_t max_bins, int32_t* b0,
+                                  uint32_t* nbins, double* sums_dev);
 /* Fourier_Autocorr::operator() Fourier_Autocorr.h:55-72: V_j = (1/N) sum_i
  * v_i exp(i 2 pi x_ij / L_j) over the solvent for j = x, y, z, with the
  * component along e_j set to zero.  modes[(j * 3 + c) * 2 + {0,1}] =

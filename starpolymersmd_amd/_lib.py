@@ -117,6 +117,8 @@ _SIGS = {
     "spmd_patches": (C.c_int, [C.c_void_p, C.c_uint32, _d, _d]),
     "spmd_statistics": (C.c_int, [C.c_void_p, C.c_uint32, _d]),
     "spmd_velocity_profile": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_uint32, _P(C.c_int32), _u32, _d]),
+    "spmd_velocity_profile_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_uint32, _P(C.c_int32),
+                                                _u32, C.c_void_p]),
     "spmd_fourier_modes": (C.c_int, [C.c_void_p, C.c_uint32, _d]),
     "spmd_speed_histogram": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_uint32, _P(C.c_uint64)]),
     "spmd_fluid_vacf_sample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -33,6 +34,21 @@ struct ProfEvents {
 
 const char* kProfNames[6] = {"md", "bin", "scan", "perm", "collide", "prologue"};
 
+/* Device scratch of spmd_velocity_profile_noalloc: the profile's fixed-point
+ * bins [2][kObsBins][3], then kStarMomentsWords doubles for the star's
+ * moments (its COM).  One copy per GPU, held until the process ends
+ * (49,408 bytes); every call on a GPU shares it. */
+constexpr size_t kStarMomentsWords = 32;
+constexpr size_t kScratchWords = 6ull * kObsBins + kStarMomentsWords;
+static_assert(kStarMomentsWords >= (size_t)kStarMoments, "moments do not fit the scratch");
+static __device__ __align__(256) unsigned long long scratch[kScratchWords];
+
+/* Its address on each device, looked up once per device with that device
+ * current (the host-side name `scratch` is not a device address). */
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch_ptr[kMaxDevices] = {};
+
 }  // namespace
 
 struct spmd_sys {
@@ -48,6 +64,7 @@ struct spmd_sys {
     bool tracking = false;
     bool mono_dirty = true;        /* device monomer buffers need (re)building */
     bool md_inited = false;        /* spmd_md_init ran: a monomer rebuild redoes it */
+    unsigned long long* scratch = nullptr;  /* this device's copy of the static scratch */
     uint32_t step_update = 1;
     double md_dt = 0.01;
     /* lazily batched work */
@@ -507,6 +524,22 @@ int check_device(spmd_sys* s) {
     return SPMD_OK;
 }
 
+/* The current device's address of the static scratch (looked up on the
+ * first handle made on that device). */
+int scratch_setup(spmd_sys* s) {
+    if (s->device < 0 || s->device >= kMaxDevices) return fail(s, SPMD_ERR_NO_DEVICE, "device index beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[s->device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(s, SPMD_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[s->device] = static_cast<unsigned long long*>(p);
+    }
+    s->scratch = g_scratch_ptr[s->device];
+    return SPMD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -538,6 +571,7 @@ int spmd_create_batch(const spmd_params* p, uint32_t n_rep, spmd_sys** out) {
     *out = nullptr;
     auto* s = new spmd_sys;
     if (int rc = check_device(s)) { delete s; return rc; }
+    if (int rc = scratch_setup(s)) { delete s; return rc; }
     s->params.assign(p, p + n_rep);
     s->step_update = p[0].step_update ? p[0].step_update : 1;
     s->md_dt = p[0].md_dt;
@@ -1200,6 +1234,45 @@ int spmd_velocity_profile(spmd_sys* s, uint32_t r, double width, uint32_t max_bi
             o[1] = (double)(long long)q[1] * unfix;
             o[2] = (double)q[2];
         }
+    return SPMD_OK;
+}
+
+int spmd_velocity_profile_noalloc(spmd_sys* s, uint32_t r, double width, uint32_t max_bins, int32_t* b0,
+                                  uint32_t* nbins, double* sums_dev) {
+    if (!valid_rep(s, r) || !(width > 0) || !b0 || !nbins || !sums_dev) return SPMD_ERR_ARG;
+    if (!s->scratch) return fail(s, SPMD_ERR_STATE, "no device scratch on this handle");
+    if (s->mono_dirty)
+        return fail(s, SPMD_ERR_STATE, "the monomer buffers await a rebuild, which allocates: call spmd_md_init first");
+    const RepDev& rp = s->reps[r];
+    ObsArgs a{};
+    a.replica = (int)r;
+    a.flags = kObsProfile;
+    a.w = width;
+    for (int ax = 0; ax < 2; ++ax) {
+        profile_bins(rp, width, ax, &a.b0[ax], &a.nb[ax]);
+        if (a.nb[ax] > kObsBins)
+            return fail(s, SPMD_ERR_ARG, "profile bin width too small: the scratch holds 1024 bins per axis");
+        if ((uint32_t)a.nb[ax] > max_bins) return fail(s, SPMD_ERR_ARG, "max_bins smaller than the number of bins");
+    }
+    for (int ax = 0; ax < 2; ++ax) { b0[ax] = a.b0[ax]; nbins[ax] = (uint32_t)a.nb[ax]; }
+    const DevState& d = s->d;
+    /* pending substeps and delrx advances first, as every observer sees them */
+    if (s->pending_md || s->pending_delrx) {
+        HIPCHK(s, launch_prologue(d, s->stream, 0, s->pending_delrx, s->pending_dt));
+        HIPCHK(s, launch_md(d, s->stream, s->pending_md, s->pending_epot, 0, 0, s->md_dt));
+        s->pending_md = 0;
+        s->pending_delrx = 0;
+    }
+    unsigned long long* acc = s->scratch;
+    double* mom = reinterpret_cast<double*>(s->scratch + 6ull * kObsBins);
+    HIPCHK(s, launch_fill_u64(acc, (uint32_t)(6 * kObsBins), s->stream));
+    if (s->stars[r].n > 0) {
+        HIPCHK(s, launch_star_moments(d, s->stream, (int)r, mom));
+        a.com = mom + 12;
+    }
+    a.prof = acc;
+    HIPCHK(s, launch_fluid_observe(d, s->stream, a, rp.n_fluid));
+    HIPCHK(s, launch_profile_export(acc, a.nb[0], a.nb[1], max_bins, sums_dev, s->stream));
     return SPMD_OK;
 }
 

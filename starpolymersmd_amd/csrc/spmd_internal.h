@@ -167,6 +167,11 @@ struct ObsArgs {
     unsigned long long* speed;    /* [nsb + 1] */
 };
 hipError_t launch_fluid_observe(const DevState& d, hipStream_t st, const ObsArgs& a, uint32_t n_fluid);
+/* zero n words; the velocity profile's bins as doubles in the layout of
+ * spmd_velocity_profile (sums[(a * max_bins + b) * 3 + c], b < nb[a]) */
+hipError_t launch_fill_u64(unsigned long long* p, uint32_t n, hipStream_t st);
+hipError_t launch_profile_export(const unsigned long long* acc, int nb0, int nb1, uint32_t max_bins, double* out,
+                                 hipStream_t st);
 /* Vel_Autocorr_Fluid: window slot `slot` <- velocities by id; with calc,
  * sums[lag] += sum over particles of v(oldest) . v(oldest + lag) */
 constexpr int kVacfMaxEntries = 1024;

@@ -2132,6 +2132,45 @@ hipError_t launch_fluid_vacf(const DevState& d, hipStream_t st, int r, uint32_t 
     return hipGetLastError();
 }
 
+/* n words of p set to zero (a kernel rather than a memset node, so that the
+ * call stays correct when a caller captures it into a graph) */
+__global__ __launch_bounds__(256) void k_fill_u64(unsigned long long* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) p[i] = 0ull;
+}
+
+hipError_t launch_fill_u64(unsigned long long* p, uint32_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const uint32_t nblk = (n + 255) / 256 < 64 ? (n + 255) / 256 : 64;
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+/* The velocity profile's fixed-point bins acc[2][kObsBins][3] (sum vx, sum
+ * vy at 2^-32, count) as doubles in the caller's layout
+ * out[(a * max_bins + b) * 3 + c], bins b < nb[a]; other entries untouched */
+__global__ __launch_bounds__(256) void k_profile_export(const unsigned long long* acc, int nb0, int nb1,
+                                                        uint32_t max_bins, double* out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t a = i >= (uint32_t)nb0 ? 1u : 0u;
+    const uint32_t b = a ? i - (uint32_t)nb0 : i;
+    if (b >= (uint32_t)(a ? nb1 : nb0)) return;
+    const unsigned long long* q = acc + 3ull * (a * kObsBins + b);
+    double* o = out + 3ull * ((size_t)a * max_bins + b);
+    o[0] = (double)(long long)q[0] * kUnfix;
+    o[1] = (double)(long long)q[1] * kUnfix;
+    o[2] = (double)q[2];
+}
+
+hipError_t launch_profile_export(const unsigned long long* acc, int nb0, int nb1, uint32_t max_bins, double* out,
+                                 hipStream_t st) {
+    if (nb0 < 1 || nb0 > kObsBins || nb1 < 1 || nb1 > kObsBins || (uint32_t)nb0 > max_bins ||
+        (uint32_t)nb1 > max_bins)
+        return hipErrorInvalidValue;
+    const uint32_t n = (uint32_t)(nb0 + nb1);
+    hipLaunchKernelGGL(k_profile_export, dim3((n + 255) / 256), dim3(256), 0, st, acc, nb0, nb1, max_bins, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_fluid_observe(const DevState& d, hipStream_t st, const ObsArgs& a, uint32_t n_fluid) {
     if (a.nb[0] < 1 || a.nb[0] > kObsBins || a.nb[1] < 1 || a.nb[1] > kObsBins || a.nsb < 0 ||
         a.nsb > kObsSpeedBins)
