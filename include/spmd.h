@@ -111,6 +111,8 @@ int spmd_num_monomers(const spmd_sys* s, uint32_t replica, uint32_t* n);
 /* Molecules[0].Monomers[i].Position/Velocity/Force, xyz interleaved */
 int spmd_set_monomers(spmd_sys* s, uint32_t replica, const double* pos, const double* vel);
 int spmd_get_monomers(spmd_sys* s, uint32_t replica, double* pos, double* vel, double* force);
+/* Molecules[0].Monomers[i].Mass (Particle.h: MDParticle::Mass) */
+int spmd_get_monomer_masses(const spmd_sys* s, uint32_t replica, double* mass);
 
 /* ---- stepping ---- */
 /* Thermostat_None ctor work (Thermostat_None.cpp:5-12): Box::update_VerletLists
@@ -148,6 +150,13 @@ int spmd_get_cell_index(spmd_sys* s, uint32_t replica, uint32_t* fluid_cells, ui
 /* MPC::calculateCurrentTemperature MPC.cpp:405-409 over the cell lists of
  * the last collision. */
 int spmd_temperature(spmd_sys* s, uint32_t replica, double* T);
+/* MPC::calculateFluidVelocity (MPC.cpp:334-342) of every cell with z in
+ * [z_start, z_stop], as MPC::print_fluid (MPC.cpp:533-553) reads it: the
+ * mean velocity of the solvent particles the last collision sorted into the
+ * cell, 0 for an empty cell (and for every cell before the first collision
+ * after the solvent was set).  out[3 * (c - Lx Ly z_start) + k] for cell
+ * c = x + Lx y + Lx Ly z, component k. */
+int spmd_cell_fluid_velocity(spmd_sys* s, uint32_t replica, int32_t z_start, int32_t z_stop, double* out);
 /* Molecule::Epot after the last propagate (Box.cpp:243-286). */
 int spmd_epot(spmd_sys* s, uint32_t replica, double* epot);
 /* gyration tensor of one replica's star about its centre of mass

@@ -74,6 +74,7 @@ _SIGS = {
     "spmd_num_monomers": (C.c_int, [C.c_void_p, C.c_uint32, _u32]),
     "spmd_set_monomers": (C.c_int, [C.c_void_p, C.c_uint32, _d, _d]),
     "spmd_get_monomers": (C.c_int, [C.c_void_p, C.c_uint32, _d, _d, _d]),
+    "spmd_get_monomer_masses": (C.c_int, [C.c_void_p, C.c_uint32, _d]),
     "spmd_md_init": (C.c_int, [C.c_void_p]),
     "spmd_md_propagate": (C.c_int, [C.c_void_p, C.c_int]),
     "spmd_mpc_step": (C.c_int, [C.c_void_p, C.c_int64, C.c_double]),
@@ -85,6 +86,7 @@ _SIGS = {
     "spmd_last_collision": (C.c_int, [C.c_void_p, C.c_uint32, _d, _P(C.c_int), _d, _d]),
     "spmd_get_cell_index": (C.c_int, [C.c_void_p, C.c_uint32, _u32, _u32]),
     "spmd_temperature": (C.c_int, [C.c_void_p, C.c_uint32, _d]),
+    "spmd_cell_fluid_velocity": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int32, C.c_int32, _d]),
     "spmd_epot": (C.c_int, [C.c_void_p, C.c_uint32, _d]),
     "spmd_md_stats": (C.c_int, [C.c_void_p, C.c_uint32, _u32, _u32]),
     "spmd_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),

@@ -88,6 +88,8 @@ struct spmd_sys {
     uint32_t cl_k = 0;             /* closure interval counter (Philox) */
     double* mom = nullptr;         /* star moments (kStarMoments) and patches (2) */
     unsigned long long* obs = nullptr;  /* solvent observer sums */
+    double* cellv = nullptr;       /* per-cell mean solvent velocities (spmd_cell_fluid_velocity) */
+    size_t cellv_n = 0;
     /* fluid velocity autocorrelation per replica: windows by particle id,
      * lag sums (64-bit fixed point), samples taken and sums made */
     struct Vacf {
@@ -685,7 +687,7 @@ int spmd_destroy(spmd_sys* s) {
     dfree(d.vl); dfree(d.bonds); dfree(d.mcell);
     dfree(d.inj_axis); dfree(d.inj_gamma); dfree(d.reps); dfree(d.ctl); dfree(d.pblk_base);
     dfree(d.tile_base); dfree(d.red);
-    dfree(s->cl_feat); dfree(s->cl_out); dfree(s->cl_range); dfree(s->mom); dfree(s->obs);
+    dfree(s->cl_feat); dfree(s->cl_out); dfree(s->cl_range); dfree(s->mom); dfree(s->obs); dfree(s->cellv);
     for (auto& v : s->vacf) { dfree(v.hist); dfree(v.sums); }
     if (s->side) (void)hipStreamSynchronize(s->side);
     if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
@@ -849,6 +851,13 @@ int spmd_set_monomers(spmd_sys* s, uint32_t r, const double* pos, const double* 
         HIPCHK(s, hipMemcpy(s->d.mvel + 3ull * mb, vel, 24ull * n, hipMemcpyHostToDevice));
         std::memcpy(s->stars[r].vel.data(), vel, 24ull * n);
     }
+    return SPMD_OK;
+}
+
+int spmd_get_monomer_masses(const spmd_sys* s, uint32_t r, double* mass) {
+    if (!s || r >= (uint32_t)s->d.n_rep || !mass) return SPMD_ERR_ARG;
+    const StarHost& st = s->stars[r];
+    std::copy(st.mass.begin(), st.mass.begin() + st.n, mass);
     return SPMD_OK;
 }
 
@@ -1042,6 +1051,28 @@ int spmd_temperature(spmd_sys* s, uint32_t r, double* T) {
     double ek = 0.0, filled = 0.0;
     for (uint32_t i = 0; i < nt; ++i) { ek += red[2 * i]; filled += red[2 * i + 1]; }
     *T = 2. * ek / (3. * ((double)rp.n_fluid - filled));
+    return SPMD_OK;
+}
+
+int spmd_cell_fluid_velocity(spmd_sys* s, uint32_t r, int32_t z_start, int32_t z_stop, double* out) {
+    if (!valid_rep(s, r) || !out) return SPMD_ERR_ARG;
+    const RepDev& rp = s->reps[r];
+    const int Lz = rp.box.iL[2];
+    if (z_start < 0 || z_stop < z_start || z_stop >= Lz) return fail(s, SPMD_ERR_ARG, "z range outside the box");
+    const uint32_t plane = (uint32_t)rp.box.iL[0] * (uint32_t)rp.box.iL[1];
+    const uint32_t n = plane * (uint32_t)(z_stop - z_start + 1);
+    if (!s->collided) {  /* no sort since the fluid was set: empty cell lists (MPC.cpp:334-342) */
+        std::fill(out, out + 3ull * n, 0.0);
+        return SPMD_OK;
+    }
+    if (3ull * n > s->cellv_n) {
+        dfree(s->cellv);
+        if (int rc = dalloc(s, &s->cellv, 3ull * n)) return rc;
+        s->cellv_n = 3ull * n;
+    }
+    HIPCHK(s, launch_cell_velocity(s->d, s->stream, (int)r, plane * (uint32_t)z_start, n, s->cellv));
+    HIPCHK(s, hipMemcpyAsync(out, s->cellv, 24ull * n, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
     return SPMD_OK;
 }
 

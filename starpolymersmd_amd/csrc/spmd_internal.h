@@ -180,6 +180,9 @@ hipError_t launch_fluid_vacf(const DevState& d, hipStream_t st, int r, uint32_t 
 hipError_t launch_closure_kick(const DevState& d, hipStream_t st, const float* out, float sy0, float sy1, float sy2,
                                uint32_t k);
 hipError_t launch_temperature(const DevState& d, hipStream_t st, int replica, uint32_t ntiles);
+/* mean solvent velocity of cells c0 .. c0+n-1 of a replica at the last
+ * collision (3 doubles per cell into out, device) */
+hipError_t launch_cell_velocity(const DevState& d, hipStream_t st, int replica, uint32_t c0, uint32_t n, double* out);
 
 }  // namespace spmd
 

@@ -2035,6 +2035,28 @@ hipError_t launch_fluid_init(const DevState& d, hipStream_t st) {
     return hipGetLastError();
 }
 
+/* MPC::calculateFluidVelocity (MPC.cpp:334-342) of cells c0 .. c0+n-1 of
+ * one replica: the mean velocity of the solvent particles the last
+ * collision sorted into the cell (its slice of the current buffer), 0 for
+ * an empty cell; summed in slice order in FP64. */
+__global__ __launch_bounds__(256) void k_cell_velocity(DevState d, int replica, uint32_t c0, uint32_t n, double* out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t g = (uint32_t)d.reps[replica].cell_base + c0 + i;
+    const uint32_t a = d.offsets[g], e = d.offsets[g + 1];
+    const int cur = d.cur;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (uint32_t p = a; p < e; ++p) { s0 += d.vx[cur][p]; s1 += d.vy[cur][p]; s2 += d.vz[cur][p]; }
+    if (e > a) { s0 /= (double)(e - a); s1 /= (double)(e - a); s2 /= (double)(e - a); }
+    out[3ull * i] = s0; out[3ull * i + 1] = s1; out[3ull * i + 2] = s2;
+}
+
+hipError_t launch_cell_velocity(const DevState& d, hipStream_t st, int replica, uint32_t c0, uint32_t n, double* out) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cell_velocity, dim3((n + 255) / 256), dim3(256), 0, st, d, replica, c0, n, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_temperature(const DevState& d, hipStream_t st, int replica, uint32_t ntiles) {
     switch (d.tile_cells) {
     case 64: hipLaunchKernelGGL(k_temperature<64>, dim3(ntiles), dim3(256), 0, st, d, replica); break;

@@ -2,38 +2,46 @@
  * spmd_facade.h -- the reference's C++ hot-path classes, re-declared on top
  * of the C ABI of libspmd.so (include/spmd.h).
  *
- * Names, constructor arguments and call order follow
+ * Names, constructor arguments, virtual interfaces and call order follow
  *   Box              star-polymers/lib/Box.h:30-121      (Box.cpp:4-46)
+ *   Hydrodynamics    star-polymers/lib/Hydrodynamics/Hydrodynamics.h:7-22
  *   MPC              star-polymers/lib/Hydrodynamics/MPC.h:25-122
- *   Thermostat_None  star-polymers/lib/Thermostats/Thermostat_None.h:14
- * so a driver written like star-polymers/src/star-polymers.cpp:351-405
- * compiles against it:
+ *   Thermostat       star-polymers/lib/Thermostats/Thermostat.h:7-22
+ *   Thermostat_None  star-polymers/lib/Thermostats/Thermostat_None.h:9-21
+ * so a driver written like star-polymers/src/star-polymers.cpp:351-536
+ * compiles against it, base-class pointers included:
  *
  *   Box box(Lx, Ly, Lz, Temperature, Lambda);
  *   MPC hydrodynamics{box, Temperature, N_c, Shear, 100, false};
  *   box.add_star(A, B, Arms, Mass);
- *   Thermostat_None thermostat{box, dt};
+ *   Thermostat* thermostat = new Thermostat_None{box, dt};
  *   hydrodynamics.initialize();
- *   for (long n = 0; n <= Steps; ++n) { thermostat.propagate(); hydrodynamics.step(n, 100 * dt); }
+ *   for (long n = 0; n <= Steps; ++n) { thermostat->propagate(); hydrodynamics.step(n, 100 * dt); }
  *
  * Differences, each forced by the device owning the state:
- *  - Box::Molecules and MPC::Fluid are host mirrors refreshed by
- *    Box::sync_molecules() / MPC::sync_fluid() (one device-to-host copy);
- *    writes go back through Box::upload_molecules() / MPC::upload_fluid().
+ *  - Box::Molecules and MPC::Fluid are host mirrors of the device state,
+ *    read from the device on the first access after the device changed it
+ *    and written back before the next device operation when the host took
+ *    a non-const reference into them (Mirror below).  A write of MPC::Fluid
+ *    replaces the solvent (spmd_set_fluid): the next collision sorts anew.
  *  - Errors the reference turns into raise(SIGSEGV) (MPC.cpp:216-219,
  *    Box.cpp:133-142) throw spmd::Error with the library's status code.
  *  - The random stream is Philox4x32-10 keyed by Box::Seed instead of the
  *    global mt19937_64(1) of Rand.cpp:6.
+ *  - Vector3d is a small value type with the Eigen members the drivers and
+ *    functors use (operator(), transpose(), norm(), printing), not Eigen.
  */
 #ifndef SPMD_FACADE_H
 #define SPMD_FACADE_H
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <complex>
-#include <deque>
 #include <cstdint>
 #include <cstdio>
+#include <deque>
+#include <functional>
 #include <iomanip>
 #include <map>
 #include <memory>
@@ -42,6 +50,8 @@
 #include <stdexcept>
 #include <string>
 #include <tuple>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/spmd.h"
@@ -60,31 +70,207 @@ inline void check(int rc, const spmd_sys* h) {
     throw Error(rc, m);
 }
 
-using Vector3d = std::array<double, 3>;
+/* Coefficients printed as Eigen's default IOFormat does (Eigen/src/Core/
+ * IO.h): each right-aligned to the widest of them in the stream's format,
+ * separated by sep. */
+inline void print_coeffs(std::ostream& os, const double* c, int n, const char* sep) {
+    std::string t[3];
+    std::size_t w = 0;
+    for (int i = 0; i < n; ++i) {
+        std::ostringstream q;
+        q.copyfmt(os);
+        q.width(0);
+        q << c[i];
+        t[i] = q.str();
+        w = std::max(w, t[i].size());
+    }
+    for (int i = 0; i < n; ++i) {
+        if (i) os << sep;
+        os << std::string(w - t[i].size(), ' ') << t[i];
+    }
+}
+
+/* The members of Eigen::Vector3d the reference's drivers and functors use. */
+struct Vector3d {
+    double v[3] = {0.0, 0.0, 0.0};
+    Vector3d() = default;
+    Vector3d(double a, double b, double c) : v{a, b, c} {}
+    static Vector3d Zero() { return Vector3d{}; }
+    double& operator[](int i) { return v[i]; }
+    const double& operator[](int i) const { return v[i]; }
+    double& operator()(int i) { return v[i]; }
+    const double& operator()(int i) const { return v[i]; }
+    double* data() { return v; }
+    const double* data() const { return v; }
+    static constexpr int size() { return 3; }
+    double* begin() { return v; }
+    double* end() { return v + 3; }
+    const double* begin() const { return v; }
+    const double* end() const { return v + 3; }
+    Vector3d& operator+=(const Vector3d& o) { for (int i = 0; i < 3; ++i) v[i] += o.v[i]; return *this; }
+    Vector3d& operator-=(const Vector3d& o) { for (int i = 0; i < 3; ++i) v[i] -= o.v[i]; return *this; }
+    Vector3d& operator*=(double a) { for (double& x : v) x *= a; return *this; }
+    Vector3d& operator/=(double a) { for (double& x : v) x /= a; return *this; }
+    friend Vector3d operator+(Vector3d a, const Vector3d& b) { return a += b; }
+    friend Vector3d operator-(Vector3d a, const Vector3d& b) { return a -= b; }
+    friend Vector3d operator-(Vector3d a) { return a *= -1.0; }
+    friend Vector3d operator*(Vector3d a, double s) { return a *= s; }
+    friend Vector3d operator*(double s, Vector3d a) { return a *= s; }
+    friend Vector3d operator/(Vector3d a, double s) { return a /= s; }
+    double dot(const Vector3d& o) const { return (v[0] * o.v[0] + v[1] * o.v[1]) + v[2] * o.v[2]; }
+    double squaredNorm() const { return dot(*this); }
+    double norm() const { return std::sqrt(squaredNorm()); }
+    /* the row view Eigen prints on one line */
+    struct Row {
+        const Vector3d& x;
+    };
+    Row transpose() const { return Row{*this}; }
+};
+inline std::ostream& operator<<(std::ostream& os, const Vector3d::Row& r) {
+    print_coeffs(os, r.x.v, 3, " ");
+    return os;
+}
+inline std::ostream& operator<<(std::ostream& os, const Vector3d& x) {
+    print_coeffs(os, x.v, 3, "\n");
+    return os;
+}
+
 /* Eigen's Matrix3d is column-major; data()[i] of the reference's printing
  * (star-polymers.cpp:477-479) is element i of this array */
 using Matrix3d = std::array<double, 9>;
 
-/* Particle.h:17-82 without the Eigen types */
+/* Particle.h:17-82 */
 struct MDParticle {
     Vector3d Position{}, Velocity{}, Force{};
     double Mass{};
 };
+struct MPCParticle {
+    Vector3d Position{}, Velocity{};
+};
 struct Molecule {
     std::vector<MDParticle> Monomers;
     unsigned NumberOfMonomers{};
+    /* Molecule::calculate_center_of_mass(_velocity) Molecule.cpp:76-92:
+     * plain means in monomer order */
+    Vector3d calculate_center_of_mass() const {
+        Vector3d c{};
+        for (auto& m : Monomers) c += m.Position;
+        return c /= NumberOfMonomers;
+    }
+    Vector3d calculate_center_of_mass_velocity() const {
+        Vector3d c{};
+        for (auto& m : Monomers) c += m.Velocity;
+        return c /= NumberOfMonomers;
+    }
+    /* Molecule::print Molecule.cpp:226-231 */
+    std::ostream& print(std::ostream& os) const {
+        for (auto& m : Monomers)
+            os << m.Position.transpose() << " ; " << m.Velocity.transpose() << " ; " << m.Force.transpose() << '\n';
+        return os;
+    }
 };
-struct MPCParticle {
-    std::array<float, 3> Position{}, Velocity{};
+inline std::ostream& operator<<(std::ostream& os, const Molecule& m) { return m.print(os); }
+
+/* A host mirror of device state: read from the device on the first access
+ * after invalidate(); a non-const access marks it written, and flush()
+ * (called before every device operation) then sends it back. */
+template <class T>
+class Mirror {
+    std::vector<T> v_;
+    std::function<void(std::vector<T>&)> pull_;
+    std::function<void(const std::vector<T>&)> push_;
+    bool stale_ = true, dirty_ = false;
+    void refresh() {
+        if (stale_ && pull_) {
+            pull_(v_);
+            stale_ = false;
+        }
+    }
+    Mirror& self() const { return const_cast<Mirror&>(*this); }
+
+public:
+    using value_type = T;
+    using iterator = typename std::vector<T>::iterator;
+    using const_iterator = typename std::vector<T>::const_iterator;
+    void bind(std::function<void(std::vector<T>&)> pull, std::function<void(const std::vector<T>&)> push) {
+        pull_ = std::move(pull);
+        push_ = std::move(push);
+        stale_ = true;
+        dirty_ = false;
+    }
+    /* the device changed the state */
+    void invalidate() { stale_ = true; }
+    /* host writes to the device (before a device operation) */
+    void flush() {
+        if (dirty_ && push_) push_(v_);
+        dirty_ = false;
+    }
+    bool written() const { return dirty_; }
+    /* read again now */
+    void sync() {
+        stale_ = true;
+        refresh();
+    }
+    std::size_t size() const { self().refresh(); return v_.size(); }
+    bool empty() const { return size() == 0; }
+    const T& operator[](std::size_t i) const { self().refresh(); return v_[i]; }
+    T& operator[](std::size_t i) { refresh(); dirty_ = true; return v_[i]; }
+    const T& at(std::size_t i) const { self().refresh(); return v_.at(i); }
+    T& at(std::size_t i) { refresh(); dirty_ = true; return v_.at(i); }
+    const_iterator begin() const { self().refresh(); return v_.cbegin(); }
+    const_iterator end() const { self().refresh(); return v_.cend(); }
+    const_iterator cbegin() const { return begin(); }
+    const_iterator cend() const { return end(); }
+    iterator begin() { refresh(); dirty_ = true; return v_.begin(); }
+    iterator end() { refresh(); return v_.end(); }
+    /* the host copy as it stands (no device read) */
+    const std::vector<T>& host() const { return v_; }
 };
+
+class MPC;
 
 class Box {
     friend class MPC;
+    friend class Thermostat;
     friend class Thermostat_None;
     std::shared_ptr<spmd_sys> h_;
+    Mirror<MPCParticle>* fluid_ = nullptr;  /* the MPC's solvent mirror */
     spmd_sys* handle() const {
         if (!h_) throw Error(SPMD_ERR_STATE, "construct the MPC on this Box first (star-polymers.cpp:353)");
         return h_.get();
+    }
+    void bind_molecules() {
+        Molecules.bind(
+            [this](std::vector<Molecule>& out) {
+                const unsigned n = numberOfMonomers();
+                out.assign(n ? 1 : 0, Molecule{});
+                if (!n) return;
+                std::vector<double> p(3 * n), v(3 * n), f(3 * n);
+                check(spmd_get_monomers(handle(), 0, p.data(), v.data(), f.data()), h_.get());
+                Molecule& m = out[0];
+                m.NumberOfMonomers = n;
+                m.Monomers.resize(n);
+                for (unsigned i = 0; i < n; ++i)
+                    for (int c = 0; c < 3; ++c) {
+                        m.Monomers[i].Position[c] = p[3 * i + c];
+                        m.Monomers[i].Velocity[c] = v[3 * i + c];
+                        m.Monomers[i].Force[c] = f[3 * i + c];
+                    }
+                std::vector<double> mass(n);
+                if (spmd_get_monomer_masses(handle(), 0, mass.data()) == SPMD_OK)
+                    for (unsigned i = 0; i < n; ++i) m.Monomers[i].Mass = mass[i];
+            },
+            [this](const std::vector<Molecule>& in) {
+                if (in.empty()) return;
+                const Molecule& m = in[0];
+                std::vector<double> p(3 * m.Monomers.size()), v(3 * m.Monomers.size());
+                for (std::size_t i = 0; i < m.Monomers.size(); ++i)
+                    for (int c = 0; c < 3; ++c) {
+                        p[3 * i + c] = m.Monomers[i].Position[c];
+                        v[3 * i + c] = m.Monomers[i].Velocity[c];
+                    }
+                check(spmd_set_monomers(handle(), 0, p.data(), v.data()), h_.get());
+            });
     }
 
 public:
@@ -93,57 +279,59 @@ public:
     std::array<int, 3> BoxSize;
     uint64_t Seed = 1;
     double DeltaT = 0.01;
-    std::vector<Molecule> Molecules;
+    Mirror<Molecule> Molecules;
 
     /* Box::Box(Lx, Ly, Lz, T, Lambda) Box.cpp:4-27 (odd sides are rounded
      * up to even by the library, Box.cpp:14-16) */
     Box(int Lx, int Ly, int Lz, double temperature, double lambda)
         : Temperature(temperature), Lambda(lambda), BoxSize{Lx + (Lx & 1), Ly + (Ly & 1), Lz + (Lz & 1)} {}
+    Box(const Box&) = delete;
+    Box& operator=(const Box&) = delete;
+
+    /* before a device operation: host writes of the mirrors go down */
+    void before_device() {
+        Molecules.flush();
+        if (fluid_) fluid_->flush();
+    }
+    /* after a device operation that may change the state */
+    void after_device() {
+        Molecules.invalidate();
+        if (fluid_) fluid_->invalidate();
+    }
 
     /* Box::add_star Box.cpp:39-46 */
     void add_star(unsigned A, unsigned B, unsigned Arms, double Mass, double Bond = 1.01, double AnchorBond = 2.0) {
+        before_device();
         check(spmd_add_star(handle(), 0, A, B, Arms, Mass, Bond, AnchorBond), h_.get());
-        sync_molecules();
+        after_device();
     }
     /* seeded random-coil arms (no reference counterpart) */
     void add_star_coil(unsigned A, unsigned B, unsigned Arms, double Mass, double Bond = 1.01,
                        double AnchorBond = 2.0) {
+        before_device();
         check(spmd_add_star_coil(handle(), 0, A, B, Arms, Mass, Bond, AnchorBond), h_.get());
-        sync_molecules();
+        after_device();
+    }
+    /* Box::add_star(filename, A, B, Arms, Mass, set_zero) Box.cpp:48-54 */
+    void add_star(const std::string& filename, unsigned A, unsigned B, unsigned Arms, double Mass, bool set_zero) {
+        before_device();
+        check(spmd_add_star_pdb(handle(), 0, filename.c_str(), A, B, Arms, Mass, set_zero ? 1 : 0), h_.get());
+        after_device();
     }
     unsigned numberOfMonomers() const {
         uint32_t n = 0;
         check(spmd_num_monomers(handle(), 0, &n), h_.get());
         return n;
     }
-    /* Molecules[0] from the device */
+    /* Molecules read from the device now / host edits written now */
     void sync_molecules() {
-        const unsigned n = numberOfMonomers();
-        std::vector<double> p(3 * n), v(3 * n), f(3 * n);
-        if (n) check(spmd_get_monomers(handle(), 0, p.data(), v.data(), f.data()), h_.get());
-        Molecules.assign(n ? 1 : 0, Molecule{});
-        if (!n) return;
-        Molecule& m = Molecules[0];
-        m.NumberOfMonomers = n;
-        m.Monomers.resize(n);
-        for (unsigned i = 0; i < n; ++i)
-            for (int c = 0; c < 3; ++c) {
-                m.Monomers[i].Position[c] = p[3 * i + c];
-                m.Monomers[i].Velocity[c] = v[3 * i + c];
-                m.Monomers[i].Force[c] = f[3 * i + c];
-            }
+        before_device();
+        Molecules.sync();
     }
-    /* Molecules[0] positions and velocities to the device */
-    void upload_molecules() {
-        if (Molecules.empty()) return;
-        const Molecule& m = Molecules[0];
-        std::vector<double> p(3 * m.Monomers.size()), v(3 * m.Monomers.size());
-        for (size_t i = 0; i < m.Monomers.size(); ++i)
-            for (int c = 0; c < 3; ++c) { p[3 * i + c] = m.Monomers[i].Position[c]; v[3 * i + c] = m.Monomers[i].Velocity[c]; }
-        check(spmd_set_monomers(handle(), 0, p.data(), v.data()), h_.get());
-    }
+    void upload_molecules() { Molecules.flush(); }
     /* Molecule::Epot after the last propagate(true) */
-    double epot() const {
+    double epot() {
+        before_device();
         double e = 0.0;
         check(spmd_epot(handle(), 0, &e), h_.get());
         return e;
@@ -154,64 +342,103 @@ public:
     /* Box::center_of_mass_reference_frame Box.cpp:81-102: COM_Pos / COM_Vel
      * (the device observers compute the frame themselves) */
     void center_of_mass_reference_frame() {
-        sync_molecules();
         COM_Pos = COM_Vel = Vector3d{};
-        if (Molecules.empty()) return;
-        for (auto& m : Molecules[0].Monomers)
-            for (int c = 0; c < 3; ++c) { COM_Pos[c] += m.Position[c]; COM_Vel[c] += m.Velocity[c]; }
-        for (int c = 0; c < 3; ++c) { COM_Pos[c] /= Molecules[0].NumberOfMonomers; COM_Vel[c] /= Molecules[0].NumberOfMonomers; }
+        const Mirror<Molecule>& mols = Molecules;
+        if (mols.empty()) return;
+        COM_Pos = mols[0].calculate_center_of_mass();
+        COM_Vel = mols[0].calculate_center_of_mass_velocity();
+    }
+    /* Box::print_center_of_mass Box.cpp:654-660 */
+    std::ostream& print_center_of_mass(std::ostream& os) const {
+        for (const Molecule& mol : Molecules)
+            os << mol.calculate_center_of_mass().transpose() << " " << mol.calculate_center_of_mass_velocity().transpose()
+               << '\n';
+        return os;
+    }
+    /* Box::print_molecules Box.cpp:517-522 */
+    std::ostream& print_molecules(std::ostream& os) const {
+        for (const Molecule& mol : Molecules) os << mol << std::endl;
+        return os;
     }
     /* the 18 values of a statistics row (spmd_statistics) */
-    std::array<double, 18> statistics() const {
+    std::array<double, 18> statistics() {
+        before_device();
         std::array<double, 18> row{};
         check(spmd_statistics(handle(), 0, row.data()), h_.get());
         return row;
     }
     /* Box::calculate_patches_new Box.cpp:434-507 */
-    std::vector<double> calculate_patches_new() const {
+    std::vector<double> calculate_patches_new() {
+        before_device();
         std::vector<double> r(2, 0.0);
         check(spmd_patches(handle(), 0, &r[0], &r[1]), h_.get());
         return r;
     }
     /* Box::calculate_gyration_tensor Box.cpp:330-352: (R_gyr, G) */
-    std::tuple<double, Matrix3d> calculate_gyration_tensor() const {
+    std::tuple<double, Matrix3d> calculate_gyration_tensor() {
+        before_device();
         Matrix3d g{};
         check(spmd_gyration_tensor(handle(), 0, g.data()), h_.get());
         return std::make_tuple(std::sqrt((g[0] + g[4]) + g[8]), g);
     }
     /* Box::calculate_rotation_frequency Box.cpp:354-379 */
-    Vector3d calculate_rotation_frequency() const {
+    Vector3d calculate_rotation_frequency() {
+        before_device();
         Vector3d w{};
         check(spmd_rotation_frequency(handle(), 0, w.data()), h_.get());
         return w;
     }
     /* Box::print_Epot Box.cpp:605-613 */
-    std::ostream& print_Epot(std::ostream& os) const { return os << statistics()[0] << " "; }
+    std::ostream& print_Epot(std::ostream& os) { return os << statistics()[0] << " "; }
     /* Box::print_Ekin_and_Temperature Box.cpp:635-646 */
-    std::ostream& print_Ekin_and_Temperature(std::ostream& os) const {
+    std::ostream& print_Ekin_and_Temperature(std::ostream& os) {
         const auto r = statistics();
         return os << r[1] << " " << r[2] << " ";
     }
     /* Box::print_PDB_with_velocity Box.cpp:563-602, to a file (appended) */
     void print_PDB_with_velocity(const std::string& path, int step, bool append = true) {
+        before_device();
         check(spmd_write_pdb(handle(), 0, path.c_str(), step, append ? 1 : 0), h_.get());
-    }
-    /* Box::add_star(filename, A, B, Arms, Mass, set_zero) Box.cpp:48-54 */
-    void add_star(const std::string& filename, unsigned A, unsigned B, unsigned Arms, double Mass, bool set_zero) {
-        check(spmd_add_star_pdb(handle(), 0, filename.c_str(), A, B, Arms, Mass, set_zero ? 1 : 0), h_.get());
-        sync_molecules();
     }
 };
 
-/* Hydrodynamics.h:7-22 interface subset used by the drivers */
-class MPC {
+/* Hydrodynamics.h:7-22: the solvent interface the driver holds */
+class Hydrodynamics {
+protected:
     Box& SimBox;
+    ~Hydrodynamics() = default;
 
 public:
-    std::vector<MPCParticle> Fluid;
+    explicit Hydrodynamics(Box& box) : SimBox(box) {}
+    virtual void initialize() {}
+    virtual void step(const long int& /*t*/, const double& /*dt*/) {}
+    virtual double calculateCurrentTemperature() { return 0.0; }
+    virtual void print_fluid(FILE*, int, int, int) {}
+    virtual void print_fluid_with_coordinates(FILE*, int, int, int) {}
+};
+
+/* MPC.h:25-122 */
+class MPC : public Hydrodynamics {
+    spmd_sys* h() const { return SimBox.handle(); }
+    /* observers with a device pass (sample(spmd_sys*)) run it; any other
+     * functor is applied to every particle of the solvent mirror */
+    template <class F>
+    auto apply(F& f, int) -> decltype(f.sample(static_cast<spmd_sys*>(nullptr)), void()) {
+        SimBox.before_device();
+        f.sample(h());
+    }
+    template <class F>
+    void apply(F& f, long) {
+        const Mirror<MPCParticle>& fl = Fluid;
+        for (auto it = fl.cbegin(), last = fl.cend(); it != last; ++it) f(*it);
+    }
+
+public:
+    Mirror<MPCParticle> Fluid;
 
     /* MPC::MPC(Box&, T, N_c, aShear, step, angular_mom) MPC.h:44 */
-    MPC(Box& box, double T, int N_c, double aShear = 0., unsigned step = 1, bool angular_mom = false) : SimBox(box) {
+    MPC(Box& box, double T, int N_c, double aShear = 0., unsigned step = 1, bool angular_mom = false)
+        : Hydrodynamics(box) {
         spmd_params p{};
         for (int c = 0; c < 3; ++c) p.L[c] = box.BoxSize[c];
         p.kT = T;
@@ -223,63 +450,168 @@ public:
         p.md_dt = box.DeltaT;
         p.seed = box.Seed;
         p.angular_momentum = angular_mom ? 1 : 0;
-        spmd_sys* h = nullptr;
-        check(spmd_create(&p, &h), nullptr);
-        box.h_ = std::shared_ptr<spmd_sys>(h, [](spmd_sys* q) { spmd_destroy(q); });
+        spmd_sys* hh = nullptr;
+        check(spmd_create(&p, &hh), nullptr);
+        box.h_ = std::shared_ptr<spmd_sys>(hh, [](spmd_sys* q) { spmd_destroy(q); });
+        box.bind_molecules();
+        box.fluid_ = &Fluid;
+        Fluid.bind(
+            [this](std::vector<MPCParticle>& out) {
+                const unsigned n = NumberOfParticles();
+                std::vector<float> pp(3ull * n), vv(3ull * n);
+                check(spmd_get_fluid(h(), 0, pp.data(), vv.data(), nullptr), h());
+                out.resize(n);
+                for (unsigned i = 0; i < n; ++i)
+                    for (int c = 0; c < 3; ++c) {
+                        out[i].Position[c] = pp[3ull * i + c];
+                        out[i].Velocity[c] = vv[3ull * i + c];
+                    }
+            },
+            [this](const std::vector<MPCParticle>& in) {
+                std::vector<float> pp(3 * in.size()), vv(3 * in.size());
+                for (std::size_t i = 0; i < in.size(); ++i)
+                    for (int c = 0; c < 3; ++c) {
+                        pp[3 * i + c] = (float)in[i].Position[c];
+                        vv[3 * i + c] = (float)in[i].Velocity[c];
+                    }
+                check(spmd_set_fluid(h(), 0, pp.data(), vv.data(), in.size()), h());
+            });
+    }
+    MPC(const MPC&) = delete;
+    MPC& operator=(const MPC&) = delete;
+    ~MPC() {
+        if (SimBox.fluid_ == &Fluid) SimBox.fluid_ = nullptr;
     }
     /* MPC::initialize() MPC.cpp:34-55 */
-    void initialize() { check(spmd_fluid_initialize(SimBox.handle()), SimBox.handle()); }
+    void initialize() override {
+        SimBox.before_device();
+        check(spmd_fluid_initialize(h()), h());
+        SimBox.after_device();
+    }
     /* MPC::initialize(filename, Box_old) MPC.cpp:57-113 */
     void initialize(const std::string& filename, std::array<int, 3> Box_old) {
+        SimBox.before_device();
         const int32_t ob[3] = {Box_old[0], Box_old[1], Box_old[2]};
-        check(spmd_read_fluid(SimBox.handle(), 0, filename.c_str(), ob, nullptr), SimBox.handle());
-    }
-    /* MPC::print_fluid_complete MPC.cpp:570-579, to a file */
-    void print_fluid_complete(const std::string& path) {
-        check(spmd_write_fluid(SimBox.handle(), 0, path.c_str()), SimBox.handle());
-    }
-    /* MPC::operator()(functor) MPC.h:110-116 for the device observers */
-    template <class Observer>
-    Observer& operator()(Observer& f) {
-        f.sample(SimBox.handle());
-        return f;
+        check(spmd_read_fluid(h(), 0, filename.c_str(), ob, nullptr), h());
+        SimBox.after_device();
     }
     /* MPC::step(t, dt) MPC.cpp:116-142 */
-    void step(const long int& t, const double& dt) { check(spmd_mpc_step(SimBox.handle(), t, dt), SimBox.handle()); }
+    void step(const long int& t, const double& dt) override {
+        SimBox.before_device();
+        check(spmd_mpc_step(h(), t, dt), h());
+        SimBox.after_device();
+    }
     /* MPC::calculateCurrentTemperature MPC.cpp:405-409.  Before the first
      * collision after the fluid was set (a continued run, MPC.cpp:57-113,
      * has not sorted yet) the reference's cell lists are empty and the
      * result is 0; the library reports that state as SPMD_ERR_STATE. */
-    double calculateCurrentTemperature() {
+    double calculateCurrentTemperature() override {
+        SimBox.before_device();
         double T = 0.0;
-        const int rc = spmd_temperature(SimBox.handle(), 0, &T);
+        const int rc = spmd_temperature(h(), 0, &T);
         if (rc == SPMD_ERR_STATE) return 0.0;
-        check(rc, SimBox.handle());
+        check(rc, h());
         return T;
+    }
+    /* MPC::print_fluid MPC.cpp:533-553: per z plane the cell centres, then
+     * the mean solvent velocity (x, y components) of every cell of the last
+     * collision, one row per x */
+    void print_fluid(FILE* fluid_file, int step, int z_start, int z_stop) override {
+        const int Lx = SimBox.BoxSize[0], Ly = SimBox.BoxSize[1];
+        std::vector<double> cv = cell_velocities(z_start, z_stop);
+        std::fprintf(fluid_file, "TIME %d \n", step);
+        for (int z = z_start; z <= z_stop; z++) {
+            std::fprintf(fluid_file, "Z %d \n", z);
+            for (int x = 0; x < Lx; x++) std::fprintf(fluid_file, "%f ", x + 0.5);
+            std::fprintf(fluid_file, "\n");
+            for (int y = 0; y < Ly; y++) std::fprintf(fluid_file, "%f ", y + 0.5);
+            std::fprintf(fluid_file, "\n");
+            for (int x = 0; x < Lx; x++) {
+                for (int y = 0; y < Ly; y++) {
+                    const double* c = &cv[3 * ((std::size_t)x + (std::size_t)Lx * (y + (std::size_t)Ly * (z - z_start)))];
+                    std::fprintf(fluid_file, "%f %f ", c[0], c[1]);
+                }
+                std::fprintf(fluid_file, "\n");
+            }
+            std::fprintf(fluid_file, "\n");
+        }
+        std::fprintf(fluid_file, "\n");
+    }
+    /* MPC::print_fluid_with_coordinates MPC.cpp:555-568 (z_stop exclusive) */
+    void print_fluid_with_coordinates(FILE* fluid_file, int step, int z_start, int z_stop) override {
+        const int Lx = SimBox.BoxSize[0], Ly = SimBox.BoxSize[1];
+        std::fprintf(fluid_file, "TIME     %d \n", step);
+        if (z_stop > z_start) {
+            std::vector<double> cv = cell_velocities(z_start, z_stop - 1);
+            for (int z = z_start; z < z_stop; z++)
+                for (int x = 0; x < Lx; x++)
+                    for (int y = 0; y < Ly; y++) {
+                        const double* c =
+                            &cv[3 * ((std::size_t)x + (std::size_t)Lx * (y + (std::size_t)Ly * (z - z_start)))];
+                        std::fprintf(fluid_file, "%f %f %f %f %f %f \n", x + 0.5, y + 0.5, z + 0.5, c[0], c[1], c[2]);
+                    }
+        }
+        std::fprintf(fluid_file, "\n");
+    }
+    /* MPC::calculateFluidVelocity (MPC.cpp:334-342) of the cells with
+     * z_start <= z <= z_stop, 3 per cell in cell-index order */
+    std::vector<double> cell_velocities(int z_start, int z_stop) {
+        SimBox.before_device();
+        std::vector<double> cv(3ull * SimBox.BoxSize[0] * SimBox.BoxSize[1] * (z_stop - z_start + 1));
+        check(spmd_cell_fluid_velocity(h(), 0, z_start, z_stop, cv.data()), h());
+        return cv;
+    }
+    /* MPC::print_fluid_complete MPC.cpp:570-579 */
+    void print_fluid_complete(FILE* fluid_file) {
+        for (const MPCParticle& part : static_cast<const Mirror<MPCParticle>&>(Fluid))
+            std::fprintf(fluid_file, "%f %f %f %f %f %f \n", part.Position(0), part.Position(1), part.Position(2),
+                         part.Velocity(0), part.Velocity(1), part.Velocity(2));
+    }
+    /* the same, written by the library to a file */
+    void print_fluid_complete(const std::string& path) {
+        SimBox.before_device();
+        check(spmd_write_fluid(h(), 0, path.c_str()), h());
+    }
+    /* MPC::operator()(UnaryFunc&) MPC.h:110-116: the functor on every
+     * solvent particle (device observers: their own pass) */
+    template <class UnaryFunc>
+    void operator()(UnaryFunc& ufunc) {
+        apply(ufunc, 0);
+    }
+    /* MPC::operator()(UnaryFunc&, BinaryFunc&) MPC.h:98-108 */
+    template <class UnaryFunc, class BinaryFunc>
+    void operator()(UnaryFunc& ufunc, BinaryFunc& bfunc) {
+        const Mirror<MPCParticle>& fl = Fluid;
+        for (auto first = fl.cbegin(), last = fl.cend(); first != last; ++first) {
+            ufunc(*first);
+            for (auto second = first + 1; second != last; ++second) bfunc(*first, *second);
+        }
+    }
+    /* MPC::unitary MPC.h:93-96 */
+    template <class UnitaryFunc>
+    UnitaryFunc unitary(UnitaryFunc& func) {
+        const Mirror<MPCParticle>& fl = Fluid;
+        return std::for_each(fl.cbegin(), fl.cend(), func);
     }
     /* MPC::NumberOfParticles MPC.h:91 */
     unsigned NumberOfParticles() const {
         uint64_t n = 0;
-        check(spmd_num_fluid(SimBox.handle(), 0, &n), SimBox.handle());
+        check(spmd_num_fluid(h(), 0, &n), h());
         return (unsigned)n;
     }
+    /* Fluid read from the device now / host edits written now */
     void sync_fluid() {
-        const unsigned n = NumberOfParticles();
-        std::vector<float> p(3ull * n), v(3ull * n);
-        check(spmd_get_fluid(SimBox.handle(), 0, p.data(), v.data(), nullptr), SimBox.handle());
-        Fluid.resize(n);
-        for (unsigned i = 0; i < n; ++i)
-            for (int c = 0; c < 3; ++c) { Fluid[i].Position[c] = p[3ull * i + c]; Fluid[i].Velocity[c] = v[3ull * i + c]; }
+        SimBox.before_device();
+        Fluid.sync();
     }
-    void upload_fluid() {
-        std::vector<float> p(3 * Fluid.size()), v(3 * Fluid.size());
-        for (size_t i = 0; i < Fluid.size(); ++i)
-            for (int c = 0; c < 3; ++c) { p[3 * i + c] = Fluid[i].Position[c]; v[3 * i + c] = Fluid[i].Velocity[c]; }
-        check(spmd_set_fluid(SimBox.handle(), 0, p.data(), v.data(), Fluid.size()), SimBox.handle());
-    }
+    void upload_fluid() { Fluid.flush(); }
     /* the star-polymers.cpp:412-536 loop body n times from t0, replayed from a HIP graph */
-    void run(long t0, long n, double dt) { check(spmd_run(SimBox.handle(), t0, n, dt), SimBox.handle()); }
-    void sync() { check(spmd_sync(SimBox.handle()), SimBox.handle()); }
+    void run(long t0, long n, double dt) {
+        SimBox.before_device();
+        check(spmd_run(h(), t0, n, dt), h());
+        SimBox.after_device();
+    }
+    void sync() { check(spmd_sync(h()), h()); }
 };
 
 /* VelocityX (Velocity_x.h:22-148): mean solvent velocity relative to the
@@ -507,27 +839,53 @@ inline RunParams parse_restart_name(const std::string& path) {
     return p;
 }
 
-/* Thermostat_None (Thermostat_None.h:14, Thermostat_None.cpp:5-38) */
-class Thermostat_None {
+/* Thermostat.h:7-22: the MD integrator interface the driver holds
+ * (star-polymers.cpp:101 Thermostat* thermostat) */
+class Thermostat {
+protected:
     Box& SimBox;
     double DeltaT;
+    ~Thermostat() = default;
 
 public:
-    Thermostat_None(Box& box, double dt, bool gaussian = false) : SimBox(box), DeltaT(dt) {
-        if (gaussian) throw Error(SPMD_ERR_UNSUPPORTED, "Gaussian chains are not on this path");
-        check(spmd_set_md_dt(box.handle(), dt), box.handle());
-        check(spmd_md_init(box.handle()), box.handle());
-    }
-    void propagate(bool calc_epot = false) {
-        check(spmd_md_propagate(SimBox.handle(), calc_epot ? 1 : 0), SimBox.handle());
-    }
+    Thermostat(Box& box, double dt) : SimBox(box), DeltaT(dt) {}
     double dtime() const { return DeltaT; }
-    void dtime(double& delta_time) {
-        DeltaT = delta_time;
+    virtual void dtime(double& new_dtime) { DeltaT = new_dtime; }
+    virtual void update_temp() = 0;
+    virtual void propagate(bool calc_epot = false) = 0;
+    virtual void propagate_gaussian(bool calc_epot = false) = 0;
+    virtual std::string name() const = 0;
+    virtual std::string info() const = 0;
+};
+
+/* Thermostat_None (Thermostat_None.h:9-21, Thermostat_None.cpp:5-38):
+ * velocity Verlet of the star on the device; the solvent is MPC's */
+class Thermostat_None final : public Thermostat {
+public:
+    Thermostat_None(Box& box, double dt, bool gaussian = false) : Thermostat(box, dt) {
+        if (gaussian) throw Error(SPMD_ERR_UNSUPPORTED, "Gaussian chains are not on this path");
+        update_temp();
+        Thermostat_None::dtime(dt);
+        box.before_device();
+        check(spmd_md_init(box.handle()), box.handle());
+        box.after_device();
+    }
+    using Thermostat::dtime;
+    void update_temp() override {}
+    void dtime(double& delta_time) override {
+        Thermostat::dtime(delta_time);
         check(spmd_set_md_dt(SimBox.handle(), delta_time), SimBox.handle());
     }
-    std::string name() const { return "None"; }
-    std::string info() const { return "velocity Verlet without thermostat (solvent: MPC)"; }
+    void propagate(bool calc_epot = false) override {
+        SimBox.before_device();
+        check(spmd_md_propagate(SimBox.handle(), calc_epot ? 1 : 0), SimBox.handle());
+        SimBox.after_device();
+    }
+    void propagate_gaussian(bool /*calc_epot*/ = false) override {
+        throw Error(SPMD_ERR_UNSUPPORTED, "Gaussian chains are not on this path");
+    }
+    std::string name() const override { return "None"; }
+    std::string info() const override { return "velocity Verlet without thermostat (solvent: MPC)"; }
 };
 
 }  // namespace spmd

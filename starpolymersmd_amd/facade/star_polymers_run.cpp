@@ -4,8 +4,12 @@
  * facade.  Prints the temperature, Epot and the star's centre of mass every
  * `Out` steps; with an output directory it also writes the reference's
  * files there: statistics (one row per output step, star-polymers.cpp:
- * 300-307, 466-482), fluid_profile (VelocityX, :486, 530-534), end_config
- * (.pdb) and end_fluid (the restart files of :446-458).
+ * 300-307, 466-482), fluid_profile (VelocityX, :486, 530-534), output
+ * (the step and Box::print_center_of_mass, :488-491), fluid
+ * (MPC::print_fluid of the middle z plane, :520-523), end_config (.pdb)
+ * and end_fluid (the restart files of :446-458).  As in the reference the
+ * loop holds the integrator as a Thermostat* (:101, 382) and steps the
+ * solvent through its Hydrodynamics interface.
  *
  *   star_polymers_run L A B Arms Shear Steps Out [dt] [N_c] [coil] [outdir] [tagged]
  *   star_polymers_run <dir>/end_config<tag>.pdb Steps Out [dt] [N_c] [outdir]
@@ -31,6 +35,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iostream>
 #include <string>
 
 #include "spmd_facade.h"
@@ -120,7 +125,10 @@ int main(int argc, char** argv) {
         if (cont) box.add_star(restart, rp.A, rp.B, rp.Arms, (double)rp.Mass, n0 == 1);
         else if (coil) box.add_star_coil(rp.A, rp.B, rp.Arms, (double)rp.Mass);
         else box.add_star(rp.A, rp.B, rp.Arms, (double)rp.Mass);
-        spmd::Thermostat_None thermostat{box, dt};
+        std::cout << '\n';
+        box.print_center_of_mass(std::cout);  /* star-polymers.cpp:371-372 */
+        spmd::Thermostat* thermostat = new spmd::Thermostat_None{box, dt};
+        spmd::Hydrodynamics& hydro = hydrodynamics;
         std::string fluid_in;
         if (cont) {
             fluid_in = restart;
@@ -133,10 +141,13 @@ int main(int argc, char** argv) {
         else
             hydrodynamics.initialize();
         spmd::VelocityX velocity_average_x{box, hydrodynamics, 0.5};
-        std::ofstream statistic_file;
+        std::ofstream statistic_file, output_file;
+        FILE* fluid_file = nullptr;
         if (!outdir.empty()) {
             statistic_file.open(name("statistics", tag, ".dat"), std::ios::out | std::ios::trunc);
             statistic_file << spmd::statistics_header();
+            output_file.open(name("output", tag, ".dat"), std::ios::out | std::ios::trunc);
+            fluid_file = std::fopen(name("fluid", tag, ".dat").c_str(), "w");
         }
         for (long n = n0; n <= Steps; ++n) {
             if (g_signal) {
@@ -159,7 +170,7 @@ int main(int argc, char** argv) {
                 return sig;
             }
             if (n > 0 && !(n % Out)) {
-                thermostat.propagate(true);
+                thermostat->propagate(true);
                 if (statistic_file.is_open()) {
                     /* star-polymers.cpp:470-482 */
                     box.center_of_mass_reference_frame();
@@ -175,22 +186,26 @@ int main(int argc, char** argv) {
                     statistic_file << "\n";
                     statistic_file.flush();
                     hydrodynamics(velocity_average_x);
+                    output_file << n << " ";
+                    box.print_center_of_mass(output_file);
+                    output_file << '\n';
+                    output_file.flush();
+                    hydro.print_fluid(fluid_file, (int)n, rp.Lz / 2, rp.Lz / 2);
+                    std::fflush(fluid_file);
                 }
-                box.sync_molecules();
-                double com[3] = {0, 0, 0};
-                for (auto& m : box.Molecules[0].Monomers)
-                    for (int c = 0; c < 3; ++c) com[c] += m.Position[c];
-                const double inv = 1.0 / box.Molecules[0].Monomers.size();
-                std::printf("%ld T=%.6f Epot=%.6f com=(%.4f,%.4f,%.4f)\n", n,
-                            hydrodynamics.calculateCurrentTemperature(), box.epot(), com[0] * inv, com[1] * inv,
-                            com[2] * inv);
+                const spmd::Mirror<spmd::Molecule>& mols = box.Molecules;  /* read only: nothing to write back */
+                const spmd::Vector3d com = mols[0].calculate_center_of_mass();
+                std::printf("%ld T=%.6f Epot=%.6f com=(%.4f,%.4f,%.4f)\n", n, hydro.calculateCurrentTemperature(),
+                            box.epot(), com[0], com[1], com[2]);
                 std::fflush(stdout);
             } else {
-                thermostat.propagate(false);
+                thermostat->propagate(false);
             }
-            hydrodynamics.step(n, 10 * dt);
+            hydro.step(n, 10 * dt);
         }
         hydrodynamics.sync();
+        if (fluid_file) std::fclose(fluid_file);
+        delete static_cast<spmd::Thermostat_None*>(thermostat);
         if (!outdir.empty()) {
             std::ofstream profile(name("fluid_profile", tag, ".dat"), std::ios::out | std::ios::trunc);
             velocity_average_x.print_result(profile);

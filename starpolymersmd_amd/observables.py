@@ -240,6 +240,44 @@ def statistics_row(step: int, row) -> str:
     return f"{int(step)} " + "".join("%g " % float(v) for v in row) + "\n"
 
 
+def eigen_row(values) -> str:
+    """A row vector as Eigen's default IOFormat prints it (Eigen/src/Core/
+    IO.h): every coefficient in the stream's default format (6 significant
+    digits), right-aligned to the widest of them, separated by spaces."""
+    t = ["%g" % float(v) for v in values]
+    w = max(len(x) for x in t)
+    return " ".join(x.rjust(w) for x in t)
+
+
+def center_of_mass_text(system, r: int = 0) -> str:
+    """Box::print_center_of_mass (Box.cpp:654-660): per molecule the mean
+    position and the mean velocity of its monomers (Molecule.cpp:76-92) on
+    one line."""
+    pos, vel, _ = system.get_monomers(r)
+    if len(pos) == 0:
+        return ""
+    n = len(pos)   # sums in monomer order, as the reference's loop
+    return eigen_row(pos.cumsum(axis=0)[-1] / n) + " " + eigen_row(vel.cumsum(axis=0)[-1] / n) + "\n"
+
+
+def print_fluid_text(system, step: int, z_start: int, z_stop: int, r: int = 0) -> str:
+    """MPC::print_fluid (MPC.cpp:533-553): per z plane the cell centres,
+    then for x = 0..Lx-1 one row of the x and y components of the mean
+    solvent velocity of every cell (x, y, z) of the last collision."""
+    cv = system.cell_fluid_velocity(z_start, z_stop, r)   # [z, y, x, 3]
+    nz, ly, lx, _ = cv.shape
+    out = ["TIME %d \n" % step]
+    for k in range(nz):
+        out.append("Z %d \n" % (z_start + k))
+        out.append("".join("%f " % (x + 0.5) for x in range(lx)) + "\n")
+        out.append("".join("%f " % (y + 0.5) for y in range(ly)) + "\n")
+        for x in range(lx):
+            out.append("".join("%f %f " % (cv[k, y, x, 0], cv[k, y, x, 1]) for y in range(ly)) + "\n")
+        out.append("\n")
+    out.append("\n")
+    return "".join(out)
+
+
 class StatisticsFile:
     """statistics*.dat (star-polymers.cpp:300-307, 470-482): the header on
     an empty file, then one row per output step."""

@@ -220,6 +220,15 @@ class MPCSystem:
         self._chk(self._lib.spmd_temperature(self.h, r, C.byref(v)))
         return v.value
 
+    def cell_fluid_velocity(self, z_start: int, z_stop: int, r: int = 0) -> np.ndarray:
+        """MPC::calculateFluidVelocity (MPC.cpp:334-342) of the cells with
+        z_start <= z <= z_stop: array [z, y, x, 3] of the mean solvent
+        velocity per cell of the last collision (0 where empty)."""
+        L = [int(l) + int(l) % 2 for l in self.params[r].L]   # even-rounded box (Box.cpp:14-19)
+        out = np.zeros(((z_stop - z_start + 1) * L[0] * L[1] * 3,), dtype=np.float64)
+        self._chk(self._lib.spmd_cell_fluid_velocity(self.h, r, z_start, z_stop, ptr(out, C.c_double)))
+        return out.reshape(z_stop - z_start + 1, L[1], L[0], 3)
+
     def epot(self, r: int = 0) -> float:
         v = C.c_double()
         self._chk(self._lib.spmd_epot(self.h, r, C.byref(v)))

@@ -176,3 +176,54 @@ def test_run_mpc_test_writes_reference_files(tmp_path):
     assert abs(lag[1:, 1]).max() < 0.05  # 10 collisions apart: decorrelated
     assert len(open(names["fourier"]).read().splitlines()) == 50
     assert open(names["maxwell"]).read().strip() and open(names["fluid"]).read().strip()
+
+
+@pytest.mark.parametrize("star,shear", [(True, 0.05), (False, 0.0)])
+def test_cell_fluid_velocity_matches_oracle(star, shear):
+    """MPC::calculateFluidVelocity (MPC.cpp:334-342), as MPC::print_fluid
+    reads it (MPC.cpp:533-553): after one collision from the same state
+    with the same injected randoms, the mean solvent velocity of every
+    cell of that collision's sort equals the oracle's mean over the
+    particles its cell index puts there (monomers excluded) within 1e-6;
+    every cell is empty before the first collision."""
+    L = (12, 12, 12)
+    g = MPCSystem(make_params(L=L, n_c=10, shear=shear, step_update=1, seed=5))
+    o = ob.Oracle(L=L, n_c=10, shear=shear, step_update=1, rng_mode=ob.RNG_INJECTED, seed=5)
+    if star:
+        g.add_star(3, 3, 6, 10.0)
+        o.add_star(3, 3, 6, 10.0)
+        pos, vel, _ = g.get_monomers()
+        o.set_monomers(pos, vel)
+    from parity_util import random_injection, seeded_fluid
+    p, v = seeded_fluid(L, 10, shear, seed=2)
+    g.set_fluid(p, v)
+    o.set_fluid(p.astype(np.float64), v.astype(np.float64))
+    assert np.all(g.cell_fluid_velocity(0, L[2] - 1) == 0.0)
+    shift, axis, gamma = random_injection(int(np.prod(L)), seed=9)
+    g.inject(shift, axis, gamma)
+    o.inject(shift, axis, gamma)
+    g.step(0, 0.1)
+    assert o.step(0, 0.1) == 0
+    g.sync()
+    fc, _ = o.cell_index()
+    _, vo = o.get_fluid()
+    nc = int(np.prod(L))
+    cnt = np.bincount(fc, minlength=nc + 1)[:nc]
+    want = np.zeros((nc, 3))
+    for k in range(3):
+        want[:, k] = np.bincount(fc, weights=vo[:, k], minlength=nc + 1)[:nc]
+    want[cnt > 0] /= cnt[cnt > 0, None]
+    got = g.cell_fluid_velocity(0, L[2] - 1).reshape(nc, 3)
+    assert (cnt == 0).sum() < nc // 2
+    assert np.abs(got - want).max() <= 1e-6, np.abs(got - want).max()
+    # a z range is the matching slab of cells
+    sub = g.cell_fluid_velocity(4, 6).reshape(-1, 3)
+    assert np.array_equal(sub, got[4 * 144:7 * 144])
+    with pytest.raises(SpmdError):
+        g.cell_fluid_velocity(5, L[2])
+    # the text of MPC::print_fluid for the middle plane
+    txt = obs.print_fluid_text(g, 7, 6, 6)
+    lines = txt.split("\n")
+    assert lines[0] == "TIME 7 " and lines[1] == "Z 6 "
+    assert len(lines[4].split()) == 2 * L[1]
+    assert float(lines[4].split()[0]) == pytest.approx(want[0 + 12 * 0 + 144 * 6, 0], abs=1e-6)
