@@ -54,6 +54,9 @@
 #include <utility>
 #include <vector>
 
+#include <stdlib.h>
+#include <unistd.h>
+
 #include "../../include/spmd.h"
 
 namespace spmd {
@@ -399,6 +402,25 @@ public:
     void print_PDB_with_velocity(const std::string& path, int step, bool append = true) {
         before_device();
         check(spmd_write_pdb(handle(), 0, path.c_str(), step, append ? 1 : 0), h_.get());
+    }
+    /* the same to an open stream, as the reference's signature takes it:
+     * the library writes the model to a temporary file, copied here */
+    void print_PDB_with_velocity(FILE* pdb, int step) {
+        char tmpl[] = "/tmp/spmd_pdb_XXXXXX";
+        const int fd = mkstemp(tmpl);
+        if (fd < 0) throw Error(SPMD_ERR_ARG, "cannot create a temporary file for the PDB model");
+        close(fd);
+        struct Unlink {
+            const char* p;
+            ~Unlink() { std::remove(p); }
+        } unlink_tmp{tmpl};
+        print_PDB_with_velocity(std::string(tmpl), step, false);
+        FILE* in = std::fopen(tmpl, "rb");
+        if (!in) throw Error(SPMD_ERR_ARG, "cannot read back the PDB model");
+        char buf[1 << 14];
+        std::size_t n;
+        while ((n = std::fread(buf, 1, sizeof buf, in)) > 0) std::fwrite(buf, 1, n, pdb);
+        std::fclose(in);
     }
 };
 

@@ -209,8 +209,13 @@ int main(int argc, char** argv) {
         if (!outdir.empty()) {
             std::ofstream profile(name("fluid_profile", tag, ".dat"), std::ios::out | std::ios::trunc);
             velocity_average_x.print_result(profile);
-            box.print_PDB_with_velocity(name("end_config", tag, ".pdb"), (int)Steps, false);
-            hydrodynamics.print_fluid_complete(name("end_fluid", tag, ".dat"));
+            /* the restart files through FILE*, as star-polymers.cpp:449-458 */
+            FILE* end_config_file = std::fopen(name("end_config", tag, ".pdb").c_str(), "w");
+            box.print_PDB_with_velocity(end_config_file, (int)Steps);
+            std::fclose(end_config_file);
+            FILE* end_fluid_file = std::fopen(name("end_fluid", tag, ".dat").c_str(), "w");
+            hydrodynamics.print_fluid_complete(end_fluid_file);
+            std::fclose(end_fluid_file);
         }
     } catch (const spmd::Error& e) {
         std::fprintf(stderr, "error %d: %s\n", e.code, e.what());

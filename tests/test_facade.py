@@ -54,6 +54,10 @@ int run(int argc) {
         } else thermostat->propagate(false);
         hyd->step(n, 100 * 0.01);
     }
+    FILE* restart = std::fopen("/dev/null", "w");
+    box.print_PDB_with_velocity(restart, argc);
+    hydrodynamics.print_fluid_complete(restart);
+    std::fclose(restart);
     struct Count { unsigned n = 0; void operator()(const MPCParticle&) { ++n; } } count;
     hydrodynamics(count);
     double dt = 0.005;
