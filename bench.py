@@ -174,38 +174,43 @@ def batch_bench(intervals=100, seq_intervals=10):
 
 
 def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_shear=0.012, check_warmup=400,
-                  check_intervals=800, check_box=80, train_stars=4):
+                  check_intervals=800, check_box=80, train_stars=4, train_range=(4e-3, 1.6e-2), infer_range=None,
+                  train_box=80):
     """Learned closure at batch (256 stars, f=30 arms of 20 A + 20 B beads,
-    1,201 monomers, shear rates uniform in [4e-3, 1.6e-2]) against explicit
-    MPC-MD of the same star at L=64, rho=10 on the same GPU (the smallest
-    box that holds the star stretched by these rates): star-steps/s (MD
-    steps x stars per second), the network trained here on explicit runs of
-    the library (8 shear rates, solvent started from the steady flow
-    profile; 4 independent stars per rate), fallbacks included.  Accuracy:
-    64 replicas each way at shear
-    0.012 in steady flow, Rg^2, alignment angle and rotation frequency
-    averaged over the run after a warm-up, standard errors from the spread
-    of replica means; each closure/explicit ratio carries the standard error
-    propagated from both sides (Rg = sqrt(<Rg^2>), so its ratio is the
-    square root of the Rg^2 ratio with half its relative error).  The same
-    check runs with the network restricted to the flow-relative velocity and
-    the shear rate (free_draining) and to those plus the neighbours' mean
-    velocity, crowding and type (local_coupling); the networks' likelihood
-    of held-out collisions is compared per sample."""
+    1,201 monomers, shear rates uniform in infer_range) against explicit
+    MPC-MD of the same star at L=64, rho=10 on the same GPU: star-steps/s
+    (MD steps x stars per second), the network trained here on explicit runs
+    of the library (8 shear rates log-spaced over train_range, solvent
+    started from the steady flow profile; train_stars independent stars per
+    rate, boxes of train_box that hold the stretched star) with the
+    headline inputs (closure.HEADLINE), fallbacks included.  The 256-star
+    run is cross-checked every 100th interval by explicit MPC at L=48,
+    rho=10 (ClosureRunner.cross_check_summary).  Accuracy: check_replicas
+    replicas each way at check_shear in steady flow, Rg^2, alignment angle
+    and rotation frequency averaged over the run after a warm-up, standard
+    errors from the spread of replica means; each closure/explicit ratio
+    carries the standard error propagated from both sides (Rg =
+    sqrt(<Rg^2>), so its ratio is the square root of the Rg^2 ratio with
+    half its relative error).  Time to converged Rg: from the coil start,
+    the intervals until the ensemble mean Rg^2 first comes within 5% of its
+    steady value, times each method's wall time per interval.  The same
+    check runs with the full 24-feature network and with the network
+    restricted to the flow-relative velocity and the shear rate
+    (free_draining); the networks' likelihood of held-out collisions is
+    compared per sample."""
     import numpy as np
 
     from starpolymersmd_amd import MPCSystem, make_params
     from starpolymersmd_amd import closure as cl
 
     arms, A, B, S, dt, L = 30, 20, 20, 10, 0.01, 64
-    lo, hi = 4e-3, 1.6e-2
+    lo, hi = train_range
+    ilo, ihi = infer_range if infer_range is not None else train_range
     shears_train = list(np.geomspace(lo, hi, 8))
     t0 = time.perf_counter()
-    data = cl.generate(shears_train, L=L, arms=arms, A=A, B=B, intervals=150, warmup=800)
-    if train_stars > 1:
-        # more independent stars per rate, in a box that holds them stretched
-        parts = [data] + [cl.generate(shears_train, L=80, arms=arms, A=A, B=B, intervals=150, warmup=800,
-                                      seed=11 + 7 * k) for k in range(1, train_stars)]
+    parts = [cl.generate(shears_train, L=train_box, arms=arms, A=A, B=B, intervals=150, warmup=800,
+                         seed=11 + 7 * k) for k in range(train_stars)]
+    if len(parts) > 1:
         xs = [q.X.download((q.n, cl.N_FEAT)) for q in parts]
         ys = [q.Y.download((q.n, 3)) for q in parts]
         for q in parts:
@@ -213,14 +218,16 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
             q.Y.close()
         data = cl.ClosureData.from_numpy(np.concatenate(xs), np.concatenate(ys))
         del xs, ys
+    else:
+        data = parts[0]
     t_gen = time.perf_counter() - t0
     net = cl.ClosureNet(1)
     t0 = time.perf_counter()
     steps = 2000
-    hist = net.fit(data, steps=steps, batch=65536, lr=2e-3)
+    hist = net.fit(data, steps=steps, batch=65536, lr=2e-3, features=cl.HEADLINE)
     t_fit = time.perf_counter() - t0
     rng = np.random.default_rng(0)
-    shears = list(rng.uniform(lo, hi, replicas))
+    shears = list(rng.uniform(ilo, ihi, replicas))
     run = cl.ClosureRunner(net, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=L, shears=shears)
     run.advance(2)
     run.sys.sync()
@@ -232,6 +239,14 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
     closure_rate = replicas * n * S / el
     fallbacks = run.fallbacks
     del run
+    # the same 256 stars with the explicit cross-check of every 100th interval
+    xrun = cl.ClosureRunner(net, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=48, shears=shears,
+                            check_every=100, check_replicas=16, check_intervals=20, seed=700)
+    xrun.advance(300, chunk=50)
+    xrun.sys.sync()
+    cross = xrun.cross_check_summary()
+    cross["fallbacks"] = xrun.fallbacks
+    xrun.close()
     # explicit MPC-MD of the same star, batched replicas
     g = MPCSystem([make_params(L=(L, L, L), n_c=10, shear=float(shears[r]), step_update=S, md_dt=dt, seed=3 + r)
                    for r in range(explicit_replicas)])
@@ -253,17 +268,28 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
     def ensemble(advance, sysm, R):
         """per-replica time averages of Rg^2, alignment angle and the
         rotation frequency about z (Box::calculate_rotation_frequency);
-        ensemble means and standard errors, and the wall time"""
+        ensemble means and standard errors, the wall time, and the ensemble
+        mean Rg^2 every 10 intervals from the coil start"""
         t0 = time.perf_counter()
-        advance(check_warmup)
+        trace = []
+        for _ in range(check_warmup // 10):
+            advance(10)
+            trace.append(np.mean([np.trace(sysm.gyration_tensor(r)) for r in range(R)]))
         acc = []
         for _ in range(check_intervals // 10):
             advance(10)
             acc.append([(*cl.shape_stats(sysm.gyration_tensor(r)), sysm.rotation_frequency(r)[2]) for r in range(R)])
+            trace.append(np.mean([a[0] for a in acc[-1]]))
         sysm.sync()
         el = time.perf_counter() - t0
         pm = np.array(acc).mean(axis=0)  # per replica (Rg^2, chi, w_z)
-        return pm.mean(axis=0), pm.std(axis=0, ddof=1) / np.sqrt(R), el
+        return pm.mean(axis=0), pm.std(axis=0, ddof=1) / np.sqrt(R), el, np.array(trace)
+
+    def converged(trace, steady):
+        """intervals from the coil start until the ensemble mean Rg^2 first
+        lies within 5% of its steady value"""
+        k = np.nonzero(np.abs(trace / steady - 1.0) <= 0.05)[0]
+        return 10 * (int(k[0]) + 1) if len(k) else None
 
     R = check_replicas
     # the stars stretch over the check: a box whose Verlet grid holds them
@@ -282,85 +308,194 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
         e.run(t[0], k * S + (1 if t[0] == 0 else 0), 0.1)
         t[0] += k * S + (1 if t[0] == 0 else 0)
 
-    em, es, et = ensemble(adv_e, e, R)
+    em, es, et, etr = ensemble(adv_e, e, R)
     e.close()
-    c = cl.ClosureRunner(net, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=L, shears=[check_shear] * R,
-                         seed=500)
-    cm, cs, ct = ensemble(lambda k: c.advance(k, chunk=k), c.sys, R)
-    c.close()
-    # time to the same standard error of Rg^2 (the error falls as 1/sqrt(time))
-    ttc = (et * es[0] ** 2) / (ct * cs[0] ** 2)
-    ratio = cm / em
-    ratio_se = np.abs(ratio) * np.sqrt((cs / cm) ** 2 + (es / em) ** 2)
-    # what the inputs gain: the same training restricted to the flow-relative
-    # velocity and the shear rate (a learned free-draining Langevin bath) and
-    # to those plus the neighbours' mean velocity, crowding and type (local
-    # coupling); the same ensemble check, and the likelihood of one
-    # collision's velocity changes held out from training (an independent
-    # run at the check's shear), compared per sample with standard errors
-    # from interval blocks (the monomers of one interval are correlated)
+    total = check_warmup + check_intervals
+    e_conv = converged(etr, em[0])
+
+    def check(network, seed=500):
+        c = cl.ClosureRunner(network, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=L,
+                             shears=[check_shear] * R, seed=seed)
+        cm, cs, ct, ctr = ensemble(lambda k: c.advance(k, chunk=k), c.sys, R)
+        fb = c.fallbacks
+        c.close()
+        ratio = cm / em
+        rse = np.abs(ratio) * np.sqrt((cs / cm) ** 2 + (es / em) ** 2)
+        c_conv = converged(ctr, cm[0])
+        conv = None
+        if e_conv is not None and c_conv is not None:
+            conv = (e_conv * et / total) / (c_conv * ct / total)
+        return {"Rg2": float(cm[0]), "Rg2_se": float(cs[0]), "chi": float(cm[1]), "chi_se": float(cs[1]),
+                "wz": float(cm[2]), "wz_se": float(cs[2]),
+                "Rg2_ratio": float(ratio[0]), "Rg2_ratio_se": float(rse[0]),
+                "Rg_ratio": float(np.sqrt(ratio[0])), "Rg_ratio_se": float(0.5 * rse[0] / np.sqrt(ratio[0])),
+                "chi_ratio": float(ratio[1]), "chi_ratio_se": float(rse[1]),
+                "wz_ratio": float(ratio[2]), "wz_ratio_se": float(rse[2]),
+                "seconds": ct, "fallbacks": fb,
+                "time_to_equal_Rg2_error_speedup": float((et * es[0] ** 2) / (ct * cs[0] ** 2)),
+                "converged_Rg_intervals": c_conv, "time_to_converged_Rg_speedup": conv,
+                "Rg2_trace": [float(x) for x in ctr[::4]]}
+
+    head = check(net)
+    # what the inputs gain: the full 24 features and the flow-relative
+    # velocity with the shear rate alone (a learned free-draining Langevin
+    # bath); the same ensemble check, and the likelihood of one collision's
+    # velocity changes held out from training (an independent run at the
+    # check's shear), compared per sample with standard errors from interval
+    # blocks (the monomers of one interval are correlated)
     held = cl.generate([check_shear], L=80, arms=arms, A=A, B=B, intervals=50, warmup=800, seed=99)
     xh, yh = held.X.download((held.n, 24)), held.Y.download((held.n, 3))
     held.X.close()
     held.Y.close()
     nmono = 1 + arms * (A + B)
-    nll = {"full": net.nll(xh, yh)}
+    nll = {"headline": net.nll(xh, yh)}
 
     def gain(a, b):
-        g = (nll[a] - nll[b]).reshape(-1, nmono).mean(axis=1)
-        return float(g.mean()), float(g.std(ddof=1) / np.sqrt(len(g)))
+        gg = (nll[a] - nll[b]).reshape(-1, nmono).mean(axis=1)
+        return float(gg.mean()), float(gg.std(ddof=1) / np.sqrt(len(gg)))
 
-    restricted = {}
-    for key, feats in (("free_draining", cl.FREE_DRAINING), ("local_coupling", cl.LOCAL_COUPLING)):
+    ablations = {}
+    for key, feats in (("full", None), ("free_draining", cl.FREE_DRAINING)):
         n2 = cl.ClosureNet(1)
         h2 = n2.fit(data, steps=steps, batch=65536, lr=2e-3, features=feats)
-        r2 = cl.ClosureRunner(n2, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=L,
-                              shears=[check_shear] * R, seed=500)
-        m2, s2, t2 = ensemble(lambda k: r2.advance(k, chunk=k), r2.sys, R)
-        fb2 = r2.fallbacks
-        r2.close()
+        ablations[key] = {"features": list(feats) if feats is not None else list(range(cl.N_FEAT)),
+                          "loss_last": float(np.mean(h2[-10:])), **check(n2)}
         nll[key] = n2.nll(xh, yh)
-        rt = m2 / em
-        rse = np.abs(rt) * np.sqrt((s2 / m2) ** 2 + (es / em) ** 2)
-        restricted[key] = {"features": list(feats), "loss_last": float(np.mean(h2[-10:])),
-                           "Rg2_ratio": float(rt[0]), "Rg2_ratio_se": float(rse[0]),
-                           "Rg_ratio": float(np.sqrt(rt[0])), "Rg_ratio_se": float(0.5 * rse[0] / np.sqrt(rt[0])),
-                           "chi_ratio": float(rt[1]), "chi_ratio_se": float(rse[1]),
-                           "wz_ratio": float(rt[2]), "wz_ratio_se": float(rse[2]),
-                           "fallbacks": fb2, "heldout_nll": float(nll[key].mean()),
-                           "time_to_equal_Rg2_error_speedup": float((et * es[0] ** 2) / (t2 * s2[0] ** 2))}
+        ablations[key]["heldout_nll"] = float(nll[key].mean())
+    g_head, g_head_se = gain("free_draining", "headline")
     g_full, g_full_se = gain("free_draining", "full")
-    g_loc, g_loc_se = gain("free_draining", "local_coupling")
     return {
         "star_steps_per_s": closure_rate, "explicit_star_steps_per_s": explicit_rate,
         "speedup": closure_rate / explicit_rate,
         "workload": f"{replicas} stars f={arms} {A}A+{B}B (1,201 monomers) closure vs {explicit_replicas} stars "
-                    f"explicit MPC L={L} rho=10, shear U[{lo},{hi}], {n} intervals of {S} MD steps",
+                    f"explicit MPC L={L} rho=10, shear U[{ilo},{ihi}], {n} intervals of {S} MD steps",
         "fallbacks": fallbacks,
-        "train": {"samples": data.n, "shear_rates": 8, "stars_per_rate": train_stars, "steps": steps, "batch": 65536,
+        "network_inputs": list(cl.HEADLINE),
+        "train": {"samples": data.n, "shear_rates": 8, "shear_range": [lo, hi], "stars_per_rate": train_stars,
+                  "box": train_box, "steps": steps, "batch": 65536,
                   "generate_seconds": t_gen, "fit_seconds": t_fit,
                   "loss_first": float(hist[0]), "loss_last": float(np.mean(hist[-10:]))},
+        "cross_check": cross,
         "check": {"shear": check_shear, "replicas": R, "explicit_box": Le, "warmup_intervals": check_warmup,
                   "intervals": check_intervals, "steady_flow": True,
                   "explicit_Rg2": float(em[0]), "explicit_Rg2_se": float(es[0]),
-                  "closure_Rg2": float(cm[0]), "closure_Rg2_se": float(cs[0]),
                   "explicit_chi": float(em[1]), "explicit_chi_se": float(es[1]),
-                  "closure_chi": float(cm[1]), "closure_chi_se": float(cs[1]),
                   "explicit_wz": float(em[2]), "explicit_wz_se": float(es[2]),
-                  "closure_wz": float(cm[2]), "closure_wz_se": float(cs[2]),
-                  "Rg2_ratio": float(ratio[0]), "Rg2_ratio_se": float(ratio_se[0]),
-                  "Rg_ratio": float(np.sqrt(ratio[0])), "Rg_ratio_se": float(0.5 * ratio_se[0] / np.sqrt(ratio[0])),
-                  "chi_ratio": float(ratio[1]), "chi_ratio_se": float(ratio_se[1]),
-                  "wz_ratio": float(ratio[2]), "wz_ratio_se": float(ratio_se[2]),
-                  "explicit_seconds": et, "closure_seconds": ct,
-                  "time_to_equal_Rg2_error_speedup": float(ttc),
-                  "closure_fallbacks": c.fallbacks},
-        "restricted_networks": restricted,
-        "heldout": {"samples": int(len(yh)), "nll_full": float(nll["full"].mean()),
-                    "nll_gain_full_over_free_draining": g_full, "nll_gain_full_over_free_draining_se": g_full_se,
-                    "nll_gain_local_coupling_over_free_draining": g_loc,
-                    "nll_gain_local_coupling_over_free_draining_se": g_loc_se},
+                  "explicit_seconds": et, "explicit_converged_Rg_intervals": e_conv,
+                  "explicit_Rg2_trace": [float(x) for x in etr[::4]],
+                  "closure": head},
+        "ablations": ablations,
+        "heldout": {"samples": int(len(yh)), "nll_headline": float(nll["headline"].mean()),
+                    "nll_gain_headline_over_free_draining": g_head,
+                    "nll_gain_headline_over_free_draining_se": g_head_se,
+                    "nll_gain_full_over_free_draining": g_full, "nll_gain_full_over_free_draining_se": g_full_se},
     }
+
+
+def closure_converge(intervals=20000, explicit_replicas=16, closure_replicas=64, shear=0.012, box=80,
+                     train_range=(4e-3, 1.6e-2), train_stars=4, every=50, history=False):
+    """Time to converged Rg from the coil start: explicit MPC-MD (L=box,
+    rho=10) and the learned closure (headline inputs) each advance an
+    ensemble of the f=30, 20A+20B star from the seeded coil in steady shear
+    flow for `intervals` collision intervals, recording the ensemble mean
+    Rg^2 every `every` intervals.  Plateau: the mean over the last quarter;
+    converged: the first record from which the running mean of 5 records
+    stays within 5% of the plateau.  Speedup: explicit over closure wall
+    time per star to that point.  The plateaus are the steady-state
+    accuracy check (Rg, alignment angle and w_z over the last quarter)."""
+    import numpy as np
+
+    from starpolymersmd_amd import MPCSystem, make_params
+    from starpolymersmd_amd import closure as cl
+
+    arms, A, B, S = 30, 20, 20, 10
+    lo, hi = train_range
+    t0 = time.perf_counter()
+    if history:
+        # one recorded interval every 150 over 15,000: the stretching, the
+        # overshoot and the steady state
+        data = cl.generate_batched(list(np.geomspace(lo, hi, 8)), stars_per_rate=2, L=box, arms=arms, A=A, B=B,
+                                   intervals=100, stride=150,
+                                   progress=lambda k, n: k % 10 == 0 and print(f"[converge] data {k}/{n}",
+                                                                               file=sys.stderr, flush=True))
+    else:
+        parts = [cl.generate(list(np.geomspace(lo, hi, 8)), L=box, arms=arms, A=A, B=B, intervals=150,
+                             warmup=800, seed=11 + 7 * k) for k in range(train_stars)]
+        xs = [q.X.download((q.n, cl.N_FEAT)) for q in parts]
+        ys = [q.Y.download((q.n, 3)) for q in parts]
+        for q in parts:
+            q.X.close()
+            q.Y.close()
+        data = cl.ClosureData.from_numpy(np.concatenate(xs), np.concatenate(ys))
+        del xs, ys
+    t_gen = time.perf_counter() - t0
+    print(f"[converge] training data: {data.n} samples in {t_gen:.1f} s", file=sys.stderr, flush=True)
+    net = cl.ClosureNet(1)
+    net.fit(data, steps=2000, batch=65536, lr=2e-3, features=cl.HEADLINE)
+
+    def trace(advance, sysm, R):
+        t0 = time.perf_counter()
+        tr, tail = [], []
+        for k in range(intervals // every):
+            advance(every)
+            st = [(*cl.shape_stats(sysm.gyration_tensor(r)), sysm.rotation_frequency(r)[2]) for r in range(R)]
+            tr.append(float(np.mean([x[0] for x in st])))
+            if k % 40 == 0:
+                print(f"[converge] {R} stars, interval {(k + 1) * every}: mean Rg^2 {tr[-1]:.2f}", file=sys.stderr,
+                      flush=True)
+            if k >= (3 * intervals // every) // 4:
+                tail.append(st)
+        sysm.sync()
+        pm = np.array(tail).mean(axis=0)
+        return np.array(tr), pm.mean(axis=0), pm.std(axis=0, ddof=1) / np.sqrt(R), time.perf_counter() - t0
+
+    def converged(tr, plateau):
+        sm = np.convolve(tr, np.ones(5) / 5, mode="valid")
+        ok = np.abs(sm / plateau - 1.0) <= 0.05
+        for k in range(len(ok)):
+            if ok[k:].all():
+                return every * (k + 5)
+        return None
+
+    e = MPCSystem([make_params(L=(box,) * 3, n_c=10, shear=shear, step_update=S, seed=100 + r)
+                   for r in range(explicit_replicas)])
+    for r in range(explicit_replicas):
+        e.add_star(A, B, arms, 10.0, r=r, coil=True)
+    e.md_init()
+    e.initialize()
+    cl.preshear(e)
+    t = [0]
+
+    def adv_e(k):
+        n = k * S + (1 if t[0] == 0 else 0)
+        e.run(t[0], n, 0.1)
+        t[0] += n
+
+    etr, em, es, et = trace(adv_e, e, explicit_replicas)
+    e.close()
+    c = cl.ClosureRunner(net, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=64,
+                         shears=[shear] * closure_replicas, seed=500)
+    ctr, cm, cs, ct = trace(lambda k: c.advance(k, chunk=k), c.sys, closure_replicas)
+    fb = c.fallbacks
+    c.close()
+    ke, kc = converged(etr, em[0]), converged(ctr, cm[0])
+    per_e = et / (intervals * explicit_replicas)
+    per_c = ct / (intervals * closure_replicas)
+    ratio = cm / em
+    rse = np.abs(ratio) * np.sqrt((cs / cm) ** 2 + (es / em) ** 2)
+    return {"shear": shear, "explicit_box": box, "intervals": intervals, "record_every": every,
+            "training_data": "history" if history else "window", "training_samples": data.n,
+            "training_seconds": t_gen,
+            "explicit_replicas": explicit_replicas, "closure_replicas": closure_replicas, "closure_fallbacks": fb,
+            "explicit_converged_intervals": ke, "closure_converged_intervals": kc,
+            "explicit_seconds_per_star_interval": per_e, "closure_seconds_per_star_interval": per_c,
+            "time_to_converged_Rg_speedup": (ke * per_e) / (kc * per_c) if ke and kc else None,
+            "steady_Rg_ratio": float(np.sqrt(ratio[0])), "steady_Rg_ratio_se": float(0.5 * rse[0] / np.sqrt(ratio[0])),
+            "steady_chi_ratio": float(ratio[1]), "steady_chi_ratio_se": float(rse[1]),
+            "steady_wz_ratio": float(ratio[2]), "steady_wz_ratio_se": float(rse[2]),
+            "explicit_plateau": [float(x) for x in em], "closure_plateau": [float(x) for x in cm],
+            "explicit_Rg2_trace": [round(float(x), 3) for x in etr],
+            "closure_Rg2_trace": [round(float(x), 3) for x in ctr]}
 
 
 def main():
@@ -378,8 +513,22 @@ def main():
                     help="independent explicit stars per shear rate in the closure's training data")
     ap.add_argument("--closure-only", action="store_true",
                     help="only run the learned-closure section (for profiling its kernels)")
+    ap.add_argument("--closure-train-range", type=float, nargs=2, default=(4e-3, 1.6e-2),
+                    help="shear rates of the closure's training runs (8, log-spaced)")
+    ap.add_argument("--closure-infer-range", type=float, nargs=2, default=None,
+                    help="shear rates of the 256 closure stars (uniform; default: the training range)")
+    ap.add_argument("--closure-train-box", type=int, default=80, help="explicit box of the training runs")
+    ap.add_argument("--closure-check", type=float, nargs=2, default=(0.012, 80),
+                    help="shear rate and explicit box of the closure's accuracy check")
+    ap.add_argument("--closure-converge", type=int, default=0,
+                    help="only run the time-to-converged-Rg comparison over N intervals")
+    ap.add_argument("--closure-history", action="store_true",
+                    help="train the converge run's network on whole-history data (closure.generate_batched)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run N MiB device-to-device copies (counter calibration)")
+    ap.add_argument("--gather-probe", type=int, default=0,
+                    help="only run gathers of N Mi floats through a permutation (counter calibration of the "
+                         "collide's access shape): random, then shuffled within 4,096-element blocks")
     a = ap.parse_args()
     if a.copy_probe:
         import torch
@@ -391,14 +540,44 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_mib": a.copy_probe, "bytes_read": x.numel(), "bytes_written": y.numel()}))
         return
+    if a.gather_probe:
+        # dst[i] = src[perm[i]] with 4-byte elements: the payload is 4 B read
+        # + 4 B of index + 4 B written per element, as the collide's gather
+        # of one float through the sort permutation
+        import torch
+
+        n = a.gather_probe << 20
+        src = torch.rand(n, device="cuda")
+        g = torch.Generator(device="cuda")
+        g.manual_seed(1)
+        perms = {"random": torch.randperm(n, device="cuda", generator=g, dtype=torch.int32)}
+        blk = torch.arange(n, device="cuda", dtype=torch.int32).view(-1, 4096)
+        perms["block4096"] = torch.gather(blk, 1, torch.argsort(torch.rand(blk.shape, device="cuda", generator=g),
+                                                                dim=1).to(torch.int64)).view(-1).to(torch.int32)
+        for name, perm in perms.items():
+            idx = perm.to(torch.int64)  # index_select takes int64 indices: 8 B per element
+            for _ in range(5):
+                dst = torch.index_select(src, 0, idx)
+            torch.cuda.synchronize()
+            print(json.dumps({"gather_probe": name, "elements": n, "bytes_data_read": 4 * n,
+                              "bytes_index_read": 8 * n, "bytes_written": 4 * dst.numel()}), flush=True)
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the library has no multi-GPU path (the MPC box would be "
               "slab-decomposed along z with per-face halo sums; not built). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
+    if a.closure_converge:
+        print(json.dumps({"closure_converge": closure_converge(
+            intervals=a.closure_converge, shear=a.closure_check[0], box=int(a.closure_check[1]),
+            train_range=tuple(a.closure_train_range), train_stars=a.train_stars,
+            history=a.closure_history)}), flush=True)
+        return
     if a.closure_only:
-        print(json.dumps({"closure": closure_bench(train_stars=a.train_stars)}), flush=True)
+        print(json.dumps({"closure": closure_bench(train_stars=a.train_stars, train_range=tuple(a.closure_train_range),
+                                                infer_range=a.closure_infer_range, train_box=a.closure_train_box,
+                                                check_shear=a.closure_check[0], check_box=int(a.closure_check[1]))}), flush=True)
         return
 
     w = WORKLOAD
@@ -466,7 +645,9 @@ def main():
     if not a.no_batch:
         out["batch"] = batch_bench()
     if not a.no_closure:
-        out["closure"] = closure_bench(train_stars=a.train_stars)
+        out["closure"] = closure_bench(train_stars=a.train_stars, train_range=tuple(a.closure_train_range),
+                                                infer_range=a.closure_infer_range, train_box=a.closure_train_box,
+                                                check_shear=a.closure_check[0], check_box=int(a.closure_check[1]))
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_intervals, star0)
         # all the host threads this process may use (OMP_NUM_THREADS, else the CPUs it sees)

@@ -288,6 +288,59 @@ def generate(shears, L=32, arms=10, A=5, B=5, mass=10.0, intervals=400, warmup=5
                                   seconds=time.perf_counter() - t_start)
 
 
+def generate_batched(shears, stars_per_rate=2, L=96, arms=30, A=20, B=20, mass=10.0, intervals=100, stride=150,
+                     warmup=0, n_c=10, step_update=10, md_dt=0.01, seed=11, presheared=True,
+                     progress=None) -> ClosureData:
+    """As generate, with every (shear rate, star) pair a replica of one
+    batched system, and one recorded interval every `stride` intervals
+    after `warmup`: the data then span the star's whole history in shear
+    flow -- the start-up stretching, the overshoot and the steady state --
+    instead of one window of it."""
+    h = md_dt * step_update
+    t_start = time.perf_counter()
+    reps = [(s, k) for s in shears for k in range(stars_per_rate)]
+    g = MPCSystem([make_params(L=(L, L, L), n_c=n_c, shear=s, step_update=step_update, md_dt=md_dt,
+                               seed=seed + 101 * i) for i, (s, k) in enumerate(reps)])
+    for r in range(len(reps)):
+        g.add_star(A, B, arms, mass, r=r, coil=True)
+    g.md_init()
+    g.initialize()
+    if presheared:
+        preshear(g)
+    nm = g.num_monomers(0)
+    R = len(reps)
+    row = R * nm * N_FEAT * 4
+    buf = DeviceBuffer(intervals * row)
+    dv = np.empty((intervals, R, nm, 3))
+    keep = np.ones((intervals, R), bool)
+    t = 0
+    n_int = 0
+    if warmup:
+        g.run(t, 1 + warmup * step_update, h)
+        t += 1 + warmup * step_update
+    for k in range(intervals):
+        n = (stride - 1) * step_update + step_update - 1 + (1 if t == 0 else 0)
+        g.run(t, n, h)
+        t += n
+        g.propagate()
+        g.closure_features(buf.ptr + k * row)
+        v0 = [g.get_monomers(r)[1] for r in range(R)]
+        g.step(t, h)
+        t += 1
+        for r in range(R):
+            dv[k, r] = g.get_monomers(r)[1] - v0[r]
+            keep[k, r] = not g.last_collision(r)[1]  # a recentring subtracts the COM velocity
+        n_int += stride
+        if progress is not None:
+            progress(k + 1, intervals)
+    g.sync()
+    g.close()
+    X = buf.download((intervals, R, nm, N_FEAT))
+    buf.close()
+    return ClosureData.from_numpy(X[keep].reshape(-1, N_FEAT), dv[keep].reshape(-1, 3), intervals=n_int * R,
+                                  seconds=time.perf_counter() - t_start)
+
+
 def shape_stats(G):
     """Rg^2 = tr G, alignment angle chi = 1/2 atan2(2 G_xy, G_xx - G_yy) (rad)
     of a gyration tensor."""
@@ -308,6 +361,12 @@ FREE_DRAINING = (6, 7, 8, 18)
 # mean velocity of the monomers within 1.5 relative to the monomer's own:
 # drag toward the local flow and a learned coupling to the neighbours
 LOCAL_COUPLING = (6, 7, 8, 15, 16, 18, 19, 20, 21)
+# the network the library runs by default: local coupling.  It predicts
+# held-out collisions better than the free-draining bath, and the
+# configuration inputs of the full 24-feature network (position relative to
+# the centre, bond vectors) buy no further likelihood and bias the
+# alignment angle of sheared stars.
+HEADLINE = LOCAL_COUPLING
 
 
 def regime_range(fmin, fmax, checked=RANGE_CHECKED):
@@ -322,14 +381,20 @@ def regime_range(fmin, fmax, checked=RANGE_CHECKED):
 class ClosureRunner:
     """Replicas of one star advanced by MD + the learned solvent kick, with
     explicit MPC fallback for replicas whose features leave the trained
-    range."""
+    range, and explicit MPC as the judge of its error: every check_every-th
+    interval, check_replicas replicas (in turn) are cross-checked by an
+    explicit MPC run at L, rho=n_c_fallback from the same star state beside
+    a closure run of the same length (the main run is not touched); the
+    Rg^2, alignment angle and rotation frequency of both are recorded
+    (cross_check_summary)."""
 
     def __init__(self, net: ClosureNet, fmin, fmax, shear, arms, A, B, mass=10.0, replicas=1, L=48,
                  step_update=10, md_dt=0.01, seed=5, shears=None, fallback_intervals=20, n_c_fallback=10,
-                 box=128):
-        """L: box of the explicit fallback runs; box: the closure systems'
-        box, which without solvent only frames the Verlet grid (it must
-        hold the stretched star: monomers within box/2 of its centre)."""
+                 box=128, check_every=0, check_replicas=8, check_intervals=20):
+        """L: box of the explicit fallback and cross-check runs (larger when
+        the star reaches further); box: the closure systems' box, which
+        without solvent only frames the Verlet grid (it must hold the
+        stretched star: monomers within box/2 of its centre)."""
         self.net = net
         self.fmin, self.fmax = regime_range(np.asarray(fmin, np.float32), np.asarray(fmax, np.float32))
         self.shears = list(shears) if shears is not None else [shear] * replicas
@@ -349,16 +414,100 @@ class ClosureRunner:
         self._pool = {}
         self.pool_size = 4
         self.t = 0
+        self.box = box
+        self.check_every, self.check_replicas, self.check_intervals = check_every, check_replicas, check_intervals
+        self.checks = []      # per cross-checked replica: (t, replica, shear, closure stats, explicit stats)
+        self._check_next = 0  # the replica the next cross-check starts from
 
     def advance(self, n_intervals: int, chunk: int = 10):
         done = 0
         while done < n_intervals:
             k = min(chunk, n_intervals - done)
+            if self.check_every:
+                k = min(k, self.check_every - self.t % self.check_every)
             oor = self.sys.closure_run(self.net, k, self.fmin, self.fmax)
             done += k
+            self.t += k
             for r in np.nonzero(oor)[0]:
                 self._explicit(int(r))
-        self.t += n_intervals
+            if self.check_every and self.t % self.check_every == 0:
+                self._cross_check()
+
+    def _star_states(self, reps):
+        """each replica's star centred, velocities relative to the local flow"""
+        out = []
+        for r in reps:
+            pos, vel, _ = self.sys.get_monomers(r)
+            com = pos.mean(axis=0)
+            vel = vel.copy()
+            vel[:, 0] -= self.shears[r] * com[1]
+            out.append((pos - com, vel))
+        return out
+
+    def _cross_check(self):
+        """Explicit MPC (L, rho) and the closure from the same star states of
+        check_replicas replicas over check_intervals intervals: per replica
+        the time means of Rg^2, alignment angle and w_z on each side."""
+        arms, A, B, mass = self.star
+        R = len(self.shears)
+        m = min(self.check_replicas, R)
+        reps = [(self._check_next + i) % R for i in range(m)]
+        self._check_next = (self._check_next + m) % R
+        states = self._star_states(reps)
+        reach = max(float(np.abs(p).max()) for p, _ in states)
+        L = max(self.L, 2 * int(math.ceil(reach + 4.0)))
+        h = self.S * self.dt
+        seed = self.seed + 7919 * (1 + len(self.checks))
+        ex = MPCSystem([make_params(L=(L,) * 3, n_c=self.n_c_fallback, shear=self.shears[r], step_update=self.S,
+                                    md_dt=self.dt, seed=seed + i) for i, r in enumerate(reps)])
+        cl = MPCSystem([make_params(L=(self.box,) * 3, n_c=0, shear=self.shears[r], step_update=self.S,
+                                    md_dt=self.dt, seed=seed + 100 + i) for i, r in enumerate(reps)])
+        try:
+            for i, (p, v) in enumerate(states):
+                for g in (ex, cl):
+                    g.add_star(A, B, arms, mass, r=i, coil=True)
+                    g.set_monomers(p, v, r=i)
+            ex.md_init()
+            cl.md_init()
+            ex.initialize()
+            preshear(ex)  # the flow the closure stands for
+            acc_e, acc_c = [], []
+            t = 0
+            for _ in range(self.check_intervals):
+                n = self.S + (1 if t == 0 else 0)
+                ex.run(t, n, h)
+                t += n
+                cl.closure_run(self.net, 1, self.fmin, self.fmax)
+                acc_e.append([(*shape_stats(ex.gyration_tensor(i)), ex.rotation_frequency(i)[2]) for i in range(m)])
+                acc_c.append([(*shape_stats(cl.gyration_tensor(i)), cl.rotation_frequency(i)[2]) for i in range(m)])
+            ex.sync()
+            me, mc = np.array(acc_e).mean(axis=0), np.array(acc_c).mean(axis=0)
+            for i, r in enumerate(reps):
+                self.checks.append((self.t, r, float(self.shears[r]), mc[i], me[i]))
+        finally:
+            ex.close()
+            cl.close()
+
+    def cross_check_summary(self):
+        """Over all cross-checks: closure / explicit ratios of the mean Rg^2,
+        Rg, alignment angle and w_z with standard errors (from the spread of
+        the per-replica differences), and the number of checked replicas."""
+        if not self.checks:
+            return None
+        c = np.array([x[3] for x in self.checks])
+        e = np.array([x[4] for x in self.checks])
+        n = len(c)
+        out = {"checks": n, "L": self.L}
+        for k, name in enumerate(("Rg2", "chi", "wz")):
+            mc, me = c[:, k].mean(), e[:, k].mean()
+            d = c[:, k] - e[:, k]
+            se = d.std(ddof=1) / np.sqrt(n) if n > 1 else float("nan")
+            out[name + "_closure"], out[name + "_explicit"] = float(mc), float(me)
+            out[name + "_ratio"] = float(mc / me) if me != 0 else float("nan")
+            out[name + "_ratio_se"] = float(se / abs(me)) if me != 0 else float("nan")
+        out["Rg_ratio"] = float(np.sqrt(out["Rg2_ratio"]))
+        out["Rg_ratio_se"] = 0.5 * out["Rg2_ratio_se"] / out["Rg_ratio"]
+        return out
 
     def _explicit(self, r):
         """Run the replica's star through explicit MPC intervals, then resume."""

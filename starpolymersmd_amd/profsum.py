@@ -47,7 +47,47 @@ def load_counter(path, counter):
     return {k: (sum(v) / len(v), len(v)) for k, v in per.items()}
 
 
+def calibrate(fetch_csv, write_csv, kernel, groups, read_bytes, write_bytes):
+    """Counter calibration on a probe of known traffic: the dispatches of
+    `kernel` (substring) in dispatch order, split into `groups` equal runs
+    (one per probe variant); per run the mean FETCH_SIZE and WRITE_SIZE
+    (KiB) against the bytes the probe must move."""
+    def per_dispatch(path, counter):
+        vals = defaultdict(float)
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] == counter and kernel in r["Kernel_Name"]:
+                    vals[int(r["Dispatch_Id"])] += float(r["Counter_Value"])
+        return [vals[k] for k in sorted(vals)]
+    fv, wv = per_dispatch(fetch_csv, "FETCH_SIZE"), per_dispatch(write_csv, "WRITE_SIZE")
+    out = []
+    for g in range(groups):
+        fg = fv[g * len(fv) // groups:(g + 1) * len(fv) // groups]
+        wg = wv[g * len(wv) // groups:(g + 1) * len(wv) // groups]
+        f, w = sum(fg) / len(fg) * 1024, sum(wg) / len(wg) * 1024
+        out.append((f, w, read_bytes / f, write_bytes / w))
+    return out
+
+
 def main():
+    import sys
+    if len(sys.argv) > 1 and sys.argv[1] == "calib":
+        ap = argparse.ArgumentParser(prog="profsum calib")
+        ap.add_argument("--fetch", required=True)
+        ap.add_argument("--write", required=True)
+        ap.add_argument("--kernel", required=True)
+        ap.add_argument("--groups", type=int, default=1)
+        ap.add_argument("--names", default="")
+        ap.add_argument("--read-bytes", type=float, required=True)
+        ap.add_argument("--write-bytes", type=float, required=True)
+        a = ap.parse_args(sys.argv[2:])
+        names = a.names.split(",") if a.names else [str(g) for g in range(a.groups)]
+        print("| probe | bytes read | FETCH_SIZE bytes | read scale | bytes written | WRITE_SIZE bytes | write scale |")
+        print("|---|---:|---:|---:|---:|---:|---:|")
+        for nm, (f, w, fs, ws) in zip(names, calibrate(a.fetch, a.write, a.kernel, a.groups, a.read_bytes,
+                                                         a.write_bytes)):
+            print(f"| {nm} | {a.read_bytes:.0f} | {f:.0f} | {fs:.3f} | {a.write_bytes:.0f} | {w:.0f} | {ws:.3f} |")
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--stats", required=True)
     ap.add_argument("--fetch")
