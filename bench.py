@@ -50,8 +50,9 @@ TRAFFIC_PROFILE = "profiles/collision_pipeline_L64_rho10.md"
 
 def measured_traffic():
     """HBM bytes per collision of the pipeline from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (scaled by a device copy of known size);
-    None when the summary is absent."""
+    FETCH_SIZE/WRITE_SIZE summary (scaled by the counter calibration of a
+    coalesced 4-byte-per-lane gather of known size, the pipeline kernels'
+    access shape); None when the summary is absent."""
     try:
         with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
             for line in f:
@@ -208,8 +209,11 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
     ilo, ihi = infer_range if infer_range is not None else train_range
     shears_train = list(np.geomspace(lo, hi, 8))
     t0 = time.perf_counter()
-    parts = [cl.generate(shears_train, L=train_box, arms=arms, A=A, B=B, intervals=150, warmup=800,
-                         seed=11 + 7 * k) for k in range(train_stars)]
+    parts = []
+    for k in range(train_stars):
+        parts.append(cl.generate(shears_train, L=train_box, arms=arms, A=A, B=B, intervals=150, warmup=800,
+                                 seed=11 + 7 * k))
+        print(f"[closure] training runs: star {k + 1} of {train_stars} per rate", file=sys.stderr, flush=True)
     if len(parts) > 1:
         xs = [q.X.download((q.n, cl.N_FEAT)) for q in parts]
         ys = [q.Y.download((q.n, 3)) for q in parts]
@@ -221,6 +225,7 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
     else:
         data = parts[0]
     t_gen = time.perf_counter() - t0
+    print(f"[closure] training data: {data.n} samples in {t_gen:.1f} s", file=sys.stderr, flush=True)
     net = cl.ClosureNet(1)
     t0 = time.perf_counter()
     steps = 2000
@@ -245,6 +250,7 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
     xrun.advance(300, chunk=50)
     xrun.sys.sync()
     cross = xrun.cross_check_summary()
+    print("[closure] throughput and cross-check done", file=sys.stderr, flush=True)
     cross["fallbacks"] = xrun.fallbacks
     xrun.close()
     # explicit MPC-MD of the same star, batched replicas
@@ -310,6 +316,7 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
 
     em, es, et, etr = ensemble(adv_e, e, R)
     e.close()
+    print(f"[closure] explicit ensemble done in {et:.1f} s", file=sys.stderr, flush=True)
     total = check_warmup + check_intervals
     e_conv = converged(etr, em[0])
 
@@ -337,6 +344,7 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
                 "Rg2_trace": [float(x) for x in ctr[::4]]}
 
     head = check(net)
+    print("[closure] headline check done", file=sys.stderr, flush=True)
     # what the inputs gain: the full 24 features and the flow-relative
     # velocity with the shear rate alone (a learned free-draining Langevin
     # bath); the same ensemble check, and the likelihood of one collision's
@@ -528,7 +536,7 @@ def main():
                     help="only run N MiB device-to-device copies (counter calibration)")
     ap.add_argument("--gather-probe", type=int, default=0,
                     help="only run gathers of N Mi floats through a permutation (counter calibration of the "
-                         "collide's access shape): random, then shuffled within 4,096-element blocks")
+                         "collide's access shape): identity, random, then shuffled within 4,096-element blocks")
     a = ap.parse_args()
     if a.copy_probe:
         import torch
@@ -550,7 +558,8 @@ def main():
         src = torch.rand(n, device="cuda")
         g = torch.Generator(device="cuda")
         g.manual_seed(1)
-        perms = {"random": torch.randperm(n, device="cuda", generator=g, dtype=torch.int32)}
+        perms = {"identity": torch.arange(n, device="cuda", dtype=torch.int32),
+                 "random": torch.randperm(n, device="cuda", generator=g, dtype=torch.int32)}
         blk = torch.arange(n, device="cuda", dtype=torch.int32).view(-1, 4096)
         perms["block4096"] = torch.gather(blk, 1, torch.argsort(torch.rand(blk.shape, device="cuda", generator=g),
                                                                 dim=1).to(torch.int64)).view(-1).to(torch.int32)

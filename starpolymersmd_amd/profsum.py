@@ -47,7 +47,7 @@ def load_counter(path, counter):
     return {k: (sum(v) / len(v), len(v)) for k, v in per.items()}
 
 
-def calibrate(fetch_csv, write_csv, kernel, groups, read_bytes, write_bytes):
+def calibrate(fetch_csv, write_csv, kernel, groups, read_bytes, write_bytes, skip=0):
     """Counter calibration on a probe of known traffic: the dispatches of
     `kernel` (substring) in dispatch order, split into `groups` equal runs
     (one per probe variant); per run the mean FETCH_SIZE and WRITE_SIZE
@@ -58,7 +58,7 @@ def calibrate(fetch_csv, write_csv, kernel, groups, read_bytes, write_bytes):
             for r in csv.DictReader(f):
                 if r["Counter_Name"] == counter and kernel in r["Kernel_Name"]:
                     vals[int(r["Dispatch_Id"])] += float(r["Counter_Value"])
-        return [vals[k] for k in sorted(vals)]
+        return [vals[k] for k in sorted(vals)][skip:]
     fv, wv = per_dispatch(fetch_csv, "FETCH_SIZE"), per_dispatch(write_csv, "WRITE_SIZE")
     out = []
     for g in range(groups):
@@ -77,6 +77,7 @@ def main():
         ap.add_argument("--write", required=True)
         ap.add_argument("--kernel", required=True)
         ap.add_argument("--groups", type=int, default=1)
+        ap.add_argument("--skip", type=int, default=0, help="leading dispatches of the kernel that are not probes")
         ap.add_argument("--names", default="")
         ap.add_argument("--read-bytes", type=float, required=True)
         ap.add_argument("--write-bytes", type=float, required=True)
@@ -85,7 +86,7 @@ def main():
         print("| probe | bytes read | FETCH_SIZE bytes | read scale | bytes written | WRITE_SIZE bytes | write scale |")
         print("|---|---:|---:|---:|---:|---:|---:|")
         for nm, (f, w, fs, ws) in zip(names, calibrate(a.fetch, a.write, a.kernel, a.groups, a.read_bytes,
-                                                         a.write_bytes)):
+                                                         a.write_bytes, a.skip)):
             print(f"| {nm} | {a.read_bytes:.0f} | {f:.0f} | {fs:.3f} | {a.write_bytes:.0f} | {w:.0f} | {ws:.3f} |")
         return
     ap = argparse.ArgumentParser()
@@ -97,7 +98,14 @@ def main():
     ap.add_argument("--copy-bytes", type=float, default=0.0)
     ap.add_argument("--particles", type=int, default=0)
     ap.add_argument("--bytes-per-particle", type=float, default=48.0)
+    ap.add_argument("--read-scale", type=float, default=0.0,
+                    help="FETCH_SIZE scale from a probe of the kernels' access shape (overrides the copy's)")
+    ap.add_argument("--write-scale", type=float, default=0.0)
+    ap.add_argument("--kernel-read-scale", action="append", default=[],
+                    help="kernel=scale: FETCH_SIZE scale for one kernel (a gather's scattered lines are counted "
+                         "whole, a coalesced stream's at half)")
     a = ap.parse_args()
+    kscale = {kv.split("=")[0]: float(kv.split("=")[1]) for kv in a.kernel_read_scale}
 
     stats = load_stats(a.stats)
     merged = defaultdict(lambda: [0, 0.0])
@@ -132,6 +140,12 @@ def main():
             fs, ws = a.copy_bytes / cf, a.copy_bytes / cw
             print(f"\ncopy calibration ({a.copy_bytes:.0f} B copied): FETCH_SIZE reads {cf:.0f} B -> scale {fs:.3f}; "
                   f"WRITE_SIZE reads {cw:.0f} B -> scale {ws:.3f}")
+        if a.read_scale:
+            fs = a.read_scale
+        if a.write_scale:
+            ws = a.write_scale
+        print(f"\nscales applied: FETCH_SIZE x {fs:.3f}"
+              + "".join(f" ({k}: x {v:.3f})" for k, v in kscale.items()) + f", WRITE_SIZE x {ws:.3f}")
         f = load_counter(a.fetch, "FETCH_SIZE")
         w = load_counter(a.write, "WRITE_SIZE")
         nf = f.get("k_collide", (0.0, 1))[1]
@@ -143,7 +157,7 @@ def main():
             fk, nk = f.get(k, (0.0, 0))
             wk, _ = w.get(k, (0.0, 0))
             per = nk / nf if k in PIPELINE else 1.0
-            rb, wb = fk * 1024 * fs * per, wk * 1024 * ws * per
+            rb, wb = fk * 1024 * kscale.get(k, fs) * per, wk * 1024 * ws * per
             if k in PIPELINE:
                 tot += rb + wb
             print(f"| {k} | {nk} | {fk * per:.0f} | {wk * per:.0f} | {rb / 1e6:.1f} | {wb / 1e6:.1f} |")
