@@ -192,9 +192,7 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
     errors from the spread of replica means; each closure/explicit ratio
     carries the standard error propagated from both sides (Rg =
     sqrt(<Rg^2>), so its ratio is the square root of the Rg^2 ratio with
-    half its relative error).  Time to converged Rg: from the coil start,
-    the intervals until the ensemble mean Rg^2 first comes within 5% of its
-    steady value, times each method's wall time per interval.  The same
+    half its relative error).  The same
     check runs with the full 24-feature network and with the network
     restricted to the flow-relative velocity and the shear rate
     (free_draining); the networks' likelihood of held-out collisions is
@@ -291,12 +289,6 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
         pm = np.array(acc).mean(axis=0)  # per replica (Rg^2, chi, w_z)
         return pm.mean(axis=0), pm.std(axis=0, ddof=1) / np.sqrt(R), el, np.array(trace)
 
-    def converged(trace, steady):
-        """intervals from the coil start until the ensemble mean Rg^2 first
-        lies within 5% of its steady value"""
-        k = np.nonzero(np.abs(trace / steady - 1.0) <= 0.05)[0]
-        return 10 * (int(k[0]) + 1) if len(k) else None
-
     R = check_replicas
     # the stars stretch over the check: a box whose Verlet grid holds them
     # (a monomer leaving it ends the run, as the reference's SIGSEGV does)
@@ -317,8 +309,6 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
     em, es, et, etr = ensemble(adv_e, e, R)
     e.close()
     print(f"[closure] explicit ensemble done in {et:.1f} s", file=sys.stderr, flush=True)
-    total = check_warmup + check_intervals
-    e_conv = converged(etr, em[0])
 
     def check(network, seed=500):
         c = cl.ClosureRunner(network, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=L,
@@ -328,10 +318,6 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
         c.close()
         ratio = cm / em
         rse = np.abs(ratio) * np.sqrt((cs / cm) ** 2 + (es / em) ** 2)
-        c_conv = converged(ctr, cm[0])
-        conv = None
-        if e_conv is not None and c_conv is not None:
-            conv = (e_conv * et / total) / (c_conv * ct / total)
         return {"Rg2": float(cm[0]), "Rg2_se": float(cs[0]), "chi": float(cm[1]), "chi_se": float(cs[1]),
                 "wz": float(cm[2]), "wz_se": float(cs[2]),
                 "Rg2_ratio": float(ratio[0]), "Rg2_ratio_se": float(rse[0]),
@@ -339,9 +325,7 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
                 "chi_ratio": float(ratio[1]), "chi_ratio_se": float(rse[1]),
                 "wz_ratio": float(ratio[2]), "wz_ratio_se": float(rse[2]),
                 "seconds": ct, "fallbacks": fb,
-                "time_to_equal_Rg2_error_speedup": float((et * es[0] ** 2) / (ct * cs[0] ** 2)),
-                "converged_Rg_intervals": c_conv, "time_to_converged_Rg_speedup": conv,
-                "Rg2_trace": [float(x) for x in ctr[::4]]}
+                "time_to_equal_Rg2_error_speedup": float((et * es[0] ** 2) / (ct * cs[0] ** 2))}
 
     head = check(net)
     print("[closure] headline check done", file=sys.stderr, flush=True)
@@ -389,9 +373,12 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
                   "explicit_Rg2": float(em[0]), "explicit_Rg2_se": float(es[0]),
                   "explicit_chi": float(em[1]), "explicit_chi_se": float(es[1]),
                   "explicit_wz": float(em[2]), "explicit_wz_se": float(es[2]),
-                  "explicit_seconds": et, "explicit_converged_Rg_intervals": e_conv,
-                  "explicit_Rg2_trace": [float(x) for x in etr[::4]],
-                  "closure": head},
+                  "explicit_seconds": et, "closure": head},
+        # the 1,200 intervals of the check are the start-up: the stars are
+        # still stretching.  The steady state takes ~13,000 intervals and is
+        # measured by --closure-converge, where the closure over-stretches
+        # and does not converge to the explicit plateau.
+        "time_to_converged_Rg_speedup": None,
         "ablations": ablations,
         "heldout": {"samples": int(len(yh)), "nll_headline": float(nll["headline"].mean()),
                     "nll_gain_headline_over_free_draining": g_head,
