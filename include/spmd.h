@@ -264,10 +264,28 @@ int spmd_snapshot_load(const char* path, spmd_sys** out, int64_t* step);
  * collide tile (64,128,256); chosen by starpolymersmd_amd.autotune. */
 int spmd_set_tuning(spmd_sys* s, int ppt, int tile_cells);
 int spmd_get_tuning(const spmd_sys* s, int* ppt, int* tile_cells);
+/* Collision path (no reference counterpart; both give the same bits).
+ * SPMD_PATH_AUTO: once a collision has put the solvent in cell order, the
+ * row-tile path (advect + row-window collide: no sort pass) whenever every
+ * replica's x-row fits a tile (L_x <= 256, L_x * N_c <= 1300); the first
+ * collision after the fluid was set, and handles whose rows do not fit,
+ * take the sort path.  SPMD_PATH_SORT: always the sort path (bin, scan,
+ * permutation, collide).  spmd_get_collision_path tells which path the
+ * next collision takes. */
+enum { SPMD_PATH_AUTO = 0, SPMD_PATH_SORT = 1, SPMD_PATH_ROWS = 2 };
+int spmd_set_collision_path(spmd_sys* s, int path);
+/* Speed bound of the row-tile path's source-row windows in units of
+ * sqrt(kT/m) (default 5): the window of rows a tile reads covers particles
+ * up to that speed per axis, faster ones come through an outlier list.
+ * Results do not depend on it; smaller is fewer rows, more outliers. */
+int spmd_set_row_window(spmd_sys* s, double speed);
+int spmd_get_collision_path(const spmd_sys* s, int* next_path);
 
 /* ---- timing (device events on the handle's stream) ---- */
 /* average duration in ms of each kernel class over the launches since the
- * last reset: names "md", "bin", "scan", "perm", "collide", "prologue". */
+ * last reset: names "md", "bin", "scan", "perm", "collide", "prologue",
+ * and for the row-tile path "advect" and "redo" ("collide" is then the
+ * row-window collide). */
 int spmd_profile_enable(spmd_sys* s, int on);
 int spmd_profile_read(spmd_sys* s, const char* name, double* avg_ms, int64_t* count);
 int spmd_profile_reset(spmd_sys* s);

@@ -32,7 +32,18 @@ struct ProfEvents {
     std::vector<hipEvent_t> ev;  /* start/stop pairs */
 };
 
-const char* kProfNames[6] = {"md", "bin", "scan", "perm", "collide", "prologue"};
+constexpr int kProfKinds = 8;
+const char* kProfNames[kProfKinds] = {"md", "bin", "scan", "perm", "collide", "prologue", "advect", "redo"};
+
+/* Row tiles: a tile's mean occupancy stays below these (cells x density),
+ * so that its particle capacity (1024 / 2048) lies more than 12 standard
+ * deviations of a Poisson count above the mean. */
+constexpr double kRowTileMeanSmall = 640.0;
+constexpr double kRowTileMean = 1300.0;
+/* Speed bound (units of sqrt(kT / m), m = 1) of the row-tile path's
+ * source-row windows: a particle whose row moves farther (per axis, about
+ * 6e-7 of them at 5) is taken through the outlier list instead. */
+constexpr double kWinSpeedDefault = 5.0;
 
 /* Device scratch of spmd_velocity_profile_noalloc: the profile's fixed-point
  * bins [2][kObsBins][3], then kStarMomentsWords doubles for the star's
@@ -74,11 +85,15 @@ struct spmd_sys {
     double pending_dt = 0.0;
     bool collided = false;
     /* graphs of one collision interval, per starting buffer */
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  /* [path][starting buffer] */
+    /* collision path: 0 automatic (row tiles when the buffer is in cell
+     * order and the replicas allow it), 1 always the sort path */
+    int path_pref = 0;
+    bool tc_dirty = true;          /* row-tile arrival counters must be cleared first */
     double graph_h = -1.0;
     /* profiling */
     bool prof = false;
-    ProfEvents pev[6];
+    ProfEvents pev[kProfKinds];
     std::vector<size_t> alloc_bytes;
     /* learned closure workspace */
     float* cl_feat = nullptr;      /* m_tot x 24 */
@@ -130,8 +145,9 @@ void dfree(T*& p) {
 }
 
 void clear_graphs(spmd_sys* s) {
-    for (auto& g : s->graph)
-        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    for (auto& gp : s->graph)
+        for (auto& g : gp)
+            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
 }
 
 int upload_tables(spmd_sys* s) {
@@ -149,7 +165,70 @@ int upload_tables(spmd_sys* s) {
     size_t need = std::max<size_t>(4ull * (d.n_pblk + d.n_rep), 2ull * d.n_tiles);
     dfree(d.red);
     if (int rc = dalloc(s, &d.red, need)) return rc;
+    /* row tiles: about tile_cells cells (whole x-rows) each, fewer rows
+     * where the mean occupancy would pass kRowTileMean; the path is off
+     * for the handle if a single row holds more */
+    std::vector<uint32_t> rb(d.n_rep + 1, 0);
+    uint32_t planes = 0;
+    double max_mean = 0.0;
+    int max_tc = 0;
+    bool ok = true;
+    for (int r = 0; r < d.n_rep; ++r) {
+        RepDev& rp = s->reps[r];
+        const int Lx = rp.box.iL[0], Ly = rp.box.iL[1], Lz = rp.box.iL[2];
+        const double rho = (double)rp.n_fluid / (double)rp.ncells;
+        int nr = std::max(1, d.tile_cells / Lx);
+        nr = std::min(std::min(nr, Ly), std::min(kRowsMax, std::max(1, kRowTileCells / Lx)));
+        while (nr > 1 && nr * Lx * rho > kRowTileMean) --nr;
+        if (Lx > kRowTileCells || Lx * rho > kRowTileMean) ok = false;
+        rp.nr = nr;
+        rp.tpp = (Ly + nr - 1) / nr;
+        rp.rt_base = rb[r];
+        rp.pl_base = planes;
+        rb[r + 1] = rb[r] + (uint32_t)(rp.tpp * Lz);
+        planes += (uint32_t)Lz;
+        max_mean = std::max(max_mean, nr * Lx * rho);
+        max_tc = std::max(max_tc, nr * Lx);
+    }
+    d.rt_cap = !ok ? 0 : (max_mean <= kRowTileMeanSmall ? 1024 : 2048);
+    d.rt_tcm = max_tc <= 128 ? 128 : 256;
+    d.n_rtiles = rb[d.n_rep];
+    d.n_planes = planes;
+    for (int b = 0; b < 2; ++b) {
+        dfree(d.tcount[b]);
+        dfree(d.pcount[b]);
+        if (int rc = dalloc(s, &d.tcount[b], d.n_rtiles)) return rc;
+        if (int rc = dalloc(s, &d.pcount[b], d.n_planes)) return rc;
+        HIPCHK(s, hipMemset(d.tcount[b], 0, 4ull * d.n_rtiles));
+        HIPCHK(s, hipMemset(d.pcount[b], 0, 4ull * d.n_planes));
+    }
+    dfree(d.tcount_r);
+    dfree(d.pcount_r);
+    if (int rc = dalloc(s, &d.tcount_r, d.n_rtiles)) return rc;
+    if (int rc = dalloc(s, &d.pcount_r, d.n_planes)) return rc;
+    HIPCHK(s, hipMemset(d.tcount_r, 0, 4ull * d.n_rtiles));
+    HIPCHK(s, hipMemset(d.pcount_r, 0, 4ull * d.n_planes));
+    HIPCHK(s, hipMemcpy(d.rtile_base, rb.data(), rb.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(d.reps, s->reps.data(), s->reps.size() * sizeof(RepDev), hipMemcpyHostToDevice));
+    s->tc_dirty = false;
     clear_graphs(s);
+    return SPMD_OK;
+}
+
+/* The row-tile path needs the current buffer in cell order (any collision
+ * leaves it so) and the replicas' rows within a tile's capacity. */
+bool use_rows(const spmd_sys* s) { return s->d.rt_cap != 0 && s->collided && s->path_pref == 0; }
+
+/* The arrival counters start from zero: after a collision of the sort
+ * path they hold stale counts (outside any graph capture). */
+int prep_rows(spmd_sys* s) {
+    if (!s->tc_dirty) return SPMD_OK;
+    const DevState& d = s->d;
+    for (int b = 0; b < 2; ++b) {
+        HIPCHK(s, hipMemsetAsync(d.tcount[b], 0, 4ull * d.n_rtiles, s->stream));
+        HIPCHK(s, hipMemsetAsync(d.pcount[b], 0, 4ull * d.n_planes, s->stream));
+    }
+    s->tc_dirty = false;
     return SPMD_OK;
 }
 
@@ -287,8 +366,27 @@ hipError_t enqueue_collision_ops(spmd_sys* s, int n_md, int epot_last, int n_del
         if ((e = hipEventRecord(s->ev_join, s->side)) != hipSuccess) return e;
     }
     prof_begin(s, 5, s->stream);
-    if ((e = launch_prologue(d, s->stream, 1, n_delrx, delrx_dt)) != hipSuccess) return e;
+    if ((e = launch_prologue(d, s->stream, 1, n_delrx, delrx_dt, h)) != hipSuccess) return e;
     prof_end(s, 5, s->stream);
+    if (use_rows(s)) {
+        /* row-tile path: advect (beside the MD, assuming no recentring),
+         * the monomers' cells and a recentred replica's advect again after
+         * it, then the row-window collide */
+        prof_begin(s, 6, s->stream);
+        if ((e = launch_advect(d, s->stream, h, d.m_tot ? kSortSpec : kSortNormal)) != hipSuccess) return e;
+        prof_end(s, 6, s->stream);
+        if (d.m_tot > 0) {
+            if ((e = hipStreamWaitEvent(s->stream, s->ev_join, 0)) != hipSuccess) return e;
+            prof_begin(s, 7, s->stream);
+            if ((e = launch_rows_redo(d, s->stream, h)) != hipSuccess) return e;
+            prof_end(s, 7, s->stream);
+        }
+        prof_begin(s, 4, s->stream);
+        if ((e = launch_collide_rows(d, s->stream, h)) != hipSuccess) return e;
+        prof_end(s, 4, s->stream);
+        return hipSuccess;
+    }
+    s->tc_dirty = true;
     if (d.m_tot == 0) {
         prof_begin(s, 1, s->stream);
         if ((e = launch_bin(d, s->stream, h, kSortNormal)) != hipSuccess) return e;
@@ -319,6 +417,8 @@ hipError_t enqueue_collision_ops(spmd_sys* s, int n_md, int epot_last, int n_del
 }
 
 int enqueue_collision(spmd_sys* s, int n_md, int epot_last, int n_delrx, double delrx_dt, double h) {
+    if (use_rows(s))
+        if (int rc = prep_rows(s)) return rc;
     const hipError_t e = enqueue_collision_ops(s, n_md, epot_last, n_delrx, delrx_dt, h);
     if (e != hipSuccess) return fail(s, SPMD_ERR_HIP, std::string("collision launch: ") + hipGetErrorString(e));
     s->d.cur ^= 1;
@@ -329,7 +429,7 @@ int enqueue_collision(spmd_sys* s, int n_md, int epot_last, int n_delrx, double 
 int flush_md(spmd_sys* s) {
     if (s->pending_md == 0 && s->pending_delrx == 0) return SPMD_OK;
     if (int rc = ensure_monomers(s)) return rc;
-    LAUNCH(s, 5, s->stream, launch_prologue(s->d, s->stream, 0, s->pending_delrx, s->pending_dt));
+    LAUNCH(s, 5, s->stream, launch_prologue(s->d, s->stream, 0, s->pending_delrx, s->pending_dt, 0.0));
     LAUNCH(s, 0, s->stream, launch_md(s->d, s->stream, s->pending_md, s->pending_epot, 0, 0, s->md_dt));
     s->pending_md = 0;
     s->pending_delrx = 0;
@@ -352,6 +452,8 @@ int check_status(spmd_sys* s) {
     if (st & ST_VERLET_OVERFLOW) return fail(s, SPMD_ERR_VERLET_OVERFLOW, std::string("Verlet list overflow; ") + buf);
     if (st & ST_SORT_MISMATCH)
         return fail(s, SPMD_ERR_SCAN, std::string("collide found a particle outside its sorted cell; ") + buf);
+    if (st & ST_TILE_OVERFLOW)
+        return fail(s, SPMD_ERR_RANK_OVERFLOW, std::string("more particles arrived in a row tile than it holds; ") + buf);
     if (st & ST_TILE_MONO)
         return fail(s, SPMD_ERR_RANK_OVERFLOW, std::string("more monomers in one collide tile than it holds; ") + buf);
     return fail(s, SPMD_ERR_RANK_OVERFLOW, std::string("cell occupancy overflow; ") + buf);
@@ -581,6 +683,7 @@ int spmd_create_batch(const spmd_params* p, uint32_t n_rep, spmd_sys** out) {
     d.n_rep = (int)n_rep;
     d.ppt = 4;
     d.tile_cells = 128;
+    d.win_speed = kWinSpeedDefault;
     uint64_t nt = 0, ct = 0;
     for (uint32_t r = 0; r < n_rep; ++r) {
         const spmd_params& q = p[r];
@@ -642,7 +745,11 @@ int spmd_create_batch(const spmd_params* p, uint32_t n_rep, spmd_sys** out) {
     rc = rc ? rc : dalloc(s, &d.pcell, nt);
     rc = rc ? rc : dalloc(s, &d.perm, nt);
     rc = rc ? rc : dalloc(s, &d.counts, ct + 16);
-    rc = rc ? rc : dalloc(s, &d.offsets, ct + 16);
+    rc = rc ? rc : dalloc(s, &d.offs[0], ct + 16);
+    rc = rc ? rc : dalloc(s, &d.offs[1], ct + 16);
+    rc = rc ? rc : dalloc(s, &d.outl, nt);
+    rc = rc ? rc : dalloc(s, &d.outl_r, nt);
+    rc = rc ? rc : dalloc(s, &d.rtile_base, n_rep + 1);
     rc = rc ? rc : dalloc(s, &d.scan_flags, d.n_scan_tiles);
     rc = rc ? rc : dalloc(s, &d.scan_ticket, 1);
     rc = rc ? rc : dalloc(s, &d.reps, n_rep);
@@ -651,7 +758,8 @@ int spmd_create_batch(const spmd_params* p, uint32_t n_rep, spmd_sys** out) {
     rc = rc ? rc : dalloc(s, &d.tile_base, n_rep + 1);
     if (rc) { spmd_destroy(s); return rc; }
     HIPCHK(s, hipMemset(d.counts, 0, (ct + 16) * 4));
-    HIPCHK(s, hipMemset(d.offsets, 0, (ct + 16) * 4));
+    HIPCHK(s, hipMemset(d.offs[0], 0, (ct + 16) * 4));
+    HIPCHK(s, hipMemset(d.offs[1], 0, (ct + 16) * 4));
     HIPCHK(s, hipMemset(d.scan_flags, 0, d.n_scan_tiles * 8ull));
     HIPCHK(s, hipMemset(d.scan_ticket, 0, 4));
     for (int b = 0; b < 2; ++b) {
@@ -682,7 +790,9 @@ int spmd_destroy(spmd_sys* s) {
         dfree(d.x[b]); dfree(d.y[b]); dfree(d.z[b]); dfree(d.vx[b]); dfree(d.vy[b]); dfree(d.vz[b]);
         dfree(d.ids[b]);
     }
-    dfree(d.rank); dfree(d.pcell); dfree(d.perm); dfree(d.counts); dfree(d.offsets); dfree(d.scan_flags); dfree(d.scan_ticket);
+    dfree(d.rank); dfree(d.pcell); dfree(d.perm); dfree(d.counts); dfree(d.scan_flags); dfree(d.scan_ticket);
+    for (int b = 0; b < 2; ++b) { dfree(d.offs[b]); dfree(d.tcount[b]); dfree(d.pcount[b]); }
+    dfree(d.tcount_r); dfree(d.pcount_r); dfree(d.outl); dfree(d.outl_r); dfree(d.rtile_base);
     dfree(d.mpos); dfree(d.mvel); dfree(d.mforce); dfree(d.mvpos); dfree(d.mmass); dfree(d.mbtype);
     dfree(d.vl); dfree(d.bonds); dfree(d.mcell);
     dfree(d.inj_axis); dfree(d.inj_gamma); dfree(d.reps); dfree(d.ctl); dfree(d.pblk_base);
@@ -721,6 +831,26 @@ int spmd_set_tuning(spmd_sys* s, int ppt, int tile_cells) {
     s->d.ppt = ppt;
     s->d.tile_cells = tile_cells;
     return upload_tables(s);
+}
+
+int spmd_set_collision_path(spmd_sys* s, int path) {
+    if (!s || (path != SPMD_PATH_AUTO && path != SPMD_PATH_SORT)) return SPMD_ERR_ARG;
+    s->path_pref = path == SPMD_PATH_SORT ? 1 : 0;
+    return SPMD_OK;
+}
+
+int spmd_set_row_window(spmd_sys* s, double speed) {
+    if (!s || !(speed >= 0.0) || speed > 1e3) return SPMD_ERR_ARG;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    s->d.win_speed = speed;
+    clear_graphs(s);
+    return SPMD_OK;
+}
+
+int spmd_get_collision_path(const spmd_sys* s, int* next_path) {
+    if (!s || !next_path) return SPMD_ERR_ARG;
+    *next_path = use_rows(s) ? SPMD_PATH_ROWS : SPMD_PATH_SORT;
+    return SPMD_OK;
 }
 
 int spmd_get_tuning(const spmd_sys* s, int* ppt, int* tile_cells) {
@@ -927,11 +1057,17 @@ int spmd_run(spmd_sys* s, int64_t t0, int64_t n, double mpc_dt) {
     if (int rc = ensure_monomers(s)) return rc;
     /* whole intervals t = kS+1 .. (k+1)S: replay the captured graph */
     while (end - t >= S && at_start()) {
-        if (s->prof) {
+        /* the first collision after the fluid was set orders the buffer
+         * (sort path): run it directly, the steady path from the graph */
+        if (s->prof || (!s->collided && s->d.rt_cap && s->path_pref == 0)) {
             if (int rc = enqueue_collision(s, (int)S, 0, (int)S, mpc_dt, mpc_dt)) return rc;
         } else {
             const int c = s->d.cur;
-            if (!s->graph[c] || s->graph_h != mpc_dt) {
+            const int path = use_rows(s) ? 1 : 0;
+            if (path)
+                if (int rc = prep_rows(s)) return rc;
+            hipGraphExec_t& gx = s->graph[path][c];
+            if (!gx || s->graph_h != mpc_dt) {
                 if (s->graph_h != mpc_dt) clear_graphs(s);
                 hipGraph_t g;
                 HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
@@ -939,11 +1075,12 @@ int spmd_run(spmd_sys* s, int64_t t0, int64_t n, double mpc_dt) {
                 hipError_t e2 = hipStreamEndCapture(s->stream, &g);
                 if (e != hipSuccess || e2 != hipSuccess)
                     return fail(s, SPMD_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e ? e : e2));
-                HIPCHK(s, hipGraphInstantiate(&s->graph[c], g, nullptr, nullptr, 0));
+                HIPCHK(s, hipGraphInstantiate(&gx, g, nullptr, nullptr, 0));
                 (void)hipGraphDestroy(g);
                 s->graph_h = mpc_dt;
             }
-            HIPCHK(s, hipGraphLaunch(s->graph[c], s->stream));
+            HIPCHK(s, hipGraphLaunch(gx, s->stream));
+            if (!path) s->tc_dirty = true;
             s->d.cur ^= 1;
             s->collided = true;
         }
@@ -1023,7 +1160,7 @@ int spmd_get_cell_index(spmd_sys* s, uint32_t r, uint32_t* fluid_cells, uint32_t
     if (fluid_cells) {
         if (!d.ids[d.cur]) return fail(s, SPMD_ERR_STATE, "particle tracking is off");
         std::vector<uint32_t> off(rp.ncells + 1), ids(rp.n_fluid);
-        HIPCHK(s, hipMemcpy(off.data(), d.offsets + rp.cell_base, 4ull * (rp.ncells + 1), hipMemcpyDeviceToHost));
+        HIPCHK(s, hipMemcpy(off.data(), d.offs[d.cur] + rp.cell_base, 4ull * (rp.ncells + 1), hipMemcpyDeviceToHost));
         HIPCHK(s, hipMemcpy(ids.data(), d.ids[d.cur] + rp.part_base, 4ull * rp.n_fluid, hipMemcpyDeviceToHost));
         for (uint32_t c = 0; c < rp.ncells; ++c)
             for (uint32_t p = off[c]; p < off[c + 1]; ++p) fluid_cells[ids[p - rp.part_base]] = c;
@@ -1289,7 +1426,7 @@ int spmd_velocity_profile_noalloc(spmd_sys* s, uint32_t r, double width, uint32_
     const DevState& d = s->d;
     /* pending substeps and delrx advances first, as every observer sees them */
     if (s->pending_md || s->pending_delrx) {
-        HIPCHK(s, launch_prologue(d, s->stream, 0, s->pending_delrx, s->pending_dt));
+        HIPCHK(s, launch_prologue(d, s->stream, 0, s->pending_delrx, s->pending_dt, 0.0));
         HIPCHK(s, launch_md(d, s->stream, s->pending_md, s->pending_epot, 0, 0, s->md_dt));
         s->pending_md = 0;
         s->pending_delrx = 0;
@@ -1629,7 +1766,7 @@ int spmd_profile_reset(spmd_sys* s) {
 int spmd_profile_read(spmd_sys* s, const char* name, double* avg_ms, int64_t* count) {
     if (!s || !name) return SPMD_ERR_ARG;
     int k = -1;
-    for (int i = 0; i < 6; ++i)
+    for (int i = 0; i < kProfKinds; ++i)
         if (!std::strcmp(name, kProfNames[i])) k = i;
     if (k < 0) return fail(s, SPMD_ERR_ARG, "unknown kernel class");
     HIPCHK(s, hipStreamSynchronize(s->stream));

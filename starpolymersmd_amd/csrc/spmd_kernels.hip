@@ -731,11 +731,22 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
 
 /* ========================================================== prologue */
 
+/* Rows a particle's shifted coordinate can cross between two collisions:
+ * the shifted coordinate moves by ds +- vh (plus FP32 rounding, covered by
+ * 1e-3), and a row index floor(a + d) - floor(a) lies in [floor(d_min),
+ * floor(d_max) + 1].  A window as wide as the box takes every row. */
+__device__ __forceinline__ void row_window(double ds, double vh, int L, int& lo, int& n) {
+    const double dmin = ds - vh - 1e-3, dmax = ds + vh + 1e-3;
+    lo = (int)floor(dmin);
+    n = (int)floor(dmax) + 1 - lo + 1;
+    if (n >= L) { lo = 0; n = L; }
+}
+
 /* Per replica, before anything else of a collision: the delrx advance of
  * every pending MPC::step call (MPC.cpp:117-118), then the collision's
  * Philox counter and grid shift (MPC.cpp:123).  Block 0 resets the scan
  * state. */
-__global__ __launch_bounds__(64) void k_prologue(DevState d, int do_coll, int n_delrx, double delrx_dt) {
+__global__ __launch_bounds__(64) void k_prologue(DevState d, int do_coll, int n_delrx, double delrx_dt, double h) {
     const int r = blockIdx.x;
     const RepDev& rep = d.reps[r];
     RepCtl* ctl = d.ctl + r;
@@ -759,7 +770,23 @@ __global__ __launch_bounds__(64) void k_prologue(DevState d, int do_coll, int n_
                 for (int c = 0; c < 3; ++c) sh[c] = u01(w[c]) - 0.5;
                 ctl->inj_active = 0;
             }
-            for (int c = 0; c < 3; ++c) ctl->shift[c] = sh[c];
+            for (int c = 0; c < 3; ++c) {
+                ctl->shift_prev[c] = ctl->shift[c];
+                ctl->shift[c] = sh[c];
+            }
+            /* source-row window of the row-tile path: the shifted y (z)
+             * of a particle moves by shift - shift_prev + v h between two
+             * collisions, so its row moves by floor(f + that) for some f in
+             * [0, 1); speeds up to win_speed sqrt(kT) keep it inside */
+            const double vh = d.win_speed * sqrt(rep.kT) * h;
+            for (int a = 1; a < 3; ++a) {
+                int lo, n;
+                row_window(sh[a] - ctl->shift_prev[a], vh, rep.box.iL[a], lo, n);
+                ctl->wlo[a - 1] = lo;
+                ctl->wn[a - 1] = n;
+            }
+            ctl->n_out = 0u;
+            ctl->n_out_redo = 0u;
             ctl->k_cur = k;
             ctl->k_done = k + 1;
             if (d.m_tot == 0) {  /* else k_md, running beside this kernel, sets them */
@@ -929,13 +956,13 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(DevState d) {
         if (timeout) atomicOr(&d.ctl[0].status, ST_SCAN_TIMEOUT);
         if (lane == 0) {
             s_prefix = acc;
-            if ((uint64_t)(tile + 1) * kScanTile >= d.c_tot) d.offsets[d.c_tot] = acc + agg;
+            if ((uint64_t)(tile + 1) * kScanTile >= d.c_tot) d.offs[d.cur ^ 1][d.c_tot] = acc + agg;
         }
     }
     __syncthreads();
     const uint32_t off = s_prefix + texcl;
     if (base + kScanItems <= d.c_tot) {
-        uint4* po = reinterpret_cast<uint4*>(d.offsets + base);
+        uint4* po = reinterpret_cast<uint4*>(d.offs[d.cur ^ 1] + base);
         uint4* pc = reinterpret_cast<uint4*>(d.counts + base);
 #pragma unroll
         for (int q = 0; q < kScanItems / 4; ++q) {
@@ -944,7 +971,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(DevState d) {
         }
     } else {
         for (int q = 0; q < kScanItems; ++q)
-            if (base + q < d.c_tot) { d.offsets[base + q] = off + v[q]; d.counts[base + q] = 0u; }
+            if (base + q < d.c_tot) { d.offs[d.cur ^ 1][base + q] = off + v[q]; d.counts[base + q] = 0u; }
     }
 }
 
@@ -1013,7 +1040,7 @@ __global__ __launch_bounds__(1024) void k_sort_redo(DevState d, double h) {
         uint32_t pre = 0, tot = 0;
         for (int q = 0; q < 16; ++q) { pre += q < w ? s_w[q] : 0u; tot += s_w[q]; }
         const uint32_t base = s_carry;
-        if (cc < rep.ncells) d.offsets[g] = base + pre + inc - v;
+        if (cc < rep.ncells) d.offs[d.cur ^ 1][g] = base + pre + inc - v;
         __syncthreads();
         if (tid == 0) s_carry = base + tot;
         __syncthreads();
@@ -1023,7 +1050,7 @@ __global__ __launch_bounds__(1024) void k_sort_redo(DevState d, double h) {
     for (uint64_t i = p0 + tid; i < pend; i += 1024) {
         const uint32_t cell = d.pcell[i];
         if (cell == ~0u) continue;
-        const uint32_t dst = d.offsets[cell] + d.rank[i];
+        const uint32_t dst = d.offs[d.cur ^ 1][cell] + d.rank[i];
         if (dst >= pend || dst < p0) continue;
         d.perm[dst] = (uint32_t)i;
     }
@@ -1046,7 +1073,7 @@ __global__ __launch_bounds__(256) void k_perm(DevState d, int mode) {
         if (i >= pend) continue;
         const uint32_t cell = d.pcell[i];
         if (cell == ~0u) continue;
-        const uint32_t dst = d.offsets[cell] + d.rank[i];
+        const uint32_t dst = d.offs[d.cur ^ 1][cell] + d.rank[i];
         if (dst >= pend || dst < rep.part_base) continue;  /* cannot happen when bin and scan agree */
         d.perm[dst] = (uint32_t)i;
     }
@@ -1109,7 +1136,7 @@ __global__ __launch_bounds__(256) void k_collide(DevState d, double h) {
     uint32_t* oids = d.ids[alt];
     const double shear = rep.box.shear, LyD = rep.box.L[1];
 
-    for (uint32_t i = tid; i <= nc; i += NT) s_off[i] = d.offsets[gc0 + i];
+    for (uint32_t i = tid; i <= nc; i += NT) s_off[i] = d.offs[d.cur ^ 1][gc0 + i];
     for (uint32_t i = tid; i < nc; i += NT) {
         s_px[i] = s_py[i] = s_pz[i] = s_e[i] = s_mm[i] = 0ull;
         s_nm[i] = 0u;
@@ -1407,6 +1434,700 @@ __global__ __launch_bounds__(256) void k_collide(DevState d, double h) {
     }
 }
 
+/* ===================================================== row-tile path */
+
+/* A row tile is nr consecutive y-rows (whole x-rows) of one z-plane of a
+ * replica, so its cells are one contiguous range of the cell order.  The
+ * current buffer is in cell order (every collision writes it so), so the
+ * particles of any set of rows are contiguous runs of it.
+ *
+ * Per collision:
+ *   k_advect        one workgroup per row tile of the current buffer:
+ *                   streaming and the new shifted cell of its particles
+ *                   (pcell, bit-identical to k_bin's), and the arrivals per
+ *                   destination row tile and z-plane (LDS counters, about
+ *                   nine global atomics per workgroup)
+ *   k_rows_redo     after the MD: the monomers' cells; for a replica the MD
+ *                   recentred, k_advect's work again with the recentring
+ *   k_collide_rows  one workgroup per row tile of the new cells: its first
+ *                   slot from the arrival counts, its particles pulled from
+ *                   the window of source rows they can have come from (a
+ *                   row moves by at most a few rows between collisions;
+ *                   faster particles come through the outlier list), a
+ *                   counting sort in LDS, the collision with cell sums
+ *                   per cell thread, and the new buffer written in cell
+ *                   order, coalesced, with its cell offsets.
+ * No bin, scan or permutation pass: each particle is read once by k_advect
+ * and once (plus its pcell word from the windows, mostly from L2) by
+ * k_collide_rows, and written once. */
+
+struct RowTile {
+    int z, ty, y0, ny;
+    uint32_t c_lo, nc;     /* first cell (replica-local) and cells */
+};
+
+__device__ __forceinline__ RowTile row_tile(const RepDev& rep, uint32_t t) {
+    RowTile T;
+    T.z = (int)(t / (uint32_t)rep.tpp);
+    T.ty = (int)t - T.z * rep.tpp;
+    T.y0 = T.ty * rep.nr;
+    T.ny = min(rep.nr, rep.box.iL[1] - T.y0);
+    T.c_lo = (uint32_t)rep.box.iL[0] * (uint32_t)(T.y0 + rep.box.iL[1] * T.z);
+    T.nc = (uint32_t)(rep.box.iL[0] * T.ny);
+    return T;
+}
+
+__device__ __forceinline__ int pmod(int a, int L) {
+    a %= L;
+    return a < 0 ? a + L : a;
+}
+
+/* The cell shifted_cell gives, and its row (y, z).  The row is read off
+ * the cell index when a coordinate sits on the far face (truncation gives
+ * L there), so it is always the row of the index. */
+__device__ __forceinline__ uint32_t shifted_cell_row(const Box& b, const RepCtl& c, double x, double y, double z,
+                                                     double vx, int& iy, int& iz) {
+    double px = x + c.shift[0], py = y + c.shift[1], pz = z + c.shift[2];
+    le_wrap(b, px, py, pz, vx);
+    const int ix = (int)(px + b.half[0]);
+    iy = (int)(py + b.half[1]);
+    iz = (int)(pz + b.half[2]);
+    const uint32_t cell = (uint32_t)(ix + b.iL[0] * iy + b.iL[0] * b.iL[1] * iz);
+    if ((unsigned)ix >= (unsigned)b.iL[0] || (unsigned)iy >= (unsigned)b.iL[1] ||
+        (unsigned)iz >= (unsigned)b.iL[2]) {
+        const uint32_t row = cell / (uint32_t)b.iL[0];
+        iy = (int)(row % (uint32_t)b.iL[1]);
+        iz = (int)(row / (uint32_t)b.iL[1]);
+    }
+    return cell;
+}
+
+/* Streaming and new cell of the particles of row tile t of the current
+ * buffer.  A particle whose row moved by (dy, dz) outside the window
+ * [wylo, wylo + wyn) x [wzlo, wzlo + wzn) (mod L) is an outlier: kOutBit in
+ * pcell and listed.  Arrivals are counted per destination row tile and
+ * z-plane: the tiles a row tile's particles reach lie in an 8 x 8 block of
+ * tiles around the window (LDS counters), others go straight to global
+ * atomics.  s_rs holds kRowsMax + 1 words, s_hist 64. */
+template <int NT>
+__device__ void advect_tile(const DevState& d, int r, uint32_t t, const RepCtl& c, int wylo, int wyn, int wzlo,
+                            int wzn, double h, uint32_t* tcnt, uint32_t* pcnt, uint32_t* outl, uint32_t* n_out,
+                            uint32_t* s_rs, uint32_t* s_hist) {
+    const RepDev& rep = d.reps[r];
+    Box b = rep.box;
+    b.delrx = c.delrx;
+    const RowTile T = row_tile(rep, t);
+    const int Lx = b.iL[0], Ly = b.iL[1], Lz = b.iL[2];
+    const uint32_t* off = d.offs[d.cur];
+    const uint32_t gc0 = (uint32_t)rep.cell_base + T.c_lo;
+    const int tid = threadIdx.x;
+    __syncthreads();  /* s_rs and s_hist of a previous tile are read out */
+    for (int j = tid; j <= T.ny; j += NT) s_rs[j] = off[gc0 + (uint32_t)(Lx * j)];
+    for (int j = tid; j < 64; j += NT) s_hist[j] = 0u;
+    __syncthreads();
+    /* offsets left by a collision that reported an error need not be
+     * monotone: stay inside the replica's particles in any case */
+    const uint32_t pr0 = (uint32_t)rep.part_base, pr1 = pr0 + rep.n_fluid;
+    const uint32_t i0 = min(max(s_rs[0], pr0), pr1), i1 = min(max(s_rs[T.ny], i0), pr1);
+    const int tyf = pmod(T.y0 + wylo, Ly) / rep.nr;  /* tile of the first destination row */
+    const int zf = pmod(T.z + wzlo, Lz);
+    const int cur = d.cur;
+    const float *X = d.x[cur], *Y = d.y[cur], *Z = d.z[cur];
+    const float *VX = d.vx[cur], *VY = d.vy[cur], *VZ = d.vz[cur];
+    bool bad = false;
+    for (uint32_t i = i0 + tid; i < i1; i += NT) {
+        int yo = T.y0;
+        for (int j = 1; j < T.ny; ++j) yo += i >= s_rs[j] ? 1 : 0;
+        double x = X[i], y = Y[i], z = Z[i], vx = VX[i], vy = VY[i], vz = VZ[i];
+        recenter_stream(b, h, c, x, y, z, vx, vy, vz);
+        int iy, iz;
+        const uint32_t cell = shifted_cell_row(b, c, x, y, z, vx, iy, iz);
+        if (cell >= rep.ncells) {
+            bad = true;
+            d.pcell[i] = ~0u;
+            continue;
+        }
+        uint32_t g = (uint32_t)rep.cell_base + cell;
+        if (pmod(iy - yo - wylo, Ly) >= wyn || pmod(iz - T.z - wzlo, Lz) >= wzn) {
+            g |= kOutBit;
+            const uint32_t k = atomicAdd(n_out, 1u);
+            if (k < rep.n_fluid) outl[rep.part_base + k] = i;
+        }
+        d.pcell[i] = g;
+        const int tyd = iy / rep.nr;
+        const int hy = pmod(tyd - tyf, rep.tpp), hz = pmod(iz - zf, Lz);
+        if (hy < 8 && hz < 8) {
+            atomicAdd(&s_hist[hz * 8 + hy], 1u);
+        } else {
+            atomicAdd(&tcnt[rep.rt_base + (uint32_t)(iz * rep.tpp + tyd)], 1u);
+            atomicAdd(&pcnt[rep.pl_base + (uint32_t)iz], 1u);
+        }
+    }
+    __syncthreads();
+    for (int j = tid; j < 64; j += NT) {
+        const uint32_t n = s_hist[j];
+        if (!n) continue;
+        const int zd = (zf + j / 8) % Lz, tyd = (tyf + j % 8) % rep.tpp;
+        atomicAdd(&tcnt[rep.rt_base + (uint32_t)(zd * rep.tpp + tyd)], n);
+        atomicAdd(&pcnt[rep.pl_base + (uint32_t)zd], n);
+    }
+    if (bad) atomicOr(&d.ctl[r].status, ST_CELL_OOB);
+}
+
+__global__ __launch_bounds__(256) void k_advect(DevState d, double h, int mode) {
+    __shared__ uint32_t s_rs[kRowsMax + 1], s_hist[64];
+    const uint32_t lt = xcd_remap(blockIdx.x, gridDim.x);
+    const int r = find_rep(d.rtile_base, d.n_rep, lt);
+    const RepDev& rep = d.reps[r];
+    const RepCtl* g = d.ctl + r;
+    RepCtl c;
+    sort_ctl(d, r, mode, c);
+    const uint32_t t = lt - rep.rt_base;
+    /* the recentring redo's counts of an earlier collision are read out:
+     * clear them for this one's redo (which runs after this kernel) */
+    if (threadIdx.x == 0) {
+        d.tcount_r[lt] = 0u;
+        if (t % (uint32_t)rep.tpp == 0u) d.pcount_r[rep.pl_base + t / (uint32_t)rep.tpp] = 0u;
+    }
+    advect_tile<256>(d, r, t, c, g->wlo[0], g->wn[0], g->wlo[1], g->wn[1], h, d.tcount[d.cur], d.pcount[d.cur],
+                     d.outl, &d.ctl[r].n_out, s_rs, s_hist);
+}
+
+/* After the MD kernel: the monomers' cells of this collision
+ * (MPC::shiftParticles MPC.cpp:306-310 + sort MPC.cpp:221-235: shifted,
+ * never wrapped), and for a replica whose MD recentred it
+ * (MPC::check_bounds MPC.cpp:160-190) the advect pass again, with the
+ * recentring, into the redo counts and outlier list; its source window
+ * moves by -COM and widens by the COM speed.  The grid strides over the
+ * replica's row tiles; with no recentring the kernel only finds the
+ * monomer cells. */
+__global__ __launch_bounds__(256) void k_rows_redo(DevState d, double h) {
+    __shared__ uint32_t s_rs[kRowsMax + 1], s_hist[64];
+    for (int r = blockIdx.x; r < d.n_rep; r += gridDim.x) {
+        const RepDev& rep = d.reps[r];
+        const double sh0 = d.ctl[r].shift[0], sh1 = d.ctl[r].shift[1], sh2 = d.ctl[r].shift[2];
+        bool oob = false;
+        for (uint32_t i = threadIdx.x; i < rep.n_mono; i += 256) {
+            const size_t g = 3 * (size_t)(rep.mono_base + i);
+            const uint32_t ci = cell_index(rep.box, d.mpos[g] + sh0, d.mpos[g + 1] + sh1, d.mpos[g + 2] + sh2);
+            oob |= ci >= rep.ncells;
+            d.mcell[rep.mono_base + i] = ci < rep.ncells ? (uint32_t)rep.cell_base + ci : ~0u;
+        }
+        if (oob) atomicOr(&d.ctl[r].status, ST_CELL_OOB);
+    }
+    for (int r = 0; r < d.n_rep; ++r) {
+        const RepCtl* g = d.ctl + r;
+        if (!g->redo) continue;
+        const RepDev& rep = d.reps[r];
+        RepCtl c;
+        sort_ctl(d, r, kSortNormal, c);
+        int lo[2], n[2];
+        const double vs = d.win_speed * sqrt(rep.kT);
+        for (int a = 1; a < 3; ++a)
+            row_window(c.shift[a] - g->shift_prev[a] - c.com[a], (vs + fabs(c.com_vel[a])) * h, rep.box.iL[a],
+                       lo[a - 1], n[a - 1]);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            d.ctl[r].wlo[0] = lo[0]; d.ctl[r].wn[0] = n[0];
+            d.ctl[r].wlo[1] = lo[1]; d.ctl[r].wn[1] = n[1];
+        }
+        const uint32_t nt = d.rtile_base[r + 1] - d.rtile_base[r];
+        for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x)
+            advect_tile<256>(d, r, t, c, lo[0], n[0], lo[1], n[1], h, d.tcount_r, d.pcount_r, d.outl_r,
+                             &d.ctl[r].n_out_redo, s_rs, s_hist);
+    }
+}
+
+/* Wave-level compaction into a row tile's selection list: the lanes that
+ * take a particle get consecutive list positions (one LDS atomic per wave)
+ * and a rank inside their cell (LDS atomic); positions from `cap` on are
+ * counted but not stored (the tile then reports an overflow). */
+__device__ __forceinline__ void rows_select(bool take, uint32_t idx, uint32_t lc, uint32_t* s_nsel, uint32_t* s_cnt,
+                                            uint32_t* sel, uint32_t* code, uint32_t cap) {
+    const uint64_t m = __ballot(take);
+    if (m == 0ull) return;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t wb = 0u;
+    if ((int)__lane_id() == leader) wb = atomicAdd(s_nsel, (uint32_t)__popcll(m));
+    wb = __shfl(wb, leader);
+    if (take) {
+        const uint32_t pos = wb + (uint32_t)__popcll(m & lanemask_lt());
+        const uint32_t rk = atomicAdd(&s_cnt[lc], 1u);
+        if (pos < cap) {
+            sel[pos] = idx;
+            code[pos] = (lc << 16) | (rk < 0xFFFFu ? rk : 0xFFFFu);
+        }
+    }
+}
+
+/* Row-window collide: one workgroup of NT threads per row tile of the new
+ * cells (at most TCM cells, CAP particles).  The arithmetic of every
+ * particle and cell is k_collide's, expression for expression, with the
+ * cell sums as 64-bit fixed point (so their order does not matter): the
+ * two paths give the same bits.  Phases:
+ *   (a) first slot = arrivals of the replica's earlier z-planes and of
+ *       this plane's earlier tiles; the tile's monomers (as k_collide)
+ *   (b) the source window: rows y0 - (wylo + wyn - 1) .. y0 + ny - 1 - wylo
+ *       of planes z - (wzlo + wzn - 1) .. z - wzlo of the current buffer,
+ *       one wave per row, 4 x 64 pcell words in flight per lane; particles
+ *       whose new cell is the tile's enter the selection list with their
+ *       rank in the cell; then the replica's outliers
+ *   (c) exclusive scan of the cell counts (the slots), the check that the
+ *       tile holds exactly the arrivals k_advect counted
+ *   (d) per particle (list order): load, stream (as k_advect), position
+ *       (and id) to the new buffer, velocity and image index staged in LDS
+ *       at the slot; per cell: Gamma variate and rotation
+ *   (e) per cell (one thread): momentum sums over its slots + monomers ->
+ *       CMV, relative kinetic energy -> MBS scale (with angular momentum:
+ *       CMP, inertia tensor, C, omega)
+ *   (f) per slot: v <- CMV + s R (v - CMV) (+ omega x r), written coalesced;
+ *       cell offsets of the new buffer; monomer velocities; the arrival
+ *       counters of the next collision cleared. */
+template <int NT, int CAP, int TCM, bool AMC>
+__global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
+    static_assert(TCM <= NT && CAP % NT == 0, "tile shape");
+    constexpr int MCAP = 2 * TCM + 64;
+    constexpr int KW = 128;               /* source rows held at once */
+    constexpr int PER = CAP / NT;
+    constexpr int NW = NT / 64;
+    constexpr int U = 4;
+    constexpr int TA = AMC ? TCM : 1;
+    constexpr size_t kSelBytes = 8ull * CAP;
+    constexpr size_t kStageBytes = (AMC ? 26ull : 14ull) * CAP;
+    __shared__ __align__(16) unsigned char s_buf[kSelBytes > kStageBytes ? kSelBytes : kStageBytes];
+    __shared__ uint32_t s_cnt[TCM], s_off[TCM + 1], s_nm[TCM];
+    __shared__ unsigned long long s_mq[4][TCM];  /* monomers: sum m v (3), sum m */
+    __shared__ unsigned long long s_e[TCM];
+    __shared__ double s_cmv[TCM][3], s_g[TCM];
+    __shared__ float s_R[TCM][9];
+    __shared__ uint32_t s_ml[MCAP];
+    __shared__ uint32_t s_ws[KW], s_wl[KW];
+    __shared__ uint32_t s_wsum[NW];
+    __shared__ uint32_t s_nsel, s_mn, s_bad, s_pre;
+    __shared__ unsigned long long s_am[12][TA];
+    __shared__ double s_cmp[TA][3], s_w[TA][3];
+    uint32_t* sel = reinterpret_cast<uint32_t*>(s_buf);
+    uint32_t* code = sel + CAP;
+    float* s_vx = reinterpret_cast<float*>(s_buf);
+    float* s_vy = s_vx + CAP;
+    float* s_vz = s_vy + CAP;
+    float* s_lx = s_vz + CAP;  /* AMC only */
+    float* s_ly = s_lx + CAP;
+    float* s_lz = s_ly + CAP;
+    int8_t* s_cy = reinterpret_cast<int8_t*>(AMC ? s_lz + CAP : s_vz + CAP);
+    uint8_t* s_lcs = reinterpret_cast<uint8_t*>(s_cy + CAP);
+
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t lt = xcd_remap(blockIdx.x, gridDim.x);
+    const int r = find_rep(d.rtile_base, d.n_rep, lt);
+    const RepDev& rep = d.reps[r];
+    RepCtl* ctl = d.ctl + r;
+    RepCtl cc;
+    sort_ctl(d, r, kSortNormal, cc);
+    Box b = rep.box;
+    b.delrx = cc.delrx;
+    const uint32_t t = lt - rep.rt_base;
+    const RowTile T = row_tile(rep, t);
+    const int Lx = b.iL[0], Ly = b.iL[1], Lz = b.iL[2];
+    const uint32_t nc = T.nc;
+    const uint32_t gc0 = (uint32_t)rep.cell_base + T.c_lo;
+    const int cur = d.cur, alt = cur ^ 1;
+    const bool rd = ctl->redo != 0;
+    const uint32_t* tcnt = rd ? d.tcount_r : d.tcount[cur];
+    const uint32_t* pcnt = rd ? d.pcount_r : d.pcount[cur];
+    const uint32_t* outl = rd ? d.outl_r : d.outl;
+    const uint32_t n_out = min(rd ? ctl->n_out_redo : ctl->n_out, rep.n_fluid);
+    const uint32_t pr0 = (uint32_t)rep.part_base, pr1 = pr0 + rep.n_fluid;
+    const int wylo = ctl->wlo[0], wyn = ctl->wn[0], wzlo = ctl->wlo[1], wzn = ctl->wn[1];
+    const uint32_t* off = d.offs[cur];
+    const float *X = d.x[cur], *Y = d.y[cur], *Z = d.z[cur];
+    const float *VX = d.vx[cur], *VY = d.vy[cur], *VZ = d.vz[cur];
+    float *OX = d.x[alt], *OY = d.y[alt], *OZ = d.z[alt];
+    float *OVX = d.vx[alt], *OVY = d.vy[alt], *OVZ = d.vz[alt];
+    const uint32_t* ids = d.ids[cur];
+    uint32_t* oids = d.ids[alt];
+    const double shear = rep.box.shear, LyD = rep.box.L[1];
+    const bool amc = AMC && rep.amc;
+
+    /* (a) */
+    for (uint32_t i = tid; i < nc; i += NT) {
+        s_cnt[i] = 0u;
+        s_nm[i] = 0u;
+        s_mq[0][i] = s_mq[1][i] = s_mq[2][i] = s_mq[3][i] = 0ull;
+        s_e[i] = 0ull;
+    }
+    if (amc)
+        for (uint32_t i = tid; i < 12u * nc; i += NT) s_am[i / nc][i % nc] = 0ull;
+    if (tid == 0) { s_nsel = 0u; s_mn = 0u; s_bad = 0u; }
+    {
+        uint32_t pre = 0u;
+        for (int q = tid; q < T.z; q += NT) pre += pcnt[rep.pl_base + (uint32_t)q];
+        for (int q = tid; q < T.ty; q += NT) pre += tcnt[rep.rt_base + (uint32_t)(T.z * rep.tpp + q)];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o);
+        if (lane == 0) s_wsum[wv] = pre;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t p = 0u;
+        for (int q = 0; q < NW; ++q) p += s_wsum[q];
+        s_pre = p;
+    }
+    for (uint32_t j = tid; j < rep.n_mono; j += NT) {
+        const uint32_t rel = d.mcell[rep.mono_base + j] - gc0;
+        if (rel < nc) {
+            const uint32_t slot = atomicAdd(&s_mn, 1u);
+            if (slot < (uint32_t)MCAP) { s_ml[slot] = (rel << 16) | j; atomicAdd(&s_nm[rel], 1u); }
+        }
+    }
+
+    /* (b) */
+    int nwy = T.ny + wyn - 1, ys0 = T.y0 - (wylo + wyn - 1);
+    if (nwy >= Ly) { nwy = Ly; ys0 = 0; }
+    const int nwz = wzn, zs0 = T.z - (wzlo + wzn - 1);
+    const int nrows = nwy * nwz;
+    for (int k0 = 0; k0 < nrows; k0 += KW) {
+        const int kn = min(KW, nrows - k0);
+        for (int k = tid; k < kn; k += NT) {
+            const int q = k0 + k, jz = q / nwy, jy = q - jz * nwy;
+            const int oy = pmod(ys0 + jy, Ly), oz = pmod(zs0 + jz, Lz);
+            const uint32_t rc = (uint32_t)rep.cell_base + (uint32_t)Lx * (uint32_t)(oy + Ly * oz);
+            /* clamped as in advect_tile */
+            const uint32_t a = min(max(off[rc], pr0), pr1);
+            s_ws[k] = a;
+            s_wl[k] = min(max(off[rc + (uint32_t)Lx], a), pr1) - a;
+        }
+        __syncthreads();
+        for (int k = wv; k < kn; k += NW) {
+            const uint32_t a = s_ws[k], e = a + s_wl[k];
+            for (uint32_t p0 = a; p0 < e; p0 += 64u * U) {
+                uint32_t pc[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t p = p0 + (uint32_t)(u * 64 + lane);
+                    pc[u] = p < e ? d.pcell[p] : ~0u;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t lc = pc[u] - gc0;
+                    rows_select(lc < nc, p0 + (uint32_t)(u * 64 + lane), lc, &s_nsel, s_cnt, sel, code, CAP);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (uint32_t e0 = 0; e0 < n_out; e0 += NT) {
+        const uint32_t e = e0 + tid;
+        uint32_t idx = 0u, lc = ~0u;
+        if (e < n_out) {
+            idx = outl[rep.part_base + e];
+            const uint32_t pc = d.pcell[idx];
+            if (pc & kOutBit) lc = (pc & ~kOutBit) - gc0;
+        }
+        rows_select(lc < nc, idx, lc, &s_nsel, s_cnt, sel, code, CAP);
+    }
+    __syncthreads();
+
+    /* (c) */
+    const uint32_t nsel = s_nsel;
+    {
+        const uint32_t v = (uint32_t)tid < nc ? s_cnt[tid] : 0u;
+        uint32_t inc = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(inc, o);
+            if (lane >= o) inc += u;
+        }
+        if (lane == 63) s_wsum[wv] = inc;
+        __syncthreads();
+        uint32_t wpre = 0u;
+        for (int q = 0; q < wv; ++q) wpre += s_wsum[q];
+        if ((uint32_t)tid < nc) s_off[tid] = wpre + inc - v;
+        if ((uint32_t)tid == nc - 1u) s_off[nc] = wpre + inc;
+    }
+    if (tid == 0) {
+        if (nsel != tcnt[lt]) atomicOr(&ctl->status, ST_SORT_MISMATCH);
+        if (nsel > (uint32_t)CAP) atomicOr(&ctl->status, ST_TILE_OVERFLOW);
+        if (s_mn > (uint32_t)MCAP) atomicOr(&ctl->status, ST_TILE_MONO);
+    }
+    const uint32_t base = (uint32_t)rep.part_base + s_pre;
+    /* uniform exits, status set: the tile cannot hold its particles, or the
+     * arrival counts would place them beyond the replica */
+    if (nsel > (uint32_t)CAP) return;
+    if (base < pr0 || (uint64_t)base + nsel > (uint64_t)pr1) {
+        if (tid == 0) atomicOr(&ctl->status, ST_SORT_MISMATCH);
+        return;
+    }
+    const uint32_t nmt = s_mn < (uint32_t)MCAP ? s_mn : (uint32_t)MCAP;
+    const uint32_t k = ctl->k_cur;
+    const bool inj = ctl->inj_active != 0;
+
+    /* (d) the list into registers, then the staging may overwrite it */
+    uint32_t my_i[PER], my_c[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const uint32_t e = (uint32_t)(tid + u * NT);
+        my_i[u] = e < nsel ? sel[e] : ~0u;
+        my_c[u] = e < nsel ? code[e] : 0u;
+    }
+    __syncthreads();
+    float fx[PER], fy[PER], fz[PER], fvx[PER], fvy[PER], fvz[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        if (my_i[u] == ~0u) continue;
+        const uint32_t i = my_i[u];
+        fx[u] = X[i]; fy[u] = Y[i]; fz[u] = Z[i];
+        fvx[u] = VX[i]; fvy[u] = VY[i]; fvz[u] = VZ[i];
+    }
+    for (uint32_t ci = tid; ci < nc; ci += NT) {
+        const uint32_t cell = T.c_lo + ci;
+        const uint32_t n = s_cnt[ci] + s_nm[ci];
+        double g = 0.0;
+        if (n >= 2)
+            g = inj ? d.inj_gamma[gc0 + ci] : gamma_mt(rep.seed, rep.replica, cell, k, 1.5 * (n - 1), rep.kT);
+        s_g[ci] = g;
+        double u1, u2;
+        if (inj) {
+            u1 = d.inj_axis[2 * (size_t)(gc0 + ci)];
+            u2 = d.inj_axis[2 * (size_t)(gc0 + ci) + 1];
+        } else {
+            uint32_t w[4];
+            draw4(rep.seed, rep.replica, TAG_AXIS, cell, k, 0, w);
+            u1 = u01(w[0]);
+            u2 = u01(w[1]);
+        }
+        const double phi = 2. * kPi * u1;
+        const double th = 2. * (u2 - 0.5);
+        const double a0 = sqrt(1 - th * th) * cos(phi);
+        const double a1 = sqrt(1 - th * th) * sin(phi);
+        const double a2 = th;
+        const double cs = rep.c, ss = rep.s;
+        float* R = s_R[ci];
+        R[0] = (float)(a0 * a0 + (1 - a0 * a0) * cs);
+        R[1] = (float)(a0 * a1 * (1 - cs) - a2 * ss);
+        R[2] = (float)(a0 * a2 * (1 - cs) + a1 * ss);
+        R[3] = (float)(a0 * a1 * (1 - cs) + a2 * ss);
+        R[4] = (float)(a1 * a1 + (1 - a1 * a1) * cs);
+        R[5] = (float)(a1 * a2 * (1 - cs) - a0 * ss);
+        R[6] = (float)(a0 * a2 * (1 - cs) - a1 * ss);
+        R[7] = (float)(a1 * a2 * (1 - cs) + a0 * ss);
+        R[8] = (float)(a2 * a2 + (1 - a2 * a2) * cs);
+    }
+    bool bad = false;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        if (my_i[u] == ~0u) continue;
+        const uint32_t lc = my_c[u] >> 16;
+        const uint32_t slot = s_off[lc] + (my_c[u] & 0xFFFFu);
+        double x = fx[u], y = fy[u], z = fz[u], vx = fvx[u], vy = fvy[u], vz = fvz[u];
+        recenter_stream(b, h, cc, x, y, z, vx, vy, vz);
+        OX[base + slot] = (float)x; OY[base + slot] = (float)y; OZ[base + slot] = (float)z;
+        if (ids) oids[base + slot] = ids[my_i[u]];
+        double px = x + cc.shift[0], py = y + cc.shift[1], pz = z + cc.shift[2], vimg = vx;
+        const int cy = (int)le_wrap(b, px, py, pz, vimg);
+        bad |= cell_index(b, px, py, pz) != T.c_lo + lc;
+        s_vx[slot] = (float)vx; s_vy[slot] = (float)vy; s_vz[slot] = (float)vz;
+        s_cy[slot] = (int8_t)cy;
+        s_lcs[slot] = (uint8_t)lc;
+        if (AMC) {  /* coordinates inside the cell, as cell_index truncates */
+            const double qx = px + b.half[0], qy = py + b.half[1], qz = pz + b.half[2];
+            s_lx[slot] = (float)(qx - (double)(int)qx);
+            s_ly[slot] = (float)(qy - (double)(int)qy);
+            s_lz[slot] = (float)(qz - (double)(int)qz);
+        }
+    }
+    if (bad) s_bad = 1u;
+    /* a monomer's coordinates inside its cell (shifted, never wrapped) */
+    auto mono_local = [&](uint32_t mi, double& lx, double& ly, double& lz) {
+        const double qx = d.mpos[3 * mi] + cc.shift[0] + b.half[0];
+        const double qy = d.mpos[3 * mi + 1] + cc.shift[1] + b.half[1];
+        const double qz = d.mpos[3 * mi + 2] + cc.shift[2] + b.half[2];
+        lx = qx - (double)(int)qx; ly = qy - (double)(int)qy; lz = qz - (double)(int)qz;
+    };
+    for (uint32_t j = tid; j < nmt; j += NT) {
+        const uint32_t e = s_ml[j];
+        const uint32_t lc = e >> 16;
+        const uint32_t mi = rep.mono_base + (e & 0xFFFFu);
+        const double mm = d.mmass[mi];
+        atomicAdd(&s_mq[0][lc], to_fix(d.mvel[3 * mi] * mm));
+        atomicAdd(&s_mq[1][lc], to_fix(d.mvel[3 * mi + 1] * mm));
+        atomicAdd(&s_mq[2][lc], to_fix(d.mvel[3 * mi + 2] * mm));
+        atomicAdd(&s_mq[3][lc], to_fix(mm));
+        if (amc) {
+            double lx, ly, lz;
+            mono_local(mi, lx, ly, lz);
+            atomicAdd(&s_am[0][lc], to_fix(mm * lx));
+            atomicAdd(&s_am[1][lc], to_fix(mm * ly));
+            atomicAdd(&s_am[2][lc], to_fix(mm * lz));
+        }
+    }
+    __syncthreads();
+    if (tid == 0 && s_bad) atomicOr(&ctl->status, ST_SORT_MISMATCH);
+
+    /* (e) */
+    const bool has_m = nmt > 0u;
+    /* MBS scale sqrt(Gamma / E_rel) for n >= 2 (MPC.cpp:287-289); with the
+     * angular-momentum flag, omega = s I^-1 C for n > 2 (MPC.cpp:260, 373) */
+    auto finalize = [&](uint32_t ci) {
+        const uint32_t n = s_cnt[ci] + s_nm[ci];
+        s_g[ci] = n >= 2 ? sqrt(s_g[ci] / (from_fix(s_e[ci]) * 0.5)) : -1.0;
+        if (amc) {
+            double w[3] = {0.0, 0.0, 0.0};
+            if (n > 2) {
+                const double A[3][3] = {{from_fix(s_am[3][ci]), from_fix(s_am[6][ci]), from_fix(s_am[7][ci])},
+                                        {from_fix(s_am[6][ci]), from_fix(s_am[4][ci]), from_fix(s_am[8][ci])},
+                                        {from_fix(s_am[7][ci]), from_fix(s_am[8][ci]), from_fix(s_am[5][ci])}};
+                const double c3[3] = {from_fix(s_am[9][ci]), from_fix(s_am[10][ci]), from_fix(s_am[11][ci])};
+                llt_solve3(A, c3, w);
+                const double sc = s_g[ci] >= 0.0 ? s_g[ci] : 1.0;
+                const bool ok = isfinite(w[0]) && isfinite(w[1]) && isfinite(w[2]);
+                for (int q = 0; q < 3; ++q) w[q] = ok ? w[q] * sc : 0.0;
+            }
+            s_w[ci][0] = w[0]; s_w[ci][1] = w[1]; s_w[ci][2] = w[2];
+        }
+    };
+    /* the angular-momentum sums of one member (MPC::calculateAngular
+     * MPC.cpp:344-372): I about CMP and C = m r x (u - R u) */
+    auto amc_terms = [&](uint32_t lc, double m, double lx, double ly, double lz, double ux, double uy, double uz,
+                         unsigned long long* acc) {
+        const double rx = lx - s_cmp[lc][0], ry = ly - s_cmp[lc][1], rz = lz - s_cmp[lc][2];
+        const float* R = s_R[lc];
+        const double qx = ux - (((double)R[0] * ux + (double)R[1] * uy) + (double)R[2] * uz);
+        const double qy = uy - (((double)R[3] * ux + (double)R[4] * uy) + (double)R[5] * uz);
+        const double qz = uz - (((double)R[6] * ux + (double)R[7] * uy) + (double)R[8] * uz);
+        acc[0] = to_fix(m * (ry * ry + rz * rz));
+        acc[1] = to_fix(m * (rx * rx + rz * rz));
+        acc[2] = to_fix(m * (ry * ry + rx * rx));
+        acc[3] = to_fix(-m * rx * ry);
+        acc[4] = to_fix(-m * rx * rz);
+        acc[5] = to_fix(-m * ry * rz);
+        acc[6] = to_fix(m * (ry * qz - rz * qy));
+        acc[7] = to_fix(m * (rz * qx - rx * qz));
+        acc[8] = to_fix(m * (rx * qy - ry * qx));
+    };
+    for (uint32_t ci = tid; ci < nc; ci += NT) {
+        const uint32_t a = s_off[ci], n = s_cnt[ci];
+        /* MPC::calculateCMV (MPC.cpp:314-322) */
+        unsigned long long qx = s_mq[0][ci], qy = s_mq[1][ci], qz = s_mq[2][ci];
+        unsigned long long l0 = 0ull, l1 = 0ull, l2 = 0ull;
+        if (amc) { l0 = s_am[0][ci]; l1 = s_am[1][ci]; l2 = s_am[2][ci]; }
+        for (uint32_t q = a; q < a + n; ++q) {
+            qx += to_fix((double)s_vx[q] - (double)s_cy[q] * shear * LyD);
+            qy += to_fix((double)s_vy[q]);
+            qz += to_fix((double)s_vz[q]);
+            if (amc) {
+                l0 += to_fix(1.0 * (double)s_lx[q]);
+                l1 += to_fix(1.0 * (double)s_ly[q]);
+                l2 += to_fix(1.0 * (double)s_lz[q]);
+            }
+        }
+        const double M = (double)n + from_fix(s_mq[3][ci]);
+        double c0 = from_fix(qx), c1 = from_fix(qy), c2 = from_fix(qz);
+        if (M > 0) { c0 /= M; c1 /= M; c2 /= M; }
+        s_cmv[ci][0] = c0; s_cmv[ci][1] = c1; s_cmv[ci][2] = c2;
+        if (amc) {  /* MPC::calculateCMP MPC.cpp:324-333, inside the cell */
+            const double inv = M > 0 ? 1.0 / M : 0.0;
+            s_cmp[ci][0] = from_fix(l0) * inv;
+            s_cmp[ci][1] = from_fix(l1) * inv;
+            s_cmp[ci][2] = from_fix(l2) * inv;
+        }
+        /* MPC::thermostat relative kinetic energy (MPC.cpp:277-279) */
+        unsigned long long e = 0ull;
+        unsigned long long am[9] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
+        for (uint32_t q = a; q < a + n; ++q) {
+            const double vx = (double)s_vx[q] - (double)s_cy[q] * shear * LyD;
+            const double dx = vx - c0, dy = (double)s_vy[q] - c1, dz = (double)s_vz[q] - c2;
+            e += to_fix((dx * dx + dy * dy) + dz * dz);
+            if (amc) {
+                unsigned long long t9[9];
+                amc_terms(ci, 1.0, s_lx[q], s_ly[q], s_lz[q], dx, dy, dz, t9);
+#pragma unroll
+                for (int j = 0; j < 9; ++j) am[j] += t9[j];
+            }
+        }
+        s_e[ci] = e;
+        if (amc)
+#pragma unroll
+            for (int j = 0; j < 9; ++j) s_am[3 + j][ci] = am[j];
+        if (!has_m) finalize(ci);
+    }
+    if (has_m) {
+        __syncthreads();
+        for (uint32_t j = tid; j < nmt; j += NT) {
+            const uint32_t e = s_ml[j];
+            const uint32_t lc = e >> 16;
+            const uint32_t mi = rep.mono_base + (e & 0xFFFFu);
+            const double dx = d.mvel[3 * mi] - s_cmv[lc][0];
+            const double dy = d.mvel[3 * mi + 1] - s_cmv[lc][1];
+            const double dz = d.mvel[3 * mi + 2] - s_cmv[lc][2];
+            atomicAdd(&s_e[lc], to_fix(d.mmass[mi] * ((dx * dx + dy * dy) + dz * dz)));
+            if (amc) {
+                double lx, ly, lz;
+                mono_local(mi, lx, ly, lz);
+                unsigned long long t9[9];
+                amc_terms(lc, d.mmass[mi], lx, ly, lz, dx, dy, dz, t9);
+                for (int q = 0; q < 9; ++q) atomicAdd(&s_am[3 + q][lc], t9[q]);
+            }
+        }
+        __syncthreads();
+        for (uint32_t ci = tid; ci < nc; ci += NT) finalize(ci);
+    }
+    __syncthreads();
+
+    /* (f) v <- s v + (1-s) CMV, then v <- CMV + R (v - CMV) (MPC.cpp:290-292,
+     * 266-267); the image offset is undone as the unshift wrap does */
+    auto apply = [&](uint32_t lc, double& vx, double& vy, double& vz) {
+        const double c0 = s_cmv[lc][0], c1 = s_cmv[lc][1], c2 = s_cmv[lc][2];
+        const double sc = s_g[lc];
+        if (sc >= 0.0) {
+            vx = sc * vx + (1 - sc) * c0;
+            vy = sc * vy + (1 - sc) * c1;
+            vz = sc * vz + (1 - sc) * c2;
+        }
+        const float* R = s_R[lc];
+        const double d0 = vx - c0, d1 = vy - c1, d2 = vz - c2;
+        vx = c0 + (((double)R[0] * d0 + (double)R[1] * d1) + (double)R[2] * d2);
+        vy = c1 + (((double)R[3] * d0 + (double)R[4] * d1) + (double)R[5] * d2);
+        vz = c2 + (((double)R[6] * d0 + (double)R[7] * d1) + (double)R[8] * d2);
+    };
+    /* v += omega x (x - CMP) (MPC.cpp:268) */
+    auto add_w = [&](uint32_t lc, double lx, double ly, double lz, double& vx, double& vy, double& vz) {
+        const double rx = lx - s_cmp[lc][0], ry = ly - s_cmp[lc][1], rz = lz - s_cmp[lc][2];
+        const double w0 = s_w[lc][0], w1 = s_w[lc][1], w2 = s_w[lc][2];
+        vx += w1 * rz - w2 * ry;
+        vy += w2 * rx - w0 * rz;
+        vz += w0 * ry - w1 * rx;
+    };
+    for (uint32_t q = tid; q < nsel; q += NT) {
+        const uint32_t lc = s_lcs[q];
+        const double offv = (double)s_cy[q] * shear * LyD;
+        double vx = (double)s_vx[q] - offv, vy = s_vy[q], vz = s_vz[q];
+        apply(lc, vx, vy, vz);
+        if (amc) add_w(lc, s_lx[q], s_ly[q], s_lz[q], vx, vy, vz);
+        vx += offv;
+        OVX[base + q] = (float)vx; OVY[base + q] = (float)vy; OVZ[base + q] = (float)vz;
+    }
+    uint32_t* offn = d.offs[alt];
+    for (uint32_t ci = tid; ci < nc; ci += NT) offn[gc0 + ci] = base + s_off[ci];
+    if (tid == 0) {
+        if (T.c_lo + nc == rep.ncells) offn[rep.cell_base + rep.ncells] = base + nsel;
+        d.tcount[alt][lt] = 0u;
+        if (T.ty == 0) d.pcount[alt][rep.pl_base + (uint32_t)T.z] = 0u;
+    }
+    for (uint32_t j = tid; j < nmt; j += NT) {
+        const uint32_t e = s_ml[j];
+        const uint32_t lc = e >> 16;
+        const uint32_t mi = rep.mono_base + (e & 0xFFFFu);
+        double vx = d.mvel[3 * mi], vy = d.mvel[3 * mi + 1], vz = d.mvel[3 * mi + 2];
+        apply(lc, vx, vy, vz);
+        if (amc) {
+            double lx, ly, lz;
+            mono_local(mi, lx, ly, lz);
+            add_w(lc, lx, ly, lz, vx, vy, vz);
+        }
+        d.mvel[3 * mi] = vx; d.mvel[3 * mi + 1] = vy; d.mvel[3 * mi + 2] = vz;
+    }
+}
+
 /* ======================================================== fluid init */
 
 /* MPC::initialize (MPC.cpp:34-55): uniform positions and U(-1/2,1/2)
@@ -1502,7 +2223,7 @@ __global__ __launch_bounds__(256) void k_temperature(DevState d, int replica) {
     __syncthreads();
     double ek = 0.0, filled = 0.0;
     for (uint32_t ci = threadIdx.x; ci < nc; ci += 256) {
-        const uint32_t a = d.offsets[gc0 + ci], e = d.offsets[gc0 + ci + 1];
+        const uint32_t a = d.offs[d.cur][gc0 + ci], e = d.offs[d.cur][gc0 + ci + 1];
         double s0 = 0.0, s1 = 0.0, s2 = 0.0, M = 0.0;
         for (uint32_t p = a; p < e; ++p) { s0 += VX[p]; s1 += VY[p]; s2 += VZ[p]; M += 1.0; }
         s0 += from_fix(s_q[0][ci]); s1 += from_fix(s_q[1][ci]); s2 += from_fix(s_q[2][ci]);
@@ -1938,8 +2659,9 @@ __global__ __launch_bounds__(256) void k_fluid_observe(DevState d, ObsArgs a) {
 
 /* ========================================================= launchers */
 
-hipError_t launch_prologue(const DevState& d, hipStream_t st, int do_collision, int n_delrx, double delrx_dt) {
-    hipLaunchKernelGGL(k_prologue, dim3(d.n_rep), dim3(64), 0, st, d, do_collision, n_delrx, delrx_dt);
+hipError_t launch_prologue(const DevState& d, hipStream_t st, int do_collision, int n_delrx, double delrx_dt,
+                           double h) {
+    hipLaunchKernelGGL(k_prologue, dim3(d.n_rep), dim3(64), 0, st, d, do_collision, n_delrx, delrx_dt, h);
     return hipGetLastError();
 }
 
@@ -2027,6 +2749,35 @@ hipError_t launch_collide(const DevState& d, hipStream_t st, double h) {
     return hipGetLastError();
 }
 
+hipError_t launch_advect(const DevState& d, hipStream_t st, double h, int mode) {
+    if (!d.rt_cap || !d.n_rtiles) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_advect, dim3(d.n_rtiles), dim3(256), 0, st, d, h, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_rows_redo(const DevState& d, hipStream_t st, double h) {
+    hipLaunchKernelGGL(k_rows_redo, dim3(256), dim3(256), 0, st, d, h);
+    return hipGetLastError();
+}
+
+template <int NT, int CAP, int TCM>
+static void launch_collide_rows_t(const DevState& d, hipStream_t st, double h) {
+    if (d.any_amc)
+        hipLaunchKernelGGL((k_collide_rows<NT, CAP, TCM, true>), dim3(d.n_rtiles), dim3(NT), 0, st, d, h);
+    else
+        hipLaunchKernelGGL((k_collide_rows<NT, CAP, TCM, false>), dim3(d.n_rtiles), dim3(NT), 0, st, d, h);
+}
+
+hipError_t launch_collide_rows(const DevState& d, hipStream_t st, double h) {
+    if (!d.n_rtiles) return hipErrorInvalidValue;
+    if (d.rt_cap == 1024 && d.rt_tcm == 128) launch_collide_rows_t<256, 1024, 128>(d, st, h);
+    else if (d.rt_cap == 1024 && d.rt_tcm == 256) launch_collide_rows_t<256, 1024, 256>(d, st, h);
+    else if (d.rt_cap == 2048 && d.rt_tcm == 128) launch_collide_rows_t<512, 2048, 128>(d, st, h);
+    else if (d.rt_cap == 2048 && d.rt_tcm == 256) launch_collide_rows_t<512, 2048, 256>(d, st, h);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 hipError_t launch_fluid_init(const DevState& d, hipStream_t st) {
     if (d.n_pblk == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fluid_init, dim3(d.n_pblk), dim3(256), 0, st, d, 0);
@@ -2043,7 +2794,7 @@ __global__ __launch_bounds__(256) void k_cell_velocity(DevState d, int replica, 
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     const uint32_t g = (uint32_t)d.reps[replica].cell_base + c0 + i;
-    const uint32_t a = d.offsets[g], e = d.offsets[g + 1];
+    const uint32_t a = d.offs[d.cur][g], e = d.offs[d.cur][g + 1];
     const int cur = d.cur;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     for (uint32_t p = a; p < e; ++p) { s0 += d.vx[cur][p]; s1 += d.vy[cur][p]; s2 += d.vz[cur][p]; }

@@ -336,6 +336,22 @@ class MPCSystem:
         self._chk(self._lib.spmd_get_tuning(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    PATH_AUTO, PATH_SORT, PATH_ROWS = 0, 1, 2
+
+    def set_collision_path(self, path: int):
+        """PATH_AUTO (row tiles once the buffer is in cell order) or PATH_SORT."""
+        self._chk(self._lib.spmd_set_collision_path(self.h, path))
+
+    def set_row_window(self, speed: float):
+        """Speed bound (units of sqrt(kT)) of the row-tile source windows."""
+        self._chk(self._lib.spmd_set_row_window(self.h, float(speed)))
+
+    def collision_path(self) -> int:
+        """The path the next collision takes: PATH_SORT or PATH_ROWS."""
+        a = C.c_int()
+        self._chk(self._lib.spmd_get_collision_path(self.h, C.byref(a)))
+        return a.value
+
     def profile(self, on: bool = True):
         self._chk(self._lib.spmd_profile_enable(self.h, int(on)))
 
