@@ -502,6 +502,8 @@ def main():
     ap.add_argument("--cpu-intervals", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (ppt 4, 128-cell tiles)")
+    ap.add_argument("--sort-path", action="store_true",
+                    help="time the sort path (bin, scan, permutation, collide) instead of the row-tile path")
     ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure throughput section")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-replicas section")
     ap.add_argument("--train-stars", type=int, default=4,
@@ -579,6 +581,8 @@ def main():
     w = WORKLOAD
     S, h = w["step_update"], w["mpc_dt"]
     g = make_system(w)
+    if a.sort_path:
+        g.set_collision_path(g.PATH_SORT)
     n_f = g.num_fluid()
     star0 = g.get_monomers()
     t = 0
@@ -607,10 +611,16 @@ def main():
     g.run(t, a.profile_steps * S, h)
     g.sync()
     t += a.profile_steps * S
-    prof = {k: g.profile_read(k)[0] for k in ("md", "bin", "scan", "perm", "collide")}
+    prof = {k: g.profile_read(k)[0] for k in ("md", "prologue", "bin", "scan", "perm", "advect", "redo", "collide")}
     g.profile(False)
-    pipe_ms = prof["bin"] + prof["scan"] + prof["perm"] + prof["collide"]
-    traffic = measured_traffic()
+    rows = g.collision_path() == g.PATH_ROWS
+    if rows:  # row-tile path: advect, the redo kernel (monomer cells), row-window collide
+        pipe_ms = prof["advect"] + prof["redo"] + prof["collide"]
+        pipe_name = "collision pipeline k_advect+k_rows_redo+k_collide_rows"
+    else:
+        pipe_ms = prof["bin"] + prof["scan"] + prof["perm"] + prof["collide"]
+        pipe_name = "collision pipeline k_bin+k_scan+k_perm+k_collide"
+    traffic = measured_traffic() if rows else None
     achieved = B_ALG * n_f / (pipe_ms * 1e-3) / 1e9
     T = g.temperature()
 
@@ -621,6 +631,7 @@ def main():
         "config": {"workload": "plain MPC-MD L=64 rho=10 (2,621,440 solvent), star f=20 15A+15B "
                                "(601 monomers), shear 0.05, h=0.1, dt=0.01, 10 MD substeps/collision",
                    "solvent_particles": n_f, "step": "one collision interval", "parallelism": "1 GPU",
+                   "collision_path": "row tiles" if rows else "sort",
                    "tuning": {"ppt": g.get_tuning()[0], "tile_cells": g.get_tuning()[1],
                               "sweep_ms": None if tune is None else tune["timings"]}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -631,7 +642,7 @@ def main():
                      "measured_frac": traffic / (pipe_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
                      "amplification": traffic / (B_ALG * n_f) if traffic else None,
                      "traffic_source": TRAFFIC_PROFILE if traffic else None,
-                     "kernel": "collision pipeline k_bin+k_scan+k_perm+k_collide",
+                     "kernel": pipe_name,
                      "alg_bytes_per_launch": B_ALG * n_f, "avg_ms": pipe_ms,
                      "per_kernel_ms": prof},
         "md_us_per_interval": prof["md"] * 1e3,

@@ -266,19 +266,16 @@ int spmd_set_tuning(spmd_sys* s, int ppt, int tile_cells);
 int spmd_get_tuning(const spmd_sys* s, int* ppt, int* tile_cells);
 /* Collision path (no reference counterpart; both give the same bits).
  * SPMD_PATH_AUTO: once a collision has put the solvent in cell order, the
- * row-tile path (advect + row-window collide: no sort pass) whenever every
- * replica's x-row fits a tile (L_x <= 256, L_x * N_c <= 1300); the first
- * collision after the fluid was set, and handles whose rows do not fit,
- * take the sort path.  SPMD_PATH_SORT: always the sort path (bin, scan,
+ * row-tile path (particles binned by row tile into buckets of indices, then
+ * one workgroup per tile gathers, sorts by cell in LDS and collides: no
+ * cell-level scan or permutation) whenever every replica's x-row fits a
+ * tile (L_x <= 256, L_x * N_c <= 1300); the first collision after the
+ * fluid was set, and handles whose rows do not fit, take the sort path.  SPMD_PATH_SORT: always the sort path (bin, scan,
  * permutation, collide).  spmd_get_collision_path tells which path the
  * next collision takes. */
 enum { SPMD_PATH_AUTO = 0, SPMD_PATH_SORT = 1, SPMD_PATH_ROWS = 2 };
 int spmd_set_collision_path(spmd_sys* s, int path);
-/* Speed bound of the row-tile path's source-row windows in units of
- * sqrt(kT/m) (default 5): the window of rows a tile reads covers particles
- * up to that speed per axis, faster ones come through an outlier list.
- * Results do not depend on it; smaller is fewer rows, more outliers. */
-int spmd_set_row_window(spmd_sys* s, double speed);
+
 int spmd_get_collision_path(const spmd_sys* s, int* next_path);
 
 /* ---- timing (device events on the handle's stream) ---- */

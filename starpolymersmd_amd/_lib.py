@@ -93,7 +93,6 @@ _SIGS = {
     "spmd_get_tuning": (C.c_int, [C.c_void_p, _P(C.c_int), _P(C.c_int)]),
     "spmd_set_collision_path": (C.c_int, [C.c_void_p, C.c_int]),
     "spmd_get_collision_path": (C.c_int, [C.c_void_p, _P(C.c_int)]),
-    "spmd_set_row_window": (C.c_int, [C.c_void_p, C.c_double]),
     "spmd_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "spmd_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, _d, _P(C.c_int64)]),
     "spmd_profile_reset": (C.c_int, [C.c_void_p]),

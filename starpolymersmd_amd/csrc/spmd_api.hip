@@ -40,10 +40,6 @@ const char* kProfNames[kProfKinds] = {"md", "bin", "scan", "perm", "collide", "p
  * deviations of a Poisson count above the mean. */
 constexpr double kRowTileMeanSmall = 640.0;
 constexpr double kRowTileMean = 1300.0;
-/* Speed bound (units of sqrt(kT / m), m = 1) of the row-tile path's
- * source-row windows: a particle whose row moves farther (per axis, about
- * 6e-7 of them at 5) is taken through the outlier list instead. */
-constexpr double kWinSpeedDefault = 5.0;
 
 /* Device scratch of spmd_velocity_profile_noalloc: the profile's fixed-point
  * bins [2][kObsBins][3], then kStarMomentsWords doubles for the star's
@@ -169,7 +165,6 @@ int upload_tables(spmd_sys* s) {
      * where the mean occupancy would pass kRowTileMean; the path is off
      * for the handle if a single row holds more */
     std::vector<uint32_t> rb(d.n_rep + 1, 0);
-    uint32_t planes = 0;
     double max_mean = 0.0;
     int max_tc = 0;
     bool ok = true;
@@ -184,30 +179,29 @@ int upload_tables(spmd_sys* s) {
         rp.nr = nr;
         rp.tpp = (Ly + nr - 1) / nr;
         rp.rt_base = rb[r];
-        rp.pl_base = planes;
         rb[r + 1] = rb[r] + (uint32_t)(rp.tpp * Lz);
-        planes += (uint32_t)Lz;
         max_mean = std::max(max_mean, nr * Lx * rho);
         max_tc = std::max(max_tc, nr * Lx);
     }
     d.rt_cap = !ok ? 0 : (max_mean <= kRowTileMeanSmall ? 1024 : 2048);
-    d.rt_tcm = max_tc <= 128 ? 128 : 256;
+    d.rt_tcm = max_tc <= 64 ? 64 : max_tc <= 128 ? 128 : 256;
     d.n_rtiles = rb[d.n_rep];
-    d.n_planes = planes;
     for (int b = 0; b < 2; ++b) {
-        dfree(d.tcount[b]);
-        dfree(d.pcount[b]);
-        if (int rc = dalloc(s, &d.tcount[b], d.n_rtiles)) return rc;
-        if (int rc = dalloc(s, &d.pcount[b], d.n_planes)) return rc;
-        HIPCHK(s, hipMemset(d.tcount[b], 0, 4ull * d.n_rtiles));
-        HIPCHK(s, hipMemset(d.pcount[b], 0, 4ull * d.n_planes));
+        dfree(d.bcount[b]);
+        if (int rc = dalloc(s, &d.bcount[b], d.n_rtiles)) return rc;
+        HIPCHK(s, hipMemset(d.bcount[b], 0, 4ull * d.n_rtiles));
     }
-    dfree(d.tcount_r);
-    dfree(d.pcount_r);
-    if (int rc = dalloc(s, &d.tcount_r, d.n_rtiles)) return rc;
-    if (int rc = dalloc(s, &d.pcount_r, d.n_planes)) return rc;
-    HIPCHK(s, hipMemset(d.tcount_r, 0, 4ull * d.n_rtiles));
-    HIPCHK(s, hipMemset(d.pcount_r, 0, 4ull * d.n_planes));
+    dfree(d.bcount_r);
+    dfree(d.tbase);
+    dfree(d.bucket);
+    dfree(d.bucket_r);
+    if (int rc = dalloc(s, &d.bcount_r, d.n_rtiles)) return rc;
+    if (int rc = dalloc(s, &d.tbase, d.n_rtiles)) return rc;
+    const size_t nb = d.rt_cap ? (size_t)d.n_rtiles * (size_t)d.rt_cap : 1;
+    if (int rc = dalloc(s, &d.bucket, nb)) return rc;
+    if (int rc = dalloc(s, &d.bucket_r, nb)) return rc;
+    HIPCHK(s, hipMemset(d.bcount_r, 0, 4ull * d.n_rtiles));
+    HIPCHK(s, hipMemset(d.tbase, 0, 4ull * d.n_rtiles));
     HIPCHK(s, hipMemcpy(d.rtile_base, rb.data(), rb.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(s, hipMemcpy(d.reps, s->reps.data(), s->reps.size() * sizeof(RepDev), hipMemcpyHostToDevice));
     s->tc_dirty = false;
@@ -224,10 +218,7 @@ bool use_rows(const spmd_sys* s) { return s->d.rt_cap != 0 && s->collided && s->
 int prep_rows(spmd_sys* s) {
     if (!s->tc_dirty) return SPMD_OK;
     const DevState& d = s->d;
-    for (int b = 0; b < 2; ++b) {
-        HIPCHK(s, hipMemsetAsync(d.tcount[b], 0, 4ull * d.n_rtiles, s->stream));
-        HIPCHK(s, hipMemsetAsync(d.pcount[b], 0, 4ull * d.n_planes, s->stream));
-    }
+    for (int b = 0; b < 2; ++b) HIPCHK(s, hipMemsetAsync(d.bcount[b], 0, 4ull * d.n_rtiles, s->stream));
     s->tc_dirty = false;
     return SPMD_OK;
 }
@@ -366,7 +357,7 @@ hipError_t enqueue_collision_ops(spmd_sys* s, int n_md, int epot_last, int n_del
         if ((e = hipEventRecord(s->ev_join, s->side)) != hipSuccess) return e;
     }
     prof_begin(s, 5, s->stream);
-    if ((e = launch_prologue(d, s->stream, 1, n_delrx, delrx_dt, h)) != hipSuccess) return e;
+    if ((e = launch_prologue(d, s->stream, 1, n_delrx, delrx_dt)) != hipSuccess) return e;
     prof_end(s, 5, s->stream);
     if (use_rows(s)) {
         /* row-tile path: advect (beside the MD, assuming no recentring),
@@ -375,12 +366,11 @@ hipError_t enqueue_collision_ops(spmd_sys* s, int n_md, int epot_last, int n_del
         prof_begin(s, 6, s->stream);
         if ((e = launch_advect(d, s->stream, h, d.m_tot ? kSortSpec : kSortNormal)) != hipSuccess) return e;
         prof_end(s, 6, s->stream);
-        if (d.m_tot > 0) {
+        if (d.m_tot > 0)
             if ((e = hipStreamWaitEvent(s->stream, s->ev_join, 0)) != hipSuccess) return e;
-            prof_begin(s, 7, s->stream);
-            if ((e = launch_rows_redo(d, s->stream, h)) != hipSuccess) return e;
-            prof_end(s, 7, s->stream);
-        }
+        prof_begin(s, 7, s->stream);
+        if ((e = launch_rows_redo(d, s->stream, h)) != hipSuccess) return e;
+        prof_end(s, 7, s->stream);
         prof_begin(s, 4, s->stream);
         if ((e = launch_collide_rows(d, s->stream, h)) != hipSuccess) return e;
         prof_end(s, 4, s->stream);
@@ -429,7 +419,7 @@ int enqueue_collision(spmd_sys* s, int n_md, int epot_last, int n_delrx, double 
 int flush_md(spmd_sys* s) {
     if (s->pending_md == 0 && s->pending_delrx == 0) return SPMD_OK;
     if (int rc = ensure_monomers(s)) return rc;
-    LAUNCH(s, 5, s->stream, launch_prologue(s->d, s->stream, 0, s->pending_delrx, s->pending_dt, 0.0));
+    LAUNCH(s, 5, s->stream, launch_prologue(s->d, s->stream, 0, s->pending_delrx, s->pending_dt));
     LAUNCH(s, 0, s->stream, launch_md(s->d, s->stream, s->pending_md, s->pending_epot, 0, 0, s->md_dt));
     s->pending_md = 0;
     s->pending_delrx = 0;
@@ -683,7 +673,6 @@ int spmd_create_batch(const spmd_params* p, uint32_t n_rep, spmd_sys** out) {
     d.n_rep = (int)n_rep;
     d.ppt = 4;
     d.tile_cells = 128;
-    d.win_speed = kWinSpeedDefault;
     uint64_t nt = 0, ct = 0;
     for (uint32_t r = 0; r < n_rep; ++r) {
         const spmd_params& q = p[r];
@@ -701,6 +690,7 @@ int spmd_create_batch(const spmd_params* p, uint32_t n_rep, spmd_sys** out) {
             rp.box.L[i] = l;
             rp.box.invL[i] = 1.0 / l;
             rp.box.half[i] = l * 0.5;
+            rp.box.inner[i] = l * 0.5 * (1.0 - 1e-9);
             rp.box.iL[i] = l;
             rp.vgrid[i] = int(l / 2.0);          /* Box.cpp:20-25 */
             rp.vside[i] = l / (double)rp.vgrid[i];
@@ -747,8 +737,6 @@ int spmd_create_batch(const spmd_params* p, uint32_t n_rep, spmd_sys** out) {
     rc = rc ? rc : dalloc(s, &d.counts, ct + 16);
     rc = rc ? rc : dalloc(s, &d.offs[0], ct + 16);
     rc = rc ? rc : dalloc(s, &d.offs[1], ct + 16);
-    rc = rc ? rc : dalloc(s, &d.outl, nt);
-    rc = rc ? rc : dalloc(s, &d.outl_r, nt);
     rc = rc ? rc : dalloc(s, &d.rtile_base, n_rep + 1);
     rc = rc ? rc : dalloc(s, &d.scan_flags, d.n_scan_tiles);
     rc = rc ? rc : dalloc(s, &d.scan_ticket, 1);
@@ -791,8 +779,8 @@ int spmd_destroy(spmd_sys* s) {
         dfree(d.ids[b]);
     }
     dfree(d.rank); dfree(d.pcell); dfree(d.perm); dfree(d.counts); dfree(d.scan_flags); dfree(d.scan_ticket);
-    for (int b = 0; b < 2; ++b) { dfree(d.offs[b]); dfree(d.tcount[b]); dfree(d.pcount[b]); }
-    dfree(d.tcount_r); dfree(d.pcount_r); dfree(d.outl); dfree(d.outl_r); dfree(d.rtile_base);
+    for (int b = 0; b < 2; ++b) { dfree(d.offs[b]); dfree(d.bcount[b]); }
+    dfree(d.bcount_r); dfree(d.tbase); dfree(d.bucket); dfree(d.bucket_r); dfree(d.rtile_base);
     dfree(d.mpos); dfree(d.mvel); dfree(d.mforce); dfree(d.mvpos); dfree(d.mmass); dfree(d.mbtype);
     dfree(d.vl); dfree(d.bonds); dfree(d.mcell);
     dfree(d.inj_axis); dfree(d.inj_gamma); dfree(d.reps); dfree(d.ctl); dfree(d.pblk_base);
@@ -836,14 +824,6 @@ int spmd_set_tuning(spmd_sys* s, int ppt, int tile_cells) {
 int spmd_set_collision_path(spmd_sys* s, int path) {
     if (!s || (path != SPMD_PATH_AUTO && path != SPMD_PATH_SORT)) return SPMD_ERR_ARG;
     s->path_pref = path == SPMD_PATH_SORT ? 1 : 0;
-    return SPMD_OK;
-}
-
-int spmd_set_row_window(spmd_sys* s, double speed) {
-    if (!s || !(speed >= 0.0) || speed > 1e3) return SPMD_ERR_ARG;
-    HIPCHK(s, hipStreamSynchronize(s->stream));
-    s->d.win_speed = speed;
-    clear_graphs(s);
     return SPMD_OK;
 }
 
@@ -1426,7 +1406,7 @@ int spmd_velocity_profile_noalloc(spmd_sys* s, uint32_t r, double width, uint32_
     const DevState& d = s->d;
     /* pending substeps and delrx advances first, as every observer sees them */
     if (s->pending_md || s->pending_delrx) {
-        HIPCHK(s, launch_prologue(d, s->stream, 0, s->pending_delrx, s->pending_dt, 0.0));
+        HIPCHK(s, launch_prologue(d, s->stream, 0, s->pending_delrx, s->pending_dt));
         HIPCHK(s, launch_md(d, s->stream, s->pending_md, s->pending_epot, 0, 0, s->md_dt));
         s->pending_md = 0;
         s->pending_delrx = 0;

@@ -10,9 +10,10 @@
  *   offs[b]   uint32[C_tot+1]  first particle of each cell in buffer b
  *                              (each collision writes the offsets of the
  *                              buffer it fills)
- *   tcount[b], pcount[b]       row-tile path: particles arriving in each
- *                              row tile / z-plane at the collision that
- *                              reads buffer b
+ *   bucket    uint32[tiles][rt_cap]  row-tile path: indices of the particles
+ *                              whose new cell lies in each row tile;
+ *                              bcount[b] their numbers at the collision
+ *                              that reads buffer b, tbase their scan
  *   monomers  pos/vel/force/vpos double[3*M_tot], mass double[M_tot],
  *             btype uint8[M_tot], Verlet half lists uint32[M_tot*64]
  *             (i | j << 16, per replica from mono_base*64), bonds
@@ -50,11 +51,9 @@ enum : uint32_t {
 };
 
 /* Row-tile path: a row tile is `nr` consecutive y-rows (whole x-rows) of
- * one z-plane; pcell carries kOutBit for a particle whose row moved beyond
- * the collision's source-row window (it is listed in the outlier list). */
+ * one z-plane. */
 constexpr int kRowsMax = 16;        /* rows per row tile */
 constexpr int kRowTileCells = 256;  /* cells per row tile */
-constexpr uint32_t kOutBit = 0x80000000u;
 
 /* bond entry: partner index | anchor-FENE flag in bit 31 */
 constexpr uint32_t kBondAnchor = 0x80000000u;
@@ -76,7 +75,6 @@ struct RepDev {
     int nr;                /* row-tile path: y-rows per tile */
     int tpp;               /* row tiles per z-plane, ceil(Ly / nr) */
     uint32_t rt_base;      /* first row tile of this replica */
-    uint32_t pl_base;      /* first z-plane of this replica (pcount index) */
 };
 
 struct RepCtl {
@@ -94,10 +92,6 @@ struct RepCtl {
     int32_t redo;                /* the speculative sort (no recentring) must be redone */
     uint32_t vl_count;           /* entries of the star's Verlet half list */
     uint32_t closure_oor;        /* monomers whose closure features left the trained range */
-    double shift_prev[3];        /* grid shift of the collision that ordered the current buffer */
-    int32_t wlo[2], wn[2];       /* row-tile path: row displacements new - old in [wlo, wlo + wn)
-                                    along y and z (mod L) are taken from the source window */
-    uint32_t n_out, n_out_redo;  /* outlier list lengths (speculative pass / recentring redo) */
 };
 
 struct DevState {
@@ -114,18 +108,15 @@ struct DevState {
     uint32_t* counts;
     uint32_t* offs[2];     /* first particle of each cell in buffer 0 / 1 (C_tot + 1) */
     /* row-tile path */
-    uint32_t* tcount[2];   /* particles arriving per row tile, by the buffer the collision reads */
-    uint32_t* pcount[2];   /* the same per z-plane */
-    uint32_t* tcount_r;    /* recount of a recentred replica (redo kernel) */
-    uint32_t* pcount_r;
-    uint32_t* outl;        /* outlier particle indices, replica r from part_base */
-    uint32_t* outl_r;      /* outliers of the recentring redo */
+    uint32_t* bcount[2];   /* particles binned per row tile, by the buffer the collision reads */
+    uint32_t* bcount_r;    /* rebinning of a recentred replica (redo kernel) */
+    uint32_t* bucket;      /* particle indices per row tile, rt_cap each */
+    uint32_t* bucket_r;    /* the rebinning's */
+    uint32_t* tbase;       /* first slot of each row tile in the new buffer */
     uint32_t* rtile_base;  /* row-tile prefix per replica (n_rep+1) */
     uint32_t n_rtiles;
-    uint32_t n_planes;     /* z-planes of all replicas */
     int rt_cap;            /* particles a row tile holds (1024 / 2048), 0: path unavailable */
-    int rt_tcm;            /* cells a row tile holds (128 / 256) */
-    double win_speed;      /* speed bound of the source-row windows, in units of sqrt(kT) */
+    int rt_tcm;            /* cells a row tile holds (64 / 128 / 256) */
     uint64_t* scan_flags;  /* per scan tile: status<<62 | value */
     uint32_t* scan_ticket;
     uint32_t n_scan_tiles;
@@ -158,8 +149,7 @@ size_t md_lds_bytes(int ns, uint32_t cap);
  * recentring flag, kSortSpec assumes none (runs beside the MD kernel),
  * (k_sort_redo redoes the replicas whose MD found a recentring). */
 enum { kSortNormal = 0, kSortSpec = 1 };
-hipError_t launch_prologue(const DevState& d, hipStream_t st, int do_collision, int n_delrx, double delrx_dt,
-                           double h);
+hipError_t launch_prologue(const DevState& d, hipStream_t st, int do_collision, int n_delrx, double delrx_dt);
 hipError_t launch_md(const DevState& d, hipStream_t st, int n_md, int epot_last, int md_init, int do_collision,
                      double md_dt);
 hipError_t launch_bin(const DevState& d, hipStream_t st, double h, int mode);

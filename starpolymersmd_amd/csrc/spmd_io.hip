@@ -45,6 +45,7 @@ spmd::Box host_box(const spmd_params& p, double delrx) {
         b.L[c] = L;
         b.invL[c] = 1.0 / L;
         b.half[c] = 0.5 * L;
+        b.inner[c] = 0.5 * L * (1.0 - 1e-9);
         b.iL[c] = L;
     }
     b.delrx = delrx;

@@ -731,22 +731,13 @@ __global__ __launch_bounds__(kMdThreads) void k_md(DevState d, int n_md, int epo
 
 /* ========================================================== prologue */
 
-/* Rows a particle's shifted coordinate can cross between two collisions:
- * the shifted coordinate moves by ds +- vh (plus FP32 rounding, covered by
- * 1e-3), and a row index floor(a + d) - floor(a) lies in [floor(d_min),
- * floor(d_max) + 1].  A window as wide as the box takes every row. */
-__device__ __forceinline__ void row_window(double ds, double vh, int L, int& lo, int& n) {
-    const double dmin = ds - vh - 1e-3, dmax = ds + vh + 1e-3;
-    lo = (int)floor(dmin);
-    n = (int)floor(dmax) + 1 - lo + 1;
-    if (n >= L) { lo = 0; n = L; }
-}
+
 
 /* Per replica, before anything else of a collision: the delrx advance of
  * every pending MPC::step call (MPC.cpp:117-118), then the collision's
  * Philox counter and grid shift (MPC.cpp:123).  Block 0 resets the scan
  * state. */
-__global__ __launch_bounds__(64) void k_prologue(DevState d, int do_coll, int n_delrx, double delrx_dt, double h) {
+__global__ __launch_bounds__(64) void k_prologue(DevState d, int do_coll, int n_delrx, double delrx_dt) {
     const int r = blockIdx.x;
     const RepDev& rep = d.reps[r];
     RepCtl* ctl = d.ctl + r;
@@ -770,23 +761,7 @@ __global__ __launch_bounds__(64) void k_prologue(DevState d, int do_coll, int n_
                 for (int c = 0; c < 3; ++c) sh[c] = u01(w[c]) - 0.5;
                 ctl->inj_active = 0;
             }
-            for (int c = 0; c < 3; ++c) {
-                ctl->shift_prev[c] = ctl->shift[c];
-                ctl->shift[c] = sh[c];
-            }
-            /* source-row window of the row-tile path: the shifted y (z)
-             * of a particle moves by shift - shift_prev + v h between two
-             * collisions, so its row moves by floor(f + that) for some f in
-             * [0, 1); speeds up to win_speed sqrt(kT) keep it inside */
-            const double vh = d.win_speed * sqrt(rep.kT) * h;
-            for (int a = 1; a < 3; ++a) {
-                int lo, n;
-                row_window(sh[a] - ctl->shift_prev[a], vh, rep.box.iL[a], lo, n);
-                ctl->wlo[a - 1] = lo;
-                ctl->wn[a - 1] = n;
-            }
-            ctl->n_out = 0u;
-            ctl->n_out_redo = 0u;
+            for (int c = 0; c < 3; ++c) ctl->shift[c] = sh[c];
             ctl->k_cur = k;
             ctl->k_done = k + 1;
             if (d.m_tot == 0) {  /* else k_md, running beside this kernel, sets them */
@@ -1437,29 +1412,25 @@ __global__ __launch_bounds__(256) void k_collide(DevState d, double h) {
 /* ===================================================== row-tile path */
 
 /* A row tile is nr consecutive y-rows (whole x-rows) of one z-plane of a
- * replica, so its cells are one contiguous range of the cell order.  The
- * current buffer is in cell order (every collision writes it so), so the
- * particles of any set of rows are contiguous runs of it.
- *
- * Per collision:
- *   k_advect        one workgroup per row tile of the current buffer:
- *                   streaming and the new shifted cell of its particles
- *                   (pcell, bit-identical to k_bin's), and the arrivals per
- *                   destination row tile and z-plane (LDS counters, about
- *                   nine global atomics per workgroup)
- *   k_rows_redo     after the MD: the monomers' cells; for a replica the MD
- *                   recentred, k_advect's work again with the recentring
- *   k_collide_rows  one workgroup per row tile of the new cells: its first
- *                   slot from the arrival counts, its particles pulled from
- *                   the window of source rows they can have come from (a
- *                   row moves by at most a few rows between collisions;
- *                   faster particles come through the outlier list), a
- *                   counting sort in LDS, the collision with cell sums
- *                   per cell thread, and the new buffer written in cell
- *                   order, coalesced, with its cell offsets.
- * No bin, scan or permutation pass: each particle is read once by k_advect
- * and once (plus its pcell word from the windows, mostly from L2) by
- * k_collide_rows, and written once. */
+ * replica, so its cells are one contiguous range of the cell order.  Per
+ * collision:
+ *   k_advect        per block of particles of the current buffer: streaming
+ *                   and the new shifted cell (bit-identical to k_bin's), and
+ *                   the particle's index appended to the bucket of its
+ *                   destination row tile (an LDS hash of the few tiles a
+ *                   block reaches, then one global atomic per tile to
+ *                   reserve its slots)
+ *   k_rows_redo     after the MD: the monomers' cells and the first slot of
+ *                   each row tile in the new buffer (scan of the bucket
+ *                   counts); a replica the MD recentred is binned again
+ *   k_collide_rows  one workgroup per row tile: its bucket, the particles
+ *                   gathered and streamed again, ranked per cell in LDS,
+ *                   the collision with the cell sums over 2-4 lanes per
+ *                   cell, and the new buffer written in cell order with its
+ *                   cell offsets.
+ * No cell-level scan or permutation: 4 bytes per particle go through the
+ * buckets.  The buffer being in cell order (any collision leaves it so)
+ * keeps a tile's gathers within a few rows of it. */
 
 struct RowTile {
     int z, ty, y0, ny;
@@ -1477,134 +1448,158 @@ __device__ __forceinline__ RowTile row_tile(const RepDev& rep, uint32_t t) {
     return T;
 }
 
-__device__ __forceinline__ int pmod(int a, int L) {
-    a %= L;
-    return a < 0 ? a + L : a;
+/* Row tile (replica-local) of a replica-local cell index. */
+__device__ __forceinline__ uint32_t tile_of_cell(const RepDev& rep, uint32_t cell) {
+    const uint32_t row = cell / (uint32_t)rep.box.iL[0];
+    const uint32_t iz = row / (uint32_t)rep.box.iL[1], iy = row - iz * (uint32_t)rep.box.iL[1];
+    return iz * (uint32_t)rep.tpp + iy / (uint32_t)rep.nr;
 }
 
-/* The cell shifted_cell gives, and its row (y, z).  The row is read off
- * the cell index when a coordinate sits on the far face (truncation gives
- * L there), so it is always the row of the index. */
-__device__ __forceinline__ uint32_t shifted_cell_row(const Box& b, const RepCtl& c, double x, double y, double z,
-                                                     double vx, int& iy, int& iz) {
-    double px = x + c.shift[0], py = y + c.shift[1], pz = z + c.shift[2];
-    le_wrap(b, px, py, pz, vx);
-    const int ix = (int)(px + b.half[0]);
-    iy = (int)(py + b.half[1]);
-    iz = (int)(pz + b.half[2]);
-    const uint32_t cell = (uint32_t)(ix + b.iL[0] * iy + b.iL[0] * b.iL[1] * iz);
-    if ((unsigned)ix >= (unsigned)b.iL[0] || (unsigned)iy >= (unsigned)b.iL[1] ||
-        (unsigned)iz >= (unsigned)b.iL[2]) {
-        const uint32_t row = cell / (uint32_t)b.iL[0];
-        iy = (int)(row % (uint32_t)b.iL[1]);
-        iz = (int)(row / (uint32_t)b.iL[1]);
-    }
-    return cell;
-}
+constexpr int kBinHash = 64;      /* distinct destination tiles a block aggregates in LDS */
+constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
 
-/* Streaming and new cell of the particles of row tile t of the current
- * buffer.  A particle whose row moved by (dy, dz) outside the window
- * [wylo, wylo + wyn) x [wzlo, wzlo + wzn) (mod L) is an outlier: kOutBit in
- * pcell and listed.  Arrivals are counted per destination row tile and
- * z-plane: the tiles a row tile's particles reach lie in an 8 x 8 block of
- * tiles around the window (LDS counters), others go straight to global
- * atomics.  s_rs holds kRowsMax + 1 words, s_hist 64. */
-template <int NT>
-__device__ void advect_tile(const DevState& d, int r, uint32_t t, const RepCtl& c, int wylo, int wyn, int wzlo,
-                            int wzn, double h, uint32_t* tcnt, uint32_t* pcnt, uint32_t* outl, uint32_t* n_out,
-                            uint32_t* s_rs, uint32_t* s_hist) {
+/* Bin one block of particles [p0, p1) (NT threads, PPT per thread) into the
+ * row-tile buckets: bucket + tile * cap holds the indices of the particles
+ * whose new cell lies in the tile, cnt[tile] their number.  A particle's
+ * slot is its rank among the block's particles of that tile (LDS hash,
+ * open addressing) plus the block's base in the tile (one global atomic
+ * per distinct tile and block); blocks that reach more than kBinHash tiles
+ * (a buffer not in cell order) take a global atomic per particle for the
+ * rest.  Slots from cap on are counted but not stored (the collide reports
+ * the overflow). */
+template <int NT, int PPT>
+__device__ void bin_block(const DevState& d, int r, uint64_t p0, uint64_t p1, const RepCtl& c, double h,
+                          uint32_t* cnt, uint32_t* bucket, uint32_t* s_key, uint32_t* s_cnt, bool& over) {
     const RepDev& rep = d.reps[r];
     Box b = rep.box;
     b.delrx = c.delrx;
-    const RowTile T = row_tile(rep, t);
-    const int Lx = b.iL[0], Ly = b.iL[1], Lz = b.iL[2];
-    const uint32_t* off = d.offs[d.cur];
-    const uint32_t gc0 = (uint32_t)rep.cell_base + T.c_lo;
     const int tid = threadIdx.x;
-    __syncthreads();  /* s_rs and s_hist of a previous tile are read out */
-    for (int j = tid; j <= T.ny; j += NT) s_rs[j] = off[gc0 + (uint32_t)(Lx * j)];
-    for (int j = tid; j < 64; j += NT) s_hist[j] = 0u;
+    const uint32_t cap = (uint32_t)d.rt_cap;
+    __syncthreads();  /* the tables of a previous block are read out */
+    for (int q = tid; q < kBinHash; q += NT) { s_key[q] = kHashEmpty; s_cnt[q] = 0u; }
     __syncthreads();
-    /* offsets left by a collision that reported an error need not be
-     * monotone: stay inside the replica's particles in any case */
-    const uint32_t pr0 = (uint32_t)rep.part_base, pr1 = pr0 + rep.n_fluid;
-    const uint32_t i0 = min(max(s_rs[0], pr0), pr1), i1 = min(max(s_rs[T.ny], i0), pr1);
-    const int tyf = pmod(T.y0 + wylo, Ly) / rep.nr;  /* tile of the first destination row */
-    const int zf = pmod(T.z + wzlo, Lz);
     const int cur = d.cur;
     const float *X = d.x[cur], *Y = d.y[cur], *Z = d.z[cur];
     const float *VX = d.vx[cur], *VY = d.vy[cur], *VZ = d.vz[cur];
+    float fx[PPT], fy[PPT], fz[PPT], fvx[PPT], fvy[PPT], fvz[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const uint64_t i = p0 + (uint64_t)(j * NT + tid);
+        if (i >= p1) continue;
+        fx[j] = X[i]; fy[j] = Y[i]; fz[j] = Z[i];
+        fvx[j] = VX[i]; fvy[j] = VY[i]; fvz[j] = VZ[i];
+    }
     bool bad = false;
-    for (uint32_t i = i0 + tid; i < i1; i += NT) {
-        int yo = T.y0;
-        for (int j = 1; j < T.ny; ++j) yo += i >= s_rs[j] ? 1 : 0;
-        double x = X[i], y = Y[i], z = Z[i], vx = VX[i], vy = VY[i], vz = VZ[i];
+    /* global row tile; hash entry << 16 | rank, or kDirect | the slot a
+     * global atomic gave */
+    constexpr uint32_t kDirect = 0x80000000u;
+    uint32_t tl[PPT], slot[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        tl[j] = kHashEmpty;
+        const uint64_t i = p0 + (uint64_t)(j * NT + tid);
+        if (i >= p1) continue;
+        double x = fx[j], y = fy[j], z = fz[j], vx = fvx[j], vy = fvy[j], vz = fvz[j];
         recenter_stream(b, h, c, x, y, z, vx, vy, vz);
-        int iy, iz;
-        const uint32_t cell = shifted_cell_row(b, c, x, y, z, vx, iy, iz);
-        if (cell >= rep.ncells) {
-            bad = true;
-            d.pcell[i] = ~0u;
-            continue;
+        const uint32_t cell = shifted_cell(b, c, x, y, z, vx);
+        if (cell >= rep.ncells) { bad = true; continue; }
+        const uint32_t g = rep.rt_base + tile_of_cell(rep, cell);
+        tl[j] = g;
+        uint32_t hs = (g * 0x9E3779B1u) >> 26;
+        int probe = 0;
+        for (; probe < kBinHash; ++probe) {
+            const uint32_t k = atomicCAS(&s_key[hs], kHashEmpty, g);
+            if (k == kHashEmpty || k == g) break;
+            hs = (hs + 1) & (kBinHash - 1);
         }
-        uint32_t g = (uint32_t)rep.cell_base + cell;
-        if (pmod(iy - yo - wylo, Ly) >= wyn || pmod(iz - T.z - wzlo, Lz) >= wzn) {
-            g |= kOutBit;
-            const uint32_t k = atomicAdd(n_out, 1u);
-            if (k < rep.n_fluid) outl[rep.part_base + k] = i;
-        }
-        d.pcell[i] = g;
-        const int tyd = iy / rep.nr;
-        const int hy = pmod(tyd - tyf, rep.tpp), hz = pmod(iz - zf, Lz);
-        if (hy < 8 && hz < 8) {
-            atomicAdd(&s_hist[hz * 8 + hy], 1u);
+        if (probe == kBinHash) {
+            slot[j] = kDirect | min(atomicAdd(&cnt[g], 1u), cap);
         } else {
-            atomicAdd(&tcnt[rep.rt_base + (uint32_t)(iz * rep.tpp + tyd)], 1u);
-            atomicAdd(&pcnt[rep.pl_base + (uint32_t)iz], 1u);
+            slot[j] = (hs << 16) | atomicAdd(&s_cnt[hs], 1u);
         }
     }
     __syncthreads();
-    for (int j = tid; j < 64; j += NT) {
-        const uint32_t n = s_hist[j];
-        if (!n) continue;
-        const int zd = (zf + j / 8) % Lz, tyd = (tyf + j % 8) % rep.tpp;
-        atomicAdd(&tcnt[rep.rt_base + (uint32_t)(zd * rep.tpp + tyd)], n);
-        atomicAdd(&pcnt[rep.pl_base + (uint32_t)zd], n);
+    for (int q = tid; q < kBinHash; q += NT) {
+        const uint32_t k = s_key[q];
+        if (k != kHashEmpty) s_cnt[q] = atomicAdd(&cnt[k], s_cnt[q]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        if (tl[j] == kHashEmpty) continue;
+        const uint64_t i = p0 + (uint64_t)(j * NT + tid);
+        const uint32_t sl = (slot[j] & kDirect) ? slot[j] & ~kDirect : s_cnt[slot[j] >> 16] + (slot[j] & 0xFFFFu);
+        if (sl < cap) bucket[(size_t)tl[j] * cap + sl] = (uint32_t)i;
+        else over = true;
     }
     if (bad) atomicOr(&d.ctl[r].status, ST_CELL_OOB);
 }
 
+template <int PPT>
 __global__ __launch_bounds__(256) void k_advect(DevState d, double h, int mode) {
-    __shared__ uint32_t s_rs[kRowsMax + 1], s_hist[64];
-    const uint32_t lt = xcd_remap(blockIdx.x, gridDim.x);
-    const int r = find_rep(d.rtile_base, d.n_rep, lt);
+    __shared__ uint32_t s_key[kBinHash], s_cnt[kBinHash];
+    const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int r = find_rep(d.pblk_base, d.n_rep, lb);
     const RepDev& rep = d.reps[r];
-    const RepCtl* g = d.ctl + r;
     RepCtl c;
     sort_ctl(d, r, mode, c);
-    const uint32_t t = lt - rep.rt_base;
     /* the recentring redo's counts of an earlier collision are read out:
      * clear them for this one's redo (which runs after this kernel) */
-    if (threadIdx.x == 0) {
-        d.tcount_r[lt] = 0u;
-        if (t % (uint32_t)rep.tpp == 0u) d.pcount_r[rep.pl_base + t / (uint32_t)rep.tpp] = 0u;
-    }
-    advect_tile<256>(d, r, t, c, g->wlo[0], g->wn[0], g->wlo[1], g->wn[1], h, d.tcount[d.cur], d.pcount[d.cur],
-                     d.outl, &d.ctl[r].n_out, s_rs, s_hist);
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < d.n_rtiles; i += gridDim.x * 256u) d.bcount_r[i] = 0u;
+    const uint64_t p0 = rep.part_base + (uint64_t)(lb - d.pblk_base[r]) * (256 * PPT);
+    const uint64_t p1 = min(p0 + 256 * PPT, rep.part_base + rep.n_fluid);
+    bool over = false;
+    bin_block<256, PPT>(d, r, p0, p1, c, h, d.bcount[d.cur], d.bucket, s_key, s_cnt, over);
+    (void)over;  /* an overflowing tile reports it from its count */
 }
 
-/* After the MD kernel: the monomers' cells of this collision
- * (MPC::shiftParticles MPC.cpp:306-310 + sort MPC.cpp:221-235: shifted,
- * never wrapped), and for a replica whose MD recentred it
- * (MPC::check_bounds MPC.cpp:160-190) the advect pass again, with the
- * recentring, into the redo counts and outlier list; its source window
- * moves by -COM and widens by the COM speed.  The grid strides over the
- * replica's row tiles; with no recentring the kernel only finds the
- * monomer cells. */
+/* After k_advect (and the MD kernel), per replica (workgroup r, strided):
+ * the monomers' cells of this collision (MPC::shiftParticles
+ * MPC.cpp:306-310 + sort MPC.cpp:221-235: shifted, never wrapped) and the
+ * first slot of every row tile in the new buffer (exclusive scan of the
+ * bucket counts).  A replica whose MD recentred it (MPC::check_bounds
+ * MPC.cpp:160-190) is binned again, with the recentring, into the redo
+ * buckets: the grid strides over its particle blocks (k_collide_rows then
+ * sums the redo counts itself). */
+template <int PPT>
 __global__ __launch_bounds__(256) void k_rows_redo(DevState d, double h) {
-    __shared__ uint32_t s_rs[kRowsMax + 1], s_hist[64];
+    __shared__ uint32_t s_key[kBinHash], s_cnt[kBinHash];
+    __shared__ uint32_t s_w[4];
+    constexpr int KS = 16;  /* counts per thread and pass of the scan */
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int r = blockIdx.x; r < d.n_rep; r += gridDim.x) {
         const RepDev& rep = d.reps[r];
+        if (!d.ctl[r].redo) {
+            const uint32_t t0 = d.rtile_base[r], nt = d.rtile_base[r + 1] - t0;
+            const uint32_t* tc = d.bcount[d.cur] + t0;
+            uint32_t carry = (uint32_t)rep.part_base;
+            for (uint32_t c0 = 0; c0 < nt; c0 += 256u * KS) {
+                /* each thread a run of KS consecutive counts, all loads first */
+                uint32_t v[KS];
+                const uint32_t i0 = c0 + threadIdx.x * KS;
+#pragma unroll
+                for (int q = 0; q < KS; ++q) v[q] = i0 + q < nt ? tc[i0 + q] : 0u;
+                uint32_t run = 0u;
+#pragma unroll
+                for (int q = 0; q < KS; ++q) { const uint32_t t = v[q]; v[q] = run; run += t; }
+                uint32_t inc = run;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const uint32_t u = __shfl_up(inc, o);
+                    if (lane >= o) inc += u;
+                }
+                __syncthreads();
+                if (lane == 63) s_w[w] = inc;
+                __syncthreads();
+                uint32_t pre = 0u, tot = 0u;
+                for (int q = 0; q < 4; ++q) { pre += q < w ? s_w[q] : 0u; tot += s_w[q]; }
+                const uint32_t base = carry + pre + inc - run;
+#pragma unroll
+                for (int q = 0; q < KS; ++q)
+                    if (i0 + q < nt) d.tbase[t0 + i0 + q] = base + v[q];
+                carry += tot;
+            }
+        }
         const double sh0 = d.ctl[r].shift[0], sh1 = d.ctl[r].shift[1], sh2 = d.ctl[r].shift[2];
         bool oob = false;
         for (uint32_t i = threadIdx.x; i < rep.n_mono; i += 256) {
@@ -1616,45 +1611,16 @@ __global__ __launch_bounds__(256) void k_rows_redo(DevState d, double h) {
         if (oob) atomicOr(&d.ctl[r].status, ST_CELL_OOB);
     }
     for (int r = 0; r < d.n_rep; ++r) {
-        const RepCtl* g = d.ctl + r;
-        if (!g->redo) continue;
+        if (!d.ctl[r].redo) continue;
         const RepDev& rep = d.reps[r];
         RepCtl c;
         sort_ctl(d, r, kSortNormal, c);
-        int lo[2], n[2];
-        const double vs = d.win_speed * sqrt(rep.kT);
-        for (int a = 1; a < 3; ++a)
-            row_window(c.shift[a] - g->shift_prev[a] - c.com[a], (vs + fabs(c.com_vel[a])) * h, rep.box.iL[a],
-                       lo[a - 1], n[a - 1]);
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            d.ctl[r].wlo[0] = lo[0]; d.ctl[r].wn[0] = n[0];
-            d.ctl[r].wlo[1] = lo[1]; d.ctl[r].wn[1] = n[1];
-        }
-        const uint32_t nt = d.rtile_base[r + 1] - d.rtile_base[r];
-        for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x)
-            advect_tile<256>(d, r, t, c, lo[0], n[0], lo[1], n[1], h, d.tcount_r, d.pcount_r, d.outl_r,
-                             &d.ctl[r].n_out_redo, s_rs, s_hist);
-    }
-}
-
-/* Wave-level compaction into a row tile's selection list: the lanes that
- * take a particle get consecutive list positions (one LDS atomic per wave)
- * and a rank inside their cell (LDS atomic); positions from `cap` on are
- * counted but not stored (the tile then reports an overflow). */
-__device__ __forceinline__ void rows_select(bool take, uint32_t idx, uint32_t lc, uint32_t* s_nsel, uint32_t* s_cnt,
-                                            uint32_t* sel, uint32_t* code, uint32_t cap) {
-    const uint64_t m = __ballot(take);
-    if (m == 0ull) return;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t wb = 0u;
-    if ((int)__lane_id() == leader) wb = atomicAdd(s_nsel, (uint32_t)__popcll(m));
-    wb = __shfl(wb, leader);
-    if (take) {
-        const uint32_t pos = wb + (uint32_t)__popcll(m & lanemask_lt());
-        const uint32_t rk = atomicAdd(&s_cnt[lc], 1u);
-        if (pos < cap) {
-            sel[pos] = idx;
-            code[pos] = (lc << 16) | (rk < 0xFFFFu ? rk : 0xFFFFu);
+        const uint32_t nb = d.pblk_base[r + 1] - d.pblk_base[r];
+        for (uint32_t q = blockIdx.x; q < nb; q += gridDim.x) {
+            const uint64_t p0 = rep.part_base + (uint64_t)q * (256 * PPT);
+            const uint64_t p1 = min(p0 + 256 * PPT, rep.part_base + rep.n_fluid);
+            bool over = false;
+            bin_block<256, PPT>(d, r, p0, p1, c, h, d.bcount_r, d.bucket_r, s_key, s_cnt, over);
         }
     }
 }
@@ -1664,57 +1630,43 @@ __device__ __forceinline__ void rows_select(bool take, uint32_t idx, uint32_t lc
  * particle and cell is k_collide's, expression for expression, with the
  * cell sums as 64-bit fixed point (so their order does not matter): the
  * two paths give the same bits.  Phases:
- *   (a) first slot = arrivals of the replica's earlier z-planes and of
- *       this plane's earlier tiles; the tile's monomers (as k_collide)
- *   (b) the source window: rows y0 - (wylo + wyn - 1) .. y0 + ny - 1 - wylo
- *       of planes z - (wzlo + wzn - 1) .. z - wzlo of the current buffer,
- *       one wave per row, 4 x 64 pcell words in flight per lane; particles
- *       whose new cell is the tile's enter the selection list with their
- *       rank in the cell; then the replica's outliers
- *   (c) exclusive scan of the cell counts (the slots), the check that the
- *       tile holds exactly the arrivals k_advect counted
- *   (d) per particle (list order): load, stream (as k_advect), position
- *       (and id) to the new buffer, velocity and image index staged in LDS
- *       at the slot; per cell: Gamma variate and rotation
- *   (e) per cell (one thread): momentum sums over its slots + monomers ->
- *       CMV, relative kinetic energy -> MBS scale (with angular momentum:
- *       CMP, inertia tensor, C, omega)
+ *   (a) the tile's count and first slot; its monomers (as k_collide)
+ *   (b) its bucket: particle indices, the particles gathered, streamed (as
+ *       k_advect), their cell and rank in the cell (LDS atomic)
+ *   (c) exclusive scan of the cell counts (the rotations were drawn in (a))
+ *   (d) per particle: position (and id) to the new buffer at its slot,
+ *       velocity and image index staged in LDS at the slot
+ *   (e) per cell, LPC lanes: momentum sums over its slots + monomers ->
+ *       CMV, relative kinetic energy, then one lane the Gamma variate ->
+ *       MBS scale (with angular momentum: CMP, inertia tensor, C, omega);
+ *       the particles' registers are free by then
  *   (f) per slot: v <- CMV + s R (v - CMV) (+ omega x r), written coalesced;
- *       cell offsets of the new buffer; monomer velocities; the arrival
- *       counters of the next collision cleared. */
+ *       cell offsets of the new buffer; monomer velocities; the bucket
+ *       counts of the next collision cleared. */
 template <int NT, int CAP, int TCM, bool AMC>
 __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
-    static_assert(TCM <= NT && CAP % NT == 0, "tile shape");
+    static_assert(TCM <= NT && NT % TCM == 0 && CAP % NT == 0, "tile shape");
     constexpr int MCAP = 2 * TCM + 64;
-    constexpr int KW = 128;               /* source rows held at once */
     constexpr int PER = CAP / NT;
     constexpr int NW = NT / 64;
-    constexpr int U = 4;
+    constexpr int LPC = NT / TCM;     /* lanes per cell in the cell sums */
     constexpr int TA = AMC ? TCM : 1;
-    constexpr size_t kSelBytes = 8ull * CAP;
-    constexpr size_t kStageBytes = (AMC ? 26ull : 14ull) * CAP;
-    __shared__ __align__(16) unsigned char s_buf[kSelBytes > kStageBytes ? kSelBytes : kStageBytes];
+    constexpr int CA = AMC ? CAP : 1;
+    __shared__ float s_vx[CAP], s_vy[CAP], s_vz[CAP];
+    __shared__ float s_lx[CA], s_ly[CA], s_lz[CA];
+    __shared__ int8_t s_cy[CAP];
+    __shared__ uint8_t s_lcs[CAP];
     __shared__ uint32_t s_cnt[TCM], s_off[TCM + 1], s_nm[TCM];
-    __shared__ unsigned long long s_mq[4][TCM];  /* monomers: sum m v (3), sum m */
-    __shared__ unsigned long long s_e[TCM];
+    __shared__ unsigned long long s_p[3][TCM];   /* monomers: m v (3) */
+    __shared__ unsigned long long s_mm[TCM];     /* monomer mass */
+    __shared__ unsigned long long s_me[TCM];     /* monomer relative kinetic energy */
     __shared__ double s_cmv[TCM][3], s_g[TCM];
     __shared__ float s_R[TCM][9];
     __shared__ uint32_t s_ml[MCAP];
-    __shared__ uint32_t s_ws[KW], s_wl[KW];
     __shared__ uint32_t s_wsum[NW];
-    __shared__ uint32_t s_nsel, s_mn, s_bad, s_pre;
+    __shared__ uint32_t s_mn, s_bad, s_pre;
     __shared__ unsigned long long s_am[12][TA];
     __shared__ double s_cmp[TA][3], s_w[TA][3];
-    uint32_t* sel = reinterpret_cast<uint32_t*>(s_buf);
-    uint32_t* code = sel + CAP;
-    float* s_vx = reinterpret_cast<float*>(s_buf);
-    float* s_vy = s_vx + CAP;
-    float* s_vz = s_vy + CAP;
-    float* s_lx = s_vz + CAP;  /* AMC only */
-    float* s_ly = s_lx + CAP;
-    float* s_lz = s_ly + CAP;
-    int8_t* s_cy = reinterpret_cast<int8_t*>(AMC ? s_lz + CAP : s_vz + CAP);
-    uint8_t* s_lcs = reinterpret_cast<uint8_t*>(s_cy + CAP);
 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint32_t lt = xcd_remap(blockIdx.x, gridDim.x);
@@ -1727,18 +1679,13 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
     b.delrx = cc.delrx;
     const uint32_t t = lt - rep.rt_base;
     const RowTile T = row_tile(rep, t);
-    const int Lx = b.iL[0], Ly = b.iL[1], Lz = b.iL[2];
     const uint32_t nc = T.nc;
     const uint32_t gc0 = (uint32_t)rep.cell_base + T.c_lo;
     const int cur = d.cur, alt = cur ^ 1;
     const bool rd = ctl->redo != 0;
-    const uint32_t* tcnt = rd ? d.tcount_r : d.tcount[cur];
-    const uint32_t* pcnt = rd ? d.pcount_r : d.pcount[cur];
-    const uint32_t* outl = rd ? d.outl_r : d.outl;
-    const uint32_t n_out = min(rd ? ctl->n_out_redo : ctl->n_out, rep.n_fluid);
+    const uint32_t* cnt = rd ? d.bcount_r : d.bcount[cur];
+    const uint32_t* bk = (rd ? d.bucket_r : d.bucket) + (size_t)lt * (uint32_t)d.rt_cap;
     const uint32_t pr0 = (uint32_t)rep.part_base, pr1 = pr0 + rep.n_fluid;
-    const int wylo = ctl->wlo[0], wyn = ctl->wn[0], wzlo = ctl->wlo[1], wzn = ctl->wn[1];
-    const uint32_t* off = d.offs[cur];
     const float *X = d.x[cur], *Y = d.y[cur], *Z = d.z[cur];
     const float *VX = d.vx[cur], *VY = d.vy[cur], *VZ = d.vz[cur];
     float *OX = d.x[alt], *OY = d.y[alt], *OZ = d.z[alt];
@@ -1749,19 +1696,20 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
     const bool amc = AMC && rep.amc;
 
     /* (a) */
+    const uint32_t n = cnt[lt];
     for (uint32_t i = tid; i < nc; i += NT) {
         s_cnt[i] = 0u;
         s_nm[i] = 0u;
-        s_mq[0][i] = s_mq[1][i] = s_mq[2][i] = s_mq[3][i] = 0ull;
-        s_e[i] = 0ull;
+        s_p[0][i] = s_p[1][i] = s_p[2][i] = 0ull;
+        s_mm[i] = 0ull;
+        s_me[i] = 0ull;
     }
     if (amc)
         for (uint32_t i = tid; i < 12u * nc; i += NT) s_am[i / nc][i % nc] = 0ull;
-    if (tid == 0) { s_nsel = 0u; s_mn = 0u; s_bad = 0u; }
-    {
+    if (tid == 0) { s_mn = 0u; s_bad = 0u; }
+    if (rd) {  /* recentred replica (rare): the redo counts of all earlier tiles */
         uint32_t pre = 0u;
-        for (int q = tid; q < T.z; q += NT) pre += pcnt[rep.pl_base + (uint32_t)q];
-        for (int q = tid; q < T.ty; q += NT) pre += tcnt[rep.rt_base + (uint32_t)(T.z * rep.tpp + q)];
+        for (uint32_t q = tid; q < t; q += NT) pre += cnt[rep.rt_base + q];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o);
         if (lane == 0) s_wsum[wv] = pre;
@@ -1769,8 +1717,9 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
     __syncthreads();
     if (tid == 0) {
         uint32_t p = 0u;
-        for (int q = 0; q < NW; ++q) p += s_wsum[q];
-        s_pre = p;
+        if (rd)
+            for (int q = 0; q < NW; ++q) p += s_wsum[q];
+        s_pre = rd ? pr0 + p : d.tbase[lt];
     }
     for (uint32_t j = tid; j < rep.n_mono; j += NT) {
         const uint32_t rel = d.mcell[rep.mono_base + j] - gc0;
@@ -1779,112 +1728,11 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
             if (slot < (uint32_t)MCAP) { s_ml[slot] = (rel << 16) | j; atomicAdd(&s_nm[rel], 1u); }
         }
     }
-
-    /* (b) */
-    int nwy = T.ny + wyn - 1, ys0 = T.y0 - (wylo + wyn - 1);
-    if (nwy >= Ly) { nwy = Ly; ys0 = 0; }
-    const int nwz = wzn, zs0 = T.z - (wzlo + wzn - 1);
-    const int nrows = nwy * nwz;
-    for (int k0 = 0; k0 < nrows; k0 += KW) {
-        const int kn = min(KW, nrows - k0);
-        for (int k = tid; k < kn; k += NT) {
-            const int q = k0 + k, jz = q / nwy, jy = q - jz * nwy;
-            const int oy = pmod(ys0 + jy, Ly), oz = pmod(zs0 + jz, Lz);
-            const uint32_t rc = (uint32_t)rep.cell_base + (uint32_t)Lx * (uint32_t)(oy + Ly * oz);
-            /* clamped as in advect_tile */
-            const uint32_t a = min(max(off[rc], pr0), pr1);
-            s_ws[k] = a;
-            s_wl[k] = min(max(off[rc + (uint32_t)Lx], a), pr1) - a;
-        }
-        __syncthreads();
-        for (int k = wv; k < kn; k += NW) {
-            const uint32_t a = s_ws[k], e = a + s_wl[k];
-            for (uint32_t p0 = a; p0 < e; p0 += 64u * U) {
-                uint32_t pc[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint32_t p = p0 + (uint32_t)(u * 64 + lane);
-                    pc[u] = p < e ? d.pcell[p] : ~0u;
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint32_t lc = pc[u] - gc0;
-                    rows_select(lc < nc, p0 + (uint32_t)(u * 64 + lane), lc, &s_nsel, s_cnt, sel, code, CAP);
-                }
-            }
-        }
-        __syncthreads();
-    }
-    for (uint32_t e0 = 0; e0 < n_out; e0 += NT) {
-        const uint32_t e = e0 + tid;
-        uint32_t idx = 0u, lc = ~0u;
-        if (e < n_out) {
-            idx = outl[rep.part_base + e];
-            const uint32_t pc = d.pcell[idx];
-            if (pc & kOutBit) lc = (pc & ~kOutBit) - gc0;
-        }
-        rows_select(lc < nc, idx, lc, &s_nsel, s_cnt, sel, code, CAP);
-    }
-    __syncthreads();
-
-    /* (c) */
-    const uint32_t nsel = s_nsel;
-    {
-        const uint32_t v = (uint32_t)tid < nc ? s_cnt[tid] : 0u;
-        uint32_t inc = v;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t u = __shfl_up(inc, o);
-            if (lane >= o) inc += u;
-        }
-        if (lane == 63) s_wsum[wv] = inc;
-        __syncthreads();
-        uint32_t wpre = 0u;
-        for (int q = 0; q < wv; ++q) wpre += s_wsum[q];
-        if ((uint32_t)tid < nc) s_off[tid] = wpre + inc - v;
-        if ((uint32_t)tid == nc - 1u) s_off[nc] = wpre + inc;
-    }
-    if (tid == 0) {
-        if (nsel != tcnt[lt]) atomicOr(&ctl->status, ST_SORT_MISMATCH);
-        if (nsel > (uint32_t)CAP) atomicOr(&ctl->status, ST_TILE_OVERFLOW);
-        if (s_mn > (uint32_t)MCAP) atomicOr(&ctl->status, ST_TILE_MONO);
-    }
-    const uint32_t base = (uint32_t)rep.part_base + s_pre;
-    /* uniform exits, status set: the tile cannot hold its particles, or the
-     * arrival counts would place them beyond the replica */
-    if (nsel > (uint32_t)CAP) return;
-    if (base < pr0 || (uint64_t)base + nsel > (uint64_t)pr1) {
-        if (tid == 0) atomicOr(&ctl->status, ST_SORT_MISMATCH);
-        return;
-    }
-    const uint32_t nmt = s_mn < (uint32_t)MCAP ? s_mn : (uint32_t)MCAP;
     const uint32_t k = ctl->k_cur;
     const bool inj = ctl->inj_active != 0;
-
-    /* (d) the list into registers, then the staging may overwrite it */
-    uint32_t my_i[PER], my_c[PER];
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const uint32_t e = (uint32_t)(tid + u * NT);
-        my_i[u] = e < nsel ? sel[e] : ~0u;
-        my_c[u] = e < nsel ? code[e] : 0u;
-    }
-    __syncthreads();
-    float fx[PER], fy[PER], fz[PER], fvx[PER], fvy[PER], fvz[PER];
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        if (my_i[u] == ~0u) continue;
-        const uint32_t i = my_i[u];
-        fx[u] = X[i]; fy[u] = Y[i]; fz[u] = Z[i];
-        fvx[u] = VX[i]; fvy[u] = VY[i]; fvz[u] = VZ[i];
-    }
+    /* per cell: the rotation about a random axis (MPC::collide MPC.cpp:241-257) */
     for (uint32_t ci = tid; ci < nc; ci += NT) {
         const uint32_t cell = T.c_lo + ci;
-        const uint32_t n = s_cnt[ci] + s_nm[ci];
-        double g = 0.0;
-        if (n >= 2)
-            g = inj ? d.inj_gamma[gc0 + ci] : gamma_mt(rep.seed, rep.replica, cell, k, 1.5 * (n - 1), rep.kT);
-        s_g[ci] = g;
         double u1, u2;
         if (inj) {
             u1 = d.inj_axis[2 * (size_t)(gc0 + ci)];
@@ -1912,30 +1760,105 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
         R[7] = (float)(a1 * a2 * (1 - cs) + a0 * ss);
         R[8] = (float)(a2 * a2 + (1 - a2 * a2) * cs);
     }
-    bool bad = false;
+    /* uniform exits, status set: the tile cannot hold its particles */
+    if (n > (uint32_t)CAP) {
+        if (tid == 0) atomicOr(&ctl->status, ST_TILE_OVERFLOW);
+        return;
+    }
+
+    /* (b) */
+    uint32_t my_i[PER];
+    float fx[PER], fy[PER], fz[PER], fvx[PER], fvy[PER], fvz[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const uint32_t e = (uint32_t)(tid + u * NT);
+        my_i[u] = e < n ? bk[e] : ~0u;
+        if (my_i[u] < pr0 || my_i[u] >= pr1) my_i[u] = ~0u;  /* only this replica's particles */
+    }
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         if (my_i[u] == ~0u) continue;
-        const uint32_t lc = my_c[u] >> 16;
-        const uint32_t slot = s_off[lc] + (my_c[u] & 0xFFFFu);
+        const uint32_t i = my_i[u];
+        fx[u] = X[i]; fy[u] = Y[i]; fz[u] = Z[i];
+        fvx[u] = VX[i]; fvy[u] = VY[i]; fvz[u] = VZ[i];
+    }
+    __syncthreads();  /* cell counters cleared, monomers listed */
+    uint32_t my_c[PER];  /* local cell << 16 | rank; image index and local coordinates */
+    int my_cy[PER];
+    float my_l[AMC ? PER : 1][3];
+    bool bad = false;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        my_c[u] = ~0u;
+        if (my_i[u] == ~0u) continue;
         double x = fx[u], y = fy[u], z = fz[u], vx = fvx[u], vy = fvy[u], vz = fvz[u];
         recenter_stream(b, h, cc, x, y, z, vx, vy, vz);
-        OX[base + slot] = (float)x; OY[base + slot] = (float)y; OZ[base + slot] = (float)z;
-        if (ids) oids[base + slot] = ids[my_i[u]];
+        fx[u] = (float)x; fy[u] = (float)y; fz[u] = (float)z;
+        fvx[u] = (float)vx; fvy[u] = (float)vy; fvz[u] = (float)vz;
         double px = x + cc.shift[0], py = y + cc.shift[1], pz = z + cc.shift[2], vimg = vx;
-        const int cy = (int)le_wrap(b, px, py, pz, vimg);
-        bad |= cell_index(b, px, py, pz) != T.c_lo + lc;
-        s_vx[slot] = (float)vx; s_vy[slot] = (float)vy; s_vz[slot] = (float)vz;
-        s_cy[slot] = (int8_t)cy;
-        s_lcs[slot] = (uint8_t)lc;
+        my_cy[u] = (int)le_wrap(b, px, py, pz, vimg);
+        const uint32_t lc = cell_index(b, px, py, pz) - T.c_lo;
+        if (lc >= nc) { bad = true; continue; }
+        my_c[u] = (lc << 16) | atomicAdd(&s_cnt[lc], 1u);
         if (AMC) {  /* coordinates inside the cell, as cell_index truncates */
             const double qx = px + b.half[0], qy = py + b.half[1], qz = pz + b.half[2];
-            s_lx[slot] = (float)(qx - (double)(int)qx);
-            s_ly[slot] = (float)(qy - (double)(int)qy);
-            s_lz[slot] = (float)(qz - (double)(int)qz);
+            my_l[AMC ? u : 0][0] = (float)(qx - (double)(int)qx);
+            my_l[AMC ? u : 0][1] = (float)(qy - (double)(int)qy);
+            my_l[AMC ? u : 0][2] = (float)(qz - (double)(int)qz);
         }
     }
     if (bad) s_bad = 1u;
+    __syncthreads();
+
+    /* (c) */
+    {
+        const uint32_t v = (uint32_t)tid < nc ? s_cnt[tid] : 0u;
+        uint32_t inc = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(inc, o);
+            if (lane >= o) inc += u;
+        }
+        if (lane == 63) s_wsum[wv] = inc;
+        __syncthreads();
+        uint32_t wpre = 0u;
+        for (int q = 0; q < wv; ++q) wpre += s_wsum[q];
+        if ((uint32_t)tid < nc) s_off[tid] = wpre + inc - v;
+        if ((uint32_t)tid == nc - 1u) s_off[nc] = wpre + inc;
+    }
+    __syncthreads();  /* every cell's offset before any particle takes its slot */
+    const uint32_t base = s_pre;
+    const uint32_t nmt = s_mn < (uint32_t)MCAP ? s_mn : (uint32_t)MCAP;
+    if (tid == 0) {
+        if (s_bad) atomicOr(&ctl->status, ST_SORT_MISMATCH);
+        if (s_mn > (uint32_t)MCAP) atomicOr(&ctl->status, ST_TILE_MONO);
+    }
+    /* the counts would place the tile beyond its replica: status set, exit */
+    if (base < pr0 || (uint64_t)base + n > (uint64_t)pr1) {
+        if (tid == 0) atomicOr(&ctl->status, ST_SORT_MISMATCH);
+        return;
+    }
+    /* (d) */
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        if (my_c[u] == ~0u) continue;
+        const uint32_t lc = my_c[u] >> 16;
+        const uint32_t slot = s_off[lc] + (my_c[u] & 0xFFFFu);
+        if (slot >= n) {  /* cannot happen: the counts and ranks agree */
+            atomicOr(&ctl->status, ST_SORT_MISMATCH);
+            continue;
+        }
+        OX[base + slot] = fx[u]; OY[base + slot] = fy[u]; OZ[base + slot] = fz[u];
+        if (ids) oids[base + slot] = ids[my_i[u]];
+        s_vx[slot] = fvx[u]; s_vy[slot] = fvy[u]; s_vz[slot] = fvz[u];
+        s_cy[slot] = (int8_t)my_cy[u];
+        s_lcs[slot] = (uint8_t)lc;
+        if (AMC) {
+            s_lx[AMC ? slot : 0] = my_l[AMC ? u : 0][0];
+            s_ly[AMC ? slot : 0] = my_l[AMC ? u : 0][1];
+            s_lz[AMC ? slot : 0] = my_l[AMC ? u : 0][2];
+        }
+    }
     /* a monomer's coordinates inside its cell (shifted, never wrapped) */
     auto mono_local = [&](uint32_t mi, double& lx, double& ly, double& lz) {
         const double qx = d.mpos[3 * mi] + cc.shift[0] + b.half[0];
@@ -1943,15 +1866,16 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
         const double qz = d.mpos[3 * mi + 2] + cc.shift[2] + b.half[2];
         lx = qx - (double)(int)qx; ly = qy - (double)(int)qy; lz = qz - (double)(int)qz;
     };
+    /* MPC::calculateCMV (MPC.cpp:314-322): the monomers' m v and m */
     for (uint32_t j = tid; j < nmt; j += NT) {
         const uint32_t e = s_ml[j];
         const uint32_t lc = e >> 16;
         const uint32_t mi = rep.mono_base + (e & 0xFFFFu);
         const double mm = d.mmass[mi];
-        atomicAdd(&s_mq[0][lc], to_fix(d.mvel[3 * mi] * mm));
-        atomicAdd(&s_mq[1][lc], to_fix(d.mvel[3 * mi + 1] * mm));
-        atomicAdd(&s_mq[2][lc], to_fix(d.mvel[3 * mi + 2] * mm));
-        atomicAdd(&s_mq[3][lc], to_fix(mm));
+        atomicAdd(&s_p[0][lc], to_fix(d.mvel[3 * mi] * mm));
+        atomicAdd(&s_p[1][lc], to_fix(d.mvel[3 * mi + 1] * mm));
+        atomicAdd(&s_p[2][lc], to_fix(d.mvel[3 * mi + 2] * mm));
+        atomicAdd(&s_mm[lc], to_fix(mm));
         if (amc) {
             double lx, ly, lz;
             mono_local(mi, lx, ly, lz);
@@ -1961,31 +1885,16 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
         }
     }
     __syncthreads();
-    if (tid == 0 && s_bad) atomicOr(&ctl->status, ST_SORT_MISMATCH);
 
-    /* (e) */
-    const bool has_m = nmt > 0u;
-    /* MBS scale sqrt(Gamma / E_rel) for n >= 2 (MPC.cpp:287-289); with the
-     * angular-momentum flag, omega = s I^-1 C for n > 2 (MPC.cpp:260, 373) */
-    auto finalize = [&](uint32_t ci) {
-        const uint32_t n = s_cnt[ci] + s_nm[ci];
-        s_g[ci] = n >= 2 ? sqrt(s_g[ci] / (from_fix(s_e[ci]) * 0.5)) : -1.0;
-        if (amc) {
-            double w[3] = {0.0, 0.0, 0.0};
-            if (n > 2) {
-                const double A[3][3] = {{from_fix(s_am[3][ci]), from_fix(s_am[6][ci]), from_fix(s_am[7][ci])},
-                                        {from_fix(s_am[6][ci]), from_fix(s_am[4][ci]), from_fix(s_am[8][ci])},
-                                        {from_fix(s_am[7][ci]), from_fix(s_am[8][ci]), from_fix(s_am[5][ci])}};
-                const double c3[3] = {from_fix(s_am[9][ci]), from_fix(s_am[10][ci]), from_fix(s_am[11][ci])};
-                llt_solve3(A, c3, w);
-                const double sc = s_g[ci] >= 0.0 ? s_g[ci] : 1.0;
-                const bool ok = isfinite(w[0]) && isfinite(w[1]) && isfinite(w[2]);
-                for (int q = 0; q < 3; ++q) w[q] = ok ? w[q] * sc : 0.0;
-            }
-            s_w[ci][0] = w[0]; s_w[ci][1] = w[1]; s_w[ci][2] = w[2];
-        }
+    /* (e) LPC consecutive lanes per cell, each summing every LPC-th slot of
+     * it; integer sums, so the split does not change a bit */
+    const uint32_t ci = (uint32_t)tid / LPC, sub = (uint32_t)tid % LPC;
+    auto group_sum = [&](unsigned long long v) {
+#pragma unroll
+        for (int o = 1; o < LPC; o <<= 1) v += __shfl_xor(v, o);
+        return v;
     };
-    /* the angular-momentum sums of one member (MPC::calculateAngular
+    /* the angular-momentum terms of one member (MPC::calculateAngular
      * MPC.cpp:344-372): I about CMP and C = m r x (u - R u) */
     auto amc_terms = [&](uint32_t lc, double m, double lx, double ly, double lz, double ux, double uy, double uz,
                          unsigned long long* acc) {
@@ -2004,62 +1913,70 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
         acc[7] = to_fix(m * (rz * qx - rx * qz));
         acc[8] = to_fix(m * (rx * qy - ry * qx));
     };
-    for (uint32_t ci = tid; ci < nc; ci += NT) {
-        const uint32_t a = s_off[ci], n = s_cnt[ci];
-        /* MPC::calculateCMV (MPC.cpp:314-322) */
-        unsigned long long qx = s_mq[0][ci], qy = s_mq[1][ci], qz = s_mq[2][ci];
-        unsigned long long l0 = 0ull, l1 = 0ull, l2 = 0ull;
-        if (amc) { l0 = s_am[0][ci]; l1 = s_am[1][ci]; l2 = s_am[2][ci]; }
-        for (uint32_t q = a; q < a + n; ++q) {
-            qx += to_fix((double)s_vx[q] - (double)s_cy[q] * shear * LyD);
-            qy += to_fix((double)s_vy[q]);
-            qz += to_fix((double)s_vz[q]);
+    const bool has_m = nmt > 0u;
+    const bool live = ci < nc;
+    const uint32_t a0 = live ? s_off[ci] : 0u, a1 = live ? s_off[ci] + s_cnt[ci] : 0u;
+    double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+    unsigned long long e = 0ull;
+    {
+        /* MPC::calculateCMV */
+        unsigned long long q0 = 0ull, q1 = 0ull, q2 = 0ull, l0 = 0ull, l1 = 0ull, l2 = 0ull;
+        for (uint32_t q = a0 + sub; q < a1; q += LPC) {
+            q0 += to_fix((double)s_vx[q] - (double)s_cy[q] * shear * LyD);
+            q1 += to_fix((double)s_vy[q]);
+            q2 += to_fix((double)s_vz[q]);
             if (amc) {
-                l0 += to_fix(1.0 * (double)s_lx[q]);
-                l1 += to_fix(1.0 * (double)s_ly[q]);
-                l2 += to_fix(1.0 * (double)s_lz[q]);
+                l0 += to_fix(1.0 * (double)s_lx[AMC ? q : 0]);
+                l1 += to_fix(1.0 * (double)s_ly[AMC ? q : 0]);
+                l2 += to_fix(1.0 * (double)s_lz[AMC ? q : 0]);
             }
         }
-        const double M = (double)n + from_fix(s_mq[3][ci]);
-        double c0 = from_fix(qx), c1 = from_fix(qy), c2 = from_fix(qz);
-        if (M > 0) { c0 /= M; c1 /= M; c2 /= M; }
-        s_cmv[ci][0] = c0; s_cmv[ci][1] = c1; s_cmv[ci][2] = c2;
-        if (amc) {  /* MPC::calculateCMP MPC.cpp:324-333, inside the cell */
-            const double inv = M > 0 ? 1.0 / M : 0.0;
-            s_cmp[ci][0] = from_fix(l0) * inv;
-            s_cmp[ci][1] = from_fix(l1) * inv;
-            s_cmp[ci][2] = from_fix(l2) * inv;
-        }
-        /* MPC::thermostat relative kinetic energy (MPC.cpp:277-279) */
-        unsigned long long e = 0ull;
-        unsigned long long am[9] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
-        for (uint32_t q = a; q < a + n; ++q) {
-            const double vx = (double)s_vx[q] - (double)s_cy[q] * shear * LyD;
-            const double dx = vx - c0, dy = (double)s_vy[q] - c1, dz = (double)s_vz[q] - c2;
-            e += to_fix((dx * dx + dy * dy) + dz * dz);
-            if (amc) {
-                unsigned long long t9[9];
-                amc_terms(ci, 1.0, s_lx[q], s_ly[q], s_lz[q], dx, dy, dz, t9);
-#pragma unroll
-                for (int j = 0; j < 9; ++j) am[j] += t9[j];
+        q0 = group_sum(q0); q1 = group_sum(q1); q2 = group_sum(q2);
+        if (amc) { l0 = group_sum(l0); l1 = group_sum(l1); l2 = group_sum(l2); }
+        if (live) {
+            q0 += s_p[0][ci]; q1 += s_p[1][ci]; q2 += s_p[2][ci];
+            const double M = (double)s_cnt[ci] + from_fix(s_mm[ci]);
+            c0 = from_fix(q0); c1 = from_fix(q1); c2 = from_fix(q2);
+            if (M > 0) { c0 /= M; c1 /= M; c2 /= M; }
+            if (sub == 0) { s_cmv[ci][0] = c0; s_cmv[ci][1] = c1; s_cmv[ci][2] = c2; }
+            if (amc && sub == 0) {  /* MPC::calculateCMP MPC.cpp:324-333, inside the cell */
+                const double inv = M > 0 ? 1.0 / M : 0.0;
+                s_cmp[ci][0] = from_fix(l0 + s_am[0][ci]) * inv;
+                s_cmp[ci][1] = from_fix(l1 + s_am[1][ci]) * inv;
+                s_cmp[ci][2] = from_fix(l2 + s_am[2][ci]) * inv;
             }
         }
-        s_e[ci] = e;
-        if (amc)
-#pragma unroll
-            for (int j = 0; j < 9; ++j) s_am[3 + j][ci] = am[j];
-        if (!has_m) finalize(ci);
     }
+    if (amc) __syncthreads();  /* CMP of every cell before the inertia sums */
+    unsigned long long am[AMC ? 9 : 1];
+#pragma unroll
+    for (int j = 0; j < (AMC ? 9 : 1); ++j) am[j] = 0ull;
+    /* MPC::thermostat relative kinetic energy (MPC.cpp:277-279) */
+    for (uint32_t q = a0 + sub; q < a1; q += LPC) {
+        const double vx = (double)s_vx[q] - (double)s_cy[q] * shear * LyD;
+        const double dx = vx - c0, dy = (double)s_vy[q] - c1, dz = (double)s_vz[q] - c2;
+        e += to_fix((dx * dx + dy * dy) + dz * dz);
+        if (amc) {
+            unsigned long long t9[9];
+            amc_terms(ci, 1.0, s_lx[AMC ? q : 0], s_ly[AMC ? q : 0], s_lz[AMC ? q : 0], dx, dy, dz, t9);
+#pragma unroll
+            for (int j = 0; j < (AMC ? 9 : 1); ++j) am[j] += t9[j];
+        }
+    }
+    e = group_sum(e);
+    if (amc)
+#pragma unroll
+        for (int j = 0; j < (AMC ? 9 : 1); ++j) am[j] = group_sum(am[j]);
     if (has_m) {
-        __syncthreads();
+        __syncthreads();  /* every CMV before the monomers' energies */
         for (uint32_t j = tid; j < nmt; j += NT) {
-            const uint32_t e = s_ml[j];
-            const uint32_t lc = e >> 16;
-            const uint32_t mi = rep.mono_base + (e & 0xFFFFu);
+            const uint32_t me = s_ml[j];
+            const uint32_t lc = me >> 16;
+            const uint32_t mi = rep.mono_base + (me & 0xFFFFu);
             const double dx = d.mvel[3 * mi] - s_cmv[lc][0];
             const double dy = d.mvel[3 * mi + 1] - s_cmv[lc][1];
             const double dz = d.mvel[3 * mi + 2] - s_cmv[lc][2];
-            atomicAdd(&s_e[lc], to_fix(d.mmass[mi] * ((dx * dx + dy * dy) + dz * dz)));
+            atomicAdd(&s_me[lc], to_fix(d.mmass[mi] * ((dx * dx + dy * dy) + dz * dz)));
             if (amc) {
                 double lx, ly, lz;
                 mono_local(mi, lx, ly, lz);
@@ -2069,25 +1986,52 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
             }
         }
         __syncthreads();
-        for (uint32_t ci = tid; ci < nc; ci += NT) finalize(ci);
+    }
+    /* MBS scale sqrt(Gamma / E_rel) for n >= 2 (MPC.cpp:287-289); with the
+     * angular-momentum flag, omega = s I^-1 C for n > 2 (MPC.cpp:260, 373) */
+    if (live && sub == 0) {
+        const uint32_t nn = s_cnt[ci] + s_nm[ci];
+        double g = 0.0;  /* the cell's Gamma variate (depends only on its member count) */
+        if (nn >= 2)
+            g = inj ? d.inj_gamma[gc0 + ci]
+                    : gamma_mt(rep.seed, rep.replica, T.c_lo + ci, k, 1.5 * (nn - 1), rep.kT);
+        const double sc = nn >= 2 ? sqrt(g / (from_fix(e + s_me[ci]) * 0.5)) : -1.0;
+        s_g[ci] = sc;
+        if (amc) {
+            double w[3] = {0.0, 0.0, 0.0};
+            if (nn > 2) {
+                unsigned long long A6[9];
+#pragma unroll
+                for (int j = 0; j < 9; ++j) A6[j] = am[AMC ? j : 0] + s_am[3 + j][ci];
+                const double A[3][3] = {{from_fix(A6[0]), from_fix(A6[3]), from_fix(A6[4])},
+                                        {from_fix(A6[3]), from_fix(A6[1]), from_fix(A6[5])},
+                                        {from_fix(A6[4]), from_fix(A6[5]), from_fix(A6[2])}};
+                const double c3[3] = {from_fix(A6[6]), from_fix(A6[7]), from_fix(A6[8])};
+                llt_solve3(A, c3, w);
+                const double scl = sc >= 0.0 ? sc : 1.0;
+                const bool ok = isfinite(w[0]) && isfinite(w[1]) && isfinite(w[2]);
+                for (int q = 0; q < 3; ++q) w[q] = ok ? w[q] * scl : 0.0;
+            }
+            s_w[ci][0] = w[0]; s_w[ci][1] = w[1]; s_w[ci][2] = w[2];
+        }
     }
     __syncthreads();
 
     /* (f) v <- s v + (1-s) CMV, then v <- CMV + R (v - CMV) (MPC.cpp:290-292,
      * 266-267); the image offset is undone as the unshift wrap does */
     auto apply = [&](uint32_t lc, double& vx, double& vy, double& vz) {
-        const double c0 = s_cmv[lc][0], c1 = s_cmv[lc][1], c2 = s_cmv[lc][2];
+        const double m0 = s_cmv[lc][0], m1 = s_cmv[lc][1], m2 = s_cmv[lc][2];
         const double sc = s_g[lc];
         if (sc >= 0.0) {
-            vx = sc * vx + (1 - sc) * c0;
-            vy = sc * vy + (1 - sc) * c1;
-            vz = sc * vz + (1 - sc) * c2;
+            vx = sc * vx + (1 - sc) * m0;
+            vy = sc * vy + (1 - sc) * m1;
+            vz = sc * vz + (1 - sc) * m2;
         }
         const float* R = s_R[lc];
-        const double d0 = vx - c0, d1 = vy - c1, d2 = vz - c2;
-        vx = c0 + (((double)R[0] * d0 + (double)R[1] * d1) + (double)R[2] * d2);
-        vy = c1 + (((double)R[3] * d0 + (double)R[4] * d1) + (double)R[5] * d2);
-        vz = c2 + (((double)R[6] * d0 + (double)R[7] * d1) + (double)R[8] * d2);
+        const double d0 = vx - m0, d1 = vy - m1, d2 = vz - m2;
+        vx = m0 + (((double)R[0] * d0 + (double)R[1] * d1) + (double)R[2] * d2);
+        vy = m1 + (((double)R[3] * d0 + (double)R[4] * d1) + (double)R[5] * d2);
+        vz = m2 + (((double)R[6] * d0 + (double)R[7] * d1) + (double)R[8] * d2);
     };
     /* v += omega x (x - CMP) (MPC.cpp:268) */
     auto add_w = [&](uint32_t lc, double lx, double ly, double lz, double& vx, double& vy, double& vz) {
@@ -2097,26 +2041,25 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
         vy += w2 * rx - w0 * rz;
         vz += w0 * ry - w1 * rx;
     };
-    for (uint32_t q = tid; q < nsel; q += NT) {
+    for (uint32_t q = tid; q < n; q += NT) {
         const uint32_t lc = s_lcs[q];
         const double offv = (double)s_cy[q] * shear * LyD;
         double vx = (double)s_vx[q] - offv, vy = s_vy[q], vz = s_vz[q];
         apply(lc, vx, vy, vz);
-        if (amc) add_w(lc, s_lx[q], s_ly[q], s_lz[q], vx, vy, vz);
+        if (amc) add_w(lc, s_lx[AMC ? q : 0], s_ly[AMC ? q : 0], s_lz[AMC ? q : 0], vx, vy, vz);
         vx += offv;
         OVX[base + q] = (float)vx; OVY[base + q] = (float)vy; OVZ[base + q] = (float)vz;
     }
     uint32_t* offn = d.offs[alt];
-    for (uint32_t ci = tid; ci < nc; ci += NT) offn[gc0 + ci] = base + s_off[ci];
+    for (uint32_t c = tid; c < nc; c += NT) offn[gc0 + c] = base + s_off[c];
     if (tid == 0) {
-        if (T.c_lo + nc == rep.ncells) offn[rep.cell_base + rep.ncells] = base + nsel;
-        d.tcount[alt][lt] = 0u;
-        if (T.ty == 0) d.pcount[alt][rep.pl_base + (uint32_t)T.z] = 0u;
+        if (T.c_lo + nc == rep.ncells) offn[rep.cell_base + rep.ncells] = base + n;
+        d.bcount[alt][lt] = 0u;
     }
     for (uint32_t j = tid; j < nmt; j += NT) {
-        const uint32_t e = s_ml[j];
-        const uint32_t lc = e >> 16;
-        const uint32_t mi = rep.mono_base + (e & 0xFFFFu);
+        const uint32_t me = s_ml[j];
+        const uint32_t lc = me >> 16;
+        const uint32_t mi = rep.mono_base + (me & 0xFFFFu);
         double vx = d.mvel[3 * mi], vy = d.mvel[3 * mi + 1], vz = d.mvel[3 * mi + 2];
         apply(lc, vx, vy, vz);
         if (amc) {
@@ -2659,9 +2602,8 @@ __global__ __launch_bounds__(256) void k_fluid_observe(DevState d, ObsArgs a) {
 
 /* ========================================================= launchers */
 
-hipError_t launch_prologue(const DevState& d, hipStream_t st, int do_collision, int n_delrx, double delrx_dt,
-                           double h) {
-    hipLaunchKernelGGL(k_prologue, dim3(d.n_rep), dim3(64), 0, st, d, do_collision, n_delrx, delrx_dt, h);
+hipError_t launch_prologue(const DevState& d, hipStream_t st, int do_collision, int n_delrx, double delrx_dt) {
+    hipLaunchKernelGGL(k_prologue, dim3(d.n_rep), dim3(64), 0, st, d, do_collision, n_delrx, delrx_dt);
     return hipGetLastError();
 }
 
@@ -2751,12 +2693,24 @@ hipError_t launch_collide(const DevState& d, hipStream_t st, double h) {
 
 hipError_t launch_advect(const DevState& d, hipStream_t st, double h, int mode) {
     if (!d.rt_cap || !d.n_rtiles) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_advect, dim3(d.n_rtiles), dim3(256), 0, st, d, h, mode);
+    switch (d.ppt) {
+    case 1: hipLaunchKernelGGL(k_advect<1>, dim3(d.n_pblk), dim3(256), 0, st, d, h, mode); break;
+    case 2: hipLaunchKernelGGL(k_advect<2>, dim3(d.n_pblk), dim3(256), 0, st, d, h, mode); break;
+    case 4: hipLaunchKernelGGL(k_advect<4>, dim3(d.n_pblk), dim3(256), 0, st, d, h, mode); break;
+    case 8: hipLaunchKernelGGL(k_advect<8>, dim3(d.n_pblk), dim3(256), 0, st, d, h, mode); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_rows_redo(const DevState& d, hipStream_t st, double h) {
-    hipLaunchKernelGGL(k_rows_redo, dim3(256), dim3(256), 0, st, d, h);
+    switch (d.ppt) {
+    case 1: hipLaunchKernelGGL(k_rows_redo<1>, dim3(256), dim3(256), 0, st, d, h); break;
+    case 2: hipLaunchKernelGGL(k_rows_redo<2>, dim3(256), dim3(256), 0, st, d, h); break;
+    case 4: hipLaunchKernelGGL(k_rows_redo<4>, dim3(256), dim3(256), 0, st, d, h); break;
+    case 8: hipLaunchKernelGGL(k_rows_redo<8>, dim3(256), dim3(256), 0, st, d, h); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -2770,8 +2724,10 @@ static void launch_collide_rows_t(const DevState& d, hipStream_t st, double h) {
 
 hipError_t launch_collide_rows(const DevState& d, hipStream_t st, double h) {
     if (!d.n_rtiles) return hipErrorInvalidValue;
-    if (d.rt_cap == 1024 && d.rt_tcm == 128) launch_collide_rows_t<256, 1024, 128>(d, st, h);
+    if (d.rt_cap == 1024 && d.rt_tcm == 64) launch_collide_rows_t<256, 1024, 64>(d, st, h);
+    else if (d.rt_cap == 1024 && d.rt_tcm == 128) launch_collide_rows_t<256, 1024, 128>(d, st, h);
     else if (d.rt_cap == 1024 && d.rt_tcm == 256) launch_collide_rows_t<256, 1024, 256>(d, st, h);
+    else if (d.rt_cap == 2048 && d.rt_tcm == 64) launch_collide_rows_t<512, 2048, 64>(d, st, h);
     else if (d.rt_cap == 2048 && d.rt_tcm == 128) launch_collide_rows_t<512, 2048, 128>(d, st, h);
     else if (d.rt_cap == 2048 && d.rt_tcm == 256) launch_collide_rows_t<512, 2048, 256>(d, st, h);
     else return hipErrorInvalidValue;

@@ -90,6 +90,7 @@ struct Box {
     double L[3];       /* even-rounded box (Box.cpp:14-19) */
     double invL[3];
     double half[3];    /* L*0.5 */
+    double inner[3];   /* L*0.5*(1 - 1e-9): inside it le_wrap is the identity */
     int iL[3];
     double delrx;      /* Lees-Edwards x offset of the y images */
     double shear;
@@ -99,6 +100,17 @@ struct Box {
 /* MPC::LEBC(Particle&) MPC.cpp:421-429 / plain wrap MPC.cpp:473-478, chosen
  * like MPC::wrap(Particle&) MPC.cpp:487-490.  Returns the y image index cy. */
 SPMD_HD double le_wrap(const Box& b, double& x, double& y, double& z, double& vx) {
+    /* Inside the central image every round_div below is a signed zero (the
+     * margin 1e-9 keeps q = c/L away from 1/2 by far more than its
+     * rounding), so the wrap only turns -0.0 into +0.0 and subtracts
+     * sign(y) 0 shear L from vx: the same bits without the divisions. */
+    if (fabs(x) < b.inner[0] && fabs(y) < b.inner[1] && fabs(z) < b.inner[2]) {
+        if (b.shear_on) vx -= copysign(0.0, y) * b.shear * b.L[1];
+        x += 0.0;
+        y += 0.0;
+        z += 0.0;
+        return 0.0;
+    }
     if (b.shear_on) {
         const double cy = round_div(y, b.L[1], b.invL[1]);
         x -= b.L[0] * round_div(x, b.L[0], b.invL[0]);

@@ -342,10 +342,6 @@ class MPCSystem:
         """PATH_AUTO (row tiles once the buffer is in cell order) or PATH_SORT."""
         self._chk(self._lib.spmd_set_collision_path(self.h, path))
 
-    def set_row_window(self, speed: float):
-        """Speed bound (units of sqrt(kT)) of the row-tile source windows."""
-        self._chk(self._lib.spmd_set_row_window(self.h, float(speed)))
-
     def collision_path(self) -> int:
         """The path the next collision takes: PATH_SORT or PATH_ROWS."""
         a = C.c_int()

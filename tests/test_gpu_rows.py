@@ -4,8 +4,7 @@ against the sort path (k_bin + k_scan + k_perm + k_collide) and the oracle.
 Both paths evaluate every particle and cell with the same expressions and
 sum the cells in 64-bit fixed point, so from the same state they must give
 the same bits: particle positions and velocities (matched by id), monomer
-state and cell indices, over many collisions -- including the outlier list
-(particles that leave their tile's source-row window), the recentring redo
+state and cell indices, over many collisions -- including the recentring redo
 after an MD check_bounds, the angular-momentum collision, batches of
 uneven replicas and densities far from 10.  The oracle check runs one
 row-tile collision of the L=64 star workload with injected randoms and
@@ -44,15 +43,13 @@ def _same_state(a, b, n_rep=1, star=False):
         assert np.array_equal(ma, mb)
 
 
-def _twins(params, star=None, window=None):
+def _twins(params, star=None):
     """Two handles from the same state: a on the automatic path, b on the
     sort path."""
     out = []
     for path in (MPCSystem.PATH_AUTO, MPCSystem.PATH_SORT):
         g = MPCSystem(params)
         g.set_collision_path(path)
-        if window is not None:
-            g.set_row_window(window)
         if star:
             g.add_star(*star)
             g.md_init()
@@ -89,21 +86,6 @@ def test_rows_equal_sort_path_star_graph():
     a.sync()
     b.sync()
     _same_state(a, b, star=True)
-
-
-@pytest.mark.parametrize("window", [0.0, 1.0])
-def test_rows_outlier_list_equal_sort_path(window):
-    """A narrow source window sends many particles (window 0: every
-    particle whose row moves by more than the grid shift) through the
-    outlier list; the result does not change."""
-    p = make_params(L=(16, 12, 20), n_c=10, shear=0.05, step_update=1, seed=8)
-    a, b = _twins(p, window=window)
-    for t in range(25):
-        a.step(t, 0.1)
-        b.step(t, 0.1)
-    a.sync()
-    b.sync()
-    _same_state(a, b)
 
 
 def test_rows_recentring_equal_sort_path():
