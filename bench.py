@@ -502,6 +502,8 @@ def main():
     ap.add_argument("--cpu-intervals", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (ppt 4, 128-cell tiles)")
+    ap.add_argument("--tuning", type=int, nargs=2, default=None, metavar=("PPT", "TILE_CELLS"),
+                    help="use this launch shape instead of the warm-up sweep")
     ap.add_argument("--sort-path", action="store_true",
                     help="time the sort path (bin, scan, permutation, collide) instead of the row-tile path")
     ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure throughput section")
@@ -589,7 +591,9 @@ def main():
     g.run(t, 1, h)  # t=0: first collision
     t += 1
     tune = None
-    if not a.no_tune:  # launch-shape sweep inside the (untimed) warm-up
+    if a.tuning:
+        g.set_tuning(*a.tuning)
+    elif not a.no_tune:  # launch-shape sweep inside the (untimed) warm-up
         from starpolymersmd_amd.autotune import autotune
 
         tune = autotune(g, t, h, cache=None)

@@ -99,6 +99,70 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+/* gamma_mt's variate with FP32 hardware transcendentals: the same Philox
+ * words and the same Marsaglia-Tsang attempts, the normal (Box-Muller: one
+ * v_log_f32, one v_cos_f32 in revolutions) and both acceptance tests in
+ * FP32.  A test whose two sides lie closer than 1e-3 (the FP32 evaluation
+ * is good to about 3e-4 in the worst case, |x| = 6.8; typically 1e-6) is
+ * decided again by gamma_attempt in FP64, so the accepted attempt is always
+ * gamma_mt's and the variate d v^3 agrees with it to the FP32 rounding of
+ * v^3 (a few 1e-7 relative).  The FP64 transcendentals were the largest
+ * single cost of a collision. */
+__device__ __forceinline__ double gamma_fast(uint64_t seed, uint32_t replica, uint32_t cell, uint32_t k,
+                                             double shape, double scale) {
+    const double d = shape - 1.0 / 3.0;
+    const double c = 1.0 / sqrt(9.0 * d);
+    const float df = (float)d, cf = (float)c;
+    constexpr float kLn2 = 0.693147180559945f;
+    constexpr float k2m32 = 2.3283064365386963e-10f;
+    for (uint32_t j = 0; j < 64; ++j) {
+        uint32_t w[4];
+        draw4(seed, replica, TAG_GAMMA, cell, k, j, w);
+        const float u0 = ((float)w[0] + 0.5f) * k2m32;
+        const float u1 = ((float)w[1] + 0.5f) * k2m32;
+        const float u2 = ((float)w[2] + 0.5f) * k2m32;
+        const float x = __builtin_sqrtf(-2.0f * kLn2 * __builtin_amdgcn_logf(u0)) * __builtin_amdgcn_cosf(u1);
+        const float v = 1.0f + cf * x;
+        const float v3 = v * v * v, x2 = x * x;
+        const float t1 = 1.0f - 0.0331f * (x2 * x2);
+        if (fabsf(v) >= 1e-3f && fabsf(u2 - t1) >= 1e-3f) {
+            if (v <= 0.0f) continue;
+            if (u2 < t1) return d * (double)v3 * scale;
+            const float lhs = kLn2 * __builtin_amdgcn_logf(u2);
+            const float rhs = 0.5f * x2 + df * (1.0f - v3 + kLn2 * __builtin_amdgcn_logf(v3));
+            if (fabsf(lhs - rhs) >= 1e-3f * (1.0f + fabsf(rhs))) {
+                if (lhs < rhs) return d * (double)v3 * scale;
+                continue;
+            }
+        }
+        double vv;
+        if (gamma_attempt(w, d, c, vv)) return d * vv * scale;
+    }
+    return d * scale;
+}
+
+/* The SRD rotation matrix about the axis (MPC::collide MPC.cpp:241-257)
+ * phi = 2 pi u1, cos(theta) = 2 (u2 - 1/2), stored as FP32.  The sine and
+ * cosine of phi are the FP32 hardware ones in revolutions (a few 1e-7
+ * absolute); sin(theta) = sqrt(1 - cos^2) and the entries stay FP64, since
+ * 1 - cos^2 cancels near the poles. */
+__device__ __forceinline__ void rotation_f32(double u1, double u2, double c, double s, float* R) {
+    const double th = 2. * (u2 - 0.5);
+    const double st = sqrt(1 - th * th);
+    const double a0 = st * (double)__builtin_amdgcn_cosf((float)u1);
+    const double a1 = st * (double)__builtin_amdgcn_sinf((float)u1);
+    const double a2 = th;
+    R[0] = (float)(a0 * a0 + (1 - a0 * a0) * c);
+    R[1] = (float)(a0 * a1 * (1 - c) - a2 * s);
+    R[2] = (float)(a0 * a2 * (1 - c) + a1 * s);
+    R[3] = (float)(a0 * a1 * (1 - c) + a2 * s);
+    R[4] = (float)(a1 * a1 + (1 - a1 * a1) * c);
+    R[5] = (float)(a1 * a2 * (1 - c) - a0 * s);
+    R[6] = (float)(a0 * a2 * (1 - c) - a1 * s);
+    R[7] = (float)(a1 * a2 * (1 - c) + a0 * s);
+    R[8] = (float)(a2 * a2 + (1 - a2 * a2) * c);
+}
+
 /* Sum over the workgroup in a fixed order (deterministic). */
 template <int NT>
 __device__ double block_sum(double v, double* scratch) {
@@ -1152,7 +1216,7 @@ __global__ __launch_bounds__(256) void k_collide(DevState d, double h) {
         const uint32_t n = (s_off[ci + 1] - s_off[ci]) + s_nm[ci];
         double g = 0.0;
         if (n >= 2)
-            g = inj ? d.inj_gamma[gc0 + ci] : gamma_mt(rep.seed, rep.replica, cell, k, 1.5 * (n - 1), rep.kT);
+            g = inj ? d.inj_gamma[gc0 + ci] : gamma_fast(rep.seed, rep.replica, cell, k, 1.5 * (n - 1), rep.kT);
         s_g[ci] = g;
         double u1, u2;
         if (inj) {
@@ -1164,22 +1228,7 @@ __global__ __launch_bounds__(256) void k_collide(DevState d, double h) {
             u1 = u01(w[0]);
             u2 = u01(w[1]);
         }
-        const double phi = 2. * kPi * u1;
-        const double th = 2. * (u2 - 0.5);
-        const double a0 = sqrt(1 - th * th) * cos(phi);
-        const double a1 = sqrt(1 - th * th) * sin(phi);
-        const double a2 = th;
-        const double cs = rep.c, ss = rep.s;
-        float* R = s_R[ci];
-        R[0] = (float)(a0 * a0 + (1 - a0 * a0) * cs);
-        R[1] = (float)(a0 * a1 * (1 - cs) - a2 * ss);
-        R[2] = (float)(a0 * a2 * (1 - cs) + a1 * ss);
-        R[3] = (float)(a0 * a1 * (1 - cs) + a2 * ss);
-        R[4] = (float)(a1 * a1 + (1 - a1 * a1) * cs);
-        R[5] = (float)(a1 * a2 * (1 - cs) - a0 * ss);
-        R[6] = (float)(a0 * a2 * (1 - cs) - a1 * ss);
-        R[7] = (float)(a1 * a2 * (1 - cs) + a0 * ss);
-        R[8] = (float)(a2 * a2 + (1 - a2 * a2) * cs);
+        rotation_f32(u1, u2, rep.c, rep.s, s_R[ci]);
     }
 
     /* slot q of the tile: its particle streamed (MPC::streaming + LEBC, as
@@ -1600,6 +1649,11 @@ __global__ __launch_bounds__(256) void k_rows_redo(DevState d, double h) {
                 carry += tot;
             }
         }
+    }
+    /* the monomer cells in other workgroups than the scans (they overlap) */
+    const int G = (int)gridDim.x;
+    for (int r = ((int)blockIdx.x - d.n_rep % G + G) % G; r < d.n_rep; r += G) {
+        const RepDev& rep = d.reps[r];
         const double sh0 = d.ctl[r].shift[0], sh1 = d.ctl[r].shift[1], sh2 = d.ctl[r].shift[2];
         bool oob = false;
         for (uint32_t i = threadIdx.x; i < rep.n_mono; i += 256) {
@@ -1634,12 +1688,12 @@ __global__ __launch_bounds__(256) void k_rows_redo(DevState d, double h) {
  *   (b) its bucket: particle indices, the particles gathered, streamed (as
  *       k_advect), their cell and rank in the cell (LDS atomic)
  *   (c) exclusive scan of the cell counts (the rotations were drawn in (a))
- *   (d) per particle: position (and id) to the new buffer at its slot,
- *       velocity and image index staged in LDS at the slot
+ *   (d) per cell (one lane each) the Gamma variate; per particle: position
+ *       (and id) to the new buffer at its slot, velocity and image index
+ *       staged in LDS at the slot
  *   (e) per cell, LPC lanes: momentum sums over its slots + monomers ->
- *       CMV, relative kinetic energy, then one lane the Gamma variate ->
- *       MBS scale (with angular momentum: CMP, inertia tensor, C, omega);
- *       the particles' registers are free by then
+ *       CMV, relative kinetic energy -> MBS scale (with angular momentum:
+ *       CMP, inertia tensor, C, omega)
  *   (f) per slot: v <- CMV + s R (v - CMV) (+ omega x r), written coalesced;
  *       cell offsets of the new buffer; monomer velocities; the bucket
  *       counts of the next collision cleared. */
@@ -1743,22 +1797,7 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
             u1 = u01(w[0]);
             u2 = u01(w[1]);
         }
-        const double phi = 2. * kPi * u1;
-        const double th = 2. * (u2 - 0.5);
-        const double a0 = sqrt(1 - th * th) * cos(phi);
-        const double a1 = sqrt(1 - th * th) * sin(phi);
-        const double a2 = th;
-        const double cs = rep.c, ss = rep.s;
-        float* R = s_R[ci];
-        R[0] = (float)(a0 * a0 + (1 - a0 * a0) * cs);
-        R[1] = (float)(a0 * a1 * (1 - cs) - a2 * ss);
-        R[2] = (float)(a0 * a2 * (1 - cs) + a1 * ss);
-        R[3] = (float)(a0 * a1 * (1 - cs) + a2 * ss);
-        R[4] = (float)(a1 * a1 + (1 - a1 * a1) * cs);
-        R[5] = (float)(a1 * a2 * (1 - cs) - a0 * ss);
-        R[6] = (float)(a0 * a2 * (1 - cs) - a1 * ss);
-        R[7] = (float)(a1 * a2 * (1 - cs) + a0 * ss);
-        R[8] = (float)(a2 * a2 + (1 - a2 * a2) * cs);
+        rotation_f32(u1, u2, rep.c, rep.s, s_R[ci]);
     }
     /* uniform exits, status set: the tile cannot hold its particles */
     if (n > (uint32_t)CAP) {
@@ -1858,6 +1897,16 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
             s_ly[AMC ? slot : 0] = my_l[AMC ? u : 0][1];
             s_lz[AMC ? slot : 0] = my_l[AMC ? u : 0][2];
         }
+    }
+    /* per cell, one lane each (a whole tile of 64 cells is one wave), once
+     * the particles' registers are free: the Gamma variate, which depends
+     * only on the member count */
+    for (uint32_t c = tid; c < nc; c += NT) {
+        const uint32_t nn = s_cnt[c] + s_nm[c];
+        double g = 0.0;
+        if (nn >= 2)
+            g = inj ? d.inj_gamma[gc0 + c] : gamma_fast(rep.seed, rep.replica, T.c_lo + c, k, 1.5 * (nn - 1), rep.kT);
+        s_g[c] = g;
     }
     /* a monomer's coordinates inside its cell (shifted, never wrapped) */
     auto mono_local = [&](uint32_t mi, double& lx, double& ly, double& lz) {
@@ -1991,11 +2040,7 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
      * angular-momentum flag, omega = s I^-1 C for n > 2 (MPC.cpp:260, 373) */
     if (live && sub == 0) {
         const uint32_t nn = s_cnt[ci] + s_nm[ci];
-        double g = 0.0;  /* the cell's Gamma variate (depends only on its member count) */
-        if (nn >= 2)
-            g = inj ? d.inj_gamma[gc0 + ci]
-                    : gamma_mt(rep.seed, rep.replica, T.c_lo + ci, k, 1.5 * (nn - 1), rep.kT);
-        const double sc = nn >= 2 ? sqrt(g / (from_fix(e + s_me[ci]) * 0.5)) : -1.0;
+        const double sc = nn >= 2 ? sqrt(s_g[ci] / (from_fix(e + s_me[ci]) * 0.5)) : -1.0;
         s_g[ci] = sc;
         if (amc) {
             double w[3] = {0.0, 0.0, 0.0};

@@ -52,6 +52,21 @@ SPMD_HD double u01(uint32_t w) { return ((double)w + 0.5) * 2.328306436538696289
 
 /* Gamma(shape >= 1, scale): Marsaglia & Tsang (ACM TOMS 26(3), 2000) with
  * Box-Muller normals; attempt j uses Philox block j of (cell, k, TAG_GAMMA). */
+/* One Marsaglia-Tsang attempt from the Philox words w (d = shape - 1/3,
+ * c = 1/sqrt(9d)): true with v^3 when it is accepted. */
+SPMD_HD bool gamma_attempt(const uint32_t w[4], double d, double c, double& v3) {
+    const double r = sqrt(-2.0 * log(u01(w[0])));
+    const double x = r * cos(6.283185307179586 * u01(w[1]));
+    double v = 1.0 + c * x;
+    if (v <= 0.0) return false;
+    v = v * v * v;
+    const double u = u01(w[2]);
+    const double x2 = x * x;
+    v3 = v;
+    if (u < 1.0 - 0.0331 * (x2 * x2)) return true;
+    return log(u) < 0.5 * x2 + d * (1.0 - v + log(v));
+}
+
 SPMD_HD double gamma_mt(uint64_t seed, uint32_t replica, uint32_t cell, uint32_t k, double shape,
                         double scale) {
     const double d = shape - 1.0 / 3.0;
@@ -59,15 +74,8 @@ SPMD_HD double gamma_mt(uint64_t seed, uint32_t replica, uint32_t cell, uint32_t
     for (uint32_t j = 0; j < 64; ++j) {
         uint32_t w[4];
         draw4(seed, replica, TAG_GAMMA, cell, k, j, w);
-        const double r = sqrt(-2.0 * log(u01(w[0])));
-        const double x = r * cos(6.283185307179586 * u01(w[1]));
-        double v = 1.0 + c * x;
-        if (v <= 0.0) continue;
-        v = v * v * v;
-        const double u = u01(w[2]);
-        const double x2 = x * x;
-        if (u < 1.0 - 0.0331 * (x2 * x2)) return d * v * scale;
-        if (log(u) < 0.5 * x2 + d * (1.0 - v + log(v))) return d * v * scale;
+        double v3;
+        if (gamma_attempt(w, d, c, v3)) return d * v3 * scale;
     }
     return d * scale;
 }
