@@ -615,12 +615,13 @@ def main():
     g.run(t, a.profile_steps * S, h)
     g.sync()
     t += a.profile_steps * S
-    prof = {k: g.profile_read(k)[0] for k in ("md", "prologue", "bin", "scan", "perm", "advect", "redo", "collide")}
+    prof = {k: g.profile_read(k)[0]
+            for k in ("md", "prologue", "bin", "scan", "perm", "advect", "gamma", "redo", "collide")}
     g.profile(False)
     rows = g.collision_path() == g.PATH_ROWS
-    if rows:  # row-tile path: advect, the redo kernel (monomer cells), row-window collide
-        pipe_ms = prof["advect"] + prof["redo"] + prof["collide"]
-        pipe_name = "collision pipeline k_advect+k_rows_redo+k_collide_rows"
+    if rows:  # row-tile path: advect, the cells' Gamma draws, the redo kernel, row-window collide
+        pipe_ms = prof["advect"] + prof["gamma"] + prof["redo"] + prof["collide"]
+        pipe_name = "collision pipeline k_advect+k_cell_gamma+k_rows_redo+k_collide_rows"
     else:
         pipe_ms = prof["bin"] + prof["scan"] + prof["perm"] + prof["collide"]
         pipe_name = "collision pipeline k_bin+k_scan+k_perm+k_collide"

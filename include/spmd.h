@@ -281,8 +281,8 @@ int spmd_get_collision_path(const spmd_sys* s, int* next_path);
 /* ---- timing (device events on the handle's stream) ---- */
 /* average duration in ms of each kernel class over the launches since the
  * last reset: names "md", "bin", "scan", "perm", "collide", "prologue",
- * and for the row-tile path "advect" and "redo" ("collide" is then the
- * row-window collide). */
+ * and for the row-tile path "advect", "gamma" (the cells' Gamma variates
+ * drawn ahead) and "redo" ("collide" is then the row-window collide). */
 int spmd_profile_enable(spmd_sys* s, int on);
 int spmd_profile_read(spmd_sys* s, const char* name, double* avg_ms, int64_t* count);
 int spmd_profile_reset(spmd_sys* s);

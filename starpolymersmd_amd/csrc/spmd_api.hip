@@ -32,8 +32,9 @@ struct ProfEvents {
     std::vector<hipEvent_t> ev;  /* start/stop pairs */
 };
 
-constexpr int kProfKinds = 8;
-const char* kProfNames[kProfKinds] = {"md", "bin", "scan", "perm", "collide", "prologue", "advect", "redo"};
+constexpr int kGraphIntervals = 8;  /* row-tile intervals per graph launch */
+constexpr int kProfKinds = 9;
+const char* kProfNames[kProfKinds] = {"md", "bin", "scan", "perm", "collide", "prologue", "advect", "redo", "gamma"};
 
 /* Row tiles: a tile's mean occupancy stays below these (cells x density),
  * so that its particle capacity (1024 / 2048) lies more than 12 standard
@@ -80,8 +81,10 @@ struct spmd_sys {
     int pending_delrx = 0;
     double pending_dt = 0.0;
     bool collided = false;
-    /* graphs of one collision interval, per starting buffer */
+    /* graphs of one collision interval, per starting buffer; on the row-tile
+     * path also of kGraphIntervals intervals in a row (one launch for all) */
     hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  /* [path][starting buffer] */
+    hipGraphExec_t graph_run[2] = {nullptr, nullptr};                       /* [starting buffer] */
     /* collision path: 0 automatic (row tiles when the buffer is in cell
      * order and the replicas allow it), 1 always the sort path */
     int path_pref = 0;
@@ -144,6 +147,8 @@ void clear_graphs(spmd_sys* s) {
     for (auto& gp : s->graph)
         for (auto& g : gp)
             if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    for (auto& g : s->graph_run)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
 }
 
 int upload_tables(spmd_sys* s) {
@@ -195,11 +200,15 @@ int upload_tables(spmd_sys* s) {
     dfree(d.tbase);
     dfree(d.bucket);
     dfree(d.bucket_r);
+    dfree(d.blc);
+    dfree(d.blc_r);
     if (int rc = dalloc(s, &d.bcount_r, d.n_rtiles)) return rc;
     if (int rc = dalloc(s, &d.tbase, d.n_rtiles)) return rc;
-    const size_t nb = d.rt_cap ? (size_t)d.n_rtiles * (size_t)d.rt_cap : 1;
+    const size_t nb = d.rt_cap ? (size_t)d.n_rtiles * (size_t)d.rt_cap : 4;
     if (int rc = dalloc(s, &d.bucket, nb)) return rc;
     if (int rc = dalloc(s, &d.bucket_r, nb)) return rc;
+    if (int rc = dalloc(s, &d.blc, nb)) return rc;
+    if (int rc = dalloc(s, &d.blc_r, nb)) return rc;
     HIPCHK(s, hipMemset(d.bcount_r, 0, 4ull * d.n_rtiles));
     HIPCHK(s, hipMemset(d.tbase, 0, 4ull * d.n_rtiles));
     HIPCHK(s, hipMemcpy(d.rtile_base, rb.data(), rb.size() * 4, hipMemcpyHostToDevice));
@@ -360,12 +369,15 @@ hipError_t enqueue_collision_ops(spmd_sys* s, int n_md, int epot_last, int n_del
     if ((e = launch_prologue(d, s->stream, 1, n_delrx, delrx_dt)) != hipSuccess) return e;
     prof_end(s, 5, s->stream);
     if (use_rows(s)) {
-        /* row-tile path: advect (beside the MD, assuming no recentring),
-         * the monomers' cells and a recentred replica's advect again after
-         * it, then the row-window collide */
+        /* row-tile path: advect and the cells' Gamma variates (beside the
+         * MD, assuming no recentring), the monomers' cells and a recentred
+         * replica's advect again after it, then the row-window collide */
         prof_begin(s, 6, s->stream);
         if ((e = launch_advect(d, s->stream, h, d.m_tot ? kSortSpec : kSortNormal)) != hipSuccess) return e;
         prof_end(s, 6, s->stream);
+        prof_begin(s, 8, s->stream);
+        if ((e = launch_cell_gamma(d, s->stream)) != hipSuccess) return e;
+        prof_end(s, 8, s->stream);
         if (d.m_tot > 0)
             if ((e = hipStreamWaitEvent(s->stream, s->ev_join, 0)) != hipSuccess) return e;
         prof_begin(s, 7, s->stream);
@@ -442,6 +454,8 @@ int check_status(spmd_sys* s) {
     if (st & ST_VERLET_OVERFLOW) return fail(s, SPMD_ERR_VERLET_OVERFLOW, std::string("Verlet list overflow; ") + buf);
     if (st & ST_SORT_MISMATCH)
         return fail(s, SPMD_ERR_SCAN, std::string("collide found a particle outside its sorted cell; ") + buf);
+    if (st & ST_GAMMA_MISS)
+        return fail(s, SPMD_ERR_SCAN, std::string("a cell's Gamma variate was not drawn for its member count; ") + buf);
     if (st & ST_TILE_OVERFLOW)
         return fail(s, SPMD_ERR_RANK_OVERFLOW, std::string("more particles arrived in a row tile than it holds; ") + buf);
     if (st & ST_TILE_MONO)
@@ -737,9 +751,12 @@ int spmd_create_batch(const spmd_params* p, uint32_t n_rep, spmd_sys** out) {
     rc = rc ? rc : dalloc(s, &d.counts, ct + 16);
     rc = rc ? rc : dalloc(s, &d.offs[0], ct + 16);
     rc = rc ? rc : dalloc(s, &d.offs[1], ct + 16);
+    rc = rc ? rc : dalloc(s, &d.cg_n, ct);
+    rc = rc ? rc : dalloc(s, &d.cg_g, ct);
     rc = rc ? rc : dalloc(s, &d.rtile_base, n_rep + 1);
     rc = rc ? rc : dalloc(s, &d.scan_flags, d.n_scan_tiles);
     rc = rc ? rc : dalloc(s, &d.scan_ticket, 1);
+    rc = rc ? rc : dalloc(s, &d.grid_bar, 1);
     rc = rc ? rc : dalloc(s, &d.reps, n_rep);
     rc = rc ? rc : dalloc(s, &d.ctl, n_rep);
     rc = rc ? rc : dalloc(s, &d.pblk_base, n_rep + 1);
@@ -748,8 +765,11 @@ int spmd_create_batch(const spmd_params* p, uint32_t n_rep, spmd_sys** out) {
     HIPCHK(s, hipMemset(d.counts, 0, (ct + 16) * 4));
     HIPCHK(s, hipMemset(d.offs[0], 0, (ct + 16) * 4));
     HIPCHK(s, hipMemset(d.offs[1], 0, (ct + 16) * 4));
+    HIPCHK(s, hipMemset(d.cg_n, 0, ct * 4));
+    HIPCHK(s, hipMemset(d.cg_g, 0, ct * 8));
     HIPCHK(s, hipMemset(d.scan_flags, 0, d.n_scan_tiles * 8ull));
     HIPCHK(s, hipMemset(d.scan_ticket, 0, 4));
+    HIPCHK(s, hipMemset(d.grid_bar, 0, 4));
     for (int b = 0; b < 2; ++b) {
         HIPCHK(s, hipMemset(d.x[b], 0, nt * 4)); HIPCHK(s, hipMemset(d.y[b], 0, nt * 4));
         HIPCHK(s, hipMemset(d.z[b], 0, nt * 4)); HIPCHK(s, hipMemset(d.vx[b], 0, nt * 4));
@@ -779,8 +799,10 @@ int spmd_destroy(spmd_sys* s) {
         dfree(d.ids[b]);
     }
     dfree(d.rank); dfree(d.pcell); dfree(d.perm); dfree(d.counts); dfree(d.scan_flags); dfree(d.scan_ticket);
+    dfree(d.grid_bar);
     for (int b = 0; b < 2; ++b) { dfree(d.offs[b]); dfree(d.bcount[b]); }
     dfree(d.bcount_r); dfree(d.tbase); dfree(d.bucket); dfree(d.bucket_r); dfree(d.rtile_base);
+    dfree(d.blc); dfree(d.blc_r); dfree(d.cg_n); dfree(d.cg_g);
     dfree(d.mpos); dfree(d.mvel); dfree(d.mforce); dfree(d.mvpos); dfree(d.mmass); dfree(d.mbtype);
     dfree(d.vl); dfree(d.bonds); dfree(d.mcell);
     dfree(d.inj_axis); dfree(d.inj_gamma); dfree(d.reps); dfree(d.ctl); dfree(d.pblk_base);
@@ -1046,12 +1068,20 @@ int spmd_run(spmd_sys* s, int64_t t0, int64_t n, double mpc_dt) {
             const int path = use_rows(s) ? 1 : 0;
             if (path)
                 if (int rc = prep_rows(s)) return rc;
-            hipGraphExec_t& gx = s->graph[path][c];
-            if (!gx || s->graph_h != mpc_dt) {
-                if (s->graph_h != mpc_dt) clear_graphs(s);
+            /* the row-tile path repeats itself: kGraphIntervals intervals in
+             * one graph while as many remain */
+            const int K = path && (end - t) / S >= kGraphIntervals ? kGraphIntervals : 1;
+            if (s->graph_h != mpc_dt) clear_graphs(s);
+            hipGraphExec_t& gx = K > 1 ? s->graph_run[c] : s->graph[path][c];
+            if (!gx) {
                 hipGraph_t g;
                 HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-                const hipError_t e = enqueue_collision_ops(s, (int)S, 0, (int)S, mpc_dt, mpc_dt);
+                hipError_t e = hipSuccess;
+                for (int q = 0; q < K && e == hipSuccess; ++q) {
+                    e = enqueue_collision_ops(s, (int)S, 0, (int)S, mpc_dt, mpc_dt);
+                    s->d.cur ^= 1;
+                }
+                s->d.cur = c;
                 hipError_t e2 = hipStreamEndCapture(s->stream, &g);
                 if (e != hipSuccess || e2 != hipSuccess)
                     return fail(s, SPMD_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e ? e : e2));
@@ -1061,8 +1091,9 @@ int spmd_run(spmd_sys* s, int64_t t0, int64_t n, double mpc_dt) {
             }
             HIPCHK(s, hipGraphLaunch(gx, s->stream));
             if (!path) s->tc_dirty = true;
-            s->d.cur ^= 1;
+            s->d.cur ^= K & 1;
             s->collided = true;
+            t += (K - 1) * S;
         }
         t += S;
     }

@@ -48,6 +48,7 @@ enum : uint32_t {
     ST_TILE_MONO = 1u << 5,      /* more monomers in a collide tile than its list holds */
     ST_SORT_MISMATCH = 1u << 6,  /* collide recomputed a cell outside its tile (internal error) */
     ST_TILE_OVERFLOW = 1u << 7,  /* more particles arrived in a row tile than it holds */
+    ST_GAMMA_MISS = 1u << 8,     /* a cell's Gamma variate was not drawn for its count (internal error) */
 };
 
 /* Row-tile path: a row tile is `nr` consecutive y-rows (whole x-rows) of
@@ -92,6 +93,7 @@ struct RepCtl {
     int32_t redo;                /* the speculative sort (no recentring) must be redone */
     uint32_t vl_count;           /* entries of the star's Verlet half list */
     uint32_t closure_oor;        /* monomers whose closure features left the trained range */
+    uint32_t cg_k;               /* k_cur + 1 of the drawn cell Gammas (0: none) */
 };
 
 struct DevState {
@@ -113,12 +115,20 @@ struct DevState {
     uint32_t* bucket;      /* particle indices per row tile, rt_cap each */
     uint32_t* bucket_r;    /* the rebinning's */
     uint32_t* tbase;       /* first slot of each row tile in the new buffer */
+    uint8_t* blc;          /* beside each bucket entry: the particle's cell in its tile */
+    uint8_t* blc_r;        /* the rebinning's */
+    /* per cell (C_tot): the Gamma variate of the coming collision drawn
+     * ahead of it (k_cell_gamma, the redo kernel) and the member count it
+     * was drawn for */
+    uint32_t* cg_n;
+    double* cg_g;
     uint32_t* rtile_base;  /* row-tile prefix per replica (n_rep+1) */
     uint32_t n_rtiles;
     int rt_cap;            /* particles a row tile holds (1024 / 2048), 0: path unavailable */
     int rt_tcm;            /* cells a row tile holds (64 / 128 / 256) */
     uint64_t* scan_flags;  /* per scan tile: status<<62 | value */
     uint32_t* scan_ticket;
+    uint32_t* grid_bar;    /* the redo kernel's grid barrier (reset by the prologue) */
     uint32_t n_scan_tiles;
     double *mpos, *mvel, *mforce, *mvpos, *mmass;
     uint8_t* mbtype;
@@ -163,6 +173,9 @@ hipError_t launch_collide(const DevState& d, hipStream_t st, double h);
 hipError_t launch_advect(const DevState& d, hipStream_t st, double h, int mode);
 hipError_t launch_rows_redo(const DevState& d, hipStream_t st, double h);
 hipError_t launch_collide_rows(const DevState& d, hipStream_t st, double h);
+/* the Gamma variate of every cell for its predicted member count (after
+ * k_advect; the redo kernel redraws the cells holding monomers) */
+hipError_t launch_cell_gamma(const DevState& d, hipStream_t st);
 hipError_t launch_fluid_init(const DevState& d, hipStream_t st);
 /* learned closure: 24 features per monomer (all replicas, global monomer
  * order) with an optional range check against [fmin, fmax] (device arrays

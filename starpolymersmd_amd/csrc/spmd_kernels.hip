@@ -836,7 +836,7 @@ __global__ __launch_bounds__(64) void k_prologue(DevState d, int do_coll, int n_
     }
     if (do_coll && r == 0) {
         for (uint32_t t = threadIdx.x; t < d.n_scan_tiles; t += 64) d.scan_flags[t] = 0ull;
-        if (threadIdx.x == 0) *d.scan_ticket = 0u;
+        if (threadIdx.x == 0) { *d.scan_ticket = 0u; *d.grid_bar = 0u; }
     }
 }
 
@@ -1497,11 +1497,15 @@ __device__ __forceinline__ RowTile row_tile(const RepDev& rep, uint32_t t) {
     return T;
 }
 
-/* Row tile (replica-local) of a replica-local cell index. */
-__device__ __forceinline__ uint32_t tile_of_cell(const RepDev& rep, uint32_t cell) {
-    const uint32_t row = cell / (uint32_t)rep.box.iL[0];
+/* Row tile (replica-local) of a replica-local cell index, and the cell's
+ * index inside the tile (row_tile's c_lo + lc is the cell). */
+__device__ __forceinline__ void tile_of_cell(const RepDev& rep, uint32_t cell, uint32_t& tile, uint32_t& lc) {
+    const uint32_t Lx = (uint32_t)rep.box.iL[0], nr = (uint32_t)rep.nr;
+    const uint32_t row = cell / Lx;
     const uint32_t iz = row / (uint32_t)rep.box.iL[1], iy = row - iz * (uint32_t)rep.box.iL[1];
-    return iz * (uint32_t)rep.tpp + iy / (uint32_t)rep.nr;
+    const uint32_t ty = iy / nr;
+    tile = iz * (uint32_t)rep.tpp + ty;
+    lc = cell - Lx * (ty * nr + (uint32_t)rep.box.iL[1] * iz);
 }
 
 constexpr int kBinHash = 64;      /* distinct destination tiles a block aggregates in LDS */
@@ -1515,10 +1519,13 @@ constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
  * per distinct tile and block); blocks that reach more than kBinHash tiles
  * (a buffer not in cell order) take a global atomic per particle for the
  * rest.  Slots from cap on are counted but not stored (the collide reports
- * the overflow). */
+ * the overflow).  Beside each index, blc holds the particle's cell in its
+ * tile (one byte: a tile has at most 256 cells), from which k_cell_gamma
+ * counts the cells' members. */
 template <int NT, int PPT>
 __device__ void bin_block(const DevState& d, int r, uint64_t p0, uint64_t p1, const RepCtl& c, double h,
-                          uint32_t* cnt, uint32_t* bucket, uint32_t* s_key, uint32_t* s_cnt, bool& over) {
+                          uint32_t* cnt, uint32_t* bucket, uint8_t* blc, uint32_t* s_key, uint32_t* s_cnt,
+                          bool& over) {
     const RepDev& rep = d.reps[r];
     Box b = rep.box;
     b.delrx = c.delrx;
@@ -1543,6 +1550,7 @@ __device__ void bin_block(const DevState& d, int r, uint64_t p0, uint64_t p1, co
      * global atomic gave */
     constexpr uint32_t kDirect = 0x80000000u;
     uint32_t tl[PPT], slot[PPT];
+    uint8_t lcl[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         tl[j] = kHashEmpty;
@@ -1552,8 +1560,11 @@ __device__ void bin_block(const DevState& d, int r, uint64_t p0, uint64_t p1, co
         recenter_stream(b, h, c, x, y, z, vx, vy, vz);
         const uint32_t cell = shifted_cell(b, c, x, y, z, vx);
         if (cell >= rep.ncells) { bad = true; continue; }
-        const uint32_t g = rep.rt_base + tile_of_cell(rep, cell);
+        uint32_t lt, lc;
+        tile_of_cell(rep, cell, lt, lc);
+        const uint32_t g = rep.rt_base + lt;
         tl[j] = g;
+        lcl[j] = (uint8_t)lc;
         uint32_t hs = (g * 0x9E3779B1u) >> 26;
         int probe = 0;
         for (; probe < kBinHash; ++probe) {
@@ -1578,8 +1589,12 @@ __device__ void bin_block(const DevState& d, int r, uint64_t p0, uint64_t p1, co
         if (tl[j] == kHashEmpty) continue;
         const uint64_t i = p0 + (uint64_t)(j * NT + tid);
         const uint32_t sl = (slot[j] & kDirect) ? slot[j] & ~kDirect : s_cnt[slot[j] >> 16] + (slot[j] & 0xFFFFu);
-        if (sl < cap) bucket[(size_t)tl[j] * cap + sl] = (uint32_t)i;
-        else over = true;
+        if (sl < cap) {
+            bucket[(size_t)tl[j] * cap + sl] = (uint32_t)i;
+            blc[(size_t)tl[j] * cap + sl] = lcl[j];
+        } else {
+            over = true;
+        }
     }
     if (bad) atomicOr(&d.ctl[r].status, ST_CELL_OOB);
 }
@@ -1598,25 +1613,131 @@ __global__ __launch_bounds__(256) void k_advect(DevState d, double h, int mode) 
     const uint64_t p0 = rep.part_base + (uint64_t)(lb - d.pblk_base[r]) * (256 * PPT);
     const uint64_t p1 = min(p0 + 256 * PPT, rep.part_base + rep.n_fluid);
     bool over = false;
-    bin_block<256, PPT>(d, r, p0, p1, c, h, d.bcount[d.cur], d.bucket, s_key, s_cnt, over);
+    bin_block<256, PPT>(d, r, p0, p1, c, h, d.bcount[d.cur], d.bucket, d.blc, s_key, s_cnt, over);
     (void)over;  /* an overflowing tile reports it from its count */
 }
 
-/* After k_advect (and the MD kernel), per replica (workgroup r, strided):
- * the monomers' cells of this collision (MPC::shiftParticles
- * MPC.cpp:306-310 + sort MPC.cpp:221-235: shifted, never wrapped) and the
- * first slot of every row tile in the new buffer (exclusive scan of the
- * bucket counts).  A replica whose MD recentred it (MPC::check_bounds
- * MPC.cpp:160-190) is binned again, with the recentring, into the redo
- * buckets: the grid strides over its particle blocks (k_collide_rows then
- * sums the redo counts itself). */
+/* The monomer cells of replica r in an LDS hash (cell -> monomers in it;
+ * open addressing, 2 x kMaxMono entries, so an insert always finds a slot);
+ * with write_mcell, each monomer's cell to mcell (MPC::shiftParticles
+ * MPC.cpp:306-310 + sort MPC.cpp:221-235: shifted, never wrapped).  Ends
+ * with the table complete for the whole workgroup. */
+constexpr uint32_t kMonoHash = 2u * kMaxMono;
+__device__ void mono_cells(const DevState& d, int r, bool write_mcell, uint32_t* s_mkey, uint32_t* s_mcnt) {
+    const RepDev& rep = d.reps[r];
+    __syncthreads();  /* the table of an earlier replica is read out */
+    for (uint32_t q = threadIdx.x; q < kMonoHash; q += blockDim.x) { s_mkey[q] = kHashEmpty; s_mcnt[q] = 0u; }
+    __syncthreads();
+    const double sh0 = d.ctl[r].shift[0], sh1 = d.ctl[r].shift[1], sh2 = d.ctl[r].shift[2];
+    bool oob = false;
+    for (uint32_t i = threadIdx.x; i < rep.n_mono; i += blockDim.x) {
+        const size_t g = 3 * (size_t)(rep.mono_base + i);
+        const uint32_t ci = cell_index(rep.box, d.mpos[g] + sh0, d.mpos[g + 1] + sh1, d.mpos[g + 2] + sh2);
+        oob |= ci >= rep.ncells;
+        const uint32_t gc = ci < rep.ncells ? (uint32_t)rep.cell_base + ci : ~0u;
+        if (write_mcell) d.mcell[rep.mono_base + i] = gc;
+        if (gc == ~0u) continue;
+        uint32_t hc = (gc * 0x9E3779B1u) >> (32 - __builtin_ctz(kMonoHash));
+        while (true) {
+            const uint32_t k = atomicCAS(&s_mkey[hc], kHashEmpty, gc);
+            if (k == kHashEmpty || k == gc) break;
+            hc = (hc + 1) & (kMonoHash - 1);
+        }
+        atomicAdd(&s_mcnt[hc], 1u);
+    }
+    if (oob && write_mcell) atomicOr(&d.ctl[r].status, ST_CELL_OOB);
+    __syncthreads();
+}
+
+/* Monomers of global cell gc in the table (0 if none). */
+__device__ __forceinline__ uint32_t mono_count(uint32_t gc, const uint32_t* s_mkey, const uint32_t* s_mcnt) {
+    uint32_t hc = (gc * 0x9E3779B1u) >> (32 - __builtin_ctz(kMonoHash));
+    for (uint32_t probe = 0; probe < kMonoHash; ++probe) {
+        const uint32_t k = s_mkey[hc];
+        if (k == gc) return s_mcnt[hc];
+        if (k == kHashEmpty) return 0u;
+        hc = (hc + 1) & (kMonoHash - 1);
+    }
+    return 0u;
+}
+
+/* A cell's Gamma variate of this collision for n members (the MBS
+ * thermostat, MPC.cpp:287-289, with k_collide's expressions), and n. */
+__device__ __forceinline__ void draw_cell_gamma(const DevState& d, const RepDev& rep, uint32_t gc, uint32_t n,
+                                                uint32_t k, bool inj) {
+    d.cg_n[gc] = n;
+    d.cg_g[gc] = n >= 2 && !inj ? gamma_fast(rep.seed, rep.replica, gc - (uint32_t)rep.cell_base, k, 1.5 * (n - 1),
+                                             rep.kT)
+                                : 0.0;
+}
+
+/* Row tile lt of replica r: its cells' members counted from the n cell
+ * bytes of its bucket (an LDS histogram, s_h of 256), plus the monomers of
+ * the table when one is given, and every cell's variate drawn for its
+ * count. */
+template <int NT>
+__device__ void tile_gammas(const DevState& d, int r, uint32_t lt, uint32_t n, const uint8_t* lcs, uint32_t* s_h,
+                            const uint32_t* s_mkey, const uint32_t* s_mcnt) {
+    const RepDev& rep = d.reps[r];
+    const RowTile T = row_tile(rep, lt - rep.rt_base);
+    const uint32_t k = d.ctl[r].k_cur;
+    const bool inj = d.ctl[r].inj_active != 0;
+    __syncthreads();  /* the histogram of an earlier tile is read out */
+    for (uint32_t c = threadIdx.x; c < 256u; c += NT) s_h[c] = 0u;
+    __syncthreads();
+    n = min(n, (uint32_t)d.rt_cap);
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(lcs);  /* a tile's bytes start 4-aligned */
+    for (uint32_t q = threadIdx.x; q < (n + 3u) / 4u; q += NT) {
+        const uint32_t v = w[q];
+#pragma unroll
+        for (uint32_t b = 0; b < 4u; ++b)
+            if (4u * q + b < n) atomicAdd(&s_h[(v >> (8u * b)) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < T.nc; c += NT) {
+        const uint32_t gc = (uint32_t)rep.cell_base + T.c_lo + c;
+        draw_cell_gamma(d, rep, gc, s_h[c] + (s_mkey ? mono_count(gc, s_mkey, s_mcnt) : 0u), k, inj);
+    }
+}
+
+/* After k_advect, beside the MD: every cell's Gamma variate of this
+ * collision drawn ahead of the collide for its solvent count (assuming no
+ * recentring, as k_advect does); the redo kernel draws the cells holding
+ * monomers again, and all cells of a recentred replica.  The collide checks
+ * that each variate was drawn for the cell's member count.  One wave per
+ * row tile. */
+__global__ __launch_bounds__(64) void k_cell_gamma(DevState d) {
+    __shared__ uint32_t s_h[256];
+    const uint32_t lt = blockIdx.x;
+    const int r = find_rep(d.rtile_base, d.n_rep, lt);
+    tile_gammas<64>(d, r, lt, d.bcount[d.cur][lt], d.blc + (size_t)lt * (uint32_t)d.rt_cap, s_h, nullptr, nullptr);
+    if (lt == d.reps[r].rt_base && threadIdx.x == 0) d.ctl[r].cg_k = d.ctl[r].k_cur + 1u;
+}
+
+/* After k_advect, k_cell_gamma and the MD kernel:
+ *   - per replica (workgroup r, strided) the first slot of every row tile
+ *     in the new buffer (exclusive scan of the bucket counts)
+ *   - in other workgroups (G / n_rep - 1 per replica when n_rep <= G / 2):
+ *     the monomers' cells of this collision, and the variate of each cell
+ *     holding monomers drawn again for its solvent count plus its monomers
+ *   - a replica whose MD recentred it (MPC::check_bounds MPC.cpp:160-190)
+ *     binned again, with the recentring, into the redo buckets (the grid
+ *     strides over its particle blocks), then -- after a grid-wide barrier
+ *     (the 256 workgroups are all resident; the wait is bounded) -- all its
+ *     cells' variates drawn for the new counts (k_collide_rows sums the
+ *     redo bucket counts itself). */
 template <int PPT>
 __global__ __launch_bounds__(256) void k_rows_redo(DevState d, double h) {
     __shared__ uint32_t s_key[kBinHash], s_cnt[kBinHash];
+    __shared__ uint32_t s_mkey[kMonoHash], s_mcnt[kMonoHash];
+    __shared__ uint32_t s_h[256];
     __shared__ uint32_t s_w[4];
     constexpr int KS = 16;  /* counts per thread and pass of the scan */
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int r = blockIdx.x; r < d.n_rep; r += gridDim.x) {
+    const int G = (int)gridDim.x;
+    bool any_redo = false;
+    for (int r = 0; r < d.n_rep; ++r) any_redo |= d.ctl[r].redo != 0;
+    for (int r = blockIdx.x; r < d.n_rep; r += G) {
         const RepDev& rep = d.reps[r];
         if (!d.ctl[r].redo) {
             const uint32_t t0 = d.rtile_base[r], nt = d.rtile_base[r + 1] - t0;
@@ -1650,20 +1771,28 @@ __global__ __launch_bounds__(256) void k_rows_redo(DevState d, double h) {
             }
         }
     }
-    /* the monomer cells in other workgroups than the scans (they overlap) */
-    const int G = (int)gridDim.x;
-    for (int r = ((int)blockIdx.x - d.n_rep % G + G) % G; r < d.n_rep; r += G) {
-        const RepDev& rep = d.reps[r];
-        const double sh0 = d.ctl[r].shift[0], sh1 = d.ctl[r].shift[1], sh2 = d.ctl[r].shift[2];
-        bool oob = false;
-        for (uint32_t i = threadIdx.x; i < rep.n_mono; i += 256) {
-            const size_t g = 3 * (size_t)(rep.mono_base + i);
-            const uint32_t ci = cell_index(rep.box, d.mpos[g] + sh0, d.mpos[g + 1] + sh1, d.mpos[g + 2] + sh2);
-            oob |= ci >= rep.ncells;
-            d.mcell[rep.mono_base + i] = ci < rep.ncells ? (uint32_t)rep.cell_base + ci : ~0u;
+    /* monomer cells: wpr workgroups per replica (those after the scanning
+     * ones when there are enough), part sub of them */
+    const bool apart = 2 * d.n_rep <= G;
+    const int wpr = apart ? (G - d.n_rep) / d.n_rep : 1;
+    const int mb = apart ? (int)blockIdx.x - d.n_rep : (int)blockIdx.x;
+    if (mb >= 0) {
+        const int sub = mb % wpr;
+        for (int r = mb / wpr; r < d.n_rep; r += apart ? d.n_rep : G) {
+            const RepDev& rep = d.reps[r];
+            if (rep.n_mono == 0) continue;
+            mono_cells(d, r, sub == 0, s_mkey, s_mcnt);
+            if (d.ctl[r].redo) continue;  /* all its cells are drawn below */
+            const uint32_t k = d.ctl[r].k_cur;
+            const bool inj = d.ctl[r].inj_active != 0;
+            for (uint32_t q = threadIdx.x; q < kMonoHash; q += 256) {
+                const uint32_t gc = s_mkey[q];
+                if (gc == kHashEmpty || (int)(((gc * 0x9E3779B1u) >> 8) % (uint32_t)wpr) != sub) continue;
+                draw_cell_gamma(d, rep, gc, d.cg_n[gc] + s_mcnt[q], k, inj);
+            }
         }
-        if (oob) atomicOr(&d.ctl[r].status, ST_CELL_OOB);
     }
+    if (!any_redo) return;
     for (int r = 0; r < d.n_rep; ++r) {
         if (!d.ctl[r].redo) continue;
         const RepDev& rep = d.reps[r];
@@ -1674,7 +1803,36 @@ __global__ __launch_bounds__(256) void k_rows_redo(DevState d, double h) {
             const uint64_t p0 = rep.part_base + (uint64_t)q * (256 * PPT);
             const uint64_t p1 = min(p0 + 256 * PPT, rep.part_base + rep.n_fluid);
             bool over = false;
-            bin_block<256, PPT>(d, r, p0, p1, c, h, d.bcount_r, d.bucket_r, s_key, s_cnt, over);
+            /* at most 2 particles per thread at a time: the registers of this
+             * kernel are those of its common path */
+            constexpr int RP = PPT > 2 ? 2 : PPT;
+            for (uint64_t s0 = p0; s0 < p1; s0 += 256 * RP)
+                bin_block<256, RP>(d, r, s0, min(s0 + 256 * RP, p1), c, h, d.bcount_r, d.bucket_r, d.blc_r, s_key,
+                                   s_cnt, over);
+        }
+    }
+    /* grid barrier: every workgroup's rebinning stored */
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(d.grid_bar, 1u);
+        uint32_t spins = 0;
+        while (__hip_atomic_load(d.grid_bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < (uint32_t)G) {
+            if (++spins > (1u << 22)) { atomicOr(&d.ctl[0].status, ST_SCAN_TIMEOUT); break; }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+    for (int r = 0; r < d.n_rep; ++r) {
+        if (!d.ctl[r].redo) continue;
+        const RepDev& rep = d.reps[r];
+        if (rep.n_mono) mono_cells(d, r, false, s_mkey, s_mcnt);
+        const uint32_t t0 = d.rtile_base[r], nt = d.rtile_base[r + 1] - t0;
+        for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+            const uint32_t lt = t0 + t;
+            const uint32_t n = __hip_atomic_load(&d.bcount_r[lt], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            tile_gammas<256>(d, r, lt, n, d.blc_r + (size_t)lt * (uint32_t)d.rt_cap, s_h,
+                             rep.n_mono ? s_mkey : nullptr, s_mcnt);
         }
     }
 }
@@ -1898,16 +2056,25 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
             s_lz[AMC ? slot : 0] = my_l[AMC ? u : 0][2];
         }
     }
-    /* per cell, one lane each (a whole tile of 64 cells is one wave), once
-     * the particles' registers are free: the Gamma variate, which depends
-     * only on the member count */
+    /* per cell, one lane each: the Gamma variate, which depends only on the
+     * member count, drawn ahead for it (k_cell_gamma, the redo kernel); a
+     * count it was not drawn for is an internal error, reported */
+    const bool drawn = ctl->cg_k == k + 1u;
+    bool miss = false;
     for (uint32_t c = tid; c < nc; c += NT) {
         const uint32_t nn = s_cnt[c] + s_nm[c];
         double g = 0.0;
-        if (nn >= 2)
-            g = inj ? d.inj_gamma[gc0 + c] : gamma_fast(rep.seed, rep.replica, T.c_lo + c, k, 1.5 * (nn - 1), rep.kT);
+        if (nn >= 2) {
+            if (inj) {
+                g = d.inj_gamma[gc0 + c];
+            } else {
+                miss |= !drawn || d.cg_n[gc0 + c] != nn;
+                g = d.cg_g[gc0 + c];
+            }
+        }
         s_g[c] = g;
     }
+    if (miss) atomicOr(&ctl->status, ST_GAMMA_MISS);
     /* a monomer's coordinates inside its cell (shifted, never wrapped) */
     auto mono_local = [&](uint32_t mi, double& lx, double& ly, double& lz) {
         const double qx = d.mpos[3 * mi] + cc.shift[0] + b.half[0];
@@ -2745,6 +2912,12 @@ hipError_t launch_advect(const DevState& d, hipStream_t st, double h, int mode) 
     case 8: hipLaunchKernelGGL(k_advect<8>, dim3(d.n_pblk), dim3(256), 0, st, d, h, mode); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_gamma(const DevState& d, hipStream_t st) {
+    if (!d.n_rtiles) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cell_gamma, dim3(d.n_rtiles), dim3(64), 0, st, d);
     return hipGetLastError();
 }
 
