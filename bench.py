@@ -45,16 +45,18 @@ def make_system(w):
     return g
 
 
-TRAFFIC_PROFILE = "profiles/collision_pipeline_L64_rho10.md"
+# rocprofv3 FETCH_SIZE/WRITE_SIZE summaries of the two collision paths
+TRAFFIC_PROFILE = {"rows": "profiles/collision_pipeline_rows_L64_rho10.md",
+                   "sort": "profiles/collision_pipeline_L64_rho10.md"}
 
 
-def measured_traffic():
-    """HBM bytes per collision of the pipeline from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (scaled by the counter calibration of a
-    coalesced 4-byte-per-lane gather of known size, the pipeline kernels'
-    access shape); None when the summary is absent."""
+def measured_traffic(path):
+    """HBM bytes per collision of the pipeline of `path` ("rows" or "sort")
+    from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (each kernel's
+    counters scaled by the calibration of its access shape); None when the
+    summary is absent."""
     try:
-        with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
+        with open(os.path.join(ROOT, TRAFFIC_PROFILE[path])) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_collision:"):
                     return int(line.split(":")[1])
@@ -625,7 +627,7 @@ def main():
     else:
         pipe_ms = prof["bin"] + prof["scan"] + prof["perm"] + prof["collide"]
         pipe_name = "collision pipeline k_bin+k_scan+k_perm+k_collide"
-    traffic = measured_traffic() if rows else None
+    traffic = measured_traffic("rows" if rows else "sort")
     achieved = B_ALG * n_f / (pipe_ms * 1e-3) / 1e9
     T = g.temperature()
 
@@ -646,7 +648,7 @@ def main():
                      "measured_gbs": traffic / (pipe_ms * 1e-3) / 1e9 if traffic else None,
                      "measured_frac": traffic / (pipe_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
                      "amplification": traffic / (B_ALG * n_f) if traffic else None,
-                     "traffic_source": TRAFFIC_PROFILE if traffic else None,
+                     "traffic_source": TRAFFIC_PROFILE["rows" if rows else "sort"] if traffic else None,
                      "kernel": pipe_name,
                      "alg_bytes_per_launch": B_ALG * n_f, "avg_ms": pipe_ms,
                      "per_kernel_ms": prof},

@@ -108,16 +108,16 @@ __device__ __forceinline__ double wave_sum(double v) {
  * gamma_mt's and the variate d v^3 agrees with it to the FP32 rounding of
  * v^3 (a few 1e-7 relative).  The FP64 transcendentals were the largest
  * single cost of a collision. */
-__device__ __forceinline__ double gamma_fast(uint64_t seed, uint32_t replica, uint32_t cell, uint32_t k,
-                                             double shape, double scale) {
+__device__ __forceinline__ double gamma_fast_w0(uint64_t seed, uint32_t replica, uint32_t cell, uint32_t k,
+                                                const uint32_t w0[4], double shape, double scale) {
     const double d = shape - 1.0 / 3.0;
     const double c = 1.0 / sqrt(9.0 * d);
     const float df = (float)d, cf = (float)c;
     constexpr float kLn2 = 0.693147180559945f;
     constexpr float k2m32 = 2.3283064365386963e-10f;
     for (uint32_t j = 0; j < 64; ++j) {
-        uint32_t w[4];
-        draw4(seed, replica, TAG_GAMMA, cell, k, j, w);
+        uint32_t w[4] = {w0[0], w0[1], w0[2], w0[3]};
+        if (j) draw4(seed, replica, TAG_GAMMA, cell, k, j, w);
         const float u0 = ((float)w[0] + 0.5f) * k2m32;
         const float u1 = ((float)w[1] + 0.5f) * k2m32;
         const float u2 = ((float)w[2] + 0.5f) * k2m32;
@@ -139,6 +139,15 @@ __device__ __forceinline__ double gamma_fast(uint64_t seed, uint32_t replica, ui
         if (gamma_attempt(w, d, c, vv)) return d * vv * scale;
     }
     return d * scale;
+}
+
+/* gamma_fast_w0 with the first attempt's words drawn here (the caller can
+ * draw them itself, e.g. while the member count is still being loaded). */
+__device__ __forceinline__ double gamma_fast(uint64_t seed, uint32_t replica, uint32_t cell, uint32_t k,
+                                             double shape, double scale) {
+    uint32_t w0[4];
+    draw4(seed, replica, TAG_GAMMA, cell, k, 0, w0);
+    return gamma_fast_w0(seed, replica, cell, k, w0, shape, scale);
 }
 
 /* The SRD rotation matrix about the axis (MPC::collide MPC.cpp:241-257)
@@ -1617,31 +1626,55 @@ __global__ __launch_bounds__(256) void k_advect(DevState d, double h, int mode) 
     (void)over;  /* an overflowing tile reports it from its count */
 }
 
-/* The monomer cells of replica r in an LDS hash (cell -> monomers in it;
- * open addressing, 2 x kMaxMono entries, so an insert always finds a slot);
- * with write_mcell, each monomer's cell to mcell (MPC::shiftParticles
- * MPC.cpp:306-310 + sort MPC.cpp:221-235: shifted, never wrapped).  Ends
- * with the table complete for the whole workgroup. */
+/* The monomer cells of replica r in an LDS hash of hs entries (cell ->
+ * monomers in it; open addressing, hs >= 2 x monomers, so an insert always
+ * finds a slot); with write_mcell, each monomer's cell to mcell
+ * (MPC::shiftParticles MPC.cpp:306-310 + sort MPC.cpp:221-235: shifted,
+ * never wrapped).  The grid shift sh comes from the caller.  Every
+ * monomer's position is loaded before any is inserted (the blockDim of 256
+ * takes kMaxMono / 256 per thread).  Ends with the table complete for the
+ * whole workgroup. */
 constexpr uint32_t kMonoHash = 2u * kMaxMono;
-__device__ void mono_cells(const DevState& d, int r, bool write_mcell, uint32_t* s_mkey, uint32_t* s_mcnt) {
+constexpr int kMonoPerThread = kMaxMono / 256;
+
+__device__ __forceinline__ uint32_t mono_hash_size(uint32_t n_mono) {
+    uint32_t hs = 64u;
+    while (hs < 2u * n_mono) hs <<= 1;
+    return hs;
+}
+
+__device__ __forceinline__ uint32_t mono_slot(uint32_t gc, uint32_t hs) {
+    return (gc * 0x9E3779B1u) >> (32 - __builtin_ctz(hs));
+}
+
+__device__ void mono_cells(const DevState& d, int r, const double sh[3], bool write_mcell, uint32_t hs,
+                           uint32_t* s_mkey, uint32_t* s_mcnt) {
     const RepDev& rep = d.reps[r];
-    __syncthreads();  /* the table of an earlier replica is read out */
-    for (uint32_t q = threadIdx.x; q < kMonoHash; q += blockDim.x) { s_mkey[q] = kHashEmpty; s_mcnt[q] = 0u; }
-    __syncthreads();
-    const double sh0 = d.ctl[r].shift[0], sh1 = d.ctl[r].shift[1], sh2 = d.ctl[r].shift[2];
+    uint32_t gcs[kMonoPerThread];
     bool oob = false;
-    for (uint32_t i = threadIdx.x; i < rep.n_mono; i += blockDim.x) {
+#pragma unroll
+    for (int u = 0; u < kMonoPerThread; ++u) {
+        const uint32_t i = threadIdx.x + 256u * u;
+        gcs[u] = ~0u;
+        if (i >= rep.n_mono) continue;
         const size_t g = 3 * (size_t)(rep.mono_base + i);
-        const uint32_t ci = cell_index(rep.box, d.mpos[g] + sh0, d.mpos[g + 1] + sh1, d.mpos[g + 2] + sh2);
+        const uint32_t ci = cell_index(rep.box, d.mpos[g] + sh[0], d.mpos[g + 1] + sh[1], d.mpos[g + 2] + sh[2]);
         oob |= ci >= rep.ncells;
-        const uint32_t gc = ci < rep.ncells ? (uint32_t)rep.cell_base + ci : ~0u;
-        if (write_mcell) d.mcell[rep.mono_base + i] = gc;
+        gcs[u] = ci < rep.ncells ? (uint32_t)rep.cell_base + ci : ~0u;
+        if (write_mcell) d.mcell[rep.mono_base + i] = gcs[u];
+    }
+    __syncthreads();  /* the table of an earlier replica is read out */
+    for (uint32_t q = threadIdx.x; q < hs; q += 256u) { s_mkey[q] = kHashEmpty; s_mcnt[q] = 0u; }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kMonoPerThread; ++u) {
+        const uint32_t gc = gcs[u];
         if (gc == ~0u) continue;
-        uint32_t hc = (gc * 0x9E3779B1u) >> (32 - __builtin_ctz(kMonoHash));
+        uint32_t hc = mono_slot(gc, hs);
         while (true) {
             const uint32_t k = atomicCAS(&s_mkey[hc], kHashEmpty, gc);
             if (k == kHashEmpty || k == gc) break;
-            hc = (hc + 1) & (kMonoHash - 1);
+            hc = (hc + 1) & (hs - 1);
         }
         atomicAdd(&s_mcnt[hc], 1u);
     }
@@ -1650,13 +1683,14 @@ __device__ void mono_cells(const DevState& d, int r, bool write_mcell, uint32_t*
 }
 
 /* Monomers of global cell gc in the table (0 if none). */
-__device__ __forceinline__ uint32_t mono_count(uint32_t gc, const uint32_t* s_mkey, const uint32_t* s_mcnt) {
-    uint32_t hc = (gc * 0x9E3779B1u) >> (32 - __builtin_ctz(kMonoHash));
-    for (uint32_t probe = 0; probe < kMonoHash; ++probe) {
+__device__ __forceinline__ uint32_t mono_count(uint32_t gc, uint32_t hs, const uint32_t* s_mkey,
+                                               const uint32_t* s_mcnt) {
+    uint32_t hc = mono_slot(gc, hs);
+    for (uint32_t probe = 0; probe < hs; ++probe) {
         const uint32_t k = s_mkey[hc];
         if (k == gc) return s_mcnt[hc];
         if (k == kHashEmpty) return 0u;
-        hc = (hc + 1) & (kMonoHash - 1);
+        hc = (hc + 1) & (hs - 1);
     }
     return 0u;
 }
@@ -1677,7 +1711,7 @@ __device__ __forceinline__ void draw_cell_gamma(const DevState& d, const RepDev&
  * count. */
 template <int NT>
 __device__ void tile_gammas(const DevState& d, int r, uint32_t lt, uint32_t n, const uint8_t* lcs, uint32_t* s_h,
-                            const uint32_t* s_mkey, const uint32_t* s_mcnt) {
+                            uint32_t hs, const uint32_t* s_mkey, const uint32_t* s_mcnt) {
     const RepDev& rep = d.reps[r];
     const RowTile T = row_tile(rep, lt - rep.rt_base);
     const uint32_t k = d.ctl[r].k_cur;
@@ -1696,100 +1730,99 @@ __device__ void tile_gammas(const DevState& d, int r, uint32_t lt, uint32_t n, c
     __syncthreads();
     for (uint32_t c = threadIdx.x; c < T.nc; c += NT) {
         const uint32_t gc = (uint32_t)rep.cell_base + T.c_lo + c;
-        draw_cell_gamma(d, rep, gc, s_h[c] + (s_mkey ? mono_count(gc, s_mkey, s_mcnt) : 0u), k, inj);
+        draw_cell_gamma(d, rep, gc, s_h[c] + (s_mkey ? mono_count(gc, hs, s_mkey, s_mcnt) : 0u), k, inj);
     }
 }
 
-/* After k_advect, beside the MD: every cell's Gamma variate of this
- * collision drawn ahead of the collide for its solvent count (assuming no
- * recentring, as k_advect does); the redo kernel draws the cells holding
- * monomers again, and all cells of a recentred replica.  The collide checks
- * that each variate was drawn for the cell's member count.  One wave per
- * row tile. */
+/* After k_advect, beside the MD:
+ *   - blocks < n_rtiles, one wave per row tile: every cell's Gamma variate
+ *     of this collision drawn ahead of the collide for its solvent count
+ *     (assuming no recentring, as k_advect does); the redo kernel draws the
+ *     cells holding monomers again, and all cells of a recentred replica;
+ *     the collide checks that each variate was drawn for the cell's member
+ *     count
+ *   - the n_rep blocks after them: the first slot of every row tile of
+ *     replica r in the new buffer (exclusive scan of the bucket counts,
+ *     16 per lane a pass); a recentred replica's collide sums its redo
+ *     counts itself. */
 __global__ __launch_bounds__(64) void k_cell_gamma(DevState d) {
     __shared__ uint32_t s_h[256];
-    const uint32_t lt = blockIdx.x;
-    const int r = find_rep(d.rtile_base, d.n_rep, lt);
-    tile_gammas<64>(d, r, lt, d.bcount[d.cur][lt], d.blc + (size_t)lt * (uint32_t)d.rt_cap, s_h, nullptr, nullptr);
-    if (lt == d.reps[r].rt_base && threadIdx.x == 0) d.ctl[r].cg_k = d.ctl[r].k_cur + 1u;
+    const uint32_t b = blockIdx.x;
+    if (b >= d.n_rtiles) {
+        constexpr int KS = 16;
+        const int r = (int)(b - d.n_rtiles);
+        const uint32_t t0 = d.rtile_base[r], nt = d.rtile_base[r + 1] - t0;
+        const uint32_t* tc = d.bcount[d.cur] + t0;
+        const int lane = threadIdx.x;
+        uint32_t carry = (uint32_t)d.reps[r].part_base;
+        for (uint32_t c0 = 0; c0 < nt; c0 += 64u * KS) {
+            uint32_t v[KS];
+            const uint32_t i0 = c0 + (uint32_t)lane * KS;
+#pragma unroll
+            for (int q = 0; q < KS; ++q) v[q] = i0 + q < nt ? tc[i0 + q] : 0u;
+            uint32_t run = 0u;
+#pragma unroll
+            for (int q = 0; q < KS; ++q) { const uint32_t t = v[q]; v[q] = run; run += t; }
+            uint32_t inc = run;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t u = __shfl_up(inc, o);
+                if (lane >= o) inc += u;
+            }
+            const uint32_t base = carry + inc - run;
+#pragma unroll
+            for (int q = 0; q < KS; ++q)
+                if (i0 + q < nt) d.tbase[t0 + i0 + q] = base + v[q];
+            carry += __shfl(inc, 63);
+        }
+        return;
+    }
+    const int r = find_rep(d.rtile_base, d.n_rep, b);
+    tile_gammas<64>(d, r, b, d.bcount[d.cur][b], d.blc + (size_t)b * (uint32_t)d.rt_cap, s_h, 0u, nullptr, nullptr);
+    if (b == d.reps[r].rt_base && threadIdx.x == 0) d.ctl[r].cg_k = d.ctl[r].k_cur + 1u;
 }
 
 /* After k_advect, k_cell_gamma and the MD kernel:
- *   - per replica (workgroup r, strided) the first slot of every row tile
- *     in the new buffer (exclusive scan of the bucket counts)
- *   - in other workgroups (G / n_rep - 1 per replica when n_rep <= G / 2):
- *     the monomers' cells of this collision, and the variate of each cell
+ *   - the monomers' cells of this collision, and the variate of each cell
  *     holding monomers drawn again for its solvent count plus its monomers
+ *     (G / n_rep workgroups per replica split its cells)
  *   - a replica whose MD recentred it (MPC::check_bounds MPC.cpp:160-190)
  *     binned again, with the recentring, into the redo buckets (the grid
  *     strides over its particle blocks), then -- after a grid-wide barrier
  *     (the 256 workgroups are all resident; the wait is bounded) -- all its
- *     cells' variates drawn for the new counts (k_collide_rows sums the
- *     redo bucket counts itself). */
+ *     cells' variates drawn for the new counts. */
 template <int PPT>
 __global__ __launch_bounds__(256) void k_rows_redo(DevState d, double h) {
     __shared__ uint32_t s_key[kBinHash], s_cnt[kBinHash];
     __shared__ uint32_t s_mkey[kMonoHash], s_mcnt[kMonoHash];
     __shared__ uint32_t s_h[256];
-    __shared__ uint32_t s_w[4];
-    constexpr int KS = 16;  /* counts per thread and pass of the scan */
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int G = (int)gridDim.x;
+    const int wpr = d.n_rep >= G ? 1 : G / d.n_rep;
+    const int sub = (int)blockIdx.x % wpr;
     bool any_redo = false;
     for (int r = 0; r < d.n_rep; ++r) any_redo |= d.ctl[r].redo != 0;
-    for (int r = blockIdx.x; r < d.n_rep; r += G) {
+    for (int r = (int)blockIdx.x / wpr; r < d.n_rep; r += G / wpr) {
         const RepDev& rep = d.reps[r];
-        if (!d.ctl[r].redo) {
-            const uint32_t t0 = d.rtile_base[r], nt = d.rtile_base[r + 1] - t0;
-            const uint32_t* tc = d.bcount[d.cur] + t0;
-            uint32_t carry = (uint32_t)rep.part_base;
-            for (uint32_t c0 = 0; c0 < nt; c0 += 256u * KS) {
-                /* each thread a run of KS consecutive counts, all loads first */
-                uint32_t v[KS];
-                const uint32_t i0 = c0 + threadIdx.x * KS;
-#pragma unroll
-                for (int q = 0; q < KS; ++q) v[q] = i0 + q < nt ? tc[i0 + q] : 0u;
-                uint32_t run = 0u;
-#pragma unroll
-                for (int q = 0; q < KS; ++q) { const uint32_t t = v[q]; v[q] = run; run += t; }
-                uint32_t inc = run;
-#pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    const uint32_t u = __shfl_up(inc, o);
-                    if (lane >= o) inc += u;
-                }
-                __syncthreads();
-                if (lane == 63) s_w[w] = inc;
-                __syncthreads();
-                uint32_t pre = 0u, tot = 0u;
-                for (int q = 0; q < 4; ++q) { pre += q < w ? s_w[q] : 0u; tot += s_w[q]; }
-                const uint32_t base = carry + pre + inc - run;
-#pragma unroll
-                for (int q = 0; q < KS; ++q)
-                    if (i0 + q < nt) d.tbase[t0 + i0 + q] = base + v[q];
-                carry += tot;
-            }
-        }
-    }
-    /* monomer cells: wpr workgroups per replica (those after the scanning
-     * ones when there are enough), part sub of them */
-    const bool apart = 2 * d.n_rep <= G;
-    const int wpr = apart ? (G - d.n_rep) / d.n_rep : 1;
-    const int mb = apart ? (int)blockIdx.x - d.n_rep : (int)blockIdx.x;
-    if (mb >= 0) {
-        const int sub = mb % wpr;
-        for (int r = mb / wpr; r < d.n_rep; r += apart ? d.n_rep : G) {
-            const RepDev& rep = d.reps[r];
-            if (rep.n_mono == 0) continue;
-            mono_cells(d, r, sub == 0, s_mkey, s_mcnt);
-            if (d.ctl[r].redo) continue;  /* all its cells are drawn below */
-            const uint32_t k = d.ctl[r].k_cur;
-            const bool inj = d.ctl[r].inj_active != 0;
-            for (uint32_t q = threadIdx.x; q < kMonoHash; q += 256) {
-                const uint32_t gc = s_mkey[q];
-                if (gc == kHashEmpty || (int)(((gc * 0x9E3779B1u) >> 8) % (uint32_t)wpr) != sub) continue;
-                draw_cell_gamma(d, rep, gc, d.cg_n[gc] + s_mcnt[q], k, inj);
-            }
+        if (rep.n_mono == 0) continue;
+        const RepCtl* c = d.ctl + r;
+        const double sh[3] = {c->shift[0], c->shift[1], c->shift[2]};
+        const uint32_t k = c->k_cur;
+        const bool inj = c->inj_active != 0, redo = c->redo != 0;
+        const uint32_t hs = mono_hash_size(rep.n_mono);
+        mono_cells(d, r, sh, sub == 0, hs, s_mkey, s_mcnt);
+        if (redo) continue;  /* all its cells are drawn below */
+        for (uint32_t q = threadIdx.x; q < hs; q += 256) {
+            const uint32_t gc = s_mkey[q];
+            if (gc == kHashEmpty || (int)(((gc * 0x9E3779B1u) >> 8) % (uint32_t)wpr) != sub) continue;
+            /* the first attempt's words drawn while the solvent count loads */
+            const uint32_t nf = d.cg_n[gc];
+            uint32_t w0[4];
+            draw4(rep.seed, rep.replica, TAG_GAMMA, gc - (uint32_t)rep.cell_base, k, 0, w0);
+            const uint32_t n = nf + s_mcnt[q];
+            d.cg_n[gc] = n;
+            d.cg_g[gc] = n >= 2 && !inj ? gamma_fast_w0(rep.seed, rep.replica, gc - (uint32_t)rep.cell_base, k, w0,
+                                                        1.5 * (n - 1), rep.kT)
+                                        : 0.0;
         }
     }
     if (!any_redo) return;
@@ -1826,12 +1859,16 @@ __global__ __launch_bounds__(256) void k_rows_redo(DevState d, double h) {
     for (int r = 0; r < d.n_rep; ++r) {
         if (!d.ctl[r].redo) continue;
         const RepDev& rep = d.reps[r];
-        if (rep.n_mono) mono_cells(d, r, false, s_mkey, s_mcnt);
+        const uint32_t hs = mono_hash_size(rep.n_mono);
+        if (rep.n_mono) {
+            const double sh[3] = {d.ctl[r].shift[0], d.ctl[r].shift[1], d.ctl[r].shift[2]};
+            mono_cells(d, r, sh, false, hs, s_mkey, s_mcnt);
+        }
         const uint32_t t0 = d.rtile_base[r], nt = d.rtile_base[r + 1] - t0;
         for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
             const uint32_t lt = t0 + t;
             const uint32_t n = __hip_atomic_load(&d.bcount_r[lt], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            tile_gammas<256>(d, r, lt, n, d.blc_r + (size_t)lt * (uint32_t)d.rt_cap, s_h,
+            tile_gammas<256>(d, r, lt, n, d.blc_r + (size_t)lt * (uint32_t)d.rt_cap, s_h, hs,
                              rep.n_mono ? s_mkey : nullptr, s_mcnt);
         }
     }
@@ -1846,22 +1883,23 @@ __global__ __launch_bounds__(256) void k_rows_redo(DevState d, double h) {
  *   (b) its bucket: particle indices, the particles gathered, streamed (as
  *       k_advect), their cell and rank in the cell (LDS atomic)
  *   (c) exclusive scan of the cell counts (the rotations were drawn in (a))
- *   (d) per cell (one lane each) the Gamma variate; per particle: position
- *       (and id) to the new buffer at its slot, velocity and image index
- *       staged in LDS at the slot
- *   (e) per cell, LPC lanes: momentum sums over its slots + monomers ->
- *       CMV, relative kinetic energy -> MBS scale (with angular momentum:
- *       CMP, inertia tensor, C, omega)
+ *   (d) per particle: position (and id) to the new buffer at its slot,
+ *       velocity and image index staged in LDS at the slot, its momentum
+ *       added to the cell's sums (LDS atomics); per cell the Gamma variate
+ *       drawn ahead; the monomers' momenta
+ *   (e) per cell: CMV (with angular momentum CMP); per particle and
+ *       monomer: relative kinetic energy (and the inertia tensor and C)
+ *       added to the cell's sums; per cell: the MBS scale (and omega)
  *   (f) per slot: v <- CMV + s R (v - CMV) (+ omega x r), written coalesced;
  *       cell offsets of the new buffer; monomer velocities; the bucket
  *       counts of the next collision cleared. */
 template <int NT, int CAP, int TCM, bool AMC>
-__global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(AMC ? 1 : 6)))
+void k_collide_rows(DevState d, double h) {
     static_assert(TCM <= NT && NT % TCM == 0 && CAP % NT == 0, "tile shape");
     constexpr int MCAP = 2 * TCM + 64;
     constexpr int PER = CAP / NT;
     constexpr int NW = NT / 64;
-    constexpr int LPC = NT / TCM;     /* lanes per cell in the cell sums */
     constexpr int TA = AMC ? TCM : 1;
     constexpr int CA = AMC ? CAP : 1;
     __shared__ float s_vx[CAP], s_vy[CAP], s_vz[CAP];
@@ -2035,7 +2073,12 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
         if (tid == 0) atomicOr(&ctl->status, ST_SORT_MISMATCH);
         return;
     }
-    /* (d) */
+    /* (d) per particle: position (and id) to the new buffer at its slot,
+     * velocity and image index staged in LDS at the slot; its momentum
+     * (MPC::calculateCMV MPC.cpp:314-322, the image offset taken off) and,
+     * with angular momentum, its coordinates inside the cell added to the
+     * cell's fixed-point sums (LDS atomics: particles arrive in bucket
+     * order, so a wave's adds spread over many cells) */
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         if (my_c[u] == ~0u) continue;
@@ -2043,6 +2086,7 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
         const uint32_t slot = s_off[lc] + (my_c[u] & 0xFFFFu);
         if (slot >= n) {  /* cannot happen: the counts and ranks agree */
             atomicOr(&ctl->status, ST_SORT_MISMATCH);
+            my_c[u] = ~0u;
             continue;
         }
         OX[base + slot] = fx[u]; OY[base + slot] = fy[u]; OZ[base + slot] = fz[u];
@@ -2054,6 +2098,19 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
             s_lx[AMC ? slot : 0] = my_l[AMC ? u : 0][0];
             s_ly[AMC ? slot : 0] = my_l[AMC ? u : 0][1];
             s_lz[AMC ? slot : 0] = my_l[AMC ? u : 0][2];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        if (my_c[u] == ~0u) continue;
+        const uint32_t lc = my_c[u] >> 16;
+        atomicAdd(&s_p[0][lc], to_fix((double)fvx[u] - (double)my_cy[u] * shear * LyD));
+        atomicAdd(&s_p[1][lc], to_fix((double)fvy[u]));
+        atomicAdd(&s_p[2][lc], to_fix((double)fvz[u]));
+        if (amc) {
+            atomicAdd(&s_am[0][lc], to_fix(1.0 * (double)my_l[AMC ? u : 0][0]));
+            atomicAdd(&s_am[1][lc], to_fix(1.0 * (double)my_l[AMC ? u : 0][1]));
+            atomicAdd(&s_am[2][lc], to_fix(1.0 * (double)my_l[AMC ? u : 0][2]));
         }
     }
     /* per cell, one lane each: the Gamma variate, which depends only on the
@@ -2102,14 +2159,21 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
     }
     __syncthreads();
 
-    /* (e) LPC consecutive lanes per cell, each summing every LPC-th slot of
-     * it; integer sums, so the split does not change a bit */
-    const uint32_t ci = (uint32_t)tid / LPC, sub = (uint32_t)tid % LPC;
-    auto group_sum = [&](unsigned long long v) {
-#pragma unroll
-        for (int o = 1; o < LPC; o <<= 1) v += __shfl_xor(v, o);
-        return v;
-    };
+    /* (e) per cell, one lane each: CMV (and with angular momentum CMP,
+     * MPC::calculateCMP MPC.cpp:324-333, inside the cell) from the sums */
+    for (uint32_t ci = tid; ci < nc; ci += NT) {
+        const double M = (double)s_cnt[ci] + from_fix(s_mm[ci]);
+        double c0 = from_fix(s_p[0][ci]), c1 = from_fix(s_p[1][ci]), c2 = from_fix(s_p[2][ci]);
+        if (M > 0) { c0 /= M; c1 /= M; c2 /= M; }
+        s_cmv[ci][0] = c0; s_cmv[ci][1] = c1; s_cmv[ci][2] = c2;
+        if (amc) {
+            const double inv = M > 0 ? 1.0 / M : 0.0;
+            s_cmp[ci][0] = from_fix(s_am[0][ci]) * inv;
+            s_cmp[ci][1] = from_fix(s_am[1][ci]) * inv;
+            s_cmp[ci][2] = from_fix(s_am[2][ci]) * inv;
+        }
+    }
+    __syncthreads();
     /* the angular-momentum terms of one member (MPC::calculateAngular
      * MPC.cpp:344-372): I about CMP and C = m r x (u - R u) */
     auto amc_terms = [&](uint32_t lc, double m, double lx, double ly, double lz, double ux, double uy, double uz,
@@ -2129,92 +2193,54 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
         acc[7] = to_fix(m * (rz * qx - rx * qz));
         acc[8] = to_fix(m * (rx * qy - ry * qx));
     };
-    const bool has_m = nmt > 0u;
-    const bool live = ci < nc;
-    const uint32_t a0 = live ? s_off[ci] : 0u, a1 = live ? s_off[ci] + s_cnt[ci] : 0u;
-    double c0 = 0.0, c1 = 0.0, c2 = 0.0;
-    unsigned long long e = 0ull;
-    {
-        /* MPC::calculateCMV */
-        unsigned long long q0 = 0ull, q1 = 0ull, q2 = 0ull, l0 = 0ull, l1 = 0ull, l2 = 0ull;
-        for (uint32_t q = a0 + sub; q < a1; q += LPC) {
-            q0 += to_fix((double)s_vx[q] - (double)s_cy[q] * shear * LyD);
-            q1 += to_fix((double)s_vy[q]);
-            q2 += to_fix((double)s_vz[q]);
-            if (amc) {
-                l0 += to_fix(1.0 * (double)s_lx[AMC ? q : 0]);
-                l1 += to_fix(1.0 * (double)s_ly[AMC ? q : 0]);
-                l2 += to_fix(1.0 * (double)s_lz[AMC ? q : 0]);
-            }
-        }
-        q0 = group_sum(q0); q1 = group_sum(q1); q2 = group_sum(q2);
-        if (amc) { l0 = group_sum(l0); l1 = group_sum(l1); l2 = group_sum(l2); }
-        if (live) {
-            q0 += s_p[0][ci]; q1 += s_p[1][ci]; q2 += s_p[2][ci];
-            const double M = (double)s_cnt[ci] + from_fix(s_mm[ci]);
-            c0 = from_fix(q0); c1 = from_fix(q1); c2 = from_fix(q2);
-            if (M > 0) { c0 /= M; c1 /= M; c2 /= M; }
-            if (sub == 0) { s_cmv[ci][0] = c0; s_cmv[ci][1] = c1; s_cmv[ci][2] = c2; }
-            if (amc && sub == 0) {  /* MPC::calculateCMP MPC.cpp:324-333, inside the cell */
-                const double inv = M > 0 ? 1.0 / M : 0.0;
-                s_cmp[ci][0] = from_fix(l0 + s_am[0][ci]) * inv;
-                s_cmp[ci][1] = from_fix(l1 + s_am[1][ci]) * inv;
-                s_cmp[ci][2] = from_fix(l2 + s_am[2][ci]) * inv;
-            }
-        }
-    }
-    if (amc) __syncthreads();  /* CMP of every cell before the inertia sums */
-    unsigned long long am[AMC ? 9 : 1];
+    /* per particle (its own, bucket order, read back from its slot): the
+     * relative kinetic energy of the MBS thermostat (MPC.cpp:277-279) and
+     * the angular-momentum terms */
 #pragma unroll
-    for (int j = 0; j < (AMC ? 9 : 1); ++j) am[j] = 0ull;
-    /* MPC::thermostat relative kinetic energy (MPC.cpp:277-279) */
-    for (uint32_t q = a0 + sub; q < a1; q += LPC) {
+    for (int u = 0; u < PER; ++u) {
+        if (my_c[u] == ~0u) continue;
+        const uint32_t lc = my_c[u] >> 16;
+        const uint32_t q = s_off[lc] + (my_c[u] & 0xFFFFu);
         const double vx = (double)s_vx[q] - (double)s_cy[q] * shear * LyD;
-        const double dx = vx - c0, dy = (double)s_vy[q] - c1, dz = (double)s_vz[q] - c2;
-        e += to_fix((dx * dx + dy * dy) + dz * dz);
+        const double dx = vx - s_cmv[lc][0], dy = (double)s_vy[q] - s_cmv[lc][1], dz = (double)s_vz[q] - s_cmv[lc][2];
+        atomicAdd(&s_me[lc], to_fix((dx * dx + dy * dy) + dz * dz));
         if (amc) {
             unsigned long long t9[9];
-            amc_terms(ci, 1.0, s_lx[AMC ? q : 0], s_ly[AMC ? q : 0], s_lz[AMC ? q : 0], dx, dy, dz, t9);
+            amc_terms(lc, 1.0, s_lx[AMC ? q : 0], s_ly[AMC ? q : 0], s_lz[AMC ? q : 0], dx, dy, dz, t9);
 #pragma unroll
-            for (int j = 0; j < (AMC ? 9 : 1); ++j) am[j] += t9[j];
+            for (int j = 0; j < 9; ++j) atomicAdd(&s_am[3 + j][lc], t9[j]);
         }
     }
-    e = group_sum(e);
-    if (amc)
-#pragma unroll
-        for (int j = 0; j < (AMC ? 9 : 1); ++j) am[j] = group_sum(am[j]);
-    if (has_m) {
-        __syncthreads();  /* every CMV before the monomers' energies */
-        for (uint32_t j = tid; j < nmt; j += NT) {
-            const uint32_t me = s_ml[j];
-            const uint32_t lc = me >> 16;
-            const uint32_t mi = rep.mono_base + (me & 0xFFFFu);
-            const double dx = d.mvel[3 * mi] - s_cmv[lc][0];
-            const double dy = d.mvel[3 * mi + 1] - s_cmv[lc][1];
-            const double dz = d.mvel[3 * mi + 2] - s_cmv[lc][2];
-            atomicAdd(&s_me[lc], to_fix(d.mmass[mi] * ((dx * dx + dy * dy) + dz * dz)));
-            if (amc) {
-                double lx, ly, lz;
-                mono_local(mi, lx, ly, lz);
-                unsigned long long t9[9];
-                amc_terms(lc, d.mmass[mi], lx, ly, lz, dx, dy, dz, t9);
-                for (int q = 0; q < 9; ++q) atomicAdd(&s_am[3 + q][lc], t9[q]);
-            }
+    /* the monomers' energies and terms, against the same CMV */
+    for (uint32_t j = tid; j < nmt; j += NT) {
+        const uint32_t me = s_ml[j];
+        const uint32_t lc = me >> 16;
+        const uint32_t mi = rep.mono_base + (me & 0xFFFFu);
+        const double dx = d.mvel[3 * mi] - s_cmv[lc][0];
+        const double dy = d.mvel[3 * mi + 1] - s_cmv[lc][1];
+        const double dz = d.mvel[3 * mi + 2] - s_cmv[lc][2];
+        atomicAdd(&s_me[lc], to_fix(d.mmass[mi] * ((dx * dx + dy * dy) + dz * dz)));
+        if (amc) {
+            double lx, ly, lz;
+            mono_local(mi, lx, ly, lz);
+            unsigned long long t9[9];
+            amc_terms(lc, d.mmass[mi], lx, ly, lz, dx, dy, dz, t9);
+            for (int q = 0; q < 9; ++q) atomicAdd(&s_am[3 + q][lc], t9[q]);
         }
-        __syncthreads();
     }
+    __syncthreads();
     /* MBS scale sqrt(Gamma / E_rel) for n >= 2 (MPC.cpp:287-289); with the
      * angular-momentum flag, omega = s I^-1 C for n > 2 (MPC.cpp:260, 373) */
-    if (live && sub == 0) {
+    for (uint32_t ci = tid; ci < nc; ci += NT) {
         const uint32_t nn = s_cnt[ci] + s_nm[ci];
-        const double sc = nn >= 2 ? sqrt(s_g[ci] / (from_fix(e + s_me[ci]) * 0.5)) : -1.0;
+        const double sc = nn >= 2 ? sqrt(s_g[ci] / (from_fix(s_me[ci]) * 0.5)) : -1.0;
         s_g[ci] = sc;
         if (amc) {
             double w[3] = {0.0, 0.0, 0.0};
             if (nn > 2) {
                 unsigned long long A6[9];
 #pragma unroll
-                for (int j = 0; j < 9; ++j) A6[j] = am[AMC ? j : 0] + s_am[3 + j][ci];
+                for (int j = 0; j < 9; ++j) A6[j] = s_am[3 + j][ci];
                 const double A[3][3] = {{from_fix(A6[0]), from_fix(A6[3]), from_fix(A6[4])},
                                         {from_fix(A6[3]), from_fix(A6[1]), from_fix(A6[5])},
                                         {from_fix(A6[4]), from_fix(A6[5]), from_fix(A6[2])}};
@@ -2228,7 +2254,6 @@ __global__ __launch_bounds__(NT) void k_collide_rows(DevState d, double h) {
         }
     }
     __syncthreads();
-
     /* (f) v <- s v + (1-s) CMV, then v <- CMV + R (v - CMV) (MPC.cpp:290-292,
      * 266-267); the image offset is undone as the unshift wrap does */
     auto apply = [&](uint32_t lc, double& vx, double& vy, double& vz) {
@@ -2917,7 +2942,7 @@ hipError_t launch_advect(const DevState& d, hipStream_t st, double h, int mode) 
 
 hipError_t launch_cell_gamma(const DevState& d, hipStream_t st) {
     if (!d.n_rtiles) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_cell_gamma, dim3(d.n_rtiles), dim3(64), 0, st, d);
+    hipLaunchKernelGGL(k_cell_gamma, dim3(d.n_rtiles + (uint32_t)d.n_rep), dim3(64), 0, st, d);
     return hipGetLastError();
 }
 

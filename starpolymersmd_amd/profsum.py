@@ -15,13 +15,19 @@ import argparse
 import csv
 from collections import defaultdict
 
-PIPELINE = ("k_bin", "k_scan", "k_perm", "k_collide")
+PIPELINES = {
+    # sort path: bin, scan, permutation, collide
+    "sort": ("k_bin", "k_scan", "k_perm", "k_collide"),
+    # row-tile path: advect, the cells' Gamma draws, the redo kernel, row-window collide
+    "rows": ("k_advect", "k_cell_gamma", "k_rows_redo", "k_collide_rows"),
+}
 
 
 def short(name: str) -> str:
     n = name.split("(")[0]
-    for k in ("k_md", "k_prologue", "k_bin", "k_sort_redo", "k_scan_redo", "k_scan", "k_scatter", "k_perm", "k_collide",
-              "k_fluid_init", "k_fluid_mean", "k_temperature"):
+    for k in ("k_md", "k_prologue", "k_bin", "k_sort_redo", "k_scan_redo", "k_scan", "k_scatter", "k_perm",
+              "k_collide_rows", "k_collide", "k_advect", "k_cell_gamma", "k_rows_redo", "k_fluid_init",
+              "k_fluid_mean", "k_temperature"):
         if k in n:
             return k
     return n[:48]
@@ -98,6 +104,8 @@ def main():
     ap.add_argument("--copy-bytes", type=float, default=0.0)
     ap.add_argument("--particles", type=int, default=0)
     ap.add_argument("--bytes-per-particle", type=float, default=48.0)
+    ap.add_argument("--pipeline", choices=sorted(PIPELINES), default="sort",
+                    help="which collision path's kernels make up the pipeline")
     ap.add_argument("--read-scale", type=float, default=0.0,
                     help="FETCH_SIZE scale from a probe of the kernels' access shape (overrides the copy's)")
     ap.add_argument("--write-scale", type=float, default=0.0)
@@ -106,6 +114,8 @@ def main():
                          "whole, a coalesced stream's at half)")
     a = ap.parse_args()
     kscale = {kv.split("=")[0]: float(kv.split("=")[1]) for kv in a.kernel_read_scale}
+    pipeline = PIPELINES[a.pipeline]
+    coll = pipeline[-1]  # the collide runs once per collision
 
     stats = load_stats(a.stats)
     merged = defaultdict(lambda: [0, 0.0])
@@ -117,16 +127,16 @@ def main():
     for k, (n, tot) in sorted(merged.items(), key=lambda kv: -kv[1][1]):
         print(f"| {k} | {n} | {tot / 1e6:.3f} | {tot / n / 1e3:.2f} |")
     # per collision: every kernel's total is divided by the number of
-    # collisions (k_collide runs once per collision)
-    ncoll = merged["k_collide"][0] if "k_collide" in merged else 1
+    # collisions of the pipeline's path
+    ncoll = merged[coll][0] if coll in merged else 1
     print(f"\ncollisions profiled: {ncoll}")
     print("\n| kernel | us per collision |")
     print("|---|---:|")
-    for k in ("k_prologue", "k_md", "k_bin", "k_scan", "k_scan_redo", "k_scatter", "k_perm", "k_sort_redo", "k_collide"):
+    for k in ("k_prologue", "k_md") + pipeline + ("k_sort_redo",):
         if k in merged:
             print(f"| {k} | {merged[k][1] / ncoll / 1e3:.2f} |")
-    pipe_us = sum(merged[k][1] / ncoll / 1e3 for k in PIPELINE if k in merged)
-    print(f"\ncollision pipeline ({' + '.join(PIPELINE)}): {pipe_us:.2f} us per collision")
+    pipe_us = sum(merged[k][1] / ncoll / 1e3 for k in pipeline if k in merged)
+    print(f"\ncollision pipeline ({' + '.join(pipeline)}): {pipe_us:.2f} us per collision")
     if a.particles:
         alg = a.particles * a.bytes_per_particle
         print(f"algorithmic bytes per collision: {alg:.0f} ({a.bytes_per_particle:g} B x {a.particles} particles)")
@@ -148,7 +158,7 @@ def main():
               + "".join(f" ({k}: x {v:.3f})" for k, v in kscale.items()) + f", WRITE_SIZE x {ws:.3f}")
         f = load_counter(a.fetch, "FETCH_SIZE")
         w = load_counter(a.write, "WRITE_SIZE")
-        nf = f.get("k_collide", (0.0, 1))[1]
+        nf = f.get(coll, (0.0, 1))[1]
         print("\nper collision (dispatch means x dispatches / collisions):")
         print("\n| kernel | dispatches | FETCH KiB | WRITE KiB | scaled read MB | scaled write MB |")
         print("|---|---:|---:|---:|---:|---:|")
@@ -156,9 +166,9 @@ def main():
         for k in sorted(set(f) | set(w)):
             fk, nk = f.get(k, (0.0, 0))
             wk, _ = w.get(k, (0.0, 0))
-            per = nk / nf if k in PIPELINE else 1.0
+            per = nk / nf if k in pipeline else 1.0
             rb, wb = fk * 1024 * kscale.get(k, fs) * per, wk * 1024 * ws * per
-            if k in PIPELINE:
+            if k in pipeline:
                 tot += rb + wb
             print(f"| {k} | {nk} | {fk * per:.0f} | {wk * per:.0f} | {rb / 1e6:.1f} | {wb / 1e6:.1f} |")
         print(f"\nHBM traffic of the collision pipeline: {tot:.0f} B per collision")
