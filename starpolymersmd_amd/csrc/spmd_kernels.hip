@@ -48,6 +48,14 @@ __device__ __forceinline__ unsigned long long to_fix(double v) {
 }
 __device__ __forceinline__ double from_fix(unsigned long long v) { return (double)(long long)v * kUnfix; }
 
+/* A word that device-scope atomics of an earlier kernel built (the row
+ * tiles' arrival counts): read with an agent-scope atomic load, like the
+ * atomics themselves, never through a copy of its line that a plain access
+ * of an earlier kernel left in this XCD's L2. */
+__device__ __forceinline__ uint32_t load_count(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 /* Consecutive logical blocks on one XCD: the dispatcher places block b on
  * XCD b % 8, so logical = start(xcd) + b / 8 keeps neighbouring particle
  * chunks and cell tiles in one XCD's L2. */
@@ -1759,7 +1767,7 @@ __global__ __launch_bounds__(64) void k_cell_gamma(DevState d) {
             uint32_t v[KS];
             const uint32_t i0 = c0 + (uint32_t)lane * KS;
 #pragma unroll
-            for (int q = 0; q < KS; ++q) v[q] = i0 + q < nt ? tc[i0 + q] : 0u;
+            for (int q = 0; q < KS; ++q) v[q] = i0 + q < nt ? load_count(tc + i0 + q) : 0u;
             uint32_t run = 0u;
 #pragma unroll
             for (int q = 0; q < KS; ++q) { const uint32_t t = v[q]; v[q] = run; run += t; }
@@ -1778,7 +1786,7 @@ __global__ __launch_bounds__(64) void k_cell_gamma(DevState d) {
         return;
     }
     const int r = find_rep(d.rtile_base, d.n_rep, b);
-    tile_gammas<64>(d, r, b, d.bcount[d.cur][b], d.blc + (size_t)b * (uint32_t)d.rt_cap, s_h, 0u, nullptr, nullptr);
+    tile_gammas<64>(d, r, b, load_count(d.bcount[d.cur] + b), d.blc + (size_t)b * (uint32_t)d.rt_cap, s_h, 0u, nullptr, nullptr);
     if (b == d.reps[r].rt_base && threadIdx.x == 0) d.ctl[r].cg_k = d.ctl[r].k_cur + 1u;
 }
 
@@ -1867,7 +1875,7 @@ __global__ __launch_bounds__(256) void k_rows_redo(DevState d, double h) {
         const uint32_t t0 = d.rtile_base[r], nt = d.rtile_base[r + 1] - t0;
         for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
             const uint32_t lt = t0 + t;
-            const uint32_t n = __hip_atomic_load(&d.bcount_r[lt], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t n = load_count(d.bcount_r + lt);
             tile_gammas<256>(d, r, lt, n, d.blc_r + (size_t)lt * (uint32_t)d.rt_cap, s_h, hs,
                              rep.n_mono ? s_mkey : nullptr, s_mcnt);
         }
@@ -1946,7 +1954,7 @@ void k_collide_rows(DevState d, double h) {
     const bool amc = AMC && rep.amc;
 
     /* (a) */
-    const uint32_t n = cnt[lt];
+    const uint32_t n = load_count(cnt + lt);
     for (uint32_t i = tid; i < nc; i += NT) {
         s_cnt[i] = 0u;
         s_nm[i] = 0u;
@@ -1959,7 +1967,7 @@ void k_collide_rows(DevState d, double h) {
     if (tid == 0) { s_mn = 0u; s_bad = 0u; }
     if (rd) {  /* recentred replica (rare): the redo counts of all earlier tiles */
         uint32_t pre = 0u;
-        for (uint32_t q = tid; q < t; q += NT) pre += cnt[rep.rt_base + q];
+        for (uint32_t q = tid; q < t; q += NT) pre += load_count(cnt + rep.rt_base + q);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o);
         if (lane == 0) s_wsum[wv] = pre;
