@@ -204,6 +204,9 @@ int upload_tables(spmd_sys* s) {
     dfree(d.blc_r);
     if (int rc = dalloc(s, &d.bcount_r, d.n_rtiles)) return rc;
     if (int rc = dalloc(s, &d.tbase, d.n_rtiles)) return rc;
+    dfree(d.ttag);
+    if (int rc = dalloc(s, &d.ttag, d.n_rtiles)) return rc;
+    HIPCHK(s, hipMemset(d.ttag, 0, 4ull * d.n_rtiles));
     const size_t nb = d.rt_cap ? (size_t)d.n_rtiles * (size_t)d.rt_cap : 4;
     if (int rc = dalloc(s, &d.bucket, nb)) return rc;
     if (int rc = dalloc(s, &d.bucket_r, nb)) return rc;
@@ -801,7 +804,7 @@ int spmd_destroy(spmd_sys* s) {
     dfree(d.rank); dfree(d.pcell); dfree(d.perm); dfree(d.counts); dfree(d.scan_flags); dfree(d.scan_ticket);
     dfree(d.grid_bar);
     for (int b = 0; b < 2; ++b) { dfree(d.offs[b]); dfree(d.bcount[b]); }
-    dfree(d.bcount_r); dfree(d.tbase); dfree(d.bucket); dfree(d.bucket_r); dfree(d.rtile_base);
+    dfree(d.bcount_r); dfree(d.tbase); dfree(d.ttag); dfree(d.bucket); dfree(d.bucket_r); dfree(d.rtile_base);
     dfree(d.blc); dfree(d.blc_r); dfree(d.cg_n); dfree(d.cg_g);
     dfree(d.mpos); dfree(d.mvel); dfree(d.mforce); dfree(d.mvpos); dfree(d.mmass); dfree(d.mbtype);
     dfree(d.vl); dfree(d.bonds); dfree(d.mcell);

@@ -115,6 +115,7 @@ struct DevState {
     uint32_t* bucket;      /* particle indices per row tile, rt_cap each */
     uint32_t* bucket_r;    /* the rebinning's */
     uint32_t* tbase;       /* first slot of each row tile in the new buffer */
+    uint32_t* ttag;        /* per row tile: k_cur + 1 of the collision whose monomers it holds */
     uint8_t* blc;          /* beside each bucket entry: the particle's cell in its tile */
     uint8_t* blc_r;        /* the rebinning's */
     /* per cell (C_tot): the Gamma variate of the coming collision drawn

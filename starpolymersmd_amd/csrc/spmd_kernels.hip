@@ -48,6 +48,11 @@ __device__ __forceinline__ unsigned long long to_fix(double v) {
 }
 __device__ __forceinline__ double from_fix(unsigned long long v) { return (double)(long long)v * kUnfix; }
 
+/* A value every lane of the wave holds alike, in a scalar register (the
+ * compiler cannot tell that a load from a uniform address is uniform when
+ * the kernel also writes that memory). */
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
 /* A word that device-scope atomics of an earlier kernel built (the row
  * tiles' arrival counts): read with an agent-scope atomic load, like the
  * atomics themselves, never through a copy of its line that a plain access
@@ -1818,6 +1823,14 @@ __global__ __launch_bounds__(256) void k_rows_redo(DevState d, double h) {
         const bool inj = c->inj_active != 0, redo = c->redo != 0;
         const uint32_t hs = mono_hash_size(rep.n_mono);
         mono_cells(d, r, sh, sub == 0, hs, s_mkey, s_mcnt);
+        if (sub == 0)  /* the row tiles holding monomers, tagged for the collide */
+            for (uint32_t q = threadIdx.x; q < hs; q += 256) {
+                const uint32_t gc = s_mkey[q];
+                if (gc == kHashEmpty) continue;
+                uint32_t t, lc;
+                tile_of_cell(rep, gc - (uint32_t)rep.cell_base, t, lc);
+                d.ttag[rep.rt_base + t] = k + 1u;
+            }
         if (redo) continue;  /* all its cells are drawn below */
         for (uint32_t q = threadIdx.x; q < hs; q += 256) {
             const uint32_t gc = s_mkey[q];
@@ -1979,7 +1992,10 @@ void k_collide_rows(DevState d, double h) {
             for (int q = 0; q < NW; ++q) p += s_wsum[q];
         s_pre = rd ? pr0 + p : d.tbase[lt];
     }
-    for (uint32_t j = tid; j < rep.n_mono; j += NT) {
+    /* the replica's monomers in this tile's cells, if the redo kernel
+     * tagged the tile as holding any this collision */
+    const bool tagged = uniform_u32(d.ttag[lt]) == ctl->k_cur + 1u;
+    for (uint32_t j = tid; tagged && j < rep.n_mono; j += NT) {
         const uint32_t rel = d.mcell[rep.mono_base + j] - gc0;
         if (rel < nc) {
             const uint32_t slot = atomicAdd(&s_mn, 1u);
