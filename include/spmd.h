@@ -316,14 +316,23 @@ int spmd_mlp_forward(spmd_mlp* m, const float* x_dev, float* out_dev, uint64_t M
  * activations kept in the workspace for a backward pass) */
 int spmd_mlp_forward_unfused(spmd_mlp* m, const float* x_dev, float* out_dev, uint64_t M);
 /* gradient of the mean Gaussian negative log-likelihood of y_dev[M][3]
- * (FP32, parameter order as above) and the loss */
+ * (FP32, parameter order as above) and the loss: one fused kernel (forward,
+ * loss, backward and weight-gradient products per 64-row chunk, partial
+ * gradients in registers) and one that sums the workgroups' partials */
 int spmd_mlp_gradient(spmd_mlp* m, const float* x_dev, const float* y_dev, uint64_t M, float* grad_host,
                       double* loss);
-/* one Adam step (beta 0.9/0.999, eps 1e-8) on the mean loss of the batch */
+/* the same gradient through the per-layer products (a cross-check of the
+ * fused pass) */
+int spmd_mlp_gradient_unfused(spmd_mlp* m, const float* x_dev, const float* y_dev, uint64_t M, float* grad_host,
+                              double* loss);
+/* one Adam step (beta 0.9/0.999, eps 1e-8) on the mean loss of the batch
+ * (the fused pass; the sum kernel applies the step and updates the BF16
+ * weight copies in place) */
 int spmd_mlp_train_step(spmd_mlp* m, const float* x_dev, const float* y_dev, uint64_t M, double lr, double* loss);
 /* `steps` Adam steps on minibatches of `batch` rows drawn with replacement
- * (Philox keyed by seed) from x_dev[N][24], y_dev[N][3]; loss_hist[steps]
- * optional (each entry costs a device round trip) */
+ * (Philox keyed by seed, inside the fused pass) from x_dev[N][24],
+ * y_dev[N][3]; loss_hist[steps] optional (kept on the device, copied once
+ * at the end) */
 int spmd_mlp_fit(spmd_mlp* m, const float* x_dev, const float* y_dev, uint64_t N, uint32_t batch, uint32_t steps,
                  double lr, uint64_t seed, double* loss_hist);
 /* plain device buffers for callers without their own device allocator */

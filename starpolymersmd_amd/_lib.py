@@ -107,6 +107,7 @@ _SIGS = {
     "spmd_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     "spmd_mlp_forward_unfused": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     "spmd_mlp_gradient": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, _f, _d]),
+    "spmd_mlp_gradient_unfused": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, _f, _d]),
     "spmd_mlp_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_double, _d]),
     "spmd_mlp_fit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double,
                                C.c_uint64, _d]),

@@ -143,13 +143,15 @@ class ClosureNet:
         self._chk(fn(self.h, C.c_void_p(xb.ptr), C.c_void_p(ob.ptr), x.shape[0]))
         return ob.download((x.shape[0], 6))
 
-    def gradient(self, x: np.ndarray, y: np.ndarray):
+    def gradient(self, x: np.ndarray, y: np.ndarray, fused: bool = True):
+        """Mean loss and its gradient (parameter order) over the rows
+        (fused=False: through the per-layer products)."""
         xb = DeviceBuffer.from_numpy(np.asarray(x, np.float32))
         yb = DeviceBuffer.from_numpy(np.asarray(y, np.float32))
         g = np.empty(self.n_params, np.float32)
         loss = C.c_double()
-        self._chk(self._lib.spmd_mlp_gradient(self.h, C.c_void_p(xb.ptr), C.c_void_p(yb.ptr), len(x),
-                                              ptr(g, C.c_float), C.byref(loss)))
+        fn = self._lib.spmd_mlp_gradient if fused else self._lib.spmd_mlp_gradient_unfused
+        self._chk(fn(self.h, C.c_void_p(xb.ptr), C.c_void_p(yb.ptr), len(x), ptr(g, C.c_float), C.byref(loss)))
         return loss.value, g
 
     def fit(self, data: "ClosureData", steps: int = 1000, batch: int = 65536, lr: float = 1e-3,

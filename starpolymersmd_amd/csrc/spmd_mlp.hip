@@ -236,34 +236,6 @@ __global__ __launch_bounds__(256) void k_reduce(const float* __restrict__ P, con
     }
 }
 
-/* minibatch of B rows drawn uniformly with replacement (Philox, counter =
- * (row, step, 7, 0), key = seed) from X (N x 24) and Y (N x 3) */
-__global__ __launch_bounds__(256) void k_gather(const float* __restrict__ X, const float* __restrict__ Y, uint64_t N,
-                                                float* __restrict__ Xb, float* __restrict__ Yb, uint32_t B,
-                                                uint64_t seed, uint32_t step) {
-    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
-    if (b >= B) return;
-    uint32_t w[4];
-    spmd::philox10(b, step, 7u, 0u, (uint32_t)seed, (uint32_t)(seed >> 32), w);
-    const uint64_t r = ((((uint64_t)w[0] << 32) | w[1]) % N);
-    for (int c = 0; c < kIn; ++c) Xb[(uint64_t)b * kIn + c] = X[r * kIn + c];
-    for (int c = 0; c < 3; ++c) Yb[(uint64_t)b * 3 + c] = Y[r * 3 + c];
-}
-
-/* Adam on the FP32 master parameters */
-__global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
-                                              const float* __restrict__ g, int n, float lr, float b1, float b2,
-                                              float c1, float c2, float eps) {
-    const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= n) return;
-    const float gg = g[e];
-    const float mm = b1 * m[e] + (1.0f - b1) * gg;
-    const float vv = b2 * v[e] + (1.0f - b2) * gg * gg;
-    m[e] = mm;
-    v[e] = vv;
-    p[e] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
-}
-
 /* BF16 copies of one layer's FP32 weights W[nv][kv]: padded W (N x K) and
  * its transpose Wt (KT x NT, with NT >= N rows of zero padding) */
 __global__ __launch_bounds__(256) void k_pack(const float* __restrict__ W, int nv, int kv, int N, int K,
@@ -460,6 +432,536 @@ __global__ __launch_bounds__(kFwdThreads, 2) void k_forward_fused(const float* _
     }
 }
 
+/* ---------------------------------------------- fused training step */
+
+/* Transposed BF16 weights for the backward data products dA = W^T dZ, laid
+ * out like the forward pack with rows and columns swapped: tile mt holds
+ * the layer's inputs 16mt..16mt+15 as the matrix-core A operand's rows and
+ * block kb its outputs kperm(kb, g, 0..7) as the K values, so the output
+ * gradients in the order the previous product leaves them (lane l: outputs
+ * 4(l/16)+r of row l%16 in each 16-output tile) are the B operand as they
+ * stand. */
+constexpr int bL3 = 0;                      /* W3^T: 8 tiles x 64 lanes x 8 (outputs padded to 32) */
+constexpr int bL2 = bL3 + 8 * 64 * 8;       /* W2^T: 8 tiles x 4 blocks x 64 x 8 */
+constexpr int bL1 = bL2 + 8 * 4 * 64 * 8;   /* W1^T */
+constexpr int bEnd = bL1 + 8 * 4 * 64 * 8;  /* 36,864 BF16 */
+
+__global__ __launch_bounds__(256) void k_pack_bwd(const float* __restrict__ p, uint16_t* __restrict__ wb) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= bEnd) return;
+    const int i = e & 7, lane = (e >> 3) & 63, s = lane & 15, g = lane >> 4;
+    float v;
+    if (e < bL2) {
+        const int mt = e >> 9, o = kperm(0, g, i);
+        v = o < kOut ? p[oW3 + o * kH + 16 * mt + s] : 0.0f;
+    } else {
+        const int q = e - (e < bL1 ? bL2 : bL1), off = e < bL1 ? oW2 : oW1;
+        const int mt = q >> 11, kb = (q >> 9) & 3;
+        v = p[off + kperm(kb, g, i) * kH + 16 * mt + s];
+    }
+    wb[e] = f2bf(v);
+}
+
+/* the block, lane group and position with kperm(kb, g, i) = k */
+__device__ __forceinline__ void kperm_inv(int k, int& kb, int& g, int& i) {
+    kb = k >> 5;
+    const int j = k & 31;
+    g = (j & 15) >> 2;
+    i = (j & 3) + (j >= 16 ? 4 : 0);
+}
+
+/* where weight parameter e sits in the forward pack (pf) and in the
+ * transposed pack (pb); -1 where it has no place (biases, W0 in pb) */
+__device__ __forceinline__ void pack_pos(int e, int& pf, int& pb) {
+    pf = -1;
+    pb = -1;
+    int kb, g, i;
+    if (e < oB0) {
+        const int n = e / kIn, k = e % kIn;
+        pf = fL0 + ((n >> 4) * 64 + 16 * (k >> 3) + (n & 15)) * 8 + (k & 7);
+    } else if ((e >= oW1 && e < oB1) || (e >= oW2 && e < oB2)) {
+        const bool l1 = e < oB1;
+        const int q = e - (l1 ? oW1 : oW2), n = q >> 7, k = q & (kH - 1);
+        kperm_inv(k, kb, g, i);
+        pf = (l1 ? fL1 : fL2) + (((n >> 4) * 4 + kb) * 64 + 16 * g + (n & 15)) * 8 + i;
+        kperm_inv(n, kb, g, i);
+        pb = (l1 ? bL1 : bL2) + (((k >> 4) * 4 + kb) * 64 + 16 * g + (k & 15)) * 8 + i;
+    } else if (e >= oW3 && e < oB3) {
+        const int q = e - oW3, o = q >> 7, k = q & (kH - 1);
+        kperm_inv(k, kb, g, i);
+        pf = fL3 + (kb * 64 + 16 * g + o) * 8 + i;
+        kperm_inv(o, kb, g, i);  /* kb = 0: o < 6 */
+        pb = bL3 + ((k >> 4) * 64 + 16 * g + (k & 15)) * 8 + i;
+    }
+}
+
+/* One workgroup of eight waves (two per SIMD) takes 64-row chunks and
+ * keeps the weight gradients of all its chunks in registers.  Waves 0-3
+ * work on rows (16 each), waves 4-7 on the weight gradients; the two roles
+ * run their own loops over the same chunks and meet at the same eight
+ * barriers per chunk, so each role's registers stay within half a SIMD's
+ * and one role's products run while the other waits on memory.
+ *   rows:  forward as k_forward_fused (weights in LDS), SiLU' kept per
+ *          value, every layer's input written transposed (feature-major) to
+ *          the staging area; the Gaussian NLL and its gradient dZ4 (lanes
+ *          exchange the mean and log-variance of a row); backward dA =
+ *          W^T dZ with the transposed pack read from L2, dZ = dA SiLU'(z),
+ *          each dZ^T staged over the activation it replaces once the
+ *          gradient waves have read that
+ *   dW:    dW += dZ^T a over the chunk's rows (rows as the products' K
+ *          dimension, both operands read from the staging); gradient wave q
+ *          owns output tiles 2q, 2q+1 of each layer (input tiles 2q, 2q+1
+ *          of dW3); bias sums from the staged dZ^T
+ * At the end each workgroup writes its partial gradient (parameter order)
+ * and its loss sum; k_train_finish adds them up. */
+constexpr int kTrThreads = 512;
+constexpr int kTrRows = 64;
+constexpr int kTS = kTrRows + 8;  /* staging row stride (BF16): 144 bytes, 16-byte reads conflict-free */
+constexpr int sA0o = 0;                      /* inputs^T (32 features) */
+constexpr int sA1o = sA0o + kInP * kTS;      /* a1^T, then dZ1^T */
+constexpr int sA2o = sA1o + kH * kTS;        /* a2^T, then dZ2^T */
+constexpr int sA3o = sA2o + kH * kTS;        /* a3^T, then dZ3^T */
+constexpr int sD4o = sA3o + kH * kTS;        /* dZ4^T (16 outputs) */
+constexpr int sEnd = sD4o + kOutP * kTS;
+/* LDS: the staging first (its constant offsets then fit the 16-bit offset
+ * field of the LDS instructions), the forward weights, the biases */
+constexpr size_t kTrLdsS = (size_t)sEnd * 2;
+constexpr size_t kTrLds = kTrLdsS + (size_t)fEnd * 2 + (size_t)fBias * 4 + (4 + 2 * kInP) * sizeof(float);
+static_assert(kTrLds <= 160 * 1024, "LDS of one CU");
+
+struct TrainArgs {
+    const float* x;   /* rows of 24 features */
+    const float* y;   /* rows of 3 targets */
+    uint64_t n_src;   /* > 0: row b of the batch is source row philox(b, step) mod n_src */
+    uint64_t M;       /* batch rows */
+    uint64_t seed;
+    uint32_t step;
+    const float* mu;
+    const float* sd;
+    const float* sdy;
+    const uint16_t* wf;
+    const uint16_t* wb;
+    const float* p;
+    float* part;      /* [gridDim.x][kParams] */
+    float* lpart;     /* [gridDim.x] */
+};
+
+__device__ __forceinline__ bf16x8 ld8(const uint16_t* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+/* z + b: SiLU into the next product's B operands and to the staging,
+ * SiLU'(z) kept (BF16) for the backward pass */
+__device__ __forceinline__ void tr_fwd_epilogue(const f32x4 (&acc)[8], const float* __restrict__ bias, int g,
+                                                uint16_t* sT, int rw, bf16x8 (&bh)[4], bf16x8 (&ds)[4]) {
+#pragma unroll
+    for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int n = mt * 16 + 4 * g + r;
+            const float z = acc[mt][r] + bias[n];
+            const float sg = __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+            const short av = bf16_rne(z * sg);
+            bh[mt >> 1][(mt & 1) * 4 + r] = av;
+            ds[mt >> 1][(mt & 1) * 4 + r] = bf16_rne(sg * (1.0f + z * (1.0f - sg)));
+            sT[n * kTS + rw] = (uint16_t)av;
+        }
+}
+
+/* dZ = dA SiLU'(z) (BF16), the next backward product's B operands */
+__device__ __forceinline__ void tr_bwd_epilogue(const f32x4 (&acc)[8], const bf16x8 (&ds)[4], bf16x8 (&bd)[4]) {
+#pragma unroll
+    for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int q = (mt & 1) * 4 + r;
+            bd[mt >> 1][q] = bf16_rne(acc[mt][r] * bf2f((uint16_t)ds[mt >> 1][q]));
+        }
+}
+
+/* dZ^T to the staging from the B operands */
+__device__ __forceinline__ void tr_stage(const bf16x8 (&v)[4], int g, uint16_t* sT, int rw) {
+#pragma unroll
+    for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sT[(mt * 16 + 4 * g + r) * kTS + rw] = (uint16_t)v[mt >> 1][(mt & 1) * 4 + r];
+}
+
+/* acc[8 jn + tk] += dZ^T (features 16(2w + jn) + .) a^T (features 16 tk + .)
+ * over the chunk's 64 rows */
+__device__ __forceinline__ void tr_wgrad(f32x4 (&acc)[16], const uint16_t* sD, const uint16_t* sA, int w, int g,
+                                         int s) {
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+        const int ro = 32 * ks + 8 * g;
+        const bf16x8 a0 = ld8(sD + (32 * w + s) * kTS + ro), a1 = ld8(sD + (32 * w + 16 + s) * kTS + ro);
+#pragma unroll
+        for (int tk = 0; tk < 8; ++tk) {
+            const bf16x8 b = ld8(sA + (16 * tk + s) * kTS + ro);
+            acc[tk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b, acc[tk], 0, 0, 0);
+            acc[8 + tk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b, acc[8 + tk], 0, 0, 0);
+        }
+    }
+}
+
+/* half the chunk's rows (32) of staged feature n */
+__device__ __forceinline__ float tr_row_sum(const uint16_t* sT, int n, int h) {
+    float t = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const bf16x8 v = ld8(sT + n * kTS + 32 * h + 8 * q);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) t += bf2f((uint16_t)v[i]);
+    }
+    return t;
+}
+
+/* The row waves' loop (16 rows each; w = 0..3). */
+__device__ __forceinline__ void tr_rows(const TrainArgs& a, uint16_t* st, const uint16_t* sw, const float* sb,
+                                        float* sred, const float* snorm, int w, int lane) {
+    const float sy[3] = {a.sdy[0], a.sdy[1], a.sdy[2]};
+    const float inv = 1.0f / (float)a.M;
+    float lsum = 0.0f;
+    const uint64_t chunks = (a.M + kTrRows - 1) / kTrRows;
+    for (uint64_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        /* opaque bases keep the weight reads and the staging addresses
+         * inside the loop (hoisted, each base + constant would take a
+         * register for the whole loop instead of an offset field) */
+        uint32_t wo = 0, so = 0;
+        int ln = lane;
+        asm volatile("" : "+v"(wo));
+        asm volatile("" : "+v"(so));
+        asm volatile("" : "+v"(ln));
+        uint16_t* sl = st + so;
+        const float* sbl = sb + so;
+        const int g = ln >> 4, s = ln & 15, rw = 16 * w + s;
+        const uint64_t row = c * kTrRows + rw;
+        const bool live = row < a.M;
+        uint64_t src = row;
+        if (a.n_src && live) {
+            uint32_t q[4];
+            spmd::philox10((uint32_t)row, a.step, 7u, 0u, (uint32_t)a.seed, (uint32_t)(a.seed >> 32), q);
+            src = ((((uint64_t)q[0] << 32) | q[1]) % a.n_src);
+        }
+        float yv[3] = {0.0f, 0.0f, 0.0f};
+        if (live)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) yv[k] = a.y[src * 3 + k] / sy[k];
+        bf16x8 bx;
+        {
+            float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (live && g < 3) {
+                const float4* xr = reinterpret_cast<const float4*>(a.x + src * kIn + 8 * g);
+                const float4 u0 = xr[0], u1 = xr[1];
+                v[0] = u0.x; v[1] = u0.y; v[2] = u0.z; v[3] = u0.w;
+                v[4] = u1.x; v[5] = u1.y; v[6] = u1.z; v[7] = u1.w;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) v[i] = (v[i] - snorm[so + 8 * g + i]) / snorm[so + kInP + 8 * g + i];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                bx[i] = bf16_rne(v[i]);
+                sl[sA0o + (8 * g + i) * kTS + rw] = (uint16_t)bx[i];
+            }
+        }
+        /* forward */
+        f32x4 acc[8];
+        bf16x8 bh[4], ds1[4], ds2[4], ds3[4];
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt)
+            acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld8(sw + wo + fL0 + (mt * 64 + ln) * 8), bx,
+                                                              f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        tr_fwd_epilogue(acc, sbl, g, sl + sA1o, rw, bh, ds1);
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            uint32_t ko = fL1 + wo;
+            asm volatile("" : "+v"(ko));
+#pragma unroll
+            for (int mt = 0; mt < 8; ++mt)
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld8(sw + ko + ((mt * 4 + kb) * 64 + ln) * 8),
+                                                                  bh[kb], acc[mt], 0, 0, 0);
+        }
+        tr_fwd_epilogue(acc, sbl + kH, g, sl + sA2o, rw, bh, ds2);
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            uint32_t ko = fL2 + wo;
+            asm volatile("" : "+v"(ko));
+#pragma unroll
+            for (int mt = 0; mt < 8; ++mt)
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld8(sw + ko + ((mt * 4 + kb) * 64 + ln) * 8),
+                                                                  bh[kb], acc[mt], 0, 0, 0);
+        }
+        tr_fwd_epilogue(acc, sbl + 2 * kH, g, sl + sA3o, rw, bh, ds3);
+        f32x4 o = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+            o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld8(sw + wo + fL3 + (kb * 64 + ln) * 8), bh[kb], o, 0, 0, 0);
+        /* W3^T for the first backward product */
+        bf16x8 wt[8];
+        uint32_t bo = bL3 + wo;
+        asm volatile("" : "+v"(bo));
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt) wt[mt] = ld8(a.wb + bo + (mt * 64 + ln) * 8);
+
+        /* loss (k_loss's expressions): lane group 0 holds outputs 0..3 of
+         * its row, group 1 outputs 4..7 */
+        float ov[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ov[r] = o[r] + sbl[3 * kH + 4 * g + r];
+        const float mu[3] = {__shfl(ov[0], s), __shfl(ov[1], s), __shfl(ov[2], s)};
+        const float lv[3] = {__shfl(ov[3], s), __shfl(ov[0], 16 + s), __shfl(ov[1], 16 + s)};
+        float gm[3] = {0.0f, 0.0f, 0.0f}, gs[3] = {0.0f, 0.0f, 0.0f};
+        if (live) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                float sv = lv[k];
+                const bool clamped = sv < -12.0f || sv > 12.0f;
+                sv = fminf(fmaxf(sv, -12.0f), 12.0f);
+                const float r = yv[k] - mu[k];
+                const float e = __expf(-sv);
+                if (g == 0) lsum += 0.5f * (r * r * e + sv);
+                gm[k] = -r * e * inv;
+                gs[k] = clamped ? 0.0f : 0.5f * (1.0f - r * r * e) * inv;
+            }
+        }
+        /* this lane's outputs 4g + r: g = 0 -> mean 0..2, log-variance 0;
+         * g = 1 -> log-variance 1, 2; the rest padding */
+        const float d4[4] = {g == 0 ? gm[0] : (g == 1 ? gs[1] : 0.0f), g == 0 ? gm[1] : (g == 1 ? gs[2] : 0.0f),
+                             g == 0 ? gm[2] : 0.0f, g == 0 ? gs[0] : 0.0f};
+        bf16x8 bd4;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            bd4[i] = bf16_rne(d4[i]);
+            bd4[4 + i] = 0;
+            sl[sD4o + (4 * g + i) * kTS + rw] = (uint16_t)bd4[i];
+        }
+        __syncthreads();  /* B1: inputs^T, a1^T..a3^T and dZ4^T staged */
+        /* dA3 = W3^T dZ4, dZ3 = dA3 SiLU'(z3) */
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt)
+            acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wt[mt], bd4, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        bf16x8 bd[4];
+        tr_bwd_epilogue(acc, ds3, bd);
+        /* W2^T, fetched once dZ3 is formed (a data dependence orders it) */
+        bf16x8 wl[4][8];
+        bo = bL2 + wo;
+        asm volatile("" : "+v"(bo) : "v"(bd[3]));
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+            for (int mt = 0; mt < 8; ++mt) wl[kb][mt] = ld8(a.wb + bo + ((mt * 4 + kb) * 64 + ln) * 8);
+        __syncthreads();  /* B2: the gradient waves have read a3^T */
+        tr_stage(bd, g, sl + sA3o, rw);
+        __syncthreads();  /* B3: dZ3^T staged */
+        /* dA2 = W2^T dZ3, dZ2 */
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+            for (int mt = 0; mt < 8; ++mt)
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[kb][mt], bd[kb], acc[mt], 0, 0, 0);
+        tr_bwd_epilogue(acc, ds2, bd);
+        bo = bL1 + wo;
+        asm volatile("" : "+v"(bo) : "v"(bd[3]));
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+            for (int mt = 0; mt < 8; ++mt) wl[kb][mt] = ld8(a.wb + bo + ((mt * 4 + kb) * 64 + ln) * 8);
+        __syncthreads();  /* B4: the gradient waves have read a2^T */
+        tr_stage(bd, g, sl + sA2o, rw);
+        __syncthreads();  /* B5: dZ2^T staged */
+        /* dA1 = W1^T dZ2, dZ1 */
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+            for (int mt = 0; mt < 8; ++mt)
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl[kb][mt], bd[kb], acc[mt], 0, 0, 0);
+        tr_bwd_epilogue(acc, ds1, bd);
+        __syncthreads();  /* B6: the gradient waves have read a1^T */
+        tr_stage(bd, g, sl + sA1o, rw);
+        __syncthreads();  /* B7: dZ1^T staged */
+        __syncthreads();  /* B8: dW0 has read the staging */
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
+    if (lane == 0) sred[w] = lsum;
+}
+
+/* The gradient waves' loop (q = 0..3; t = 0..255 their thread index). */
+__device__ __forceinline__ void tr_grads(const TrainArgs& a, const uint16_t* st, int q, int lane, int t) {
+    const int bn = t >> 1, bhf = t & 1;  /* bias sums: feature and half of the rows */
+    f32x4 gw0[4], gw1[16], gw2[16], gw3[2];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) gw1[j] = gw2[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) gw0[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    gw3[0] = gw3[1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float db0 = 0.0f, db1 = 0.0f, db2 = 0.0f, db3 = 0.0f;
+    const uint64_t chunks = (a.M + kTrRows - 1) / kTrRows;
+    for (uint64_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        uint32_t so = 0;
+        int ln = lane;
+        asm volatile("" : "+v"(so));
+        asm volatile("" : "+v"(ln));
+        const uint16_t* sl = st + so;
+        const int g = ln >> 4, s = ln & 15;
+        __syncthreads();  /* B1 */
+        /* dW3 (outputs 0..15 x inputs 16(2q + j)..), b3 */
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const int ro = 32 * ks + 8 * g;
+            const bf16x8 av = ld8(sl + sD4o + s * kTS + ro);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                gw3[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, ld8(sl + sA3o + (16 * (2 * q + j) + s) * kTS + ro),
+                                                                 gw3[j], 0, 0, 0);
+        }
+        if (t < 2 * kOutP) db3 += tr_row_sum(sl + sD4o, bn, bhf);
+        __syncthreads();  /* B2 */
+        __syncthreads();  /* B3 */
+        tr_wgrad(gw2, sl + sA3o, sl + sA2o, q, g, s);
+        db2 += tr_row_sum(sl + sA3o, bn, bhf);
+        __syncthreads();  /* B4 */
+        __syncthreads();  /* B5 */
+        tr_wgrad(gw1, sl + sA2o, sl + sA1o, q, g, s);
+        db1 += tr_row_sum(sl + sA2o, bn, bhf);
+        __syncthreads();  /* B6 */
+        __syncthreads();  /* B7 */
+        /* dW0 = dZ1^T inputs, b0 */
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const int ro = 32 * ks + 8 * g;
+#pragma unroll
+            for (int jn = 0; jn < 2; ++jn) {
+                const bf16x8 av = ld8(sl + sA1o + (32 * q + 16 * jn + s) * kTS + ro);
+#pragma unroll
+                for (int tk = 0; tk < 2; ++tk)
+                    gw0[2 * jn + tk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        av, ld8(sl + sA0o + (16 * tk + s) * kTS + ro), gw0[2 * jn + tk], 0, 0, 0);
+            }
+        }
+        db0 += tr_row_sum(sl + sA1o, bn, bhf);
+        __syncthreads();  /* B8 */
+    }
+    /* this workgroup's partial gradient, parameter order */
+    const int g = lane >> 4, s = lane & 15;
+    float* P = a.part + (size_t)blockIdx.x * kParams;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int n = 16 * (2 * q + (j >> 1)) + 4 * g + r, k = 16 * (j & 1) + s;
+            if (k < kIn) P[oW0 + n * kIn + k] = gw0[j][r];
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int n = 16 * (2 * q + (j >> 3)) + 4 * g + r, k = 16 * (j & 7) + s;
+            P[oW1 + n * kH + k] = gw1[j][r];
+            P[oW2 + n * kH + k] = gw2[j][r];
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int o = 4 * g + r, k = 16 * (2 * q + j) + s;
+            if (o < kOut) P[oW3 + o * kH + k] = gw3[j][r];
+        }
+    }
+    db0 += __shfl_xor(db0, 1);
+    db1 += __shfl_xor(db1, 1);
+    db2 += __shfl_xor(db2, 1);
+    db3 += __shfl_xor(db3, 1);
+    if (bhf == 0) {
+        P[oB0 + bn] = db0;
+        P[oB1 + bn] = db1;
+        P[oB2 + bn] = db2;
+        if (bn < kOut) P[oB3 + bn] = db3;
+    }
+}
+
+__global__ __launch_bounds__(kTrThreads, 2) void k_train_fused(TrainArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint16_t* st = reinterpret_cast<uint16_t*>(smem);
+    uint16_t* sw = reinterpret_cast<uint16_t*>(smem + kTrLdsS);
+    float* sb = reinterpret_cast<float*>(smem + kTrLdsS + (size_t)fEnd * 2);
+    float* sred = sb + fBias;
+    float* snorm = sred + 4;  /* feature means (32), then standard deviations (32) */
+    const int tid = threadIdx.x;
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(a.wf);
+        uint4* dst = reinterpret_cast<uint4*>(sw);
+        for (int e = tid; e < fEnd / 8; e += kTrThreads) dst[e] = src[e];
+        if (tid < 2 * kInP) {
+            const int c = tid & (kInP - 1);
+            snorm[tid] = tid < kInP ? (c < kIn ? a.mu[c] : 0.0f) : (c < kIn ? a.sd[c] : 1.0f);
+        }
+        for (int e = tid; e < fBias; e += kTrThreads) {
+            float v = 0.0f;
+            if (e < kH) v = a.p[oB0 + e];
+            else if (e < 2 * kH) v = a.p[oB1 + e - kH];
+            else if (e < 3 * kH) v = a.p[oB2 + e - 2 * kH];
+            else if (e - 3 * kH < kOut) v = a.p[oB3 + e - 3 * kH];
+            sb[e] = v;
+        }
+    }
+    __syncthreads();
+    const int lane = tid & 63;
+    const int wv = (int)__builtin_amdgcn_readfirstlane((uint32_t)(tid >> 6));  /* uniform: a scalar branch */
+    if (wv < 4) tr_rows(a, st, sw, sb, sred, snorm, wv, lane);
+    else tr_grads(a, st, wv - 4, lane, tid - 256);
+    __syncthreads();  /* the row waves' loss sums */
+    if (tid == 0) a.lpart[blockIdx.x] = ((sred[0] + sred[1]) + sred[2]) + sred[3];
+}
+
+/* The gradient: the nb workgroups' partials summed in a fixed order (64
+ * parameters per workgroup, each wave a quarter of the partials) and the
+ * mean loss; with lr > 0 the Adam step on the FP32 master copy, and the two
+ * BF16 packs updated in place. */
+__global__ __launch_bounds__(256) void k_train_finish(const float* __restrict__ part, int nb,
+                                                      const float* __restrict__ lpart, float inv_m,
+                                                      float* __restrict__ g, float* __restrict__ loss_out,
+                                                      float* __restrict__ p, float* __restrict__ m,
+                                                      float* __restrict__ v, uint16_t* __restrict__ wf,
+                                                      uint16_t* __restrict__ wb, float lr, float c1, float c2) {
+    __shared__ float sp[4][64];
+    const int t = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int e = blockIdx.x * 64 + t;
+    float acc = 0.0f;
+    if (e < kParams) {
+#pragma unroll 8
+        for (int b = q; b < nb; b += 4) acc += part[(size_t)b * kParams + e];
+    }
+    sp[q][t] = acc;
+    __syncthreads();
+    if (q == 0 && e < kParams) {
+        const float gg = ((sp[0][t] + sp[1][t]) + sp[2][t]) + sp[3][t];
+        g[e] = gg;
+        if (lr > 0.0f) {
+            const float b1 = 0.9f, b2 = 0.999f;
+            const float mm = b1 * m[e] + (1.0f - b1) * gg;
+            const float vv = b2 * v[e] + (1.0f - b2) * gg * gg;
+            m[e] = mm;
+            v[e] = vv;
+            const float pn = p[e] - lr * (mm / c1) / (sqrtf(vv / c2) + 1e-8f);
+            p[e] = pn;
+            int pf, pb;
+            pack_pos(e, pf, pb);
+            if (pf >= 0) wf[pf] = f2bf(pn);
+            if (pb >= 0) wb[pb] = f2bf(pn);
+        }
+    }
+    if (blockIdx.x == 0 && q == 1 && loss_out) {
+        float l = 0.0f;
+        for (int b = t; b < nb; b += 64) l += lpart[b];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+        if (t == 0) *loss_out = l * inv_m;
+    }
+}
+
 }  // namespace mlpk
 
 using namespace mlpk;
@@ -471,14 +973,16 @@ struct spmd_mlp {
     uint16_t *W0 = nullptr, *W1 = nullptr, *W2 = nullptr, *W3 = nullptr;
     uint16_t *W1t = nullptr, *W2t = nullptr, *W3t = nullptr;
     uint16_t* Wf = nullptr;  /* lane-ready weights of the fused inference pass */
+    uint16_t* Wb = nullptr;  /* lane-ready transposed weights of the fused training pass */
     int n_cu = 0;
     /* workspace */
     uint64_t cap = 0;
     uint16_t *X0 = nullptr, *Z1 = nullptr, *A1 = nullptr, *Z2 = nullptr, *A2 = nullptr, *Z3 = nullptr, *A3 = nullptr;
     uint16_t *dZ4 = nullptr, *dZ = nullptr;
     float *O = nullptr, *dA = nullptr, *P = nullptr, *Pb = nullptr, *lpart = nullptr;
-    float *Xb = nullptr, *Yb = nullptr;  /* minibatch of spmd_mlp_fit */
-    uint32_t bcap = 0;
+    /* fused training: per-workgroup partial gradients and loss sums, loss per step */
+    float *tpart = nullptr, *tloss = nullptr, *lossd = nullptr;
+    uint32_t tcap = 0, lcap = 0;
     hipStream_t st = nullptr;
     long step = 0;
 };
@@ -526,7 +1030,10 @@ int ensure(spmd_mlp* h, uint64_t M) {
     return SPMD_OK;
 }
 
-int pack_all(spmd_mlp* h) {
+/* the BF16 copies of the per-layer products (the unfused path): the fused
+ * training step does not keep them current, so the unfused entry points
+ * pack them from the FP32 parameters on every call */
+int pack_unfused(spmd_mlp* h) {
     const float* p = h->p;
     auto g = [](int n) { return dim3((n + 255) / 256); };
     hipLaunchKernelGGL(k_pack, g(kH * kInP), dim3(256), 0, h->st, p + oW0, kH, kIn, kH, kInP, h->W0,
@@ -535,7 +1042,55 @@ int pack_all(spmd_mlp* h) {
     hipLaunchKernelGGL(k_pack, g(kH * kH), dim3(256), 0, h->st, p + oW2, kH, kH, kH, kH, h->W2, h->W2t, kH);
     hipLaunchKernelGGL(k_pack, g(kH * kOutK), dim3(256), 0, h->st, p + oW3, kOut, kH, kOutP, kH, h->W3, h->W3t,
                        kOutK);
-    hipLaunchKernelGGL(k_pack_fused, g(fEnd), dim3(256), 0, h->st, p, h->Wf);
+    MCHK(h, hipGetLastError());
+    return SPMD_OK;
+}
+
+/* every BF16 copy from the FP32 parameters (after they were set) */
+int pack_all(spmd_mlp* h) {
+    auto g = [](int n) { return dim3((n + 255) / 256); };
+    if (int rc = pack_unfused(h)) return rc;
+    hipLaunchKernelGGL(k_pack_fused, g(fEnd), dim3(256), 0, h->st, h->p, h->Wf);
+    hipLaunchKernelGGL(k_pack_bwd, g(bEnd), dim3(256), 0, h->st, h->p, h->Wb);
+    MCHK(h, hipGetLastError());
+    return SPMD_OK;
+}
+
+int ensure_loss(spmd_mlp* h, uint32_t n) {
+    if (n <= h->lcap) return SPMD_OK;
+    dfr(h->lossd);
+    MCHK(h, dal(&h->lossd, n));
+    h->lcap = n;
+    return SPMD_OK;
+}
+
+/* One fused training pass over M rows (x, y; with n_src > 0 the rows are
+ * drawn from n_src source rows by Philox(seed, step)): the gradient into
+ * h->g, the mean loss into loss_dev (device, optional), and with lr > 0
+ * the Adam step with bias corrections c1, c2.  Two launches on h->st. */
+int train_fused(spmd_mlp* h, const float* x, const float* y, uint64_t n_src, uint64_t M, uint64_t seed,
+                uint32_t step, float lr, float c1, float c2, float* loss_dev) {
+    static bool attr = false;
+    if (!attr) {
+        MCHK(h, hipFuncSetAttribute((const void*)k_train_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)kTrLds));
+        attr = true;
+    }
+    const uint64_t chunks = (M + kTrRows - 1) / kTrRows;
+    const uint64_t cap = (uint64_t)(h->n_cu > 0 ? h->n_cu : 256);
+    const unsigned nb = (unsigned)(chunks < cap ? chunks : cap);
+    if (nb > h->tcap) {
+        MCHK(h, hipStreamSynchronize(h->st));
+        dfr(h->tpart);
+        dfr(h->tloss);
+        MCHK(h, dal(&h->tpart, (size_t)nb * kParams));
+        MCHK(h, dal(&h->tloss, nb));
+        h->tcap = nb;
+    }
+    TrainArgs a{x, y, n_src, M, seed, step, h->mu, h->sd, h->sdy, h->Wf, h->Wb, h->p, h->tpart, h->tloss};
+    hipLaunchKernelGGL(k_train_fused, dim3(nb), dim3(kTrThreads), kTrLds, h->st, a);
+    hipLaunchKernelGGL(k_train_finish, dim3((kParams + 63) / 64), dim3(256), 0, h->st, h->tpart, (int)nb, h->tloss,
+                       1.0f / (float)M, h->g, loss_dev, h->p, h->m, h->v, h->Wf, h->Wb, lr, c1, c2);
     MCHK(h, hipGetLastError());
     return SPMD_OK;
 }
@@ -651,7 +1206,7 @@ int spmd_mlp_create(uint64_t seed, spmd_mlp** out) {
         bad(dal(&h->m, kParams)) || bad(dal(&h->v, kParams)) || bad(dal(&h->g, kParams)) || bad(dal(&h->mu, kIn)) ||
         bad(dal(&h->sd, kIn)) || bad(dal(&h->sdy, 3)) || bad(dal(&h->W0, kH * kInP)) || bad(dal(&h->W1, kH * kH)) ||
         bad(dal(&h->W2, kH * kH)) || bad(dal(&h->W3, kOutP * kH)) || bad(dal(&h->W1t, kH * kH)) ||
-        bad(dal(&h->W2t, kH * kH)) || bad(dal(&h->W3t, kH * kOutK)) || bad(dal(&h->Wf, fEnd))) {
+        bad(dal(&h->W2t, kH * kH)) || bad(dal(&h->W3t, kH * kOutK)) || bad(dal(&h->Wf, fEnd)) || bad(dal(&h->Wb, bEnd))) {
         spmd_mlp_destroy(h);
         return SPMD_ERR_OOM;
     }
@@ -691,10 +1246,10 @@ int spmd_mlp_destroy(spmd_mlp* h) {
     if (!h) return SPMD_OK;
     if (h->st) (void)hipStreamSynchronize(h->st);
     dfr(h->p); dfr(h->m); dfr(h->v); dfr(h->g); dfr(h->mu); dfr(h->sd); dfr(h->sdy);
-    dfr(h->W0); dfr(h->W1); dfr(h->W2); dfr(h->W3); dfr(h->W1t); dfr(h->W2t); dfr(h->W3t); dfr(h->Wf);
+    dfr(h->W0); dfr(h->W1); dfr(h->W2); dfr(h->W3); dfr(h->W1t); dfr(h->W2t); dfr(h->W3t); dfr(h->Wf); dfr(h->Wb);
     dfr(h->X0); dfr(h->Z1); dfr(h->A1); dfr(h->Z2); dfr(h->A2); dfr(h->Z3); dfr(h->A3);
     dfr(h->dZ4); dfr(h->dZ); dfr(h->O); dfr(h->dA); dfr(h->P); dfr(h->Pb); dfr(h->lpart);
-    dfr(h->Xb); dfr(h->Yb);
+    dfr(h->tpart); dfr(h->tloss); dfr(h->lossd);
     if (h->st) (void)hipStreamDestroy(h->st);
     delete h;
     return SPMD_OK;
@@ -764,12 +1319,13 @@ int spmd_mlp_forward(spmd_mlp* h, const float* x_dev, float* out_dev, uint64_t M
     return SPMD_OK;
 }
 
-/* the training-path forward (four row-tiled products through the
- * workspace, the activations kept for the backward pass) */
+/* the per-layer path's forward (four row-tiled products through the
+ * workspace, the activations kept for a backward pass) */
 int spmd_mlp_forward_unfused(spmd_mlp* h, const float* x_dev, float* out_dev, uint64_t M) {
     if (!h || !x_dev || !out_dev) return SPMD_ERR_ARG;
     if (M == 0) return SPMD_OK;
     uint64_t Mp;
+    if (int rc = pack_unfused(h)) return rc;
     if (int rc = to_rows(h, x_dev, M, &Mp)) return rc;
     if (int rc = forward_rows(h, Mp)) return rc;
     MCHK(h, hipMemcpy2DAsync(out_dev, kOut * 4, h->O, kOutP * 4, kOut * 4, M, hipMemcpyDeviceToDevice, h->st));
@@ -777,10 +1333,27 @@ int spmd_mlp_forward_unfused(spmd_mlp* h, const float* x_dev, float* out_dev, ui
     return SPMD_OK;
 }
 
+/* the fused training pass without the Adam step */
 int spmd_mlp_gradient(spmd_mlp* h, const float* x_dev, const float* y_dev, uint64_t M, float* grad_host,
                       double* loss) {
     if (!h || !x_dev || !y_dev || M == 0) return SPMD_ERR_ARG;
+    if (int rc = ensure_loss(h, 1)) return rc;
+    if (int rc = train_fused(h, x_dev, y_dev, 0, M, 0, 0, 0.0f, 1.0f, 1.0f, h->lossd)) return rc;
+    if (grad_host) MCHK(h, hipMemcpyAsync(grad_host, h->g, kParams * 4ull, hipMemcpyDeviceToHost, h->st));
+    float l = 0.0f;
+    MCHK(h, hipMemcpyAsync(&l, h->lossd, 4, hipMemcpyDeviceToHost, h->st));
+    MCHK(h, hipStreamSynchronize(h->st));
+    if (loss) *loss = l;
+    return SPMD_OK;
+}
+
+/* the same gradient through the per-layer products (loss, data and weight
+ * products, SiLU' and the partial sums as separate kernels) */
+int spmd_mlp_gradient_unfused(spmd_mlp* h, const float* x_dev, const float* y_dev, uint64_t M, float* grad_host,
+                              double* loss) {
+    if (!h || !x_dev || !y_dev || M == 0) return SPMD_ERR_ARG;
     uint64_t Mp;
+    if (int rc = pack_unfused(h)) return rc;
     if (int rc = to_rows(h, x_dev, M, &Mp)) return rc;
     if (int rc = forward_rows(h, Mp)) return rc;
     if (int rc = backward_rows(h, y_dev, M, Mp, loss)) return rc;
@@ -791,37 +1364,37 @@ int spmd_mlp_gradient(spmd_mlp* h, const float* x_dev, const float* y_dev, uint6
 
 int spmd_mlp_train_step(spmd_mlp* h, const float* x_dev, const float* y_dev, uint64_t M, double lr, double* loss) {
     if (!h || !x_dev || !y_dev || M == 0 || !(lr > 0)) return SPMD_ERR_ARG;
-    uint64_t Mp;
-    if (int rc = to_rows(h, x_dev, M, &Mp)) return rc;
-    if (int rc = forward_rows(h, Mp)) return rc;
-    if (int rc = backward_rows(h, y_dev, M, Mp, loss)) return rc;
+    if (int rc = ensure_loss(h, 1)) return rc;
     h->step += 1;
-    const float b1 = 0.9f, b2 = 0.999f;
-    const float c1 = 1.0f - powf(b1, (float)h->step), c2 = 1.0f - powf(b2, (float)h->step);
-    hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, h->st, h->p, h->m, h->v, h->g, kParams,
-                       (float)lr, b1, b2, c1, c2, 1e-8f);
-    MCHK(h, hipGetLastError());
-    if (int rc = pack_all(h)) return rc;
-    if (!loss) MCHK(h, hipStreamSynchronize(h->st));
+    const float c1 = 1.0f - powf(0.9f, (float)h->step), c2 = 1.0f - powf(0.999f, (float)h->step);
+    if (int rc = train_fused(h, x_dev, y_dev, 0, M, 0, 0, (float)lr, c1, c2, loss ? h->lossd : nullptr)) return rc;
+    float l = 0.0f;
+    if (loss) MCHK(h, hipMemcpyAsync(&l, h->lossd, 4, hipMemcpyDeviceToHost, h->st));
+    MCHK(h, hipStreamSynchronize(h->st));
+    if (loss) *loss = l;
     return SPMD_OK;
 }
 
+/* Each step: one fused pass over `batch` rows drawn on the device (Philox
+ * counter (row, step, 7, 0), key seed) and the Adam step; the losses stay
+ * on the device until the end. */
 int spmd_mlp_fit(spmd_mlp* h, const float* x_dev, const float* y_dev, uint64_t N, uint32_t batch, uint32_t steps,
                  double lr, uint64_t seed, double* loss_hist) {
     if (!h || !x_dev || !y_dev || N == 0 || batch == 0 || !(lr > 0)) return SPMD_ERR_ARG;
-    if (batch > h->bcap) {
-        dfr(h->Xb); dfr(h->Yb);
-        MCHK(h, dal(&h->Xb, (size_t)batch * kIn));
-        MCHK(h, dal(&h->Yb, (size_t)batch * 3));
-        h->bcap = batch;
-    }
+    if (int rc = ensure_loss(h, loss_hist && steps ? steps : 1)) return rc;
     for (uint32_t it = 0; it < steps; ++it) {
-        hipLaunchKernelGGL(k_gather, dim3((batch + 255) / 256), dim3(256), 0, h->st, x_dev, y_dev, N, h->Xb, h->Yb,
-                           batch, seed, (uint32_t)h->step);
-        MCHK(h, hipGetLastError());
-        double l = 0.0;
-        if (int rc = spmd_mlp_train_step(h, h->Xb, h->Yb, batch, lr, loss_hist ? &l : nullptr)) return rc;
-        if (loss_hist) loss_hist[it] = l;
+        const uint32_t draw = (uint32_t)h->step;
+        h->step += 1;
+        const float c1 = 1.0f - powf(0.9f, (float)h->step), c2 = 1.0f - powf(0.999f, (float)h->step);
+        if (int rc = train_fused(h, x_dev, y_dev, N, batch, seed, draw, (float)lr, c1, c2,
+                                 loss_hist ? h->lossd + it : nullptr))
+            return rc;
+    }
+    if (loss_hist && steps) {
+        std::vector<float> l(steps);
+        MCHK(h, hipMemcpyAsync(l.data(), h->lossd, steps * 4ull, hipMemcpyDeviceToHost, h->st));
+        MCHK(h, hipStreamSynchronize(h->st));
+        for (uint32_t it = 0; it < steps; ++it) loss_hist[it] = l[it];
     }
     MCHK(h, hipStreamSynchronize(h->st));
     return SPMD_OK;

@@ -88,6 +88,51 @@ def test_gradient_matches_oracle():
         assert abs(np.linalg.norm(ga) / np.linalg.norm(gb) - 1) < 0.05, (a, np.linalg.norm(ga), np.linalg.norm(gb))
 
 
+@pytest.mark.parametrize("M", [1, 63, 64, 65, 3000, 65536])
+def test_fused_gradient_matches_unfused(M):
+    """The fused training pass (forward, loss, backward and weight-gradient
+    products in one kernel per 64-row chunk) against the per-layer products:
+    the same BF16 roundings of activations and gradients up to the order of
+    FP32 sums, so per layer the gradients agree to cosine 0.9999 and norm 1%.
+    Ragged row counts cover the partial chunk (padding rows contribute
+    nothing); 65,536 rows take four chunks per workgroup."""
+    net = _net()
+    p = (omlp.init_params(13) * 0.7).astype(np.float32)
+    net.set_params(p)
+    net.set_norm(np.linspace(-0.3, 0.3, 24).astype(np.float32), np.linspace(0.7, 1.3, 24).astype(np.float32),
+                 np.array([1.0, 2.0, 0.5], np.float32))
+    rng = np.random.default_rng(M)
+    x = rng.normal(size=(M, 24)).astype(np.float32)
+    y = (rng.normal(size=(M, 3)) * [1.0, 2.0, 0.5]).astype(np.float32)
+    lf, gf = net.gradient(x, y)
+    lu, gu = net.gradient(x, y, fused=False)
+    assert np.all(np.isfinite(gf))
+    assert abs(lf - lu) <= 1e-3 * abs(lu) + 1e-6, (lf, lu)
+    for a, b in ((0, 3200), (3200, 19712), (19712, 36224), (36224, 36998)):
+        ga, gb = gf[a:b].astype(np.float64), gu[a:b].astype(np.float64)
+        cos = ga @ gb / (np.linalg.norm(ga) * np.linalg.norm(gb))
+        assert cos > 0.9999, (M, a, cos)
+        assert abs(np.linalg.norm(ga) / np.linalg.norm(gb) - 1) < 0.01, (M, a)
+
+
+def test_train_step_keeps_weight_copies_current():
+    """The Adam step updates the FP32 parameters and, in place, the BF16
+    copies the fused kernels read: the forward pass after training equals,
+    bit for bit, the forward pass after the same parameters are set afresh
+    (which packs every copy from the FP32 parameters)."""
+    from starpolymersmd_amd.closure import ClosureData
+
+    net = _net(5)
+    rng = np.random.default_rng(8)
+    x = rng.normal(size=(20000, 24)).astype(np.float32)
+    y = (x[:, :3] * 0.5 + 0.1 * rng.normal(size=(20000, 3))).astype(np.float32)
+    hist = net.fit(ClosureData.from_numpy(x, y), steps=7, batch=4096, lr=1e-3)
+    assert np.all(np.isfinite(hist)) and hist[-1] < hist[0]
+    out_trained = net.forward(x[:3000])
+    net.set_params(net.get_params())
+    assert np.array_equal(out_trained, net.forward(x[:3000]))
+
+
 def test_adam_learns_a_known_map():
     """y = A x (3 of the 24 inputs), noise sd 0.1: the network must reach
     the noise floor's log-likelihood region (mean NLL per component of
