@@ -2477,10 +2477,36 @@ constexpr int kNbHash = 4096;
 constexpr float kNbSide = 1.5f * 1.001f;
 constexpr int kClThreads = 1024;
 constexpr int kClPer = kMaxMono / kClThreads;
-constexpr size_t kClLds = (size_t)kMaxMono * (16 + 16 + 12) + (size_t)(kNbHash + 1) * 4;
+constexpr size_t kClLds = (size_t)kMaxMono * (16 + 16 + 16) + (size_t)(kNbHash + 1) * 4;
 
 __device__ __forceinline__ uint32_t nb_bucket(int cx, int cy, int cz) {
     return (uint32_t)(cx & 15) | ((uint32_t)(cy & 15) << 4) | ((uint32_t)(cz & 15) << 8);
+}
+
+/* The six centre-of-mass sums of the workgroup at once: each is the fixed
+ * order block_sum takes (wave sums, then the waves' partials in order), so
+ * the values are block_sum's bit for bit.  Thread c < 6 adds up sum c;
+ * scratch holds 6 (NT / 64 + 1) doubles. */
+template <int NT>
+__device__ void block_sum6(double (&v)[6], double* scratch) {
+    constexpr int W = NT / 64;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) v[c] = wave_sum(v[c]);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) scratch[c * W + w] = v[c];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < W; ++i) t += scratch[threadIdx.x * W + i];
+        scratch[6 * W + threadIdx.x] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 6; ++c) v[c] = scratch[6 * W + c];
 }
 
 __global__ __launch_bounds__(kClThreads) void k_closure_features(DevState d, float* feat, const float* fmin,
@@ -2496,28 +2522,42 @@ __global__ __launch_bounds__(kClThreads) void k_closure_features(DevState d, flo
     uint32_t* prv = reinterpret_cast<uint32_t*>(svel + kMaxMono);  /* N */
     uint32_t* nxt = prv + kMaxMono;                              /* N */
     uint32_t* deg = nxt + kMaxMono;                              /* N */
-    uint32_t* hb = deg + kMaxMono;                               /* kNbHash + 1: bucket ends */
-    __shared__ double sred[NT / 64];
-    float px[kClPer], py[kClPer], pz[kClPer];
+    uint32_t* smap = deg + kMaxMono;                             /* N: each monomer's sorted slot */
+    uint32_t* hb = smap + kMaxMono;                              /* kNbHash + 1: bucket ends */
+    __shared__ double sred[6 * (NT / 64) + 6];
+    __shared__ float sbound[2 * kClosureFeatures];  /* trained range widened by 25%: low, high */
+    /* each owned monomer's state, loaded once */
+    double x[kClPer], y[kClPer], z[kClPer], vx[kClPer], vy[kClPer], vz[kClPer];
+    float im[kClPer], bt[kClPer];
     int cx[kClPer], cy[kClPer], cz[kClPer];
     uint32_t slot[kClPer];
     double cs[6] = {0, 0, 0, 0, 0, 0};
     for (int c = tid; c <= kNbHash; c += NT) hb[c] = 0u;
+    if (fmin && tid < kClosureFeatures) {
+        const float lo = fmin[tid], hi = fmax[tid], pad = 0.25f * (hi - lo);
+        sbound[tid] = lo - pad;
+        sbound[kClosureFeatures + tid] = hi + pad;
+    }
 #pragma unroll
     for (int k = 0; k < kClPer; ++k) {
         const int i = tid + k * NT;
+        x[k] = y[k] = z[k] = vx[k] = vy[k] = vz[k] = 0.0;
+        im[k] = bt[k] = 0.0f;
         if (i < N) {
             const size_t g = 3 * (size_t)(mb + i);
-            const double x = d.mpos[g], y = d.mpos[g + 1], z = d.mpos[g + 2];
-            cs[0] += x; cs[1] += y; cs[2] += z;
-            cs[3] += d.mvel[g]; cs[4] += d.mvel[g + 1]; cs[5] += d.mvel[g + 2];
-            px[k] = (float)x; py[k] = (float)y; pz[k] = (float)z;
+            x[k] = d.mpos[g]; y[k] = d.mpos[g + 1]; z[k] = d.mpos[g + 2];
+            vx[k] = d.mvel[g]; vy[k] = d.mvel[g + 1]; vz[k] = d.mvel[g + 2];
+            im[k] = (float)(1.0 / d.mmass[mb + i]);
+            bt[k] = d.mbtype[mb + i] ? 1.0f : 0.0f;
+            cs[0] += x[k]; cs[1] += y[k]; cs[2] += z[k];
+            cs[3] += vx[k]; cs[4] += vy[k]; cs[5] += vz[k];
             prv[i] = (uint32_t)i; nxt[i] = (uint32_t)i; deg[i] = 0u;
         }
     }
-    double com[6];
-    for (int c = 0; c < 6; ++c) com[c] = block_sum<NT>(cs[c], sred) / (double)(N > 0 ? N : 1);
-    __syncthreads();
+    double com[6] = {cs[0], cs[1], cs[2], cs[3], cs[4], cs[5]};
+    block_sum6<NT>(com, sred);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) com[c] /= (double)(N > 0 ? N : 1);
     const float ox = (float)com[0], oy = (float)com[1], oz = (float)com[2], inv = 1.0f / kNbSide;
     for (uint32_t q = tid; q < rep.n_bond; q += NT) {
         const uint32_t e = d.bonds[rep.bond_base + q];
@@ -2529,9 +2569,9 @@ __global__ __launch_bounds__(kClThreads) void k_closure_features(DevState d, flo
 #pragma unroll
     for (int k = 0; k < kClPer; ++k)
         if (tid + k * NT < N) {
-            cx[k] = (int)floorf((px[k] - ox) * inv);
-            cy[k] = (int)floorf((py[k] - oy) * inv);
-            cz[k] = (int)floorf((pz[k] - oz) * inv);
+            cx[k] = (int)floorf(((float)x[k] - ox) * inv);
+            cy[k] = (int)floorf(((float)y[k] - oy) * inv);
+            cz[k] = (int)floorf(((float)z[k] - oz) * inv);
             atomicAdd(&hb[nb_bucket(cx[k], cy[k], cz[k])], 1u);
         }
     __syncthreads();
@@ -2540,29 +2580,37 @@ __global__ __launch_bounds__(kClThreads) void k_closure_features(DevState d, flo
     for (int k = 0; k < kClPer; ++k) {
         const int i = tid + k * NT;
         if (i < N) {
-            const size_t g = 3 * (size_t)(mb + i);
             slot[k] = atomicAdd(&hb[nb_bucket(cx[k], cy[k], cz[k])], 1u);
-            spos[slot[k]] = make_float4(px[k], py[k], pz[k], 0.0f);
-            svel[slot[k]] = make_float4((float)d.mvel[g], (float)d.mvel[g + 1], (float)d.mvel[g + 2], 0.0f);
+            spos[slot[k]] = make_float4((float)x[k], (float)y[k], (float)z[k], 0.0f);
+            svel[slot[k]] = make_float4((float)vx[k], (float)vy[k], (float)vz[k], 0.0f);
+            smap[i] = slot[k];
         }
     }
     __syncthreads();  /* hb[b] now ends bucket b */
     const double shear = rep.box.shear;
     uint32_t oor = 0;
     static_assert(kClPer == 2, "two monomers per thread");
-    const uint32_t slot0 = slot[0], slot1 = slot[1];
+    /* the loop stays rolled (its body is large); the owned values are
+     * selected, never indexed by k, so they stay in registers */
+#define SEL(a) (k ? a[1] : a[0])
 #pragma unroll 1
     for (int k = 0; k < kClPer; ++k) {
         const int i = tid + k * NT;
         if (i >= N) continue;
-        const uint32_t mys = k ? slot1 : slot0;
-        const float4 me = spos[mys];  /* own position and cell, as sorted */
-        const float mx = me.x, my = me.y, mz = me.z;
-        const int qx = (int)floorf((mx - ox) * inv), qy = (int)floorf((my - oy) * inv),
-                  qz = (int)floorf((mz - oz) * inv);
-        const size_t g = 3 * (size_t)(mb + i);
+        const uint32_t mys = SEL(slot);
+        const double X = SEL(x), Y = SEL(y), Z = SEL(z), VX = SEL(vx), VY = SEL(vy), VZ = SEL(vz);
+        const float mx = (float)X, my = (float)Y, mz = (float)Z;
+        const int qx = SEL(cx), qy = SEL(cy), qz = SEL(cz);
         int cnt = 0;
         unsigned long long nvx = 0ull, nvy = 0ull, nvz = 0ull;
+        auto take = [&](uint32_t t, const float4& o) {
+            const float dx = o.x - mx, dy = o.y - my, dz = o.z - mz;
+            if (t != mys && __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < 2.25f) {
+                const float4 ov = svel[t];
+                ++cnt;
+                nvx += to_fix(ov.x); nvy += to_fix(ov.y); nvz += to_fix(ov.z);
+            }
+        };
         const uint32_t x0 = (uint32_t)(qx - 1) & 15u;
 #pragma unroll 1
         for (int c = 0; c < 18; ++c) {
@@ -2575,53 +2623,56 @@ __global__ __launch_bounds__(kClThreads) void k_closure_features(DevState d, flo
             else if (x0 + 2 >= 16u) { b0 = row; b1 = row + x0 + 2 - 16u; }
             else continue;
             const uint32_t s1 = hb[b1];
-            for (uint32_t t = b0 ? hb[b0 - 1] : 0u; t < s1; ++t) {
-                const float4 o = spos[t];
-                const float dx = o.x - mx, dy = o.y - my, dz = o.z - mz;
-                if (t != mys && __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) < 2.25f) {
-                    const float4 ov = svel[t];
-                    ++cnt;
-                    nvx += to_fix(ov.x); nvy += to_fix(ov.y); nvz += to_fix(ov.z);
-                }
+            uint32_t t = b0 ? hb[b0 - 1] : 0u;
+            /* two candidates' loads in flight */
+            for (; t + 1 < s1; t += 2) {
+                const float4 o0 = spos[t], o1 = spos[t + 1];
+                take(t, o0);
+                take(t + 1, o1);
             }
+            if (t < s1) take(t, spos[t]);
         }
-        const double x = d.mpos[g], y = d.mpos[g + 1], z = d.mpos[g + 2];
-        const double vx = d.mvel[g], vy = d.mvel[g + 1], vz = d.mvel[g + 2];
         const bool core = deg[i] > 2u;
         float f[kClosureFeatures];
-        f[0] = (float)(x - com[0]); f[1] = (float)(y - com[1]); f[2] = (float)(z - com[2]);
-        f[3] = (float)(vx - com[3]); f[4] = (float)(vy - com[4]); f[5] = (float)(vz - com[5]);
-        const double ux = vx - shear * y;
-        f[6] = (float)ux; f[7] = (float)vy; f[8] = (float)vz;
-        const int ip = core ? i : (int)prv[i], in = core ? i : (int)nxt[i];
-        const size_t gp = 3 * (size_t)(mb + ip), gn = 3 * (size_t)(mb + in);
-        f[9] = (float)d.mpos[gp] - mx; f[10] = (float)d.mpos[gp + 1] - my; f[11] = (float)d.mpos[gp + 2] - mz;
-        f[12] = (float)d.mpos[gn] - mx; f[13] = (float)d.mpos[gn + 1] - my; f[14] = (float)d.mpos[gn + 2] - mz;
+        f[0] = (float)(X - com[0]); f[1] = (float)(Y - com[1]); f[2] = (float)(Z - com[2]);
+        f[3] = (float)(VX - com[3]); f[4] = (float)(VY - com[4]); f[5] = (float)(VZ - com[5]);
+        const double ux = VX - shear * Y;
+        f[6] = (float)ux; f[7] = (float)VY; f[8] = (float)VZ;
+        /* the bonded neighbours' positions from their sorted LDS copies
+         * (the same FP32 values as their FP64 positions rounded) */
+        const float4 pp = spos[smap[core ? i : (int)prv[i]]], pn = spos[smap[core ? i : (int)nxt[i]]];
+        f[9] = pp.x - mx; f[10] = pp.y - my; f[11] = pp.z - mz;
+        f[12] = pn.x - mx; f[13] = pn.y - my; f[14] = pn.z - mz;
         f[15] = (float)cnt;
-        f[16] = d.mbtype[mb + i] ? 1.0f : 0.0f;
+        f[16] = SEL(bt);
         f[17] = core ? 1.0f : 0.0f;
         f[18] = (float)shear;
-        const float mvx = (float)vx, mvy = (float)vy, mvz = (float)vz;
+        const float mvx = (float)VX, mvy = (float)VY, mvz = (float)VZ;
         const double ic = cnt ? 1.0 / (double)cnt : 0.0;
         f[19] = cnt ? (float)(from_fix(nvx) * ic) - mvx : 0.0f;
         f[20] = cnt ? (float)(from_fix(nvy) * ic) - mvy : 0.0f;
         f[21] = cnt ? (float)(from_fix(nvz) * ic) - mvz : 0.0f;
-        f[22] = (float)(1.0 / d.mmass[mb + i]);
+        f[22] = SEL(im);
         f[23] = (float)deg[i];
-        /* six 16-byte stores per row (rows are 96 bytes, 16-byte aligned) */
+        /* six 16-byte stores per row (rows are 96 bytes, 16-byte aligned),
+         * each group checked against the widened range as it goes (bounds
+         * read from LDS through an opaque offset: hoisted, the 48 bounds
+         * would hold registers through the neighbour scan) */
         float4* o = reinterpret_cast<float4*>(feat + (size_t)(mb + i) * kClosureFeatures);
-#pragma unroll
-        for (int c = 0; c < kClosureFeatures / 4; ++c) o[c] = make_float4(f[4 * c], f[4 * c + 1], f[4 * c + 2], f[4 * c + 3]);
+        uint32_t bo = 0;
+        asm volatile("" : "+v"(bo));
         bool out = false;
 #pragma unroll
-        for (int c = 0; c < kClosureFeatures; ++c) {
-            if (fmin) {
-                const float lo = fmin[c], hi = fmax[c], pad = 0.25f * (hi - lo);
-                out |= f[c] < lo - pad || f[c] > hi + pad;
-            }
+        for (int c = 0; c < kClosureFeatures / 4; ++c) {
+            o[c] = make_float4(f[4 * c], f[4 * c + 1], f[4 * c + 2], f[4 * c + 3]);
+            if (fmin)
+#pragma unroll
+                for (int j = 4 * c; j < 4 * c + 4; ++j)
+                    out |= f[j] < sbound[bo + j] || f[j] > sbound[bo + kClosureFeatures + j];
         }
         oor += out ? 1u : 0u;
     }
+#undef SEL
     if (fmin && oor) atomicAdd(&d.ctl[r].closure_oor, oor);
 }
 
