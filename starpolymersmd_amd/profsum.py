@@ -3,6 +3,7 @@
     python -m starpolymersmd_amd.profsum --stats <dir>/<name>_kernel_stats.csv \
         [--fetch <fetch counter_collection.csv>] [--write <write counter_collection.csv>] \
         [--copy-fetch <csv> --copy-write <csv> --copy-bytes N] [--particles N]
+    python -m starpolymersmd_amd.profsum closure --trace <dir>/<name>_kernel_trace.csv [--stars 256]
 
 Kernel statistics come from `rocprofv3 --kernel-trace --stats --output-format
 csv`; byte counters from `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE`
@@ -75,8 +76,50 @@ def calibrate(fetch_csv, write_csv, kernel, groups, read_bytes, write_bytes, ski
     return out
 
 
+def closure_summary(trace_csv, stars=256, top=24):
+    """Markdown of a kernel trace of the closure benchmark: the whole run's
+    kernels, then the closure kernels (MD at `stars` workgroups, features,
+    forward pass, kick; by grid size) launched between the first and the
+    last `stars`-workgroup k_md launch, the closure intervals of `stars`
+    stars."""
+    rows = []
+    with open(trace_csv) as f:
+        for r in csv.DictReader(f):
+            wg = int(r.get("Workgroup_Size_X") or 1)
+            gx = int(r.get("Grid_Size_X") or 0)
+            rows.append((r["Kernel_Name"].split("(")[0].replace("void ", ""), int(r["Start_Timestamp"]),
+                         int(r["End_Timestamp"]), gx // max(wg, 1)))
+    rows.sort(key=lambda t: t[1])
+    out = []
+    tot = defaultdict(lambda: [0, 0])
+    for n, a, b, _ in rows:
+        tot[n][0] += 1
+        tot[n][1] += b - a
+    out.append("| kernel | calls | total ms | avg us |\n|---|---:|---:|---:|")
+    for n, (c, t) in sorted(tot.items(), key=lambda kv: -kv[1][1])[:top]:
+        out.append(f"| {n} | {c} | {t / 1e6:.2f} | {t / c / 1e3:.2f} |")
+    md = [i for i, r in enumerate(rows) if "k_md" in r[0] and r[3] == stars]
+    if md:
+        win = defaultdict(list)
+        for n, a, b, g in rows[md[0]:md[-1] + 1]:
+            if ("k_md" in n and g == stars) or any(k in n for k in ("k_closure_", "k_forward_fused")):
+                win[(n, g)].append(b - a)
+        out.append(f"\nLaunches of the {stars}-star intervals (first to last {stars}-workgroup k_md):\n")
+        out.append("| kernel | workgroups | launches | avg us |\n|---|---:|---:|---:|")
+        for (n, g), v in sorted(win.items(), key=lambda kv: -sum(kv[1]) / len(kv[1])):
+            out.append(f"| {n} | {g} | {len(v)} | {sum(v) / len(v) / 1e3:.2f} |")
+    return "\n".join(out)
+
+
 def main():
     import sys
+    if len(sys.argv) > 1 and sys.argv[1] == "closure":
+        ap = argparse.ArgumentParser(prog="profsum closure")
+        ap.add_argument("--trace", required=True, help="<dir>/<name>_kernel_trace.csv")
+        ap.add_argument("--stars", type=int, default=256)
+        a = ap.parse_args(sys.argv[2:])
+        print(closure_summary(a.trace, a.stars))
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "calib":
         ap = argparse.ArgumentParser(prog="profsum calib")
         ap.add_argument("--fetch", required=True)
