@@ -559,9 +559,10 @@ __device__ __forceinline__ void tr_fwd_epilogue(const f32x4 (&acc)[8], const flo
             const int n = mt * 16 + 4 * g + r;
             const float z = acc[mt][r] + bias[n];
             const float sg = __builtin_amdgcn_rcpf(1.0f + __expf(-z));
-            const short av = bf16_rne(z * sg);
+            const float a = z * sg;  /* SiLU; its derivative sg (1 + z (1 - sg)) = sg + a (1 - sg) */
+            const short av = bf16_rne(a);
             bh[mt >> 1][(mt & 1) * 4 + r] = av;
-            ds[mt >> 1][(mt & 1) * 4 + r] = bf16_rne(sg * (1.0f + z * (1.0f - sg)));
+            ds[mt >> 1][(mt & 1) * 4 + r] = bf16_rne(__builtin_fmaf(a, 1.0f - sg, sg));
             sT[n * kTS + rw] = (uint16_t)av;
         }
 }
