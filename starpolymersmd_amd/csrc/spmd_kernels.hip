@@ -1968,6 +1968,21 @@ void k_collide_rows(DevState d, double h) {
 
     /* (a) */
     const uint32_t n = load_count(cnt + lt);
+    const uint32_t tb0 = d.tbase[lt];                /* the tile's first slot in the new buffer */
+    const uint32_t tg = uniform_u32(d.ttag[lt]);     /* tagged by the redo kernel: holds monomers */
+    /* the bucket's indices, issued beside the count (the tile's CAP
+     * entries are all in bounds; those past the count are dropped below,
+     * before any is used), and this lane's cell's drawn variate and count:
+     * no later phase waits on a load these could have started */
+    uint32_t my_i[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) my_i[u] = bk[tid + u * NT];
+    uint32_t pre_n = 0u;
+    double pre_g = 0.0;
+    if ((uint32_t)tid < nc) {
+        pre_n = d.cg_n[gc0 + tid];
+        pre_g = d.cg_g[gc0 + tid];
+    }
     for (uint32_t i = tid; i < nc; i += NT) {
         s_cnt[i] = 0u;
         s_nm[i] = 0u;
@@ -1990,11 +2005,11 @@ void k_collide_rows(DevState d, double h) {
         uint32_t p = 0u;
         if (rd)
             for (int q = 0; q < NW; ++q) p += s_wsum[q];
-        s_pre = rd ? pr0 + p : d.tbase[lt];
+        s_pre = rd ? pr0 + p : tb0;
     }
     /* the replica's monomers in this tile's cells, if the redo kernel
      * tagged the tile as holding any this collision */
-    const bool tagged = uniform_u32(d.ttag[lt]) == ctl->k_cur + 1u;
+    const bool tagged = tg == ctl->k_cur + 1u;
     for (uint32_t j = tid; tagged && j < rep.n_mono; j += NT) {
         const uint32_t rel = d.mcell[rep.mono_base + j] - gc0;
         if (rel < nc) {
@@ -2026,13 +2041,11 @@ void k_collide_rows(DevState d, double h) {
     }
 
     /* (b) */
-    uint32_t my_i[PER];
     float fx[PER], fy[PER], fz[PER], fvx[PER], fvy[PER], fvz[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         const uint32_t e = (uint32_t)(tid + u * NT);
-        my_i[u] = e < n ? bk[e] : ~0u;
-        if (my_i[u] < pr0 || my_i[u] >= pr1) my_i[u] = ~0u;  /* only this replica's particles */
+        if (e >= n || my_i[u] < pr0 || my_i[u] >= pr1) my_i[u] = ~0u;  /* the tile's, this replica's */
     }
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
@@ -2142,15 +2155,16 @@ void k_collide_rows(DevState d, double h) {
      * count it was not drawn for is an internal error, reported */
     const bool drawn = ctl->cg_k == k + 1u;
     bool miss = false;
-    for (uint32_t c = tid; c < nc; c += NT) {
+    if ((uint32_t)tid < nc) {  /* TCM <= NT: one cell a lane */
+        const uint32_t c = tid;
         const uint32_t nn = s_cnt[c] + s_nm[c];
         double g = 0.0;
         if (nn >= 2) {
             if (inj) {
                 g = d.inj_gamma[gc0 + c];
             } else {
-                miss |= !drawn || d.cg_n[gc0 + c] != nn;
-                g = d.cg_g[gc0 + c];
+                miss |= !drawn || pre_n != nn;
+                g = pre_g;
             }
         }
         s_g[c] = g;
