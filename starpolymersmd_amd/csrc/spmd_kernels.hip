@@ -1748,22 +1748,22 @@ __device__ void tile_gammas(const DevState& d, int r, uint32_t lt, uint32_t n, c
 }
 
 /* After k_advect, beside the MD:
- *   - blocks < n_rtiles, one wave per row tile: every cell's Gamma variate
- *     of this collision drawn ahead of the collide for its solvent count
- *     (assuming no recentring, as k_advect does); the redo kernel draws the
- *     cells holding monomers again, and all cells of a recentred replica;
- *     the collide checks that each variate was drawn for the cell's member
- *     count
- *   - the n_rep blocks after them: the first slot of every row tile of
- *     replica r in the new buffer (exclusive scan of the bucket counts,
- *     16 per lane a pass); a recentred replica's collide sums its redo
- *     counts itself. */
+ *   - blocks < n_rep: the first slot of every row tile of replica r in the
+ *     new buffer (exclusive scan of the bucket counts, 32 per lane a
+ *     pass); a recentred replica's collide sums its redo counts itself
+ *   - the n_rtiles blocks after them, one wave per row tile: every cell's
+ *     Gamma variate of this collision drawn ahead of the collide for its
+ *     solvent count (assuming no recentring, as k_advect does); the redo
+ *     kernel draws the cells holding monomers again, and all cells of a
+ *     recentred replica; the collide checks that each variate was drawn
+ *     for the cell's member count. */
 __global__ __launch_bounds__(64) void k_cell_gamma(DevState d) {
     __shared__ uint32_t s_h[256];
-    const uint32_t b = blockIdx.x;
-    if (b >= d.n_rtiles) {
-        constexpr int KS = 16;
-        const int r = (int)(b - d.n_rtiles);
+    if (blockIdx.x < (uint32_t)d.n_rep) {
+        /* the scans first: dispatched before the tile blocks, they run
+         * beside them (32 counts a lane: two passes for 4,096 tiles) */
+        constexpr int KS = 32;
+        const int r = (int)blockIdx.x;
         const uint32_t t0 = d.rtile_base[r], nt = d.rtile_base[r + 1] - t0;
         const uint32_t* tc = d.bcount[d.cur] + t0;
         const int lane = threadIdx.x;
@@ -1790,6 +1790,7 @@ __global__ __launch_bounds__(64) void k_cell_gamma(DevState d) {
         }
         return;
     }
+    const uint32_t b = blockIdx.x - (uint32_t)d.n_rep;
     const int r = find_rep(d.rtile_base, d.n_rep, b);
     tile_gammas<64>(d, r, b, load_count(d.bcount[d.cur] + b), d.blc + (size_t)b * (uint32_t)d.rt_cap, s_h, 0u, nullptr, nullptr);
     if (b == d.reps[r].rt_base && threadIdx.x == 0) d.ctl[r].cg_k = d.ctl[r].k_cur + 1u;
