@@ -111,8 +111,50 @@ def closure_summary(trace_csv, stars=256, top=24):
     return "\n".join(out)
 
 
+def sq_summary(csvs, kernels):
+    """Markdown table of SQ counters per kernel (dispatch means, merged over
+    the passes in `csvs`): resident waves per SIMD = SQ_WAVE_CYCLES /
+    SQ_BUSY_CU_CYCLES; waiting, issuing and VALU active over
+    SQ_WAVE_CYCLES; VALU instructions per wave = SQ_INSTS_VALU / SQ_WAVES;
+    conflict cycles per LDS access = SQ_LDS_BANK_CONFLICT /
+    (SQ_LDS_IDX_ACTIVE - SQ_LDS_BANK_CONFLICT); conflict share of the
+    kernel's time = SQ_LDS_BANK_CONFLICT / (GRBM_GUI_ACTIVE / 8) / 256."""
+    per = defaultdict(lambda: defaultdict(list))
+    names = {}
+    for path in csvs:
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                full = r["Kernel_Name"].split("(")[0].replace("void ", "").replace("spmd::", "")
+                k = short(r["Kernel_Name"])
+                names.setdefault(k, full)
+                per[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    out = ["| kernel | waves/SIMD resident | waiting | issuing | VALU active | VALU instructions per wave "
+           "| LDS conflict cycles per access | LDS conflict share of kernel time |",
+           "|---|---:|---:|---:|---:|---:|---:|---:|"]
+    for k in kernels:
+        if k not in per:
+            continue
+        m = {c: sum(v) / len(v) for c, v in per[k].items()}
+        wc = m.get("SQ_WAVE_CYCLES", 0.0) or 1.0
+        lds = m.get("SQ_LDS_IDX_ACTIVE", 0.0) - m.get("SQ_LDS_BANK_CONFLICT", 0.0)
+        share = m.get("SQ_LDS_BANK_CONFLICT", 0.0) / max(m.get("GRBM_GUI_ACTIVE", 0.0) / 8.0, 1.0) / 256.0
+        out.append(f"| {names[k]} | {wc / max(m.get('SQ_BUSY_CU_CYCLES', 1.0), 1.0):.1f} "
+                   f"| {m.get('SQ_WAIT_ANY', 0.0) / wc:.0%} | {m.get('SQ_ACTIVE_INST_ANY', 0.0) / wc:.0%} "
+                   f"| {m.get('SQ_ACTIVE_INST_VALU', 0.0) / wc:.0%} "
+                   f"| {m.get('SQ_INSTS_VALU', 0.0) / max(m.get('SQ_WAVES', 1.0), 1.0):.0f} "
+                   f"| {m.get('SQ_LDS_BANK_CONFLICT', 0.0) / max(lds, 1.0):.2f} | {share:.1%} |")
+    return "\n".join(out)
+
+
 def main():
     import sys
+    if len(sys.argv) > 1 and sys.argv[1] == "sq":
+        ap = argparse.ArgumentParser(prog="profsum sq")
+        ap.add_argument("csvs", nargs="+", help="counter_collection.csv of each --pmc pass")
+        ap.add_argument("--kernels", default="k_advect,k_cell_gamma,k_rows_redo,k_collide_rows,k_md")
+        a = ap.parse_args(sys.argv[2:])
+        print(sq_summary(a.csvs, a.kernels.split(",")))
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "closure":
         ap = argparse.ArgumentParser(prog="profsum closure")
         ap.add_argument("--trace", required=True, help="<dir>/<name>_kernel_trace.csv")
