@@ -146,8 +146,40 @@ def sq_summary(csvs, kernels):
     return "\n".join(out)
 
 
+def timeline(trace_csv, last="k_collide_rows", intervals=(150, 151, 152)):
+    """Kernel start/end (us) of whole intervals, each from the end of the
+    previous `last` kernel to the end of its own, with the hardware queue
+    each ran on: shows the gaps the dependencies between queues cost."""
+    rows = []
+    with open(trace_csv) as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]),
+                         r["Kernel_Name"].split("(")[0].replace("void ", "").replace("spmd::", ""),
+                         r.get("Queue_Id") or r.get("Stream_Id") or "?"))
+    rows.sort()
+    ends = [i for i, r in enumerate(rows) if last in r[2]]
+    out = []
+    for c in intervals:
+        if c < 1 or c >= len(ends):
+            continue
+        t0 = rows[ends[c - 1]][1]
+        out.append(f"\ninterval ending with {last} #{c} (us after the previous one ended):\n")
+        out.append("| kernel | queue | start | end | duration |\n|---|---:|---:|---:|---:|")
+        for a, b, n, q in rows[ends[c - 1] + 1:ends[c] + 1]:
+            out.append(f"| {n} | {q} | {(a - t0) / 1e3:.2f} | {(b - t0) / 1e3:.2f} | {(b - a) / 1e3:.2f} |")
+    return "\n".join(out)
+
+
 def main():
     import sys
+    if len(sys.argv) > 1 and sys.argv[1] == "timeline":
+        ap = argparse.ArgumentParser(prog="profsum timeline")
+        ap.add_argument("--trace", required=True, help="<dir>/<name>_kernel_trace.csv")
+        ap.add_argument("--last", default="k_collide_rows")
+        ap.add_argument("--intervals", default="150,151,152")
+        a = ap.parse_args(sys.argv[2:])
+        print(timeline(a.trace, a.last, [int(x) for x in a.intervals.split(",")]))
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "sq":
         ap = argparse.ArgumentParser(prog="profsum sq")
         ap.add_argument("csvs", nargs="+", help="counter_collection.csv of each --pmc pass")
