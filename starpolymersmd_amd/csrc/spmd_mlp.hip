@@ -268,7 +268,7 @@ constexpr int fL2 = fL1 + 8 * 4 * 64 * 8;
 constexpr int fL3 = fL2 + 8 * 4 * 64 * 8;       /* 4 blocks x 64 x 8 */
 constexpr int fEnd = fL3 + 4 * 64 * 8;          /* 38,912 BF16 */
 constexpr int fBias = 3 * kH + 16;              /* b0, b1, b2, b3 (padded) */
-constexpr int kFwdThreads = 256;
+constexpr int kFwdThreads = 512;  /* two workgroups per CU (LDS): four waves per SIMD */
 constexpr size_t kFwdLds = (size_t)fEnd * 2 + (size_t)fBias * 4;
 
 __host__ __device__ __forceinline__ int kperm(int kb, int g, int i) {
@@ -319,7 +319,7 @@ __device__ __forceinline__ void fwd_epilogue(const f32x4 (&acc)[8][NS], const fl
  * 16 NS rows at a time, wave-strided over the grid (no synchronisation
  * after the weights are staged). */
 template <int NS>
-__global__ __launch_bounds__(kFwdThreads, 2) void k_forward_fused(const float* __restrict__ x,
+__global__ __launch_bounds__(kFwdThreads, 4) void k_forward_fused(const float* __restrict__ x,
                                                                    const float* __restrict__ mu,
                                                                    const float* __restrict__ sd,
                                                                    const uint16_t* __restrict__ wf,
@@ -1097,7 +1097,7 @@ int train_fused(spmd_mlp* h, const float* x, const float* y, uint64_t n_src, uin
 }
 
 /* the fused inference pass of M rows on stream st: x (M x 24) -> out (M x 6) */
-constexpr int kFwdRows = 2;  /* 16-row tiles per wave and pass (168 VGPRs, no spills) */
+constexpr int kFwdRows = 1;  /* 16-row tiles per wave and pass (128 VGPRs: four waves per SIMD) */
 int forward_fused(spmd_mlp* h, const float* x, float* out, uint64_t M, hipStream_t st) {
     static bool attr = false;
     if (!attr) {
