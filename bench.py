@@ -367,7 +367,7 @@ def closure_bench(replicas=256, explicit_replicas=16, check_replicas=64, check_s
         "network_inputs": list(cl.HEADLINE),
         "train": {"samples": data.n, "shear_rates": 8, "shear_range": [lo, hi], "stars_per_rate": train_stars,
                   "box": train_box, "steps": steps, "batch": 65536,
-                  "generate_seconds": t_gen, "fit_seconds": t_fit,
+                  "generate_seconds": t_gen, "fit_seconds": t_fit, "ms_per_adam_step": t_fit / steps * 1e3,
                   "loss_first": float(hist[0]), "loss_last": float(np.mean(hist[-10:]))},
         "cross_check": cross,
         "check": {"shear": check_shear, "replicas": R, "explicit_box": Le, "warmup_intervals": check_warmup,
