@@ -2713,15 +2713,18 @@ __global__ __launch_bounds__(256) void k_closure_kick(DevState d, const float* o
     const RepDev& rep = d.reps[r];
     uint32_t w[4];
     draw4(rep.seed, rep.replica, TAG_CLOSURE, m - rep.mono_base, k, 0, w);
-    const double r0 = sqrt(-2.0 * log(u01(w[0]))), r1 = sqrt(-2.0 * log(u01(w[2])));
-    const double xi[3] = {r0 * cos(2.0 * kPi * u01(w[1])), r0 * sin(2.0 * kPi * u01(w[1])),
-                          r1 * cos(2.0 * kPi * u01(w[3]))};
+    /* Box-Muller normals in FP32 with the hardware transcendentals: noise,
+     * whose few-ulp errors are far below anything the sampled distribution
+     * can show; the velocity update itself stays FP64 */
+    const float r0 = sqrtf(-2.0f * __logf((float)u01(w[0]))), r1 = sqrtf(-2.0f * __logf((float)u01(w[2])));
+    const float a1 = 6.283185307f * (float)u01(w[1]), a3 = 6.283185307f * (float)u01(w[3]);
+    const float xi[3] = {r0 * __cosf(a1), r0 * __sinf(a1), r1 * __cosf(a3)};
     const float* o = out + (size_t)m * 6;
     const float sy[3] = {sy0, sy1, sy2};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const double s = fmin(fmax((double)o[3 + c], -12.0), 12.0);
-        d.mvel[3 * (size_t)m + c] += ((double)o[c] + exp(0.5 * s) * xi[c]) * sy[c];
+        const float s = fminf(fmaxf(o[3 + c], -12.0f), 12.0f);
+        d.mvel[3 * (size_t)m + c] += ((double)o[c] + (double)(__expf(0.5f * s) * xi[c])) * sy[c];
     }
 }
 
