@@ -383,7 +383,7 @@ def regime_range(fmin, fmax, checked=RANGE_CHECKED):
 class ClosureRunner:
     """Replicas of one star advanced by MD + the learned solvent kick, with
     explicit MPC fallback for replicas whose features leave the trained
-    range, and explicit MPC as the judge of its error: every check_every-th
+    range, and explicit MPC as the measure of its error: every check_every-th
     interval, check_replicas replicas (in turn) are cross-checked by an
     explicit MPC run at L, rho=n_c_fallback from the same star state beside
     a closure run of the same length (the main run is not touched); the

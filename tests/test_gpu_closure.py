@@ -357,7 +357,7 @@ def test_closure_tracks_explicit_in_shear_flow():
     assert abs(ratio[1] - 1.0) <= 0.03 + 2 * rse[1], (ratio[1], rse[1])
     assert abs(ratio[2] - 1.0) <= 0.05 + 2 * rse[2], (ratio[2], rse[2])
 
-    # the explicit judge inside the run: every 100th interval, 8 replicas
+    # the explicit cross-check inside the run: every 100th interval, 8 replicas
     x = cl.ClosureRunner(net, data.fmin, data.fmax, shear=None, arms=arms, A=A, B=B, L=48, shears=[shear] * 16,
                          seed=700, check_every=100, check_replicas=8, check_intervals=20)
     x.advance(200, chunk=50)
