@@ -133,6 +133,40 @@ def test_train_step_keeps_weight_copies_current():
     assert np.array_equal(out_trained, net.forward(x[:3000]))
 
 
+def test_fit_draws_the_philox_rows():
+    """spmd_mlp_fit draws row b of step t's minibatch inside the fused
+    kernel as Philox4x32-10(counter (b, t, 7, 0), key = the fit's seed) mod
+    N.  The first step's recorded loss comes from the parameters before any
+    update, so it equals the FP64 oracle's loss on exactly those rows (BF16
+    tolerance); the rows of another step, from these data whose per-row
+    scale spreads 30-fold, give a loss farther away than that tolerance."""
+    from starpolymersmd_amd.closure import ClosureData, ClosureNet
+
+    rng = np.random.default_rng(21)
+    N, B = 5000, 256
+    x = rng.normal(size=(N, 24)).astype(np.float32)
+    y = (rng.normal(size=(N, 3)) * np.linspace(0.1, 3.0, N)[:, None]).astype(np.float32)
+    data = ClosureData.from_numpy(x, y)
+    net = ClosureNet(4)
+    p = net.get_params().astype(np.float64)
+    hist = net.fit(data, steps=1, batch=B, lr=1e-3)
+    seed = 4 * 1000003 + 1  # ClosureNet.fit: seed * 1000003 + its call count
+
+    def rows(step):
+        out = []
+        for b in range(B):
+            w = ob.philox([b, step, 7, 0], [seed & 0xFFFFFFFF, seed >> 32])
+            out.append(((int(w[0]) << 32) | int(w[1])) % N)
+        return np.array(out)
+
+    def loss(r):
+        return omlp.loss_and_grad(p, x[r], y[r], data.sd_y, data.mu_x, data.sd_x)[0]
+
+    l0 = loss(rows(0))
+    assert abs(hist[0] - l0) <= 2e-2 * abs(l0), (hist[0], l0)
+    assert abs(loss(rows(1)) - l0) > 4e-2 * abs(l0)
+
+
 def test_adam_learns_a_known_map():
     """y = A x (3 of the 24 inputs), noise sd 0.1: the network must reach
     the noise floor's log-likelihood region (mean NLL per component of
