@@ -1589,7 +1589,11 @@ __device__ void bin_block(const DevState& d, int r, uint64_t p0, uint64_t p1, co
         lcl[j] = (uint8_t)lc;
         uint32_t hs = (g * 0x9E3779B1u) >> 26;
         int probe = 0;
-        for (; probe < kBinHash; ++probe) {
+        /* a key, once set, never changes: a lane that reads its tile at
+         * the home slot (set by an earlier lane) skips the CAS, whose
+         * same-address lanes serialise */
+        const bool home = __hip_atomic_load(&s_key[hs], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == g;
+        for (; !home && probe < kBinHash; ++probe) {
             const uint32_t k = atomicCAS(&s_key[hs], kHashEmpty, g);
             if (k == kHashEmpty || k == g) break;
             hs = (hs + 1) & (kBinHash - 1);
